@@ -1,0 +1,1213 @@
+// fg_device.hip -- device-resident BAL evaluator and GPU Schur backend behind the C ABI.
+//
+// fg_ba mirrors the BAL Frame batch evaluator (examples/bal/model.rs:194-237) and the state
+// pools' trial buffers (src/storage.rs:282-318); fg_schur mirrors the Schur backend
+// (src/schur.rs) through the LeastSquaresBackend calls (src/optimization.rs:131-169).
+// All device memory is allocated at creation / prepare; the LM loop allocates nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+#include "../../include/fagra_gpu.h"
+#include "fg_kernels.h"
+
+using namespace fg;
+
+#define FG_TRY(expr)                                  \
+  do {                                                \
+    hipError_t e_ = (expr);                           \
+    if (e_ != hipSuccess) return FG_DEVICE_ERROR;     \
+  } while (0)
+
+#define FG_CHECK(expr)                \
+  do {                                \
+    int s_ = (expr);                  \
+    if (s_ != FG_OK) return (fg_status)s_; \
+  } while (0)
+
+namespace {
+
+const double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+template <typename P>
+hipError_t dalloc(P** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, count * sizeof(P));
+}
+template <typename P>
+void dfree(P*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+inline int grid_for(size_t n, int block = kBlock) { return (int)((n + block - 1) / block); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Problem (evaluator) state
+// ------------------------------------------------------------------------------------------
+struct fg_ba {
+  int precision = FG_FP64;
+  size_t nc = 0, np = 0, no = 0;
+  double huber = 0.0;
+  // Bit-exact reference layout (host).
+  std::vector<int64_t> cam_order, cam_off, pt_perm, pt_off;
+  // Point slots: points by track length, longest first (stable).
+  std::vector<int32_t> point_of_slot, slot_of_point;
+  int nchunks = 0;
+  size_t nslots = 0;
+  // Device static data.
+  int *d_chunk_base = nullptr, *d_chunk_len = nullptr, *d_slot_cam = nullptr, *d_slot_row = nullptr;
+  int *d_cam_off = nullptr, *d_k_slot = nullptr, *d_k_pslot = nullptr, *d_pt_len = nullptr;
+  void* d_slot_pix = nullptr;  // Vec2<T>[nslots]
+  // State: two buffers (accepted / trial) of f64 values plus compute-precision points and
+  // camera tables.
+  double *d_cams[2] = {nullptr, nullptr}, *d_pts[2] = {nullptr, nullptr};
+  void* d_pts_t[2] = {nullptr, nullptr};
+  void* d_tab[2] = {nullptr, nullptr};
+  bool tab_valid[2] = {false, false};
+  int cur = 0;
+  bool trial_active = false;
+  hipStream_t stream = nullptr;
+  // Reduction scratch.
+  double* d_parts = nullptr;
+  int* d_flags = nullptr;
+  unsigned* d_counter = nullptr;
+  double* d_out = nullptr;
+  double* h_pinned = nullptr;  // 8 doubles
+  int* h_flags = nullptr;
+  std::vector<double> h_delta_scratch;
+  double* d_delta_stage = nullptr;  // 9nc + 3np, for fg_ba_stage from host
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double cost_seconds = 0.0;
+  bool timing = false;
+};
+
+struct fg_schur {
+  fg_schur_config cfg;
+  fg_ba* ba = nullptr;
+  size_t dimension = 0;
+  bool prepared = false, linearized = false, ready = false;
+  // Model (typed device arrays stored untyped).
+  void *rec = nullptr, *res = nullptr, *rec_cm = nullptr, *res_cm = nullptr;
+  void *hpp = nullptr, *gp = nullptr, *hcc = nullptr, *gc = nullptr;
+  void *dc = nullptr, *dp = nullptr, *bs = nullptr, *cs = nullptr, *lc = nullptr, *mp = nullptr;
+  void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  void *rhs = nullptr, *zero = nullptr;
+  double *d_delta = nullptr;  // [9nc cameras | 3np points in slot order]
+  double *d_rows = nullptr;   // 2no scratch for operator products
+  double *d_parts = nullptr;  // max(nc, blocks) x kParts
+  double *d_sums = nullptr;   // kParts
+  DevScalars* d_sc = nullptr;
+  int* d_fail = nullptr;
+  int* d_flags = nullptr;
+  double* d_scalar = nullptr;
+  DevScalars* h_sc = nullptr;  // pinned mirrors
+  double* h_sums = nullptr;
+  int* h_int = nullptr;
+  std::vector<double> h_delta;  // reference order
+  // statistics / diagnostics / timings
+  size_t solves = 0, iterations = 0;
+  fg_schur_diagnostics diag;
+  fg_schur_timings tim;
+  std::vector<hipEvent_t> events;  // matvec kernel timing pool (pairs)
+  size_t events_used = 0;
+  hipEvent_t ph_a = nullptr, ph_b = nullptr;
+  double last_predicted = 0.0, last_step_norm = 0.0;
+  bool step_valid = false;
+};
+
+namespace {
+
+void reset_diag(fg_schur* s, double damping) {
+  s->diag.termination = FG_CG_NONE;
+  s->diag.iterations = 0;
+  s->diag.damping = damping;
+  s->diag.estimated_relative_residual = kNaN;
+  s->diag.reduced_relative_residual = kNaN;
+  s->diag.scaled_relative_residual = kNaN;
+  s->diag.original_relative_residual = kNaN;
+  s->diag.used_fallback = 0;
+}
+
+template <typename T>
+size_t tsize() { return sizeof(T); }
+
+// Phase timing with events on the problem stream (only when collect_timings).
+struct Phase {
+  fg_schur* s;
+  double* slot;
+  bool on;
+  Phase(fg_schur* s_, double* slot_) : s(s_), slot(slot_), on(s_->cfg.collect_timings != 0) {
+    if (on) (void)hipEventRecord(s->ph_a, s->ba->stream);
+  }
+  ~Phase() {
+    if (!on) return;
+    (void)hipEventRecord(s->ph_b, s->ba->stream);
+    (void)hipEventSynchronize(s->ph_b);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s->ph_a, s->ph_b) == hipSuccess) *slot += 1e-3 * ms;
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// Evaluator kernels launches
+// ------------------------------------------------------------------------------------------
+template <typename T>
+fg_status ensure_tables(fg_ba* b, int buf) {
+  if (b->tab_valid[buf]) return FG_OK;
+  hipLaunchKernelGGL(k_camera_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     b->d_cams[buf], (T*)b->d_tab[buf]);
+  FG_TRY(hipGetLastError());
+  b->tab_valid[buf] = true;
+  return FG_OK;
+}
+
+template <typename T>
+fg_status cost_impl(fg_ba* b, int buf, double* out) {
+  FG_CHECK(ensure_tables<T>(b, buf));
+  if (b->timing) (void)hipEventRecord(b->ev_a, b->stream);
+  FG_TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int), b->stream));
+  const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  hipLaunchKernelGGL(k_cost<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np, b->d_chunk_base,
+                     b->d_chunk_len, b->d_slot_cam, (const Vec2<T>*)b->d_slot_pix, (const T*)b->d_tab[buf],
+                     (const T*)b->d_pts_t[buf], (T)b->huber, b->d_parts, b->d_flags, b->d_counter, b->d_out);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(b->h_pinned, b->d_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  if (b->timing) (void)hipEventRecord(b->ev_b, b->stream);
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (b->timing) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, b->ev_a, b->ev_b) == hipSuccess) b->cost_seconds += 1e-3 * ms;
+  }
+  if (b->h_flags[0] != 0) return FG_INVALID_EVALUATION;
+  const double v = b->h_pinned[0];
+  if (!std::isfinite(v) || v < 0.0) return FG_INVALID_EVALUATION;
+  *out = v;
+  return FG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Backend launches
+// ------------------------------------------------------------------------------------------
+template <typename T>
+PointArgs point_args(const fg_ba* b) {
+  PointArgs a;
+  a.nchunks = b->nchunks;
+  a.np = (int)b->np;
+  a.chunk_base = b->d_chunk_base;
+  a.chunk_len = b->d_chunk_len;
+  a.slot_cam = b->d_slot_cam;
+  a.slot_row = b->d_slot_row;
+  return a;
+}
+
+template <typename T, int MODE>
+void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, T lambda,
+                   double* delta_out, double* rows_out, double* parts, const DevScalars* sc) {
+  fg_ba* b = s->ba;
+  const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
+                     (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur], fw,
+                     (const T*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda, (T*)s->tp,
+                     delta_out, rows_out, parts, sc);
+}
+
+template <typename T, int MODE>
+void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const T* fw, const T* vin,
+                    const double* delta_c, const T* rhs, T lambda, T* out, double* out_d) {
+  fg_ba* b = s->ba;
+  hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
+                     (const T*)b->d_tab[b->cur], (const T*)s->tp, delta_p, rows_in, fw, (const T*)s->bs,
+                     (const T*)s->dc, (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, s->d_sc);
+}
+
+template <typename T>
+fg_status linearize_impl(fg_schur* s) {
+  fg_ba* b = s->ba;
+  const int buf = b->cur;
+  FG_CHECK(ensure_tables<T>(b, buf));
+  FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
+  const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  hipLaunchKernelGGL(k_linearize_points<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np,
+                     b->d_chunk_base, b->d_chunk_len, b->d_slot_cam, (const Vec2<T>*)b->d_slot_pix,
+                     (const T*)b->d_tab[buf], (const T*)b->d_pts_t[buf], (T)b->huber, (Vec4<T>*)s->rec,
+                     (Vec2<T>*)s->res, (T*)s->hpp, (T*)s->gp, s->d_flags);
+  FG_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
+                     b->d_k_slot, (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[buf],
+                     (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (s->h_int[0] != 0) return FG_INVALID_EVALUATION;
+  return FG_OK;
+}
+
+template <typename T>
+fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m, double* out) {
+  fg_ba* b = s->ba;
+  FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
+  const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
+  hipLaunchKernelGGL(k_norm_inf<T>, dim3(blocks), dim3(kBlock), 0, b->stream, (int)n, a, (int)m, c, s->d_parts,
+                     s->d_flags, &s->d_sc->counter_c, s->d_scalar);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(s->h_sums, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (s->h_int[0] != 0) {
+    *out = kNaN;
+    return FG_INVALID_EVALUATION;
+  }
+  *out = s->h_sums[0];
+  return FG_OK;
+}
+
+template <typename T>
+fg_status prepare_damping_impl(fg_schur* s, double floor) {
+  fg_ba* b = s->ba;
+  Phase ph(s, &s->tim.preparation);
+  FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
+  hipLaunchKernelGGL(k_scales<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (int)b->np, (const T*)s->hcc, (const T*)s->hpp, (T)floor, (T*)s->dc, (T*)s->dp, (T*)s->bs,
+                     (T*)s->cs, s->d_flags);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (s->h_int[0] != 0) return FG_INVALID_EVALUATION;
+  return FG_OK;
+}
+
+// Factor the blocks at lambda; returns the number of failed blocks through *fails.
+template <typename T>
+fg_status factor_impl(fg_schur* s, T lambda, int* fails) {
+  fg_ba* b = s->ba;
+  FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
+  hipLaunchKernelGGL(k_factor<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (int)b->np, b->d_cam_off, b->d_pt_len, (const T*)s->bs, (const T*)s->cs, (const T*)s->dp,
+                     lambda, (T*)s->lc, (T*)s->mp, s->d_fail);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  *fails = s->h_int[0];
+  return FG_OK;
+}
+
+// S x for x in scaled retained coordinates (device), into out; also the residual against rhs
+// when resid != nullptr (sums of ||Sx - rhs||^2 and ||rhs||^2 land in s->h_sums[0..1]).
+template <typename T>
+fg_status schur_product(fg_schur* s, const T* x, T lambda, T* out, const T* resid_rhs) {
+  fg_ba* b = s->ba;
+  hipLaunchKernelGGL((k_forward_tables<T, T>), dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw);
+  launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
+  launch_cameras<T, cResid>(s, nullptr, nullptr, nullptr, x, nullptr, resid_rhs ? resid_rhs : (const T*)s->zero,
+                            lambda, out, nullptr);
+  FG_TRY(hipGetLastError());
+  s->tim.matvecs += 1;
+  return FG_OK;
+}
+
+template <typename T>
+fg_status sum_parts(fg_schur* s, int rows) {
+  fg_ba* b = s->ba;
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_parts, rows, s->d_sums);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(s->h_sums, s->d_sums, kParts * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  return FG_OK;
+}
+
+// PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
+template <typename T>
+fg_status cg_impl(fg_schur* s, T lambda) {
+  fg_ba* b = s->ba;
+  const int nc = (int)b->nc;
+  const int cblocks = grid_for(nc);
+  DevScalars init{};
+  init.max_iters = (int)s->cfg.max_iterations;
+  init.lambda = lambda;
+  FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off, (const T*)s->lc,
+                     lambda, (const T*)s->rhs, (T*)s->x, (T*)s->r, (T*)s->z, (T*)s->p, s->d_parts, s->d_sc,
+                     s->cfg.relative_tolerance);
+  FG_TRY(hipGetLastError());
+  const bool timed = s->cfg.collect_timings != 0;
+  const size_t max_it = s->cfg.max_iterations;
+  size_t launched = 0;
+  int batch = 0;
+  const int kBatch = 4;
+  bool stop = false;
+  // Pipelined batches: batch i's done flag is read while batch i+1 is queued; iterations after
+  // convergence are no-ops (every kernel checks the device done flag).
+  hipEvent_t evs[2];
+  FG_TRY(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+  FG_TRY(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+  int* flags = s->h_int;  // [2 slots x 2]
+  // Initial residual may already satisfy the tolerance.
+  FG_TRY(hipMemcpyAsync(flags + 4, &s->d_sc->done, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (flags[4]) stop = true;
+  while (!stop && launched < max_it) {
+    const size_t nb = std::min<size_t>(kBatch, max_it - launched);
+    for (size_t it = 0; it < nb; ++it) {
+      hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, (const T*)b->d_tab[b->cur],
+                         (const T*)s->dc, (const T*)s->z, (T*)s->p, (T*)s->fw, (const DevScalars*)s->d_sc);
+      const bool ev = timed && s->events_used + 2 <= s->events.size();
+      if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
+      launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, s->d_sc);
+      launch_cameras<T, cSchur>(s, nullptr, nullptr, nullptr, (const T*)s->p, nullptr, nullptr, lambda, (T*)s->q,
+                                nullptr);
+      if (ev) {
+        (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
+        s->events_used += 2;
+      }
+      hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                         (const T*)s->lc, lambda, (const T*)s->p, (const T*)s->q, (T*)s->x, (T*)s->r, (T*)s->z,
+                         s->d_parts, s->d_sc);
+    }
+    FG_TRY(hipGetLastError());
+    launched += nb;
+    const int slot = batch & 1;
+    FG_TRY(hipMemcpyAsync(flags + 2 * slot, &s->d_sc->done, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    FG_TRY(hipEventRecord(evs[slot], b->stream));
+    if (batch > 0) {
+      FG_TRY(hipEventSynchronize(evs[slot ^ 1]));
+      if (flags[2 * (slot ^ 1)]) stop = true;
+    }
+    ++batch;
+  }
+  FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  (void)hipEventDestroy(evs[0]);
+  (void)hipEventDestroy(evs[1]);
+  const DevScalars& h = *s->h_sc;
+  s->iterations += (size_t)h.iters;
+  s->diag.iterations = (size_t)h.iters;
+  if (h.term == 0) {
+    s->diag.termination = FG_CG_CONVERGED;
+    s->diag.estimated_relative_residual = h.bb == 0.0 ? std::sqrt(h.rr) : std::sqrt(h.rr / h.bb);
+  } else if (h.term == 2) {
+    s->diag.termination = FG_CG_NOT_POSITIVE_DEFINITE;
+  } else {
+    s->diag.termination = FG_CG_ITERATION_LIMIT;
+  }
+  s->tim.matvecs += (size_t)h.iters;
+  return FG_OK;
+}
+
+// Schur::attempt (schur.rs:192-202): factor + reduced rhs, CG, recover, verify.
+template <typename T>
+fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
+  fg_ba* b = s->ba;
+  const T lambda = (T)lambda_d;
+  const int nc = (int)b->nc;
+  {
+    Phase ph(s, &s->tim.factorization);
+    int fails = 0;
+    FG_CHECK(factor_impl<T>(s, lambda, &fails));
+    if (fails != 0) {
+      s->diag.termination = FG_CG_NOT_POSITIVE_DEFINITE;
+      return FG_LINEAR_SOLVE_FAILED;
+    }
+    // rhs = Dc^-1 (-gc + sum Jc^T Jp M gp)
+    launch_points<T, kRhs>(s, nullptr, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
+    launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (T*)s->rhs, nullptr);
+    FG_TRY(hipGetLastError());
+  }
+  {
+    Phase ph(s, &s->tim.linear_solve);
+    FG_CHECK(cg_impl<T>(s, lambda));
+  }
+  Phase ph(s, &s->tim.verification);
+  // recover_step (schur.rs:312-344)
+  hipLaunchKernelGGL((k_forward_tables<T, T>), dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
+                     (const T*)b->d_tab[b->cur], (const T*)s->x, (const T*)s->dc, 1, (T*)s->fw);
+  launch_points<T, kRecover>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr, nullptr,
+                             nullptr);
+  hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)(9 * b->nc),
+                     (const T*)s->x, (const T*)s->dc, s->d_delta);
+  FG_TRY(hipGetLastError());
+  // verify_step (schur.rs:346-397): reduced residual
+  FG_CHECK(schur_product<T>(s, (const T*)s->x, lambda, (T*)s->q, (const T*)s->rhs));
+  FG_CHECK(sum_parts<T>(s, nc));
+  const double resid2 = s->h_sums[0], rhs2 = s->h_sums[1];
+  s->diag.reduced_relative_residual = rhs2 == 0.0 ? std::sqrt(resid2) : std::sqrt(resid2 / rhs2);
+  // full-model verification in original coordinates (linear_model.rs:254-282)
+  hipLaunchKernelGGL((k_forward_tables<T, double>), dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+  const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
+                            s->d_parts, nullptr);
+  FG_CHECK(sum_parts<T>(s, pblocks));
+  double pt[kParts];
+  std::memcpy(pt, s->h_sums, sizeof(pt));
+  launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, nullptr, s->d_delta, nullptr,
+                             lambda, nullptr, nullptr);
+  FG_CHECK(sum_parts<T>(s, nc));
+  const double* cm = s->h_sums;
+  // pt: n2, n2s, g.d, g2, g2s, ||J delta||^2 ; cm: n2, n2s, g.d, g2, g2s
+  const double nr2 = pt[0] + cm[0], nr2s = pt[1] + cm[1], gd = pt[2] + cm[2], g2 = pt[3] + cm[3],
+               g2s = pt[4] + cm[4], jd2 = pt[5];
+  const double pred = -gd - 0.5 * jd2;
+  double step_norm = 0.0;
+  const fg_status nst = norm_inf_impl<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0, &step_norm);
+  if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
+  const bool finite_step = nst == FG_OK;
+  s->diag.original_relative_residual = g2 == 0.0 ? std::sqrt(nr2) : std::sqrt(nr2 / g2);
+  s->diag.scaled_relative_residual = g2s == 0.0 ? std::sqrt(nr2s) : std::sqrt(nr2s / g2s);
+  s->last_step_norm = finite_step ? step_norm : kNaN;
+  if (!finite_step || !std::isfinite(pred) || !std::isfinite(s->diag.reduced_relative_residual) ||
+      !std::isfinite(s->diag.scaled_relative_residual) || !std::isfinite(s->diag.original_relative_residual)) {
+    s->diag.termination = FG_CG_NON_FINITE;
+    return FG_LINEAR_SOLVE_FAILED;
+  }
+  if (s->diag.termination != FG_CG_CONVERGED) return FG_LINEAR_SOLVE_FAILED;
+  *predicted = pred;
+  return FG_OK;
+}
+
+template <typename T>
+fg_status alloc_model(fg_schur* s) {
+  fg_ba* b = s->ba;
+  const size_t nc = b->nc, np = b->np, no = b->no, ns = b->nslots;
+  FG_TRY(dalloc((Vec4<T>**)&s->rec, ns));
+  FG_TRY(dalloc((Vec2<T>**)&s->res, ns));
+  FG_TRY(hipMemset(s->rec, 0, ns * sizeof(Vec4<T>)));
+  FG_TRY(hipMemset(s->res, 0, ns * sizeof(Vec2<T>)));
+  FG_TRY(dalloc((Vec4<T>**)&s->rec_cm, no));
+  FG_TRY(dalloc((Vec2<T>**)&s->res_cm, no));
+  FG_TRY(dalloc((T**)&s->hpp, 6 * np));
+  FG_TRY(dalloc((T**)&s->gp, 3 * np));
+  FG_TRY(dalloc((T**)&s->hcc, 45 * nc));
+  FG_TRY(dalloc((T**)&s->gc, 9 * nc));
+  FG_TRY(dalloc((T**)&s->dc, 9 * nc));
+  FG_TRY(dalloc((T**)&s->dp, 3 * np));
+  FG_TRY(dalloc((T**)&s->bs, 45 * nc));
+  FG_TRY(dalloc((T**)&s->cs, 6 * np));
+  FG_TRY(dalloc((T**)&s->lc, 45 * nc));
+  FG_TRY(dalloc((T**)&s->mp, 6 * np));
+  FG_TRY(dalloc((T**)&s->tp, 3 * np));
+  FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
+  for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
+    FG_TRY(dalloc((T**)v, 9 * nc));
+    FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(T)));
+  }
+  FG_TRY(dalloc(&s->d_delta, 9 * nc + 3 * np));
+  FG_TRY(dalloc(&s->d_rows, 2 * no));
+  const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  const size_t rows = std::max({nc, pblocks, (size_t)1024});
+  FG_TRY(dalloc(&s->d_parts, kParts * rows));
+  return FG_OK;
+}
+
+void free_model(fg_schur* s) {
+  for (void** v : {&s->rec, &s->res, &s->rec_cm, &s->res_cm, &s->hpp, &s->gp, &s->hcc, &s->gc, &s->dc, &s->dp,
+                   &s->bs, &s->cs, &s->lc, &s->mp, &s->tp, &s->fw, &s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs,
+                   &s->zero}) {
+    if (*v) (void)hipFree(*v);
+    *v = nullptr;
+  }
+  dfree(s->d_delta);
+  dfree(s->d_rows);
+  dfree(s->d_parts);
+}
+
+// Reference-order delta from the device layout (points in slot order).
+fg_status download_delta(fg_schur* s, double* host) {
+  fg_ba* b = s->ba;
+  const size_t nc = b->nc, np = b->np;
+  std::vector<double>& tmp = b->h_delta_scratch;
+  tmp.resize(9 * nc + 3 * np);
+  FG_TRY(hipMemcpyAsync(tmp.data(), s->d_delta, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  std::memcpy(host, tmp.data(), 9 * nc * sizeof(double));
+  for (size_t j = 0; j < np; ++j) {
+    const size_t sl = (size_t)b->slot_of_point[j];
+    for (int a = 0; a < 3; ++a) host[9 * nc + 3 * j + a] = tmp[9 * nc + 3 * sl + a];
+  }
+  return FG_OK;
+}
+fg_status upload_vector(fg_ba* b, const double* host, double* dev) {
+  const size_t nc = b->nc, np = b->np;
+  std::vector<double>& tmp = b->h_delta_scratch;
+  tmp.resize(9 * nc + 3 * np);
+  std::memcpy(tmp.data(), host, 9 * nc * sizeof(double));
+  for (size_t j = 0; j < np; ++j) {
+    const size_t sl = (size_t)b->slot_of_point[j];
+    for (int a = 0; a < 3; ++a) tmp[9 * nc + 3 * sl + a] = host[9 * nc + 3 * j + a];
+  }
+  FG_TRY(hipMemcpyAsync(dev, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  return FG_OK;
+}
+
+template <typename T>
+fg_status stage_impl(fg_ba* b, const double* d_delta) {
+  const int other = b->cur ^ 1;
+  const size_t n = 9 * b->nc + 3 * b->np;
+  hipLaunchKernelGGL(k_stage<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, (int)b->nc, (int)b->np,
+                     b->d_cams[b->cur], b->d_pts[b->cur], d_delta, d_delta + 9 * b->nc, b->d_cams[other],
+                     b->d_pts[other], (T*)b->d_pts_t[other]);
+  FG_TRY(hipGetLastError());
+  b->tab_valid[other] = false;
+  b->trial_active = true;
+  return FG_OK;
+}
+
+template <typename T>
+fg_status reduced_apply_impl(fg_schur* s, double lambda, const double* x, double* y) {
+  fg_ba* b = s->ba;
+  const size_t n = 9 * b->nc;
+  int fails = 0;
+  FG_CHECK(factor_impl<T>(s, (T)lambda, &fails));
+  if (fails) return FG_LINEAR_SOLVE_FAILED;
+  std::vector<T> hx(x, x + n), hy(n);
+  FG_TRY(hipMemcpyAsync(s->p, hx.data(), n * sizeof(T), hipMemcpyHostToDevice, b->stream));
+  FG_CHECK(schur_product<T>(s, (const T*)s->p, (T)lambda, (T*)s->q, nullptr));
+  FG_TRY(hipMemcpyAsync(hy.data(), s->q, n * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < n; ++i) y[i] = (double)hy[i];
+  return FG_OK;
+}
+
+template <typename T>
+fg_status jacobian_apply_impl(fg_schur* s, const double* x, double* y) {
+  fg_ba* b = s->ba;
+  FG_CHECK(upload_vector(b, x, s->d_delta));
+  hipLaunchKernelGGL((k_forward_tables<T, double>), dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+  launch_points<T, kJv>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, (T)0, nullptr, s->d_rows, nullptr,
+                        nullptr);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(y, s->d_rows, 2 * b->no * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  return FG_OK;
+}
+
+template <typename T>
+fg_status jacobian_transpose_apply_impl(fg_schur* s, const double* y, double* x) {
+  fg_ba* b = s->ba;
+  FG_TRY(hipMemcpyAsync(s->d_rows, y, 2 * b->no * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  launch_points<T, kJtu>(s, nullptr, nullptr, s->d_rows, (T)0, s->d_delta + 9 * b->nc, nullptr, nullptr, nullptr);
+  launch_cameras<T, cJtu>(s, nullptr, s->d_rows, nullptr, nullptr, nullptr, nullptr, (T)0, nullptr, s->d_delta);
+  FG_TRY(hipGetLastError());
+  return download_delta(s, x);
+}
+
+template <typename T>
+fg_status model_impl(fg_schur* s, double* gradient, double* scales, double* negres) {
+  fg_ba* b = s->ba;
+  const size_t nc = b->nc, np = b->np, no = b->no;
+  std::vector<T> gc(9 * nc), gp(3 * np), dc(9 * nc), dp(3 * np);
+  std::vector<Vec2<T>> res(no);
+  FG_TRY(hipMemcpyAsync(gc.data(), s->gc, gc.size() * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(gp.data(), s->gp, gp.size() * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(dc.data(), s->dc, dc.size() * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(dp.data(), s->dp, dp.size() * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(res.data(), s->res_cm, no * sizeof(Vec2<T>), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < 9 * nc; ++i) {
+    if (gradient) gradient[i] = gc[i];
+    if (scales) scales[i] = dc[i];
+  }
+  for (size_t j = 0; j < np; ++j) {
+    const size_t sl = (size_t)b->slot_of_point[j];
+    for (int a = 0; a < 3; ++a) {
+      if (gradient) gradient[9 * nc + 3 * j + a] = gp[3 * sl + a];
+      if (scales) scales[9 * nc + 3 * j + a] = dp[3 * sl + a];
+    }
+  }
+  if (negres)
+    for (size_t k = 0; k < no; ++k) {
+      negres[2 * k] = -(double)res[k].x;
+      negres[2 * k + 1] = -(double)res[k].y;
+    }
+  return FG_OK;
+}
+
+template <typename T>
+fg_status gram_impl(fg_schur* s, double* gram) {
+  fg_ba* b = s->ba;
+  std::vector<T> bs(45 * b->nc);
+  FG_TRY(hipMemcpyAsync(bs.data(), s->bs, bs.size() * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  for (size_t c = 0; c < b->nc; ++c)
+    for (int i = 0; i < 9; ++i)
+      for (int j = 0; j < 9; ++j) gram[81 * c + 9 * i + j] = bs[45 * c + sym9(i, j)];
+  return FG_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+fg_status fg_device_info(char* name, size_t name_len, int* cus, size_t* lds_bytes) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  FG_TRY(hipGetDevice(&dev));
+  FG_TRY(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len) {
+    std::strncpy(name, prop.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (lds_bytes) *lds_bytes = prop.sharedMemPerBlock;
+  return FG_OK;
+}
+
+fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, const double* points,
+                       const int64_t* obs_camera, const int64_t* obs_point, const double* obs_pixel, double huber,
+                       int precision, fg_ba** out) {
+  *out = nullptr;
+  if (precision != FG_FP64 && precision != FG_FP32) return FG_INVALID_OPTIONS;
+  if (!std::isfinite(huber) || huber < 0.0) return FG_INVALID_OPTIONS;
+  if (nc == 0 || np == 0 || no == 0) return FG_DIMENSION_MISMATCH;  // "empty BAL problem"
+  if (no > (size_t)std::numeric_limits<int>::max() / 2 || np > (size_t)std::numeric_limits<int>::max() / 4)
+    return FG_DIMENSION_MISMATCH;
+  for (size_t o = 0; o < no; ++o)
+    if (obs_camera[o] < 0 || (size_t)obs_camera[o] >= nc || obs_point[o] < 0 || (size_t)obs_point[o] >= np ||
+        !std::isfinite(obs_pixel[2 * o]) || !std::isfinite(obs_pixel[2 * o + 1]))
+      return FG_INVALID_EMISSION;
+  fg_ba* b = new fg_ba();
+  b->precision = precision;
+  b->nc = nc;
+  b->np = np;
+  b->no = no;
+  b->huber = huber;
+  // Row order: one Frame batch per camera, payloads in insertion order (examples/bal.rs:189-201).
+  b->cam_off.assign(nc + 1, 0);
+  for (size_t o = 0; o < no; ++o) b->cam_off[obs_camera[o] + 1]++;
+  for (size_t c = 0; c < nc; ++c) b->cam_off[c + 1] += b->cam_off[c];
+  b->cam_order.assign(no, 0);
+  {
+    std::vector<int64_t> fill(b->cam_off.begin(), b->cam_off.end() - 1);
+    for (size_t o = 0; o < no; ++o) b->cam_order[fill[obs_camera[o]]++] = (int64_t)o;
+  }
+  // Point permutation: stable by point over the camera-ordered rows (block_factors.rs:52-98).
+  b->pt_off.assign(np + 1, 0);
+  for (size_t k = 0; k < no; ++k) b->pt_off[obs_point[b->cam_order[k]] + 1]++;
+  for (size_t p = 0; p < np; ++p) b->pt_off[p + 1] += b->pt_off[p];
+  b->pt_perm.assign(no, 0);
+  {
+    std::vector<int64_t> fill(b->pt_off.begin(), b->pt_off.end() - 1);
+    for (size_t k = 0; k < no; ++k) b->pt_perm[fill[obs_point[b->cam_order[k]]]++] = (int64_t)k;
+  }
+  // Point slots: stable counting sort by track length, longest first.
+  int64_t maxlen = 0;
+  for (size_t p = 0; p < np; ++p) maxlen = std::max(maxlen, b->pt_off[p + 1] - b->pt_off[p]);
+  {
+    std::vector<int64_t> cnt(maxlen + 2, 0);
+    for (size_t p = 0; p < np; ++p) cnt[maxlen - (b->pt_off[p + 1] - b->pt_off[p]) + 1]++;
+    for (int64_t l = 0; l <= maxlen; ++l) cnt[l + 1] += cnt[l];
+    b->point_of_slot.assign(np, 0);
+    b->slot_of_point.assign(np, 0);
+    for (size_t p = 0; p < np; ++p) {
+      const int64_t key = maxlen - (b->pt_off[p + 1] - b->pt_off[p]);
+      const int64_t sl = cnt[key]++;
+      b->point_of_slot[sl] = (int32_t)p;
+      b->slot_of_point[p] = (int32_t)sl;
+    }
+  }
+  b->nchunks = (int)((np + kWave - 1) / kWave);
+  std::vector<int> chunk_base(b->nchunks + 1, 0), chunk_len(b->nchunks, 0);
+  for (int q = 0; q < b->nchunks; ++q) {
+    const int32_t p = b->point_of_slot[(size_t)q * kWave];
+    chunk_len[q] = (int)(b->pt_off[p + 1] - b->pt_off[p]);
+    chunk_base[q + 1] = chunk_base[q] + chunk_len[q] * kWave;
+  }
+  b->nslots = (size_t)chunk_base[b->nchunks];
+  std::vector<int> slot_cam(b->nslots, -1), slot_row(b->nslots, -1), k_slot(no), k_pslot(no), pt_len(np);
+  std::vector<double> slot_pix(2 * b->nslots, 0.0);
+  for (size_t sl = 0; sl < np; ++sl) {
+    const int32_t p = b->point_of_slot[sl];
+    const size_t q = sl / kWave, lane = sl % kWave;
+    pt_len[sl] = (int)(b->pt_off[p + 1] - b->pt_off[p]);
+    for (int64_t j = 0; j < pt_len[sl]; ++j) {
+      const int64_t k = b->pt_perm[b->pt_off[p] + j];
+      const size_t s = (size_t)chunk_base[q] + (size_t)j * kWave + lane;
+      const int64_t o = b->cam_order[k];
+      slot_cam[s] = (int)obs_camera[o];
+      slot_row[s] = (int)k;
+      slot_pix[2 * s] = obs_pixel[2 * o];
+      slot_pix[2 * s + 1] = obs_pixel[2 * o + 1];
+      k_slot[k] = (int)s;
+      k_pslot[k] = (int)sl;
+    }
+  }
+  std::vector<int> cam_off32(nc + 1);
+  for (size_t c = 0; c <= nc; ++c) cam_off32[c] = (int)b->cam_off[c];
+  auto fail = [&](fg_status st) {
+    fg_ba_destroy(b);
+    return st;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  auto up = [&](int** dst, const std::vector<int>& v) {
+    return dalloc(dst, v.size()) == hipSuccess &&
+           hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up(&b->d_chunk_base, chunk_base) || !up(&b->d_chunk_len, chunk_len) || !up(&b->d_slot_cam, slot_cam) ||
+      !up(&b->d_slot_row, slot_row) || !up(&b->d_cam_off, cam_off32) || !up(&b->d_k_slot, k_slot) ||
+      !up(&b->d_k_pslot, k_pslot) || !up(&b->d_pt_len, pt_len))
+    return fail(FG_DEVICE_ERROR);
+  const size_t ts = precision == FG_FP32 ? sizeof(float) : sizeof(double);
+  if (hipMalloc(&b->d_slot_pix, std::max<size_t>(1, 2 * b->nslots) * ts) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  if (precision == FG_FP32) {
+    std::vector<float> f(slot_pix.begin(), slot_pix.end());
+    if (hipMemcpy(b->d_slot_pix, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(FG_DEVICE_ERROR);
+  } else if (hipMemcpy(b->d_slot_pix, slot_pix.data(), slot_pix.size() * sizeof(double), hipMemcpyHostToDevice) !=
+             hipSuccess) {
+    return fail(FG_DEVICE_ERROR);
+  }
+  for (int i = 0; i < 2; ++i) {
+    if (dalloc(&b->d_cams[i], 9 * nc) != hipSuccess || dalloc(&b->d_pts[i], 3 * np) != hipSuccess ||
+        hipMalloc(&b->d_pts_t[i], 3 * np * ts) != hipSuccess ||
+        hipMalloc(&b->d_tab[i], kCamTab * nc * ts) != hipSuccess)
+      return fail(FG_DEVICE_ERROR);
+  }
+  const size_t rows = std::max<size_t>(1024, (size_t)grid_for(b->nchunks, kChunksPerBlock));
+  if (dalloc(&b->d_parts, rows) != hipSuccess || dalloc(&b->d_flags, 4) != hipSuccess ||
+      dalloc(&b->d_counter, 4) != hipSuccess || dalloc(&b->d_out, 4) != hipSuccess ||
+      dalloc(&b->d_delta_stage, 9 * nc + 3 * np) != hipSuccess)
+    return fail(FG_DEVICE_ERROR);
+  if (hipMemset(b->d_counter, 0, 4 * sizeof(unsigned)) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  if (hipHostMalloc((void**)&b->h_pinned, 8 * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_flags, 8 * sizeof(int), 0) != hipSuccess)
+    return fail(FG_DEVICE_ERROR);
+  if (hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  fg_status st = fg_ba_set_state(b, cameras, points);
+  if (st != FG_OK) return fail(st);
+  *out = b;
+  return FG_OK;
+}
+
+void fg_ba_destroy(fg_ba* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  dfree(b->d_chunk_base);
+  dfree(b->d_chunk_len);
+  dfree(b->d_slot_cam);
+  dfree(b->d_slot_row);
+  dfree(b->d_cam_off);
+  dfree(b->d_k_slot);
+  dfree(b->d_k_pslot);
+  dfree(b->d_pt_len);
+  if (b->d_slot_pix) (void)hipFree(b->d_slot_pix);
+  for (int i = 0; i < 2; ++i) {
+    dfree(b->d_cams[i]);
+    dfree(b->d_pts[i]);
+    if (b->d_pts_t[i]) (void)hipFree(b->d_pts_t[i]);
+    if (b->d_tab[i]) (void)hipFree(b->d_tab[i]);
+  }
+  dfree(b->d_parts);
+  dfree(b->d_flags);
+  dfree(b->d_counter);
+  dfree(b->d_out);
+  dfree(b->d_delta_stage);
+  if (b->h_pinned) (void)hipHostFree(b->h_pinned);
+  if (b->h_flags) (void)hipHostFree(b->h_flags);
+  if (b->ev_a) (void)hipEventDestroy(b->ev_a);
+  if (b->ev_b) (void)hipEventDestroy(b->ev_b);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+fg_status fg_ba_layout(const fg_ba* b, int64_t* cam_order, int64_t* cam_offsets, int64_t* pt_perm,
+                       int64_t* pt_offsets) {
+  if (cam_order) std::copy(b->cam_order.begin(), b->cam_order.end(), cam_order);
+  if (cam_offsets) std::copy(b->cam_off.begin(), b->cam_off.end(), cam_offsets);
+  if (pt_perm) std::copy(b->pt_perm.begin(), b->pt_perm.end(), pt_perm);
+  if (pt_offsets) std::copy(b->pt_off.begin(), b->pt_off.end(), pt_offsets);
+  return FG_OK;
+}
+
+fg_status fg_ba_set_state(fg_ba* b, const double* cameras, const double* points) {
+  const size_t nc = b->nc, np = b->np;
+  for (size_t i = 0; i < 9 * nc; ++i)
+    if (!std::isfinite(cameras[i])) return FG_INVALID_EVALUATION;
+  for (size_t i = 0; i < 3 * np; ++i)
+    if (!std::isfinite(points[i])) return FG_INVALID_EVALUATION;
+  std::vector<double> pts(3 * np);
+  for (size_t j = 0; j < np; ++j)
+    for (int a = 0; a < 3; ++a) pts[3 * (size_t)b->slot_of_point[j] + a] = points[3 * j + a];
+  b->trial_active = false;
+  b->cur = 0;
+  FG_TRY(hipMemcpyAsync(b->d_cams[0], cameras, 9 * nc * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  FG_TRY(hipMemcpyAsync(b->d_pts[0], pts.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  const int n = (int)(3 * np);
+  if (b->precision == FG_FP32)
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, n, b->d_pts[0],
+                       (float*)b->d_pts_t[0]);
+  else
+    hipLaunchKernelGGL(k_convert<double>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, n, b->d_pts[0],
+                       (double*)b->d_pts_t[0]);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipStreamSynchronize(b->stream));
+  b->tab_valid[0] = b->tab_valid[1] = false;
+  return FG_OK;
+}
+
+fg_status fg_ba_get_state(fg_ba* b, double* cameras, double* points) {
+  const size_t nc = b->nc, np = b->np;
+  std::vector<double> pts(3 * np);
+  FG_TRY(hipMemcpyAsync(cameras, b->d_cams[b->cur], 9 * nc * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(pts.data(), b->d_pts[b->cur], 3 * np * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  for (size_t j = 0; j < np; ++j)
+    for (int a = 0; a < 3; ++a) points[3 * j + a] = pts[3 * (size_t)b->slot_of_point[j] + a];
+  return FG_OK;
+}
+
+fg_status fg_ba_cost(fg_ba* b, int trial, double* out) {
+  int buf = b->cur;
+  if (trial) {
+    if (!b->trial_active) return FG_INVALID_OPTIONS;
+    buf ^= 1;
+  }
+  return b->precision == FG_FP32 ? cost_impl<float>(b, buf, out) : cost_impl<double>(b, buf, out);
+}
+
+fg_status fg_ba_stage(fg_ba* b, const double* delta) {
+  FG_CHECK(upload_vector(b, delta, b->d_delta_stage));
+  return b->precision == FG_FP32 ? stage_impl<float>(b, b->d_delta_stage) : stage_impl<double>(b, b->d_delta_stage);
+}
+
+void fg_ba_accept(fg_ba* b) {
+  if (b->trial_active) {
+    b->cur ^= 1;
+    b->trial_active = false;
+  }
+}
+void fg_ba_reject(fg_ba* b) { b->trial_active = false; }
+
+fg_status fg_schur_config_default(fg_schur_config* cfg, int precision) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->max_iterations = 1000;
+  // max(1e-10, 128 eps) of the scalar (src/schur.rs:156-157)
+  cfg->relative_tolerance = precision == FG_FP32 ? 128.0 * 1.1920928955078125e-07 : 1e-10;
+  cfg->collect_timings = 0;
+  cfg->precision = precision;
+  cfg->preconditioner = FG_PRECOND_BLOCK_JACOBI;
+  cfg->fallback_factor = 0.5;
+  cfg->fallback_iterations = 10;
+  return FG_OK;
+}
+
+fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
+  *out = nullptr;
+  if (cfg->precision != FG_FP64 && cfg->precision != FG_FP32) return FG_INVALID_OPTIONS;
+  fg_schur* s = new fg_schur();
+  s->cfg = *cfg;
+  reset_diag(s, 0.0);
+  std::memset(&s->tim, 0, sizeof(s->tim));
+  auto fail = [&]() {
+    fg_schur_destroy(s);
+    return FG_DEVICE_ERROR;
+  };
+  if (dalloc(&s->d_sc, 1) != hipSuccess || dalloc(&s->d_fail, 4) != hipSuccess ||
+      dalloc(&s->d_flags, 4) != hipSuccess || dalloc(&s->d_scalar, 4) != hipSuccess ||
+      dalloc(&s->d_sums, kParts) != hipSuccess)
+    return fail();
+  if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess) return fail();
+  if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_sums, 2 * kParts * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_int, 8 * sizeof(int), 0) != hipSuccess)
+    return fail();
+  if (hipEventCreate(&s->ph_a) != hipSuccess || hipEventCreate(&s->ph_b) != hipSuccess) return fail();
+  *out = s;
+  return FG_OK;
+}
+
+void fg_schur_destroy(fg_schur* s) {
+  if (!s) return;
+  if (s->ba && s->ba->stream) (void)hipStreamSynchronize(s->ba->stream);
+  free_model(s);
+  dfree(s->d_sc);
+  dfree(s->d_fail);
+  dfree(s->d_flags);
+  dfree(s->d_scalar);
+  dfree(s->d_sums);
+  if (s->h_sc) (void)hipHostFree(s->h_sc);
+  if (s->h_sums) (void)hipHostFree(s->h_sums);
+  if (s->h_int) (void)hipHostFree(s->h_int);
+  for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+  if (s->ph_a) (void)hipEventDestroy(s->ph_a);
+  if (s->ph_b) (void)hipEventDestroy(s->ph_b);
+  delete s;
+}
+
+fg_status fg_schur_set_config(fg_schur* s, const fg_schur_config* cfg) {
+  if (cfg->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
+  s->cfg = *cfg;
+  return FG_OK;
+}
+
+// Bind a problem: (re)allocate the model arrays for its sizes. No allocation when the same
+// problem is bound again.
+static fg_status bind(fg_schur* s, fg_ba* b) {
+  if (b == nullptr || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
+  if (s->ba == b) return FG_OK;
+  if (s->ba) free_model(s);
+  s->ba = b;
+  fg_status st = s->cfg.precision == FG_FP32 ? alloc_model<float>(s) : alloc_model<double>(s);
+  if (st != FG_OK) {
+    free_model(s);
+    s->ba = nullptr;
+  }
+  return st;
+}
+
+// prepare (schur.rs:409-431): resets statistics, validates the partition and options.
+fg_status fg_schur_prepare(fg_schur* s, fg_ba* b, size_t dimension) {
+  s->prepared = s->linearized = s->ready = false;
+  s->solves = s->iterations = 0;
+  reset_diag(s, 0.0);
+  std::memset(&s->tim, 0, sizeof(s->tim));
+  if (b == nullptr || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
+  const size_t retained = 9 * b->nc;
+  if (retained > dimension || dimension - retained != 3 * b->np || s->cfg.max_iterations == 0 ||
+      !(s->cfg.relative_tolerance >= 0.0 && s->cfg.relative_tolerance < 1.0) ||
+      s->cfg.max_iterations > (size_t)std::numeric_limits<int>::max())
+    return FG_INVALID_OPTIONS;
+  FG_CHECK(bind(s, b));
+  if (s->cfg.collect_timings && s->events.empty()) {
+    s->events.resize(8192);
+    for (auto& e : s->events)
+      if (hipEventCreate(&e) != hipSuccess) return FG_DEVICE_ERROR;
+  }
+  b->timing = s->cfg.collect_timings != 0;
+  b->cost_seconds = 0.0;
+  b->trial_active = false;
+  s->dimension = dimension;
+  s->h_delta.assign(dimension, 0.0);
+  s->prepared = true;
+  return FG_OK;
+}
+
+void fg_schur_clear(fg_schur* s) {
+  s->linearized = false;
+  s->ready = false;
+}
+
+fg_status fg_schur_linearize(fg_schur* s) {
+  if (!s->prepared) return FG_INVALID_OPTIONS;
+  s->ready = false;
+  s->linearized = false;
+  double t0 = s->tim.linearization;
+  Phase ph(s, &s->tim.linearization);
+  (void)t0;
+  fg_status st = s->cfg.precision == FG_FP32 ? linearize_impl<float>(s) : linearize_impl<double>(s);
+  if (st == FG_OK) s->linearized = true;
+  return st;
+}
+
+fg_status fg_schur_gradient_norm(fg_schur* s, double* out) {
+  if (!s->linearized) return FG_INVALID_OPTIONS;
+  fg_ba* b = s->ba;
+  if (s->cfg.precision == FG_FP32)
+    return norm_inf_impl<float>(s, (const float*)s->gc, 9 * b->nc, (const float*)s->gp, 3 * b->np, out);
+  return norm_inf_impl<double>(s, (const double*)s->gc, 9 * b->nc, (const double*)s->gp, 3 * b->np, out);
+}
+
+fg_status fg_schur_prepare_damping(fg_schur* s, double floor) {
+  s->ready = false;
+  if (!s->linearized) return FG_INVALID_OPTIONS;
+  if (!std::isfinite(floor) || floor <= 0.0) return FG_INVALID_OPTIONS;
+  fg_status st = s->cfg.precision == FG_FP32 ? prepare_damping_impl<float>(s, floor)
+                                             : prepare_damping_impl<double>(s, floor);
+  if (st == FG_OK) s->ready = true;
+  return st;
+}
+
+static fg_status solve_damped_device(fg_schur* s, double lambda, double* predicted) {
+  reset_diag(s, lambda);
+  s->step_valid = false;
+  if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
+  s->solves += 1;
+  s->events_used = 0;
+  double pred = 0.0;
+  fg_status st = s->cfg.precision == FG_FP32 ? attempt_impl<float>(s, lambda, &pred)
+                                             : attempt_impl<double>(s, lambda, &pred);
+  if (s->cfg.collect_timings && s->events_used > 0) {
+    (void)hipEventSynchronize(s->events[s->events_used - 1]);
+    for (size_t i = 0; i + 1 < s->events_used; i += 2) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, s->events[i], s->events[i + 1]) == hipSuccess) {
+        s->tim.matvec_kernel_seconds += 1e-3 * ms;
+        s->tim.products += 1e-3 * ms;
+      }
+    }
+  }
+  if (s->cfg.on_solve) s->cfg.on_solve(&s->diag, s->cfg.on_solve_user);
+  if (st == FG_OK) {
+    *predicted = pred;
+    s->last_predicted = pred;
+    s->step_valid = true;
+  }
+  return st;
+}
+
+fg_status fg_schur_solve_damped(fg_schur* s, double lambda, const double** delta, double* predicted) {
+  double pred = 0.0;
+  FG_CHECK(solve_damped_device(s, lambda, &pred));
+  FG_CHECK(download_delta(s, s->h_delta.data()));
+  if (delta) *delta = s->h_delta.data();
+  if (predicted) *predicted = pred;
+  return FG_OK;
+}
+
+fg_status fg_schur_solve(fg_schur* s, const double** delta) {
+  FG_CHECK(fg_schur_prepare_damping(s, 1e-6));
+  return fg_schur_solve_damped(s, 0.0, delta, nullptr);
+}
+
+void fg_schur_diagnostics_get(const fg_schur* s, fg_schur_diagnostics* d) { *d = s->diag; }
+void fg_schur_statistics(const fg_schur* s, size_t* solves, size_t* iterations) {
+  if (solves) *solves = s->solves;
+  if (iterations) *iterations = s->iterations;
+}
+void fg_schur_timings_get(const fg_schur* s, fg_schur_timings* t) {
+  *t = s->tim;
+  if (s->ba) t->cost = s->ba->cost_seconds;
+}
+
+fg_status fg_schur_model(fg_schur* s, double* gradient, double* scales, double* negative_residual) {
+  if (!s->linearized) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? model_impl<float>(s, gradient, scales, negative_residual)
+                                     : model_impl<double>(s, gradient, scales, negative_residual);
+}
+fg_status fg_schur_camera_gram(fg_schur* s, double* gram) {
+  if (!s->ready) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? gram_impl<float>(s, gram) : gram_impl<double>(s, gram);
+}
+fg_status fg_schur_reduced_apply(fg_schur* s, double lambda, const double* x, double* y) {
+  if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? reduced_apply_impl<float>(s, lambda, x, y)
+                                     : reduced_apply_impl<double>(s, lambda, x, y);
+}
+fg_status fg_schur_jacobian_apply(fg_schur* s, const double* x, double* y) {
+  if (!s->linearized) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? jacobian_apply_impl<float>(s, x, y) : jacobian_apply_impl<double>(s, x, y);
+}
+fg_status fg_schur_jacobian_transpose_apply(fg_schur* s, const double* y, double* x) {
+  if (!s->linearized) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? jacobian_transpose_apply_impl<float>(s, y, x)
+                                     : jacobian_transpose_apply_impl<double>(s, y, x);
+}
+
+// Damped::apply (src/lsmr.rs:147-174): [J D^-1; sqrt(l) I] x (scaled) or [J; sqrt(l) D] x.
+fg_status fg_schur_damped_apply(fg_schur* s, double lambda, int scaled, const double* x, double* y) {
+  if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
+  const size_t n = s->dimension, rows = 2 * s->ba->no;
+  std::vector<double> d(n), xs(x, x + n);
+  FG_CHECK(fg_schur_model(s, nullptr, d.data(), nullptr));
+  if (scaled)
+    for (size_t j = 0; j < n; ++j) xs[j] = x[j] / d[j];
+  FG_CHECK(fg_schur_jacobian_apply(s, xs.data(), y));
+  const double root = std::sqrt(lambda);
+  for (size_t j = 0; j < n; ++j) y[rows + j] = (scaled ? root : root * d[j]) * x[j];
+  return FG_OK;
+}
+// Damped::transpose_apply (src/lsmr.rs:183-205).
+fg_status fg_schur_damped_transpose_apply(fg_schur* s, double lambda, int scaled, const double* y, double* x) {
+  if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
+  const size_t n = s->dimension, rows = 2 * s->ba->no;
+  std::vector<double> d(n);
+  FG_CHECK(fg_schur_model(s, nullptr, d.data(), nullptr));
+  FG_CHECK(fg_schur_jacobian_transpose_apply(s, y, x));
+  const double root = std::sqrt(lambda);
+  for (size_t j = 0; j < n; ++j) {
+    if (scaled) x[j] /= d[j];
+    x[j] += (scaled ? root : root * d[j]) * y[rows + j];
+  }
+  return FG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// LM / GN on the device problem
+// ------------------------------------------------------------------------------------------
+static int cb_prepare(void* c) {
+  fg_schur* s = (fg_schur*)c;
+  s->ba->trial_active = false;
+  return fg_schur_prepare(s, s->ba, 9 * s->ba->nc + 3 * s->ba->np);
+}
+static int cb_cost(void* c, int trial, double* out) { return fg_ba_cost(((fg_schur*)c)->ba, trial, out); }
+static int cb_linearize(void* c) { return fg_schur_linearize((fg_schur*)c); }
+static int cb_gradient_norm(void* c, double* out) { return fg_schur_gradient_norm((fg_schur*)c, out); }
+static int cb_prepare_damping(void* c, double f) { return fg_schur_prepare_damping((fg_schur*)c, f); }
+static int cb_solve_damped(void* c, double lambda, double* norm, double* pred) {
+  fg_schur* s = (fg_schur*)c;
+  fg_status st = solve_damped_device(s, lambda, pred);
+  if (st == FG_OK) *norm = s->last_step_norm;
+  return st;
+}
+static int cb_solve(void* c, double* norm) {
+  fg_schur* s = (fg_schur*)c;
+  FG_CHECK(fg_schur_prepare_damping(s, 1e-6));
+  double pred = 0.0;
+  FG_CHECK(solve_damped_device(s, 0.0, &pred));
+  *norm = s->last_step_norm;
+  return FG_OK;
+}
+static int cb_stage(void* c) {
+  fg_schur* s = (fg_schur*)c;
+  if (!s->step_valid) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? stage_impl<float>(s->ba, s->d_delta) : stage_impl<double>(s->ba, s->d_delta);
+}
+static void cb_accept(void* c) {
+  fg_schur* s = (fg_schur*)c;
+  fg_ba_accept(s->ba);
+  s->linearized = false;
+  s->ready = false;
+}
+static void cb_reject(void* c) { fg_ba_reject(((fg_schur*)c)->ba); }
+static void cb_work(void* c, size_t* solves, size_t* iterations) { fg_schur_statistics((fg_schur*)c, solves, iterations); }
+
+static fg_problem device_problem(fg_schur* s) {
+  fg_problem pb;
+  pb.ctx = s;
+  pb.dimension = 9 * s->ba->nc + 3 * s->ba->np;
+  pb.prepare = cb_prepare;
+  pb.cost = cb_cost;
+  pb.linearize = cb_linearize;
+  pb.gradient_norm = cb_gradient_norm;
+  pb.prepare_damping = cb_prepare_damping;
+  pb.solve_damped = cb_solve_damped;
+  pb.solve = cb_solve;
+  pb.stage = cb_stage;
+  pb.accept = cb_accept;
+  pb.reject = cb_reject;
+  pb.linear_work = cb_work;
+  return pb;
+}
+
+fg_status fg_lm_solve_batch(fg_schur* s, fg_ba* b, const fg_optimize_options* options, const fg_lm_options* lm,
+                            fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out) {
+  if (!b || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
+  FG_CHECK(bind(s, b));
+  fg_problem pb = device_problem(s);
+  fg_lm_optimize(&pb, options, lm, on_accept, user, out);
+  return (fg_status)out->status;
+}
+
+fg_status fg_gn_solve_batch(fg_schur* s, fg_ba* b, const fg_optimize_options* options, fg_lm_outcome* out) {
+  if (!b || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
+  FG_CHECK(bind(s, b));
+  fg_problem pb = device_problem(s);
+  fg_gn_optimize(&pb, options, out);
+  return (fg_status)out->status;
+}
+
+}  // extern "C"

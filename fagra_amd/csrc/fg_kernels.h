@@ -1,0 +1,956 @@
+// fg_kernels.h -- HIP kernels of the bundle-adjustment path (gfx950, wave64).
+//
+// Data layouts (all device memory):
+//   * Points are processed in "point slots": points sorted by track length (longest first).
+//     A chunk of 64 consecutive point slots is one wavefront; lane = point. The chunk's
+//     observations are stored SELL-style: entry (j, lane) at chunk_base + 64 j + lane, so
+//     every per-observation load of a point-major kernel is a coalesced 64-lane access.
+//     Padding entries have camera -1.
+//   * Camera-major kernels walk the reference's camera-stable rows k in [off[c], off[c+1])
+//     with one workgroup per camera and keep a camera-major copy of the records.
+//   * Camera tables: kCamTab scalars per camera (see fg_math.h).
+#pragma once
+#include "fg_math.h"
+
+namespace fg {
+
+constexpr int kBlock = 256;          // threads per workgroup
+constexpr int kParts = 6;            // partial sums per workgroup / camera
+constexpr int kChunksPerBlock = kBlock / kWave;
+
+// Device-side scalars of one linear solve; counters reset by the host before use.
+struct DevScalars {
+  double rz, pq, alpha, beta, rr, bb, tol2, lambda;
+  double learned_target;             // residual^2 the learned preconditioner must reach
+  int done, iters, term, max_iters;
+  int fallback_at, use_learned, fallback_taken, pad;
+  unsigned counter_a, counter_b, counter_c, counter_d;
+};
+
+// ---------------------------------------------------------------------------------------
+// Small reduction helpers
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* smem /* kBlock/kWave */) {
+  v = wave_sum(v);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  __syncthreads();
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  T r = T(0);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) r += smem[i];
+  return r;  // valid in thread 0
+}
+
+// True in every thread of the last workgroup to finish (after all earlier writes are visible).
+__device__ __forceinline__ bool last_block(unsigned* counter) {
+  __shared__ bool is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(counter, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+// Deterministic sum of parts[0..n) by one workgroup (fixed assignment, fixed tree).
+__device__ __forceinline__ double block_sum_parts(const double* parts, int n, double* smem) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[i];
+  return block_sum(s, smem);
+}
+// Same over column col of a (n x kParts_) partial array; result broadcast to all threads.
+__device__ __forceinline__ double block_sum_parts_col(const double* parts, int n, int col, double* smem) {
+  __shared__ double bcast;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[6 * i + col];
+  s = block_sum(s, smem);
+  if (threadIdx.x == 0) bcast = s;
+  __syncthreads();
+  return bcast;
+}
+
+// ---------------------------------------------------------------------------------------
+// Camera tables: Projection::new (examples/bal/model.rs:126-146) in f64, stored in T.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_camera_tables(int nc, const double* __restrict__ cams, T* __restrict__ tab) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double* cam = cams + 9 * c;
+  const double w0 = cam[0], w1 = cam[1], w2 = cam[2];
+  const double t2 = w0 * w0 + w1 * w1 + w2 * w2;
+  double a, b;
+  if (t2 < 1e-8) {
+    a = 0.5 - t2 / 24. + t2 * t2 / 720.;
+    b = 1. / 6. - t2 / 120. + t2 * t2 / 5040.;
+  } else {
+    const double t = sqrt(t2);
+    a = (1. - cos(t)) / t2;
+    b = (t - sin(t)) / (t2 * t);
+  }
+  // K = [w]x ; Jr = I - a K + b K^2
+  const double K[9] = {0, -w2, w1, w2, 0, -w0, -w1, w0, 0};
+  double Jr[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double kk = 0;
+      for (int k = 0; k < 3; ++k) kk += K[3 * i + k] * K[3 * k + j];
+      Jr[3 * i + j] = (i == j ? 1.0 : 0.0) - a * K[3 * i + j] + b * kk;
+    }
+  // Rotation of the unit quaternion exp(w/2) (textbook quaternion-to-matrix formula).
+  const double v0 = w0 / 2, v1 = w1 / 2, v2 = w2 / 2;
+  const double nn = v0 * v0 + v1 * v1 + v2 * v2;
+  const double eps = 2.220446049250313e-16;
+  double qi = 0, qj = 0, qk = 0, qw = 1;
+  if (!(nn <= eps * eps)) {
+    const double n = sqrt(nn);
+    const double s = sin(n) / n;
+    qi = v0 * s; qj = v1 * s; qk = v2 * s; qw = cos(n);
+  }
+  const double ww = qw * qw, ii = qi * qi, jj = qj * qj, kk2 = qk * qk;
+  const double ij = qi * qj * 2, wk = qw * qk * 2, wj = qw * qj * 2, ik = qi * qk * 2, jk = qj * qk * 2,
+               wi = qw * qi * 2;
+  const double R[9] = {ww + ii - jj - kk2, ij - wk, wj + ik, wk + ij, ww - ii + jj - kk2,
+                       jk - wi, ik - wj, wi + jk, ww - ii - jj + kk2};
+  T* o = tab + kCamTab * c;
+  for (int i = 0; i < 9; ++i) o[i] = T(R[i]);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += R[3 * i + k] * Jr[3 * k + j];
+      o[9 + 3 * i + j] = T(s);
+    }
+  o[18] = T(cam[3]); o[19] = T(cam[4]); o[20] = T(cam[5]);
+  o[21] = T(cam[6]); o[22] = T(cam[7]); o[23] = T(cam[8]);
+}
+
+// Blockwise Huber (src/factors.rs:42-52): cost and sqrt(weight).
+template <typename T>
+__device__ __forceinline__ void huber_weight(T rx, T ry, T delta, T& cost, T& sw) {
+  const T s2 = rx * rx + ry * ry;
+  if (!(delta > T(0)) || s2 <= delta * delta) {
+    cost = T(0.5) * s2;
+    sw = T(1);
+  } else {
+    const T s = sqrt(s2);
+    cost = delta * (s - T(0.5) * delta);
+    sw = sqrt(delta / s);
+  }
+}
+
+// Projection::value (model.rs:147-156). Returns false on z == 0 or a nonfinite pixel.
+template <typename T>
+__device__ __forceinline__ bool project(const T* __restrict__ tab, T x0, T x1, T x2, T& px, T& py, T& qx,
+                                        T& qy, T& z) {
+  const T p0 = tab[0] * x0 + tab[1] * x1 + tab[2] * x2 + tab[18];
+  const T p1 = tab[3] * x0 + tab[4] * x1 + tab[5] * x2 + tab[19];
+  const T p2 = tab[6] * x0 + tab[7] * x1 + tab[8] * x2 + tab[20];
+  qx = -p0 / p2;
+  qy = -p1 / p2;
+  z = p2;
+  const T r2 = qx * qx + qy * qy;
+  const T s = tab[21] * (T(1) + tab[22] * r2 + tab[23] * r2 * r2);
+  px = s * qx;
+  py = s * qy;
+  return p2 != T(0) && isfinite(px) && isfinite(py);
+}
+
+// ---------------------------------------------------------------------------------------
+// Cost: sum over observations of 0.5||r||^2 or Huber, accumulated in f64 (Frame::cost,
+// model.rs:201-213). Deterministic: per-lane sequential, fixed block tree, last block sums.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int* __restrict__ chunk_base,
+                                                 const int* __restrict__ chunk_len, const int* __restrict__ slot_cam,
+                                                 const Vec2<T>* __restrict__ slot_pix, const T* __restrict__ tab,
+                                                 const T* __restrict__ pts, T huber, double* parts, int* flags,
+                                                 unsigned* counter, double* out) {
+  __shared__ double smem[kBlock / kWave];
+  const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  double acc = 0.0;
+  bool bad = false;
+  if (chunk < nchunks) {
+    const int i = chunk * kWave + lane;
+    if (i < np) {
+      const T x0 = pts[3 * i], x1 = pts[3 * i + 1], x2 = pts[3 * i + 2];
+      const int base = chunk_base[chunk], len = chunk_len[chunk];
+      for (int j = 0; j < len; ++j) {
+        const int s = base + j * kWave + lane;
+        const int c = slot_cam[s];
+        if (c < 0) continue;
+        T px, py, qx, qy, z;
+        bool ok = project(tab + kCamTab * c, x0, x1, x2, px, py, qx, qy, z);
+        const Vec2<T> ob = slot_pix[s];
+        T cost, sw;
+        huber_weight(px - ob.x, py - ob.y, huber, cost, sw);
+        bad |= !ok;
+        acc += double(cost);
+      }
+    }
+  }
+  if (bad) atomicOr(flags, 1);
+  const double b = block_sum(acc, smem);
+  if (threadIdx.x == 0) parts[blockIdx.x] = b;
+  if (last_block(counter)) {
+    const double total = block_sum_parts(parts, gridDim.x, smem);
+    if (threadIdx.x == 0) {
+      *out = total;
+      *counter = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Linearization, point-major pass: projection, residual, IRLS weight, the compact record,
+// weighted residual, the point Gram block Hpp = Jp^T Jp and gradient gp = Jp^T r.
+// (Frame::linearize, model.rs:214-236; LinearizedModel::accumulate, linear_model.rs:180-210)
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_linearize_points(
+    int nchunks, int np, const int* __restrict__ chunk_base, const int* __restrict__ chunk_len,
+    const int* __restrict__ slot_cam, const Vec2<T>* __restrict__ slot_pix, const T* __restrict__ tab,
+    const T* __restrict__ pts, T huber, Vec4<T>* __restrict__ rec, Vec2<T>* __restrict__ res,
+    T* __restrict__ hpp, T* __restrict__ gp, int* flags) {
+  const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (chunk >= nchunks) return;
+  const int i = chunk * kWave + lane;
+  if (i >= np) return;
+  const T x0 = pts[3 * i], x1 = pts[3 * i + 1], x2 = pts[3 * i + 2];
+  const int base = chunk_base[chunk], len = chunk_len[chunk];
+  T h[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  bool bad = false;
+  for (int j = 0; j < len; ++j) {
+    const int s = base + j * kWave + lane;
+    const int c = slot_cam[s];
+    if (c < 0) continue;
+    const T* ct = tab + kCamTab * c;
+    T px, py, qx, qy, z;
+    bool ok = project(ct, x0, x1, x2, px, py, qx, qy, z);
+    const Vec2<T> ob = slot_pix[s];
+    const T rx = px - ob.x, ry = py - ob.y;
+    T cost, sw;
+    huber_weight(rx, ry, huber, cost, sw);
+    Vec4<T> r4 = {qx, qy, z, sw};
+    rec[s] = r4;
+    const T wrx = sw * rx, wry = sw * ry;
+    res[s] = Vec2<T>{wrx, wry};
+    ObsGeom<T> geo;
+    obs_geom(r4, ct, geo);
+    // Jp columns: Jp[:, k] = sw * dp * R[:, k]
+    T jp0[3], jp1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      jp0[k] = sw * (geo.dp0[0] * ct[k] + geo.dp0[1] * ct[3 + k] + geo.dp0[2] * ct[6 + k]);
+      jp1[k] = sw * (geo.dp1[0] * ct[k] + geo.dp1[1] * ct[3 + k] + geo.dp1[2] * ct[6 + k]);
+      ok = ok && isfinite(jp0[k]) && isfinite(jp1[k]);
+    }
+    ok = ok && isfinite(wrx) && isfinite(wry);
+    bad |= !ok;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = a; b < 3; ++b) h[sym3(b, a)] += jp0[a] * jp0[b] + jp1[a] * jp1[b];
+      g[a] += jp0[a] * wrx + jp1[a] * wry;
+    }
+  }
+  if (bad) atomicOr(flags, 1);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) hpp[6 * i + k] = h[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gp[3 * i + k] = g[k];
+}
+
+// ---------------------------------------------------------------------------------------
+// Linearization, camera-major pass (one workgroup per camera): camera-major copies of the
+// records, the camera Gram block Hcc = Jc^T Jc (45 packed) and gradient gc = Jc^T r.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_linearize_cameras(
+    const int* __restrict__ cam_off, const int* __restrict__ k_slot, const Vec4<T>* __restrict__ rec,
+    const Vec2<T>* __restrict__ res, const T* __restrict__ tab, Vec4<T>* __restrict__ rec_cm,
+    Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc, T* __restrict__ gc, int* flags) {
+  __shared__ T ct[kCamTab];
+  __shared__ T red[54][kBlock / kWave];
+  const int c = blockIdx.x;
+  if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
+  __syncthreads();
+  T acc[54];
+#pragma unroll
+  for (int i = 0; i < 54; ++i) acc[i] = T(0);
+  bool bad = false;
+  const int k0 = cam_off[c], k1 = cam_off[c + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
+    const int s = k_slot[k];
+    const Vec4<T> r4 = rec[s];
+    const Vec2<T> rw = res[s];
+    rec_cm[k] = r4;
+    res_cm[k] = rw;
+    ObsGeom<T> geo;
+    obs_geom(r4, ct, geo);
+    T jc[18];
+    jc_block(geo, ct, jc);
+#pragma unroll
+    for (int a = 0; a < 18; ++a) bad |= !isfinite(jc[a]);
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+#pragma unroll
+      for (int b = a; b < 9; ++b) acc[sym9(b, a)] += jc[a] * jc[b] + jc[9 + a] * jc[9 + b];
+      acc[45 + a] += jc[a] * rw.x + jc[9 + a] * rw.y;
+    }
+  }
+  if (bad) atomicOr(flags, 1);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int i = 0; i < 54; ++i) {
+    const T v = wave_sum(acc[i]);
+    if (lane == 0) red[i][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 54) {
+    T s = T(0);
+    for (int w = 0; w < kBlock / kWave; ++w) s += red[threadIdx.x][w];
+    if (threadIdx.x < 45) hcc[45 * c + threadIdx.x] = s;
+    else gc[9 * c + threadIdx.x - 45] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Infinity norm with nonfinite detection (norm_inf, src/optimization.rs:653-662).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_norm_inf(int n, const T* __restrict__ a, int m, const T* __restrict__ b,
+                                                     double* parts, int* flags, unsigned* counter, double* out) {
+  __shared__ double smem[kBlock / kWave];
+  double v = 0.0;
+  bool bad = false;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n + m; i += gridDim.x * blockDim.x) {
+    const double x = double(i < n ? a[i] : b[i - n]);
+    bad |= !isfinite(x);
+    v = fmax(v, fabs(x));
+  }
+  if (bad) atomicOr(flags, 2);
+  v = wave_max(v);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int i = 0; i < kBlock / kWave; ++i) r = fmax(r, smem[i]);
+    parts[blockIdx.x] = r;
+  }
+  if (last_block(counter)) {
+    if (threadIdx.x == 0) {
+      double r = 0;
+      for (int i = 0; i < (int)gridDim.x; ++i) r = fmax(r, parts[i]);
+      *out = r;
+      *counter = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// prepare_scales (linear_model.rs:211-230) and the normalized Gram blocks
+// (block_factors.rs:47-111): D = max(column norm, floor); B = D^-1 Hcc D^-1; C = D^-1 Hpp D^-1.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const T* __restrict__ hpp, T floor,
+                         T* __restrict__ dc, T* __restrict__ dp, T* __restrict__ bs, T* __restrict__ cs,
+                         int* flags) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < nc) {
+    const T* h = hcc + 45 * idx;
+    T d[9];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      const T n = sqrt(h[sym9(j, j)]);
+      bad |= !isfinite(n);
+      d[j] = max(n, floor);
+      dc[9 * idx + j] = d[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+#pragma unroll
+      for (int i = j; i < 9; ++i) bs[45 * idx + sym9(i, j)] = (h[sym9(i, j)] / d[i]) / d[j];
+    if (bad) atomicOr(flags, 4);
+  } else if (idx < nc + np) {
+    const int p = idx - nc;
+    const T* h = hpp + 6 * p;
+    T d[3];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const T n = sqrt(h[sym3(j, j)]);
+      bad |= !isfinite(n);
+      d[j] = max(n, floor);
+      dp[3 * p + j] = d[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int i = j; i < 3; ++i) cs[6 * p + sym3(i, j)] = (h[sym3(i, j)] / d[i]) / d[j];
+    if (bad) atomicOr(flags, 4);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// BlockFactors::factor (block_factors.rs:113-138): Cholesky of B + lambda I per camera and
+// of C + lambda I per point. Point blocks are stored as the unscaled inverse
+// M = (Hpp + lambda D^2)^-1 = D^-1 (C + lambda I)^-1 D^-1 (packed 6) used by elimination.
+// Unobserved blocks are skipped (no Gram block exists for them in the reference).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const int* __restrict__ pt_len,
+                         const T* __restrict__ bs, const T* __restrict__ cs, const T* __restrict__ dp, T lambda,
+                         T* __restrict__ lc, T* __restrict__ mp, int* fail_count) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < nc) {
+    if (cam_off[idx + 1] == cam_off[idx]) return;
+    T a[45];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 45; ++i) {
+      a[i] = bs[45 * idx + i];
+      ok = ok && isfinite(a[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) a[sym9(j, j)] += lambda;
+    ok = ok && chol_packed<T, 9>(a);
+#pragma unroll
+    for (int i = 0; i < 45; ++i) lc[45 * idx + i] = a[i];
+    if (!ok) atomicAdd(fail_count, 1);
+  } else if (idx < nc + np) {
+    const int p = idx - nc;
+    T* m = mp + 6 * p;
+    if (pt_len[p] == 0) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) m[i] = T(0);
+      return;
+    }
+    T a[6];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      a[i] = cs[6 * p + i];
+      ok = ok && isfinite(a[i]);
+    }
+    a[0] += lambda; a[3] += lambda; a[5] += lambda;
+    ok = ok && chol_packed<T, 3>(a);
+    // inverse of C + lambda I from its factor: columns of L^-T L^-1
+    T inv[6];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T e[3] = {T(0), T(0), T(0)};
+      e[j] = T(1);
+      solve_lower_packed<T, 3>(a, e);
+      solve_upper_packed<T, 3>(a, e);
+#pragma unroll
+      for (int i = j; i < 3; ++i) inv[sym3(i, j)] = e[i];
+    }
+    const T d0 = dp[3 * p], d1 = dp[3 * p + 1], d2 = dp[3 * p + 2];
+    const T d[3] = {d0, d1, d2};
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int i = j; i < 3; ++i) m[sym3(i, j)] = (inv[sym3(i, j)] / d[i]) / d[j];
+    if (!ok) atomicAdd(fail_count, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Forward camera tables from a retained-coordinate vector. scaled != 0: z = x / Dc (the
+// reduced system's scaled coordinates); otherwise z = x. fw = (RJ z_rot, z_t, z_intr).
+// ---------------------------------------------------------------------------------------
+template <typename T, typename S>
+__device__ __forceinline__ void forward_table(const T* __restrict__ ct, const S* x, const T* __restrict__ d,
+                                              bool scaled, T* __restrict__ fw, T z[9]) {
+#pragma unroll
+  for (int j = 0; j < 9; ++j) z[j] = scaled ? T(x[j]) / d[j] : T(x[j]);
+  const T* rj = ct + 9;
+  fw[0] = rj[0] * z[0] + rj[1] * z[1] + rj[2] * z[2];
+  fw[1] = rj[3] * z[0] + rj[4] * z[1] + rj[5] * z[2];
+  fw[2] = rj[6] * z[0] + rj[7] * z[1] + rj[8] * z[2];
+#pragma unroll
+  for (int j = 3; j < 9; ++j) fw[j] = z[j];
+}
+
+template <typename T, typename S>
+__global__ void k_forward_tables(int nc, const T* __restrict__ tab, const S* __restrict__ x, const T* __restrict__ dc,
+                                 int scaled, T* __restrict__ fw) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  T z[9];
+  forward_table(tab + kCamTab * c, x + 9 * c, dc + 9 * c, scaled != 0, fw + kFwd * c, z);
+}
+
+// ---------------------------------------------------------------------------------------
+// Point-major half of the Schur products (one lane per point, SELL chunks).
+//   kSchur:   t_p = M_p sum_k Jp^T Jc z        (eliminated-block solve of E^T z)
+//   kRecover: delta_p = -M_p (gp + sum_k Jp^T Jc z)     (Schur::recover_step, schur.rs:312-344)
+//   kRhs:     t_p = M_p gp                      (factor_and_reduce_rhs, schur.rs:204-247)
+//   kVerify:  normal residual Jp^T (J delta + r) + lambda Dp^2 delta_p and ||J delta||^2
+//   kJv:      rows 2k,2k+1 of J x               (Jacobian::apply, linear_model.rs:39-55)
+//   kJtu:     x_p = sum_k Jp^T u_k              (Jacobian::transpose_apply, :64-87)
+// ---------------------------------------------------------------------------------------
+enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu = 5 };
+
+struct PointArgs {
+  int nchunks, np;
+  const int* chunk_base;
+  const int* chunk_len;
+  const int* slot_cam;
+  const int* slot_row;
+};
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) k_points(
+    PointArgs pa, const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
+    const T* __restrict__ fw, const T* __restrict__ mp, const T* __restrict__ gp, const T* __restrict__ dpv,
+    const double* __restrict__ delta_p_in, const double* __restrict__ rows_in, T lambda, T* __restrict__ t_out,
+    double* __restrict__ delta_out, double* __restrict__ rows_out, double* parts, const DevScalars* sc) {
+  __shared__ double smem[kBlock / kWave];
+  if (MODE == kSchur && sc != nullptr && sc->done) return;
+  const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  double uu = 0.0, n2 = 0.0, n2s = 0.0, gd = 0.0, g2 = 0.0, g2s = 0.0;  // verify partials
+  if (chunk < pa.nchunks) {
+    const int i = chunk * kWave + lane;
+    if (i < pa.np) {
+      T w[3] = {T(0), T(0), T(0)};
+      T dpx[3] = {T(0), T(0), T(0)};
+      if (MODE == kVerify || MODE == kJv) {
+        dpx[0] = T(delta_p_in[3 * i]);
+        dpx[1] = T(delta_p_in[3 * i + 1]);
+        dpx[2] = T(delta_p_in[3 * i + 2]);
+      }
+      if (MODE != kRhs) {
+        const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
+        for (int j = 0; j < len; ++j) {
+          const int s = base + j * kWave + lane;
+          const int c = pa.slot_cam[s];
+          if (c < 0) continue;
+          const T* ct = tab + kCamTab * c;
+          ObsGeom<T> geo;
+          obs_geom(rec[s], ct, geo);
+          if (MODE == kSchur || MODE == kRecover) {
+            T u0, u1;
+            jc_apply(geo, fw + kFwd * c, u0, u1);
+            jpt_accumulate(geo, ct, u0, u1, w);
+          } else if (MODE == kVerify || MODE == kJv) {
+            T u0, u1, v0, v1;
+            jc_apply(geo, fw + kFwd * c, u0, u1);
+            jp_apply(geo, ct, dpx, v0, v1);
+            u0 += v0;
+            u1 += v1;
+            if (MODE == kJv) {
+              const int row = pa.slot_row[s];
+              rows_out[2 * row] = double(u0);
+              rows_out[2 * row + 1] = double(u1);
+            } else {
+              uu += double(u0) * double(u0) + double(u1) * double(u1);
+              const Vec2<T> r = res[s];
+              jpt_accumulate(geo, ct, u0 + r.x, u1 + r.y, w);
+            }
+          } else if (MODE == kJtu) {
+            const int row = pa.slot_row[s];
+            jpt_accumulate(geo, ct, T(rows_in[2 * row]), T(rows_in[2 * row + 1]), w);
+          }
+        }
+      }
+      if (MODE == kSchur || MODE == kRhs || MODE == kRecover) {
+        const T* m = mp + 6 * i;
+        T v[3];
+        if (MODE == kRhs) {
+          v[0] = gp[3 * i]; v[1] = gp[3 * i + 1]; v[2] = gp[3 * i + 2];
+        } else if (MODE == kRecover) {
+          v[0] = gp[3 * i] + w[0]; v[1] = gp[3 * i + 1] + w[1]; v[2] = gp[3 * i + 2] + w[2];
+        } else {
+          v[0] = w[0]; v[1] = w[1]; v[2] = w[2];
+        }
+        T t[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) t[a] = m[sym3(a, 0)] * v[0] + m[sym3(a, 1)] * v[1] + m[sym3(a, 2)] * v[2];
+        if (MODE == kRecover) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -double(t[a]);
+        } else {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) t_out[3 * i + a] = t[a];
+        }
+      } else if (MODE == kVerify) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const T d = dpv[3 * i + a];
+          const T nr = w[a] + (lambda * (d * dpx[a])) * d;
+          n2 += double(nr) * double(nr);
+          n2s += double(nr / d) * double(nr / d);
+          const double g = double(gp[3 * i + a]);
+          gd += g * delta_p_in[3 * i + a];
+          g2 += g * g;
+          g2s += (g / double(d)) * (g / double(d));
+        }
+      } else if (MODE == kJtu) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = double(w[a]);
+      }
+    }
+  }
+  if (MODE == kVerify) {
+    const double v[6] = {n2, n2s, gd, g2, g2s, uu};
+    for (int a = 0; a < 6; ++a) {
+      const double b = block_sum(v[a], smem);
+      if (threadIdx.x == 0) parts[kParts * blockIdx.x + a] = b;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Camera-major half (one workgroup per camera). Sums the factors of Jc^T v over the camera's
+// rows and finishes the camera block:
+//   kSchur:   q = B p + lambda p - Dc^-1 T(acc),  v = Jp t  (Reduced::apply, schur.rs:606-652),
+//             plus p.q partials and, in the last workgroup, alpha = rz / p.q.
+//   kResid:   q as kSchur, minus rhs, with ||q - rhs||^2 and ||rhs||^2 partials
+//   kRhs:     rhs = Dc^-1 (-gc + T(acc)),  v = Jp t
+//   kVerify:  nr = T(acc) + lambda Dc^2 delta_c, v = J delta + r; norms and ||J delta||^2
+//   kJtu:     x_c = T(acc), v = u_k
+// T(acc) maps the rotation slots by (R Jr)^T.
+// ---------------------------------------------------------------------------------------
+enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4 };
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) k_cameras(
+    int nc, const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
+    const Vec2<T>* __restrict__ res_cm, const T* __restrict__ tab, const T* __restrict__ tp,
+    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
+    const T* __restrict__ bs, const T* __restrict__ dc, const T* __restrict__ gc, const T* __restrict__ vin,
+    const double* __restrict__ delta_c, const T* __restrict__ rhs, T lambda, T* __restrict__ out,
+    double* __restrict__ out_d, double* parts, DevScalars* sc) {
+  __shared__ T ct[kCamTab];
+  __shared__ T cfw[kFwd];
+  __shared__ T red[9][kBlock / kWave];
+  __shared__ double smem[kBlock / kWave];
+  if ((MODE == cSchur) && sc->done) return;
+  const int c = blockIdx.x;
+  if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
+  if (MODE == cVerify && threadIdx.x < kFwd) cfw[threadIdx.x] = fw[kFwd * c + threadIdx.x];
+  __syncthreads();
+  T acc[9];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) acc[a] = T(0);
+  const int k0 = cam_off[c], k1 = cam_off[c + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
+    ObsGeom<T> geo;
+    obs_geom(rec_cm[k], ct, geo);
+    T v0, v1;
+    if (MODE == cJtu) {
+      v0 = T(rows_in[2 * k]);
+      v1 = T(rows_in[2 * k + 1]);
+    } else if (MODE == cVerify) {
+      const int ps = k_pslot[k];
+      const T x[3] = {T(delta_p[3 * ps]), T(delta_p[3 * ps + 1]), T(delta_p[3 * ps + 2])};
+      T u0, u1, w0, w1;
+      jc_apply(geo, cfw, u0, u1);
+      jp_apply(geo, ct, x, w0, w1);
+      u0 += w0;
+      u1 += w1;
+      const Vec2<T> r = res_cm[k];
+      v0 = u0 + r.x;
+      v1 = u1 + r.y;
+    } else {
+      const int ps = k_pslot[k];
+      const T x[3] = {tp[3 * ps], tp[3 * ps + 1], tp[3 * ps + 2]};
+      jp_apply(geo, ct, x, v0, v1);
+    }
+    jct_factors(geo, v0, v1, acc);
+  }
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int a = 0; a < 9; ++a) {
+    const T v = wave_sum(acc[a]);
+    if (lane == 0) red[a][wave] = v;
+  }
+  __syncthreads();
+  double part0 = 0.0, part1 = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
+  if (threadIdx.x == 0) {
+    T s[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      s[a] = T(0);
+      for (int w = 0; w < kBlock / kWave; ++w) s[a] += red[a][w];
+    }
+    // T(acc): rotation slots through (R Jr)^T
+    T y[9];
+    const T* rj = ct + 9;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) y[j] = rj[j] * s[0] + rj[3 + j] * s[1] + rj[6 + j] * s[2];
+#pragma unroll
+    for (int j = 3; j < 9; ++j) y[j] = s[j];
+    const T* d = dc + 9 * c;
+    if (MODE == cSchur || MODE == cResid) {
+      const T* p = vin + 9 * c;
+      const T* b = bs + 45 * c;
+      const bool observed = k1 > k0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        T q = lambda * p[i] - y[i] / d[i];
+        if (observed) {
+#pragma unroll
+          for (int j = 0; j < 9; ++j) q += b[sym9(i, j)] * p[j];
+        }
+        out[9 * c + i] = q;
+        if (MODE == cSchur) {
+          part0 += double(p[i]) * double(q);
+        } else {
+          const double e = double(q) - double(rhs[9 * c + i]);
+          part0 += e * e;
+          part1 += double(rhs[9 * c + i]) * double(rhs[9 * c + i]);
+        }
+      }
+    } else if (MODE == cRhs) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const T r = (-gc[9 * c + i] + y[i]) / d[i];
+        out[9 * c + i] = r;
+        part0 += double(r) * double(r);
+      }
+    } else if (MODE == cVerify) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const T dd = d[i];
+        const T nr = y[i] + (lambda * (dd * T(delta_c[9 * c + i]))) * dd;
+        part0 += double(nr) * double(nr);
+        part1 += double(nr / dd) * double(nr / dd);
+        const double g = double(gc[9 * c + i]);
+        part2 += g * delta_c[9 * c + i];
+        part3 += g * g;
+        part4 += (g / double(dd)) * (g / double(dd));
+      }
+    } else if (MODE == cJtu) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) out_d[9 * c + i] = double(y[i]);
+    }
+    if (MODE != cJtu) {
+      double* pc = parts + kParts * c;
+      pc[0] = part0;
+      pc[1] = part1;
+      pc[2] = part2;
+      pc[3] = part3;
+      pc[4] = part4;
+    }
+  }
+  if (MODE == cSchur) {
+    if (last_block(&sc->counter_a)) {
+      double pq = 0.0;
+      for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += parts[kParts * i];
+      pq = block_sum(pq, smem);
+      if (threadIdx.x == 0) {
+        sc->pq = pq;
+        if (!(pq > 0.0) || !isfinite(pq)) {
+          sc->done = 1;
+          sc->term = 2;  // NotPositiveDefinite
+        } else {
+          sc->alpha = sc->rz / pq;
+        }
+        sc->counter_a = 0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Column sums of a (rows x kParts) partial array in fixed order (deterministic).
+// ---------------------------------------------------------------------------------------
+__global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* __restrict__ out) {
+  __shared__ double smem[kBlock / kWave];
+  for (int col = 0; col < kParts; ++col) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < rows; i += blockDim.x) s += parts[kParts * i + col];
+    s = block_sum(s, smem);
+    if (threadIdx.x == 0) out[col] = s;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Reduced-system PCG (textbook; replaces faer's conjugate_gradient at schur.rs:249-310).
+// One thread per camera block for the vector work; dot products and norms in f64.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const T* __restrict__ lc,
+                                             T lambda, const T r[9], T z[9]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) z[i] = r[i];
+  if (cam_off[c + 1] > cam_off[c]) {
+    T l[45];
+#pragma unroll
+    for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
+    solve_lower_packed<T, 9>(l, z);
+    solve_upper_packed<T, 9>(l, z);
+  } else if (lambda != T(0)) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) z[i] = r[i] / lambda;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
+                                                    T lambda, const T* __restrict__ rhs, T* __restrict__ x,
+                                                    T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+                                                    double* parts, DevScalars* sc, double rel_tol) {
+  __shared__ double smem[kBlock / kWave];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rz = 0.0;
+  if (c < nc) {
+    T rv[9], zv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      rv[i] = rhs[9 * c + i];
+      x[9 * c + i] = T(0);
+      r[9 * c + i] = rv[i];
+    }
+    precondition(c, cam_off, lc, lambda, rv, zv);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      z[9 * c + i] = zv[i];
+      p[9 * c + i] = zv[i];
+      rr += double(rv[i]) * double(rv[i]);
+      rz += double(rv[i]) * double(zv[i]);
+    }
+  }
+  const double b0 = block_sum(rr, smem);
+  const double b1 = block_sum(rz, smem);
+  if (threadIdx.x == 0) {
+    parts[kParts * blockIdx.x] = b0;
+    parts[kParts * blockIdx.x + 1] = b1;
+  }
+  if (last_block(&sc->counter_b)) {
+    const double bb = block_sum_parts_col(parts, gridDim.x, 0, smem);
+    const double rz0 = block_sum_parts_col(parts, gridDim.x, 1, smem);
+    if (threadIdx.x == 0) {
+      sc->bb = bb;
+      sc->rr = bb;
+      sc->rz = rz0;
+      sc->beta = 0.0;
+      sc->iters = 0;
+      sc->tol2 = rel_tol * rel_tol * bb;
+      if (bb <= sc->tol2) {
+        sc->done = 1;
+        sc->term = 0;  // Converged
+      } else {
+        sc->done = 0;
+        sc->term = -1;
+      }
+      sc->counter_b = 0;
+    }
+  }
+}
+
+// p = z + beta p and the forward table of p (scaled coordinates).
+template <typename T>
+__global__ void k_cg_direction(int nc, const T* __restrict__ tab, const T* __restrict__ dc, const T* __restrict__ z,
+                               T* __restrict__ p, T* __restrict__ fw, const DevScalars* sc) {
+  if (sc->done) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const T beta = T(sc->beta);
+  T pv[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    pv[i] = z[9 * c + i] + beta * p[9 * c + i];
+    p[9 * c + i] = pv[i];
+  }
+  T zz[9];
+  forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
+}
+
+// x += alpha p; r -= alpha q; z = M r; rr, rz; convergence / exhaustion in the last block.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
+                                                      T lambda, const T* __restrict__ p, const T* __restrict__ q,
+                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                      double* parts, DevScalars* sc) {
+  __shared__ double smem[kBlock / kWave];
+  if (sc->done) return;
+  const T alpha = T(sc->alpha);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rz = 0.0;
+  if (c < nc) {
+    T rv[9], zv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      x[9 * c + i] += alpha * p[9 * c + i];
+      rv[i] = r[9 * c + i] - alpha * q[9 * c + i];
+      r[9 * c + i] = rv[i];
+    }
+    precondition(c, cam_off, lc, lambda, rv, zv);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      z[9 * c + i] = zv[i];
+      rr += double(rv[i]) * double(rv[i]);
+      rz += double(rv[i]) * double(zv[i]);
+    }
+  }
+  const double b0 = block_sum(rr, smem);
+  const double b1 = block_sum(rz, smem);
+  if (threadIdx.x == 0) {
+    parts[kParts * blockIdx.x] = b0;
+    parts[kParts * blockIdx.x + 1] = b1;
+  }
+  if (last_block(&sc->counter_b)) {
+    const double rrt = block_sum_parts_col(parts, gridDim.x, 0, smem);
+    const double rzt = block_sum_parts_col(parts, gridDim.x, 1, smem);
+    if (threadIdx.x == 0) {
+      sc->rr = rrt;
+      sc->iters += 1;
+      if (rrt <= sc->tol2) {
+        sc->done = 1;
+        sc->term = 0;  // Converged
+      } else if (sc->iters >= sc->max_iters) {
+        sc->done = 1;
+        sc->term = 1;  // IterationLimit
+      }
+      sc->beta = rzt / sc->rz;
+      sc->rz = rzt;
+      sc->counter_b = 0;
+    }
+  }
+}
+
+// delta_c = x / Dc (original coordinates, f64)
+template <typename T>
+__global__ void k_recover_cameras(int n, const T* __restrict__ x, const T* __restrict__ dc, double* __restrict__ delta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) delta[i] = double(x[i]) / double(dc[i]);
+}
+
+// State staging: x_trial = x + delta (additive BAL retraction, model.rs:20-28), plus the
+// compute-precision point copy.
+template <typename T>
+__global__ void k_stage(int nc, int np, const double* __restrict__ cams, const double* __restrict__ pts,
+                        const double* __restrict__ dcam, const double* __restrict__ dpt, double* __restrict__ tcams,
+                        double* __restrict__ tpts, T* __restrict__ tpts_t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 9 * nc) {
+    tcams[i] = cams[i] + dcam[i];
+  } else if (i < 9 * nc + 3 * np) {
+    const int j = i - 9 * nc;
+    const double v = pts[j] + dpt[j];
+    tpts[j] = v;
+    tpts_t[j] = T(v);
+  }
+}
+
+template <typename T>
+__global__ void k_convert(int n, const double* __restrict__ a, T* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = T(a[i]);
+}
+
+}  // namespace fg

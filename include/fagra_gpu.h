@@ -1,0 +1,226 @@
+/* fagra_gpu.h -- C ABI of the MI355X (gfx950) bundle-adjustment backend.
+ *
+ * The library is a drop-in for one path of oleflb/fagra: the LM/GN bundle-adjustment
+ * inner loop. It mirrors three reference interfaces:
+ *
+ *   LeastSquaresBackend (src/optimization.rs:131-169)  -> fg_schur_*   (GPU Schur-PCG)
+ *   FactorBatch for the BAL Frame (examples/bal/model.rs:194-237) -> fg_ba_*  (device evaluator)
+ *   LevenbergMarquardt / GaussNewton solve_batch
+ *     (src/optimization/levenberg_marquardt.rs:252-262, gauss_newton.rs:63-74) -> fg_lm_*, fg_gn_*
+ *
+ * Every call returns an fg_status whose values correspond one-to-one to the reference's
+ * SolverError / EvaluationError variants (src/error.rs:21-76). Device pointers never cross
+ * the ABI: arrays are host memory in the reference's coordinate order (camera i at
+ * [9i, 9i+9), point j at [9nc+3j, 9nc+3j+3), Jacobian rows of observation k at [2k, 2k+2)
+ * with k the camera-stable row order of examples/bal.rs:189-201). One HIP stream per
+ * backend; not thread-safe (the reference is &mut self, src/schur.rs:527).
+ */
+#ifndef FAGRA_GPU_H
+#define FAGRA_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/error.rs:21-76 */
+typedef enum {
+  FG_OK = 0,
+  FG_INVALID_OPTIONS = 1,        /* SolverError::InvalidOptions */
+  FG_LINEAR_SOLVE_FAILED = 2,    /* SolverError::LinearSolveFailed */
+  FG_NO_CONVERGENCE = 3,         /* SolverError::NoConvergence */
+  FG_NO_PROGRESS = 4,            /* SolverError::NoProgress */
+  FG_INVALID_EVALUATION = 5,     /* EvaluationError::InvalidEvaluation */
+  FG_DIMENSION_MISMATCH = 6,     /* EvaluationError::DimensionMismatch */
+  FG_INVALID_EMISSION = 7,       /* EvaluationError::InvalidEmission */
+  FG_UNSUPPORTED_STRUCTURE = 8,  /* EvaluationError::UnsupportedStructure */
+  FG_DEVICE_ERROR = 100          /* HIP runtime failure; no reference counterpart */
+} fg_status;
+
+/* Arithmetic of the bulk path. FG_FP32: Jacobians, Schur products and PCG matvecs in FP32,
+ * with FP64 cost sums, gain ratio, damping and PCG dot products/norms. */
+typedef enum { FG_FP64 = 0, FG_FP32 = 1 } fg_precision;
+
+/* Reduced-camera PCG preconditioner. BLOCK_JACOBI is the reference's (B_i + lambda I)^-1
+ * (src/schur.rs:658-682). LEARNED applies L^-T G G^T L^-1 with G from the per-camera MLP
+ * and falls back to BLOCK_JACOBI when the residual does not fall by fallback_factor within
+ * fallback_iterations. */
+typedef enum { FG_PRECOND_BLOCK_JACOBI = 0, FG_PRECOND_LEARNED = 1 } fg_preconditioner;
+
+/* src/schur.rs:32-41 */
+typedef enum {
+  FG_CG_NONE = -1,
+  FG_CG_CONVERGED = 0,
+  FG_CG_ITERATION_LIMIT = 1,
+  FG_CG_NOT_POSITIVE_DEFINITE = 2,
+  FG_CG_NON_FINITE = 3
+} fg_schur_termination;
+
+/* SchurDiagnostics (src/schur.rs:44-60); NaN encodes None. */
+typedef struct {
+  int termination;
+  size_t iterations;
+  double damping;
+  double estimated_relative_residual;
+  double reduced_relative_residual;
+  double scaled_relative_residual;
+  double original_relative_residual;
+  int used_fallback;            /* learned preconditioner replaced by block-Jacobi */
+} fg_schur_diagnostics;
+
+/* SchurTimings (src/schur.rs:77-91), seconds, from HIP events on the backend stream. */
+typedef struct {
+  double preparation, factorization, linear_solve, products, preconditioner, verification;
+  double linearization, cost;   /* evaluator time; not part of the reference struct */
+  size_t matvecs;               /* reduced-system products, including verification */
+  double matvec_kernel_seconds; /* summed duration of the two Schur matvec kernels */
+} fg_schur_timings;
+
+typedef void (*fg_on_solve_fn)(const fg_schur_diagnostics*, void* user);
+
+/* Schur public fields (src/schur.rs:116-125) plus the device choices. */
+typedef struct {
+  size_t max_iterations;        /* default 1000 (src/schur.rs:155) */
+  double relative_tolerance;    /* default max(1e-10, 128 eps) of the precision (:156-157) */
+  int collect_timings;          /* default 0 */
+  int precision;                /* fg_precision */
+  int preconditioner;           /* fg_preconditioner */
+  double fallback_factor;       /* learned: required residual reduction ... */
+  size_t fallback_iterations;   /* ... within this many iterations */
+  fg_on_solve_fn on_solve;      /* observe every numerically attempted solve */
+  void* on_solve_user;
+} fg_schur_config;
+
+typedef struct fg_schur fg_schur;  /* GPU Schur backend (src/schur.rs:116) */
+typedef struct fg_ba fg_ba;        /* device-resident BAL problem and batch evaluator */
+
+/* ---- batch evaluator: BAL Frame batches (examples/bal/model.rs:194-237) ---- */
+
+/* Upload a problem: cameras nc x 9 [angle-axis, t, f, k1, k2], points np x 3, observations
+ * (camera, point, pixel). huber > 0 selects blockwise Huber with that threshold via IRLS
+ * (src/factors.rs:42-52); 0 is plain least squares. Builds the camera-stable row order,
+ * camera offsets and point permutation on the host (examples/bal.rs:176-201). */
+fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, const double* points,
+                       const int64_t* obs_camera, const int64_t* obs_point, const double* obs_pixel,
+                       double huber, int precision, fg_ba** out);
+void fg_ba_destroy(fg_ba*);
+/* Bit-exact layout: camera-stable row order (no entries: original observation index of row
+ * k), camera offsets (nc+1), point permutation of rows (no) and point offsets (np+1). */
+fg_status fg_ba_layout(const fg_ba*, int64_t* cam_order, int64_t* cam_offsets, int64_t* pt_perm,
+                       int64_t* pt_offsets);
+fg_status fg_ba_get_state(fg_ba*, double* cameras, double* points);
+fg_status fg_ba_set_state(fg_ba*, const double* cameras, const double* points);
+/* Total cost of the accepted (trial=0) or staged (trial=1) state: sum over Frame batches of
+ * 0.5||r||^2 or the Huber cost (Frame::cost, model.rs:201-213; checked_cost storage.rs:612). */
+fg_status fg_ba_cost(fg_ba*, int trial, double* out);
+/* StatePool::stage (src/storage.rs:288-306) with the additive BAL retraction. */
+fg_status fg_ba_stage(fg_ba*, const double* delta);
+void fg_ba_accept(fg_ba*);  /* StatePool::accept_trial (storage.rs:308-313) */
+void fg_ba_reject(fg_ba*);  /* StatePool::reject_trial (storage.rs:315-318) */
+
+/* ---- LeastSquaresBackend mirror (src/optimization.rs:131-169, src/schur.rs:400-546) ---- */
+
+fg_status fg_schur_config_default(fg_schur_config* cfg, int precision);
+fg_status fg_schur_create(const fg_schur_config*, fg_schur** out);
+void fg_schur_destroy(fg_schur*);
+fg_status fg_schur_set_config(fg_schur*, const fg_schur_config*);
+/* prepare(dimension): validates the 9/3 partition for the bound problem (schur.rs:409-431). */
+fg_status fg_schur_prepare(fg_schur*, fg_ba*, size_t dimension);
+void fg_schur_clear(fg_schur*);
+/* Bulk replacement of Frame::linearize + Schur::accumulate for every observation of the
+ * bound problem at its accepted state (model.rs:214-236, schur.rs:438-482, with the sink's
+ * finiteness checks of optimization.rs:556,597). */
+fg_status fg_schur_linearize(fg_schur*);
+fg_status fg_schur_gradient_norm(fg_schur*, double* out);
+fg_status fg_schur_prepare_damping(fg_schur*, double min_column_norm);
+/* delta (host, 9nc+3np) stays valid until the next mutating call. */
+fg_status fg_schur_solve_damped(fg_schur*, double lambda, const double** delta, double* predicted);
+fg_status fg_schur_solve(fg_schur*, const double** delta);
+void fg_schur_diagnostics_get(const fg_schur*, fg_schur_diagnostics*);
+void fg_schur_statistics(const fg_schur*, size_t* solves, size_t* iterations);
+void fg_schur_timings_get(const fg_schur*, fg_schur_timings*);
+
+/* Model and operator access for parity checks (host buffers, reference order). */
+fg_status fg_schur_model(fg_schur*, double* gradient, double* scales, double* negative_residual);
+fg_status fg_schur_camera_gram(fg_schur*, double* gram /* nc x 81, row-major full */);
+/* Reduced::apply (src/schur.rs:606-652): y = S x on the 9nc retained coordinates. */
+fg_status fg_schur_reduced_apply(fg_schur*, double lambda, const double* x, double* y);
+/* Jacobian::apply / transpose_apply (src/linear_model.rs:39-87). */
+fg_status fg_schur_jacobian_apply(fg_schur*, const double* x, double* y);
+fg_status fg_schur_jacobian_transpose_apply(fg_schur*, const double* y, double* x);
+/* Damped::apply / transpose_apply (src/lsmr.rs:147-205). */
+fg_status fg_schur_damped_apply(fg_schur*, double lambda, int scaled, const double* x, double* y);
+fg_status fg_schur_damped_transpose_apply(fg_schur*, double lambda, int scaled, const double* y,
+                                          double* x);
+
+/* ---- nonlinear optimizers ---- */
+
+/* OptimizeOptions (src/optimization.rs:39-67) */
+typedef struct {
+  size_t max_iterations;
+  double gradient_tolerance, step_tolerance, cost_tolerance;
+} fg_optimize_options;
+/* LmOptions (src/optimization/levenberg_marquardt.rs:17-46) */
+typedef struct {
+  double initial_damping, min_damping, max_damping, min_column_norm, acceptance_threshold;
+  size_t max_trials;
+} fg_lm_options;
+/* TerminationReason (src/optimization.rs:89-98); TrialFailure (levenberg_marquardt.rs:71-80) */
+enum { FG_TERM_NONE = -1, FG_TERM_NO_VARIABLES = 0, FG_TERM_GRADIENT = 1, FG_TERM_STEP = 2,
+       FG_TERM_COST = 3 };
+enum { FG_REJ_NONE = -1, FG_REJ_LINEAR_SOLVE = 0, FG_REJ_PREDICTION = 1,
+       FG_REJ_INVALID_EVALUATION = 2, FG_REJ_POOR_AGREEMENT = 3 };
+/* LmOutcome = result + LmStatistics + LmWork (levenberg_marquardt.rs:84-184); NaN = None. */
+typedef struct {
+  int status;
+  int termination;
+  size_t iterations;
+  double initial_cost, final_cost;
+  size_t linearizations, attempts, rejected_steps, accepted_steps, cost_evaluations;
+  double damping, cost, gradient_norm, step_norm;
+  int last_rejection;
+  size_t linear_solves, linear_iterations;
+} fg_lm_outcome;
+
+typedef void (*fg_on_accept_fn)(const fg_lm_outcome* statistics, void* user);
+
+fg_status fg_optimize_options_default(fg_optimize_options*);
+fg_status fg_lm_options_default(fg_lm_options*);
+
+/* LevenbergMarquardt::<Schur>::solve_batch_with_outcome on the device problem; the problem's
+ * state holds the accepted estimates afterwards, also on error. */
+fg_status fg_lm_solve_batch(fg_schur*, fg_ba*, const fg_optimize_options*, const fg_lm_options*,
+                            fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out);
+/* GaussNewton::<Schur>::solve_batch on the device problem. */
+fg_status fg_gn_solve_batch(fg_schur*, fg_ba*, const fg_optimize_options*, fg_lm_outcome* out);
+
+/* The same LM/GN loops over caller-supplied state and backend callbacks. */
+typedef struct {
+  void* ctx;
+  size_t dimension;
+  int (*prepare)(void* ctx);
+  int (*cost)(void* ctx, int trial, double* out);
+  int (*linearize)(void* ctx);
+  int (*gradient_norm)(void* ctx, double* out);
+  int (*prepare_damping)(void* ctx, double min_column_norm);
+  /* step_norm = infinity norm of the step, NaN when it is nonfinite */
+  int (*solve_damped)(void* ctx, double lambda, double* step_norm, double* predicted);
+  int (*solve)(void* ctx, double* step_norm);
+  int (*stage)(void* ctx);  /* stage the last solved step */
+  void (*accept)(void* ctx);
+  void (*reject)(void* ctx);
+  void (*linear_work)(void* ctx, size_t* solves, size_t* iterations);
+} fg_problem;
+void fg_lm_optimize(const fg_problem*, const fg_optimize_options*, const fg_lm_options*,
+                    fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out);
+void fg_gn_optimize(const fg_problem*, const fg_optimize_options*, fg_lm_outcome* out);
+
+/* ---- device information ---- */
+fg_status fg_device_info(char* name, size_t name_len, int* cus, size_t* lds_bytes);
+const char* fg_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,177 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (GPU only).
+
+Tolerances: FP64 GPU path -- products 1e-10 relative, solves and final LM cost 1e-9 relative;
+FP32 path (FP32 Jacobians, Schur products and PCG matvecs, FP64 reductions) -- products 2e-5
+relative, steps 3e-3 relative (the reference's own f32 Schur bound, src/schur.rs:788), final
+LM cost 1e-5 relative. Integer layout is bit-exact.
+"""
+import numpy as np
+import pytest
+
+import fagra_amd as fa
+from fagra_amd import synthetic
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = [fa.FP64, fa.FP32]
+PROD_TOL = {fa.FP64: 1e-10, fa.FP32: 2e-5}
+
+
+def small(seed=0, nc=8, npts=300, huber=0.0, **kw):
+    return synthetic.generate("ring16", seed=seed, n_cameras=nc, n_points=npts, huber=huber, **kw)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def linearized_pair(p, precision, floor=1e-6):
+    ba_o = oracle.BA(p, huber=p.huber)
+    st, s_o = oracle.ba_schur(ba_o)
+    assert st == oracle.OK
+    assert s_o.prepare_damping(floor) == oracle.OK
+    ba_g = fa.BalProblem(p, precision=precision)
+    s_g = fa.Schur(precision=precision)
+    assert s_g.prepare(ba_g) == fa.OK
+    assert s_g.linearize() == fa.OK
+    assert s_g.prepare_damping(floor) == fa.OK
+    return ba_o, s_o, ba_g, s_g
+
+
+def test_layout_is_bit_exact():
+    for seed, nc, npts in [(0, 8, 300), (3, 16, 2000), (5, 3, 50)]:
+        p = small(seed=seed, nc=nc, npts=npts)
+        g = fa.BalProblem(p).layout()
+        o = oracle.BA(p).layout()
+        for a, b in zip(g, o):
+            assert a.dtype == b.dtype and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("huber", [0.0, 1.0])
+def test_cost_matches_oracle(precision, huber):
+    p = small(seed=1, huber=huber)
+    st_o, c_o = oracle.BA(p, huber=huber).cost()
+    st_g, c_g = fa.BalProblem(p, precision=precision).cost()
+    assert st_o == st_g == 0
+    tol = 1e-12 if precision == fa.FP64 else 1e-5
+    assert abs(c_g - c_o) <= tol * c_o
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("huber", [0.0, 1.0])
+def test_linearized_model_matches_oracle(precision, huber):
+    p = small(seed=2, huber=huber)
+    ba_o, s_o, ba_g, s_g = linearized_pair(p, precision)
+    g, d, nr = s_g.model()
+    tol = PROD_TOL[precision]
+    assert rel(g, s_o.gradient()) < 10 * tol
+    assert rel(d, s_o.scales()) < tol
+    assert rel(nr, s_o.negative_residual()) < tol
+    st, gn = s_g.gradient_norm()
+    st_o, gn_o = s_o.gradient_norm()
+    assert st == st_o == 0 and abs(gn - gn_o) <= 10 * tol * gn_o
+    gram = s_g.camera_gram()
+    for c in range(p.n_cameras):
+        st, go = s_o.gram_block(9 * c, 9)
+        go = np.tril(go) + np.tril(go, -1).T
+        assert rel(gram[c], go) < tol
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_reduced_apply_matches_oracle(precision):
+    p = small(seed=3, huber=1.0)
+    ba_o, s_o, ba_g, s_g = linearized_pair(p, precision)
+    rng = np.random.default_rng(0)
+    for lam in [1e-3, 1.0, 100.0]:
+        x = rng.normal(size=9 * p.n_cameras)
+        st_o, y_o = s_o.reduced_apply(lam, x)
+        st_g, y_g = s_g.reduced_apply(lam, x)
+        assert st_o == st_g == 0
+        assert rel(y_g, y_o) < 5 * PROD_TOL[precision], lam
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_jacobian_products_match_oracle(precision):
+    p = small(seed=4, huber=1.0)
+    ba_o, s_o, ba_g, s_g = linearized_pair(p, precision)
+    rng = np.random.default_rng(1)
+    n, rows = 9 * p.n_cameras + 3 * p.n_points, 2 * p.n_observations
+    x, y = rng.normal(size=n), rng.normal(size=rows)
+    tol = PROD_TOL[precision]
+    assert rel(s_g.jacobian_apply(x), s_o.jacobian_apply(x)) < tol
+    assert rel(s_g.jacobian_transpose_apply(y), s_o.jacobian_transpose_apply(y)) < tol
+    yy = rng.normal(size=rows + n)
+    for lam in [0.0, 0.01, 1e8]:
+        for scaled in [False, True]:
+            assert rel(s_g.damped_apply(lam, scaled, x), s_o.damped_apply(lam, scaled, x)) < tol
+            assert rel(s_g.damped_transpose_apply(lam, scaled, yy), s_o.damped_transpose_apply(lam, scaled, yy)) < tol
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_solve_damped_matches_oracle(precision):
+    p = small(seed=5, huber=1.0)
+    ba_o, s_o, ba_g, s_g = linearized_pair(p, precision)
+    tol = 1e-10 if precision == fa.FP64 else 1e-6
+    s_o.relative_tolerance = tol
+    s_g.relative_tolerance = tol
+    for lam in [1e-3, 1.0, 100.0, 1e-2]:
+        st_o, d_o, pr_o = s_o.solve_damped(lam)
+        st_g, d_g, pr_g = s_g.solve_damped(lam)
+        assert st_o == st_g == 0, (lam, s_g.diagnostics().termination)
+        dg = s_g.diagnostics()
+        assert dg.termination == fa.CG_CONVERGED
+        bound = 1e-8 if precision == fa.FP64 else 3e-3
+        assert rel(d_g, d_o) < bound, lam
+        assert abs(pr_g - pr_o) <= bound * abs(pr_o), lam
+        # iteration counts agree in order of magnitude (faer's exact CG is not reachable)
+        do = s_o.diagnostics()
+        assert dg.iterations <= 3 * do.iterations + 3
+        assert dg.reduced_relative_residual < 10 * tol * max(1, 10 if precision == fa.FP32 else 1)
+
+
+def test_iteration_limit_and_invalid_lambda():
+    p = small(seed=6)
+    ba_o, s_o, ba_g, s_g = linearized_pair(p, fa.FP64)
+    s_g.max_iterations = 1
+    st, d, pr = s_g.solve_damped(1e-3)
+    assert st == fa.LINEAR_SOLVE_FAILED
+    diag = s_g.diagnostics()
+    assert diag.termination == fa.CG_ITERATION_LIMIT and diag.iterations == 1
+    assert np.isnan(diag.estimated_relative_residual)
+    assert np.isfinite(diag.reduced_relative_residual) and diag.reduced_relative_residual > s_g.relative_tolerance
+    s_g.max_iterations = 1000
+    assert s_g.solve_damped(float("nan"))[0] == fa.INVALID_OPTIONS
+    assert s_g.solve_damped(-1.0)[0] == fa.INVALID_OPTIONS
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_lm_final_cost_matches_oracle(precision):
+    p = small(seed=7, nc=8, npts=400, huber=1.0)
+    opts = oracle.OptimizeOptions(30, 1e-8, 0.0, 0.0)
+    lm_o = oracle.LmOptions.default()
+    out_o, _ = oracle.BA(p, huber=1.0).lm(options=opts, lm_options=lm_o, tol=1e-10)
+    ba = fa.BalProblem(p, precision=precision)
+    lm = fa.LevenbergMarquardt(fa.Schur(precision=precision,
+                                        relative_tolerance=1e-10 if precision == fa.FP64 else None))
+    o = fa.OptimizeOptions(30, 1e-8, 0.0, 0.0)
+    out = lm.solve_batch(ba, o)
+    assert out.status in (fa.OK, fa.NO_PROGRESS, fa.NO_CONVERGENCE)
+    assert out_o.status in (oracle.OK, oracle.NO_PROGRESS, oracle.NO_CONVERGENCE)
+    tol = 1e-9 if precision == fa.FP64 else 1e-5
+    assert abs(out.cost - out_o.cost) <= tol * out_o.cost, (out.cost, out_o.cost)
+    # LmStatistics invariants (tests/levenberg_marquardt.rs:253-256)
+    assert out.attempts == out.accepted_steps + out.rejected_steps or out.status != fa.OK
+    st, c = ba.cost()
+    assert abs(c - out.cost) <= 1e-12 * c
+
+
+def test_gauss_newton_runs_and_decreases_cost():
+    p = small(seed=8, nc=6, npts=200)
+    ba = fa.BalProblem(p)
+    gn = fa.GaussNewton(fa.Schur())
+    out = gn.solve_batch(ba, fa.OptimizeOptions(5, 1e-8, 1e-10, 1e-12))
+    out_o = oracle.BA(p).gn(options=oracle.OptimizeOptions(5, 1e-8, 1e-10, 1e-12))
+    assert out.status == out_o.status
+    assert abs(out.cost - out_o.cost) <= 1e-9 * out_o.cost
