@@ -1080,7 +1080,9 @@ void orc_lm_optimize(const orc_problem* pb, const orc_optimize_options* options,
                         (options->cost_tolerance == 0.0 ||
                          actual / std::max(current, 1.0) <= options->cost_tolerance);
               current = value;
-              double update = std::max(1.0 - std::pow(2.0 * std::min(ratio, 1.0) - 1.0, 3), 1.0 / 3.0);
+              // f64::powi(3) is repeated multiplication (x * x) * x
+              double c = 2.0 * std::min(ratio, 1.0) - 1.0;
+              double update = std::max(1.0 - c * c * c, 1.0 / 3.0);
               lambda = std::min(std::max(lambda * update, lm->min_damping), lm->max_damping);
               accepted = true;
               if (pb->on_accept) pb->on_accept(pb->ctx, out);
