@@ -25,6 +25,7 @@ _d = ctypes.c_double
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _szp = ctypes.POINTER(ctypes.c_size_t)
+_fp = ctypes.POINTER(ctypes.c_float)
 _VP = ctypes.c_void_p
 
 
@@ -131,6 +132,16 @@ SIGNATURES = {
     "fg_gn_optimize": (None, [ctypes.POINTER(Problem), ctypes.POINTER(OptimizeOptions),
                               ctypes.POINTER(LmOutcome)]),
     "fg_device_info": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_int), _szp]),
+    "fg_mlp_param_count": (_sz, []),
+    "fg_mlp_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_VP)]),
+    "fg_mlp_destroy": (None, [_VP]),
+    "fg_mlp_set_params": (ctypes.c_int, [_VP, _fp]),
+    "fg_mlp_get_params": (ctypes.c_int, [_VP, _fp]),
+    "fg_mlp_forward": (ctypes.c_int, [_VP, _sz, _fp, _fp]),
+    "fg_mlp_loss_grad": (ctypes.c_int, [_VP, _sz, _fp, _fp, _dp, _fp]),
+    "fg_mlp_train": (ctypes.c_int, [_VP, _sz, _fp, _fp, _sz, _sz, _d, ctypes.c_uint64, _dp]),
+    "fg_schur_set_mlp": (ctypes.c_int, [_VP, _VP]),
+    "fg_schur_capture": (ctypes.c_int, [_VP, _d, _fp, _fp, ctypes.POINTER(ctypes.c_int)]),
     "fg_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }
 

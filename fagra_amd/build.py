@@ -23,7 +23,7 @@ FLAGS = [
     "-shared",
     # Packed FP32 VALU forms stay off: ordinary float code is compiled to scalar VALU ops.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall",
+    "-Wall", "-Wno-unused-function",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 

@@ -14,7 +14,9 @@
 #include <vector>
 
 #include "../../include/fagra_gpu.h"
+#include "fg_internal.h"
 #include "fg_kernels.h"
+#include "fg_mlp_kernels.h"
 
 using namespace fg;
 
@@ -121,6 +123,13 @@ struct fg_schur {
   hipEvent_t ph_a = nullptr, ph_b = nullptr;
   double last_predicted = 0.0, last_step_norm = 0.0;
   bool step_valid = false;
+  // learned preconditioner
+  fg_mlp* mlp = nullptr;
+  void* gmat = nullptr;        // T [nc][45]
+  uint16_t* mx = nullptr;      // bf16 [ncpad][64] features
+  float* my = nullptr;         // f32 [ncpad][48] network output
+  float* mxf = nullptr;        // f32 [nc][47] features (capture)
+  float* mtarget = nullptr;    // f32 [nc][45] targets (capture)
 };
 
 namespace {
@@ -328,17 +337,19 @@ fg_status sum_parts(fg_schur* s, int rows) {
 
 // PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
 template <typename T>
-fg_status cg_impl(fg_schur* s, T lambda) {
+fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc;
   const int cblocks = grid_for(nc);
   DevScalars init{};
   init.max_iters = (int)s->cfg.max_iterations;
   init.lambda = lambda;
+  init.use_learned = learned ? 1 : 0;
+  init.fallback_at = (int)std::min<size_t>(s->cfg.fallback_iterations, (size_t)std::numeric_limits<int>::max());
   FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off, (const T*)s->lc,
-                     lambda, (const T*)s->rhs, (T*)s->x, (T*)s->r, (T*)s->z, (T*)s->p, s->d_parts, s->d_sc,
-                     s->cfg.relative_tolerance);
+                     (const T*)s->gmat, lambda, (const T*)s->rhs, (T*)s->x, (T*)s->r, (T*)s->z, (T*)s->p, s->d_parts,
+                     s->d_sc, s->cfg.relative_tolerance, s->cfg.fallback_factor);
   FG_TRY(hipGetLastError());
   const bool timed = s->cfg.collect_timings != 0;
   const size_t max_it = s->cfg.max_iterations;
@@ -359,8 +370,9 @@ fg_status cg_impl(fg_schur* s, T lambda) {
   while (!stop && launched < max_it) {
     const size_t nb = std::min<size_t>(kBatch, max_it - launched);
     for (size_t it = 0; it < nb; ++it) {
-      hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, (const T*)b->d_tab[b->cur],
-                         (const T*)s->dc, (const T*)s->z, (T*)s->p, (T*)s->fw, (const DevScalars*)s->d_sc);
+      hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                         (const T*)s->lc, lambda, (const T*)b->d_tab[b->cur], (const T*)s->dc, (const T*)s->r,
+                         (T*)s->z, (T*)s->p, (T*)s->fw, s->d_parts, s->d_sc);
       const bool ev = timed && s->events_used + 2 <= s->events.size();
       if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
       launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, s->d_sc);
@@ -371,8 +383,8 @@ fg_status cg_impl(fg_schur* s, T lambda) {
         s->events_used += 2;
       }
       hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                         (const T*)s->lc, lambda, (const T*)s->p, (const T*)s->q, (T*)s->x, (T*)s->r, (T*)s->z,
-                         s->d_parts, s->d_sc);
+                         (const T*)s->lc, (const T*)s->gmat, lambda, (const T*)s->p, (const T*)s->q, (T*)s->x,
+                         (T*)s->r, (T*)s->z, s->d_parts, s->d_sc);
     }
     FG_TRY(hipGetLastError());
     launched += nb;
@@ -392,6 +404,7 @@ fg_status cg_impl(fg_schur* s, T lambda) {
   const DevScalars& h = *s->h_sc;
   s->iterations += (size_t)h.iters;
   s->diag.iterations = (size_t)h.iters;
+  s->diag.used_fallback = h.fallback_taken;
   if (h.term == 0) {
     s->diag.termination = FG_CG_CONVERGED;
     s->diag.estimated_relative_residual = h.bb == 0.0 ? std::sqrt(h.rr) : std::sqrt(h.rr / h.bb);
@@ -423,9 +436,21 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
     launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (T*)s->rhs, nullptr);
     FG_TRY(hipGetLastError());
   }
+  const bool learned = s->cfg.preconditioner == FG_PRECOND_LEARNED;
+  if (learned) {
+    Phase ph(s, &s->tim.preconditioner);
+    if (!s->mlp) return FG_INVALID_OPTIONS;
+    // per-camera features -> network -> G = I + s tril(y)
+    hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                       (const T*)s->hcc, lambda_d, s->mx, (float*)nullptr);
+    mlp_forward(s->mlp, b->stream, pad64(nc), s->mx, s->my);
+    hipLaunchKernelGGL(k_mlp_build_g<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, (const float*)s->my,
+                       (T*)s->gmat);
+    FG_TRY(hipGetLastError());
+  }
   {
     Phase ph(s, &s->tim.linear_solve);
-    FG_CHECK(cg_impl<T>(s, lambda));
+    FG_CHECK(cg_impl<T>(s, lambda, learned));
   }
   Phase ph(s, &s->tim.verification);
   // recover_step (schur.rs:312-344)
@@ -501,6 +526,12 @@ fg_status alloc_model(fg_schur* s) {
     FG_TRY(dalloc((T**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(T)));
   }
+  FG_TRY(dalloc((T**)&s->gmat, 45 * nc));
+  FG_TRY(dalloc(&s->mx, pad64(nc) * kInPad));
+  FG_TRY(hipMemset(s->mx, 0, pad64(nc) * kInPad * sizeof(uint16_t)));
+  FG_TRY(dalloc(&s->my, pad64(nc) * kOutPad));
+  FG_TRY(dalloc(&s->mxf, kIn * nc));
+  FG_TRY(dalloc(&s->mtarget, 45 * nc));
   FG_TRY(dalloc(&s->d_delta, 9 * nc + 3 * np));
   FG_TRY(dalloc(&s->d_rows, 2 * no));
   const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
@@ -512,13 +543,17 @@ fg_status alloc_model(fg_schur* s) {
 void free_model(fg_schur* s) {
   for (void** v : {&s->rec, &s->res, &s->rec_cm, &s->res_cm, &s->hpp, &s->gp, &s->hcc, &s->gc, &s->dc, &s->dp,
                    &s->bs, &s->cs, &s->lc, &s->mp, &s->tp, &s->fw, &s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs,
-                   &s->zero}) {
+                   &s->zero, &s->gmat}) {
     if (*v) (void)hipFree(*v);
     *v = nullptr;
   }
   dfree(s->d_delta);
   dfree(s->d_rows);
   dfree(s->d_parts);
+  dfree(s->mx);
+  dfree(s->my);
+  dfree(s->mxf);
+  dfree(s->mtarget);
 }
 
 // Reference-order delta from the device layout (points in slot order).
@@ -643,6 +678,29 @@ fg_status gram_impl(fg_schur* s, double* gram) {
   for (size_t c = 0; c < b->nc; ++c)
     for (int i = 0; i < 9; ++i)
       for (int j = 0; j < 9; ++j) gram[81 * c + 9 * i + j] = bs[45 * c + sym9(i, j)];
+  return FG_OK;
+}
+
+// Per-camera network features and training targets at damping lambda.
+template <typename T>
+fg_status capture_impl(fg_schur* s, double lambda_d, float* features, float* targets, int* counts) {
+  fg_ba* b = s->ba;
+  const int nc = (int)b->nc;
+  const T lambda = (T)lambda_d;
+  int fails = 0;
+  FG_CHECK(factor_impl<T>(s, lambda, &fails));
+  if (fails) return FG_LINEAR_SOLVE_FAILED;
+  hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     (const T*)s->hcc, lambda_d, s->mx, s->mxf);
+  hipLaunchKernelGGL(k_schur_diagonal<T>, dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const T*)b->d_tab[b->cur], (const T*)s->mp,
+                     (const T*)s->bs, (const T*)s->dc, (const T*)s->lc, lambda, s->mtarget);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(features, s->mxf, (size_t)nc * kIn * 4, hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(targets, s->mtarget, (size_t)nc * 45 * 4, hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (counts)
+    for (int c = 0; c < nc; ++c) counts[c] = (int)(b->cam_off[c + 1] - b->cam_off[c]);
   return FG_OK;
 }
 
@@ -1092,6 +1150,17 @@ fg_status fg_schur_camera_gram(fg_schur* s, double* gram) {
   if (!s->ready) return FG_INVALID_OPTIONS;
   return s->cfg.precision == FG_FP32 ? gram_impl<float>(s, gram) : gram_impl<double>(s, gram);
 }
+fg_status fg_schur_set_mlp(fg_schur* s, fg_mlp* m) {
+  s->mlp = m;
+  return FG_OK;
+}
+
+fg_status fg_schur_capture(fg_schur* s, double lambda, float* features, float* targets, int* counts) {
+  if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? capture_impl<float>(s, lambda, features, targets, counts)
+                                     : capture_impl<double>(s, lambda, features, targets, counts);
+}
+
 fg_status fg_schur_reduced_apply(fg_schur* s, double lambda, const double* x, double* y) {
   if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
   return s->cfg.precision == FG_FP32 ? reduced_apply_impl<float>(s, lambda, x, y)

@@ -1,0 +1,37 @@
+// fg_internal.h -- shared host-side declarations between the backend and the network code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/fagra_gpu.h"
+
+// Preconditioner network: f32 master parameters, bf16 operand copies, Adam state and
+// batch scratch (grown on demand outside the solve loop).
+struct fg_mlp {
+  float* params = nullptr;   // kParams
+  float* grads = nullptr;
+  float* adam_m = nullptr;
+  float* adam_v = nullptr;
+  uint16_t *w1 = nullptr, *w1t = nullptr, *w2 = nullptr, *w2t = nullptr, *w3 = nullptr, *w3t = nullptr;
+  size_t cap = 0;            // padded rows of the scratch below
+  uint16_t* x = nullptr;     // [cap][64]
+  float *z1 = nullptr, *z2 = nullptr;
+  uint16_t *h1 = nullptr, *h2 = nullptr;
+  float* y = nullptr;        // [cap][48]
+  uint16_t *dy = nullptr, *dz2 = nullptr, *dz1 = nullptr;
+  float* target = nullptr;   // [cap][45]
+  double* parts = nullptr;
+  double* h_parts = nullptr;
+  hipStream_t stream = nullptr;
+  int step = 0;
+};
+
+namespace fg {
+// Ensure scratch for n rows (padded to 64); allocates only when growing.
+fg_status mlp_reserve(fg_mlp* m, size_t n);
+// y[n][48] = network(x[n][64]) on the given stream; x and y are device buffers of >= npad rows.
+void mlp_forward(fg_mlp* m, hipStream_t stream, size_t npad, const uint16_t* x, float* y);
+inline size_t pad64(size_t n) { return (n + 63) / 64 * 64; }
+}  // namespace fg

@@ -23,7 +23,7 @@ struct DevScalars {
   double rz, pq, alpha, beta, rr, bb, tol2, lambda;
   double learned_target;             // residual^2 the learned preconditioner must reach
   int done, iters, term, max_iters;
-  int fallback_at, use_learned, fallback_taken, pad;
+  int fallback_at, use_learned, fallback_taken, restart;
   unsigned counter_a, counter_b, counter_c, counter_d;
 };
 
@@ -767,7 +767,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
 // ---------------------------------------------------------------------------------------
 // Column sums of a (rows x kParts) partial array in fixed order (deterministic).
 // ---------------------------------------------------------------------------------------
-__global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* __restrict__ out) {
+static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* __restrict__ out) {
   __shared__ double smem[kBlock / kWave];
   for (int col = 0; col < kParts; ++col) {
     double s = 0.0;
@@ -782,9 +782,12 @@ __global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* 
 // Reduced-system PCG (textbook; replaces faer's conjugate_gradient at schur.rs:249-310).
 // One thread per camera block for the vector work; dot products and norms in f64.
 // ---------------------------------------------------------------------------------------
+// Block-Jacobi z = (B + lambda I)^-1 r per camera (RetainedPreconditioner, schur.rs:670-677;
+// unobserved blocks divide by lambda, block_factors.rs:193-203), or with the learned
+// correction z = L^-T G G^T L^-1 r, G packed lower (9x9) per camera.
 template <typename T>
 __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const T* __restrict__ lc,
-                                             T lambda, const T r[9], T z[9]) {
+                                             const T* __restrict__ gmat, bool learned, T lambda, const T r[9], T z[9]) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) z[i] = r[i];
   if (cam_off[c + 1] > cam_off[c]) {
@@ -792,6 +795,24 @@ __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_
 #pragma unroll
     for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
     solve_lower_packed<T, 9>(l, z);
+    if (learned) {
+      const T* g = gmat + 45 * c;
+      T y[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {  // y = G^T z
+        T s = T(0);
+#pragma unroll
+        for (int j = i; j < 9; ++j) s += g[sym_index(9, j, i)] * z[j];
+        y[i] = s;
+      }
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {  // z = G y
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j <= i; ++j) s += g[sym_index(9, i, j)] * y[j];
+        z[i] = s;
+      }
+    }
     solve_upper_packed<T, 9>(l, z);
   } else if (lambda != T(0)) {
 #pragma unroll
@@ -801,10 +822,12 @@ __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
-                                                    T lambda, const T* __restrict__ rhs, T* __restrict__ x,
-                                                    T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
-                                                    double* parts, DevScalars* sc, double rel_tol) {
+                                                    const T* __restrict__ gmat, T lambda, const T* __restrict__ rhs,
+                                                    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                    T* __restrict__ p, double* parts, DevScalars* sc, double rel_tol,
+                                                    double fallback_factor) {
   __shared__ double smem[kBlock / kWave];
+  const bool learned = sc->use_learned != 0;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0;
   if (c < nc) {
@@ -815,7 +838,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
       x[9 * c + i] = T(0);
       r[9 * c + i] = rv[i];
     }
-    precondition(c, cam_off, lc, lambda, rv, zv);
+    precondition(c, cam_off, lc, gmat, learned, lambda, rv, zv);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
@@ -840,6 +863,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
       sc->beta = 0.0;
       sc->iters = 0;
       sc->tol2 = rel_tol * rel_tol * bb;
+      sc->learned_target = fallback_factor * fallback_factor * bb;
       if (bb <= sc->tol2) {
         sc->done = 1;
         sc->term = 0;  // Converged
@@ -852,32 +876,67 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
   }
 }
 
-// p = z + beta p and the forward table of p (scaled coordinates).
+// p = z + beta p and the forward table of p (scaled coordinates). After a fallback from the
+// learned preconditioner the direction restarts: z = block-Jacobi r, p = z, rz = r.z.
 template <typename T>
-__global__ void k_cg_direction(int nc, const T* __restrict__ tab, const T* __restrict__ dc, const T* __restrict__ z,
-                               T* __restrict__ p, T* __restrict__ fw, const DevScalars* sc) {
+__global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __restrict__ cam_off,
+                                                         const T* __restrict__ lc, T lambda, const T* __restrict__ tab,
+                                                         const T* __restrict__ dc, const T* __restrict__ r,
+                                                         T* __restrict__ z, T* __restrict__ p, T* __restrict__ fw,
+                                                         double* parts, DevScalars* sc) {
+  __shared__ double smem[kBlock / kWave];
   if (sc->done) return;
+  const bool restart = sc->restart != 0;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  const T beta = T(sc->beta);
-  T pv[9];
+  double rz = 0.0;
+  if (c < nc) {
+    T pv[9];
+    if (restart) {
+      T rv[9], zv[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    pv[i] = z[9 * c + i] + beta * p[9 * c + i];
-    p[9 * c + i] = pv[i];
+      for (int i = 0; i < 9; ++i) rv[i] = r[9 * c + i];
+      precondition(c, cam_off, lc, (const T*)nullptr, false, lambda, rv, zv);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        z[9 * c + i] = zv[i];
+        pv[i] = zv[i];
+        p[9 * c + i] = pv[i];
+        rz += double(rv[i]) * double(zv[i]);
+      }
+    } else {
+      const T beta = T(sc->beta);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        pv[i] = z[9 * c + i] + beta * p[9 * c + i];
+        p[9 * c + i] = pv[i];
+      }
+    }
+    T zz[9];
+    forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
   }
-  T zz[9];
-  forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
+  if (restart) {
+    const double b = block_sum(rz, smem);
+    if (threadIdx.x == 0) parts[kParts * blockIdx.x] = b;
+    if (last_block(&sc->counter_d)) {
+      const double t = block_sum_parts_col(parts, gridDim.x, 0, smem);
+      if (threadIdx.x == 0) {
+        sc->rz = t;
+        sc->restart = 0;
+        sc->counter_d = 0;
+      }
+    }
+  }
 }
 
 // x += alpha p; r -= alpha q; z = M r; rr, rz; convergence / exhaustion in the last block.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
-                                                      T lambda, const T* __restrict__ p, const T* __restrict__ q,
-                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                      double* parts, DevScalars* sc) {
+                                                      const T* __restrict__ gmat, T lambda, const T* __restrict__ p,
+                                                      const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
+                                                      T* __restrict__ z, double* parts, DevScalars* sc) {
   __shared__ double smem[kBlock / kWave];
   if (sc->done) return;
+  const bool learned = sc->use_learned != 0;
   const T alpha = T(sc->alpha);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0;
@@ -889,7 +948,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
       rv[i] = r[9 * c + i] - alpha * q[9 * c + i];
       r[9 * c + i] = rv[i];
     }
-    precondition(c, cam_off, lc, lambda, rv, zv);
+    precondition(c, cam_off, lc, gmat, learned, lambda, rv, zv);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
@@ -915,6 +974,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
       } else if (sc->iters >= sc->max_iters) {
         sc->done = 1;
         sc->term = 1;  // IterationLimit
+      } else if (learned && sc->iters == sc->fallback_at && rrt > sc->learned_target) {
+        sc->use_learned = 0;  // fall back to block-Jacobi, restart the direction
+        sc->fallback_taken = 1;
+        sc->restart = 1;
       }
       sc->beta = rzt / sc->rz;
       sc->rz = rzt;

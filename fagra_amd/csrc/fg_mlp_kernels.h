@@ -1,0 +1,326 @@
+// fg_mlp_kernels.h -- forward, backward and Adam kernels of the preconditioner network, plus
+// the per-camera feature and training-target kernels. Batches are padded to a multiple of 64
+// rows (one 16-row tile per wave, four waves per workgroup); padded rows are zero.
+#pragma once
+#include "fg_kernels.h"
+#include "fg_mlp.h"
+
+namespace fg {
+
+// Upper-triangle (row-major, i <= j) position k of a 9x9 block -> (i, j).
+__host__ __device__ inline void upper_index(int k, int& i, int& j) {
+  int r = 0, start = 0;
+  while (k >= start + (9 - r)) {
+    start += 9 - r;
+    ++r;
+  }
+  i = r;
+  j = r + (k - start);
+}
+
+// Features of one camera block: for the 45 upper-triangle entries of the unnormalized Gram
+// block, log(H_ii) on the diagonal and the correlation H_ij / sqrt(H_ii H_jj) off it; then
+// log(observations + 1) and log10(lambda), all scaled to O(1).
+template <typename T>
+__global__ void k_mlp_features(int nc, const int* __restrict__ cam_off, const T* __restrict__ hcc, double lambda,
+                               uint16_t* __restrict__ x, float* __restrict__ xf) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const T* h = hcc + 45 * c;
+  float f[kInPad];
+  for (int k = 0; k < kInPad; ++k) f[k] = 0.f;
+  for (int k = 0; k < 45; ++k) {
+    int i, j;
+    upper_index(k, i, j);
+    const double hii = double(h[sym9(i, i)]), hjj = double(h[sym9(j, j)]);
+    if (i == j) {
+      f[k] = float(log(fmax(hii, 1e-30)) / 8.0);
+    } else {
+      const double d = sqrt(hii * hjj);
+      f[k] = d > 0.0 ? float(double(h[sym9(i, j)]) / d) : 0.f;
+    }
+  }
+  f[45] = float(log(double(cam_off[c + 1] - cam_off[c]) + 1.0) / 8.0);
+  f[46] = float(log10(fmax(lambda, 1e-30)) / 8.0);
+  for (int k = 0; k < kInPad; ++k) x[(size_t)kInPad * c + k] = f2bf(f[k]);
+  if (xf)
+    for (int k = 0; k < kIn; ++k) xf[(size_t)kIn * c + k] = f[k];
+}
+
+// Fused three-layer forward for 16 rows per wave. Z1/H1/Z2/H2 are stored when non-null
+// (training); Y receives the f32 output rows (kOutPad wide).
+static __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w1,
+                                                     const float* __restrict__ b1, const uint16_t* __restrict__ w2,
+                                                     const float* __restrict__ b2, const uint16_t* __restrict__ w3,
+                                                     const float* __restrict__ b3, float* __restrict__ z1,
+                                                     uint16_t* __restrict__ h1, float* __restrict__ z2,
+                                                     uint16_t* __restrict__ h2, float* __restrict__ y) {
+  __shared__ uint16_t stage[2][4][16][kHidden];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const size_t m0 = ((size_t)blockIdx.x * 4 + wave) * 16;
+  // layer 1
+  for (int nt = 0; nt < kHidden / 16; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(x + m0 * kInPad, kInPad, w1 + (size_t)nt * 16 * kInPad, kInPad, kInPad, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      const float z = acc[i] + b1[col];
+      const uint16_t hb = f2bf(gelu(z));
+      stage[0][wave][row][col] = hb;
+      if (z1) z1[(m0 + row) * kHidden + col] = z;
+      if (h1) h1[(m0 + row) * kHidden + col] = hb;
+    }
+  }
+  __syncthreads();
+  // layer 2
+  for (int nt = 0; nt < kHidden / 16; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&stage[0][wave][0][0], kHidden, w2 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      const float z = acc[i] + b2[col];
+      const uint16_t hb = f2bf(gelu(z));
+      stage[1][wave][row][col] = hb;
+      if (z2) z2[(m0 + row) * kHidden + col] = z;
+      if (h2) h2[(m0 + row) * kHidden + col] = hb;
+    }
+  }
+  __syncthreads();
+  // layer 3 (linear)
+  for (int nt = 0; nt < kOutPad / 16; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&stage[1][wave][0][0], kHidden, w3 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      y[(m0 + row) * kOutPad + col] = acc[i] + b3[col];
+    }
+  }
+}
+
+// G = I + s tril(y): packed lower column-major 9x9 per camera.
+__device__ __forceinline__ void build_g(const float* yrow, float g[45]) {
+  for (int k = 0; k < 45; ++k) g[k] = kOutScale * yrow[k];
+  for (int j = 0; j < 9; ++j) g[sym_index(9, j, j)] += 1.f;
+}
+
+template <typename T>
+__global__ void k_mlp_build_g(int nc, const float* __restrict__ y, T* __restrict__ gout) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  float g[45];
+  build_g(y + (size_t)kOutPad * c, g);
+  for (int k = 0; k < 45; ++k) gout[45 * c + k] = T(g[k]);
+}
+
+// Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).
+// A is the packed (lower, column-major) 9x9 target; rows >= n get zero gradient.
+static __global__ void k_mlp_loss(int n, int npad, const float* __restrict__ y, const float* __restrict__ a,
+                           uint16_t* __restrict__ dy, double* __restrict__ parts) {
+  __shared__ double smem[kBlock / kWave];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double loss = 0.0;
+  if (b < npad) {
+    float dyv[kOutPad];
+    for (int k = 0; k < kOutPad; ++k) dyv[k] = 0.f;
+    if (b < n) {
+      float g[45], A[9][9], G[9][9], AG[9][9], E[9][9];
+      build_g(y + (size_t)kOutPad * b, g);
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+          A[i][j] = a[(size_t)45 * b + sym9(i, j)];
+          G[i][j] = i >= j ? g[sym_index(9, i, j)] : 0.f;
+        }
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+          float s = 0.f;
+          for (int k = 0; k < 9; ++k) s += A[i][k] * G[k][j];
+          AG[i][j] = s;
+        }
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+          float s = 0.f;
+          for (int k = 0; k < 9; ++k) s += G[k][i] * AG[k][j];
+          E[i][j] = s - (i == j ? 1.f : 0.f);
+          loss += double(E[i][j]) * double(E[i][j]);
+        }
+      // dL/dG = 4 A G E, restricted to the lower triangle; dy = s dG / n
+      const float scale = 4.f * kOutScale / float(n);
+      for (int j = 0; j < 9; ++j)
+        for (int i = j; i < 9; ++i) {
+          float s = 0.f;
+          for (int k = 0; k < 9; ++k) s += AG[i][k] * E[k][j];
+          dyv[sym_index(9, i, j)] = scale * s;
+        }
+      loss /= double(n);
+    }
+    for (int k = 0; k < kOutPad; ++k) dy[(size_t)kOutPad * b + k] = f2bf(dyv[k]);
+  }
+  const double s = block_sum(loss, smem);
+  if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// dH = dZ W (per 16-row tile; wt = W transposed, [in][out]); then dZ_prev = dH * gelu'(Z_prev).
+static __global__ void __launch_bounds__(256) k_mlp_backward_data(int nout, int nin, const uint16_t* __restrict__ dz,
+                                                           const uint16_t* __restrict__ wt, const float* __restrict__ zprev,
+                                                           uint16_t* __restrict__ dzprev) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const size_t m0 = ((size_t)blockIdx.x * 4 + wave) * 16;
+  for (int nt = 0; nt < nin / 16; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(dz + m0 * nout, nout, wt + (size_t)nt * 16 * nout, nout, nout, acc);
+    for (int i = 0; i < 4; ++i) {
+      const size_t row = m0 + 4 * q + i;
+      const int col = nt * 16 + r;
+      dzprev[row * nin + col] = f2bf(acc[i] * gelu_grad(zprev[row * nin + col]));
+    }
+  }
+}
+
+// dW[o][k] += sum_b dZ[b][o] X[b][k] and db[o] += sum_b dZ[b][o] over a 256-row chunk per
+// workgroup (MFMA over the batch dimension, f32 atomics into the gradient).
+static __global__ void __launch_bounds__(256) k_mlp_wgrad(int npad, int nout, int nin, const uint16_t* __restrict__ dz,
+                                                   const uint16_t* __restrict__ xin, float* __restrict__ dw,
+                                                   float* __restrict__ db) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const int b0 = blockIdx.x * 256;
+  const int b1 = min(b0 + 256, npad);
+  const int tiles_o = nout / 16, tiles_k = nin / 16;
+  for (int t = wave; t < tiles_o * tiles_k; t += 4) {
+    const int o0 = (t / tiles_k) * 16, k0 = (t % tiles_k) * 16;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int bb = b0; bb < b1; bb += 16) {
+      bf16x4 av, bv;
+      for (int i = 0; i < 4; ++i) {
+        av[i] = (short)dz[(size_t)(bb + 4 * q + i) * nout + o0 + r];
+        bv[i] = (short)xin[(size_t)(bb + 4 * q + i) * nin + k0 + r];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    for (int i = 0; i < 4; ++i) atomicAdd(dw + (size_t)(o0 + 4 * q + i) * nin + k0 + r, acc[i]);
+  }
+  for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+    float s = 0.f;
+    for (int bb = b0; bb < b1; ++bb) s += bf2f(dz[(size_t)bb * nout + o]);
+    atomicAdd(db + o, s);
+  }
+}
+
+// Adam on the f32 master parameters (bias-corrected), then bf16 copies W [out][in] and
+// WT [in][out] of every weight matrix.
+static __global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float beta1, float beta2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+  const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+static __global__ void k_mlp_convert(const float* __restrict__ p, uint16_t* __restrict__ w1, uint16_t* __restrict__ w1t,
+                              uint16_t* __restrict__ w2, uint16_t* __restrict__ w2t, uint16_t* __restrict__ w3,
+                              uint16_t* __restrict__ w3t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kHidden * kInPad) {
+    const int o = i / kInPad, k = i % kInPad;
+    const uint16_t h = f2bf(p[kW1 + i]);
+    w1[i] = h;
+    w1t[k * kHidden + o] = h;
+  }
+  if (i < kHidden * kHidden) {
+    const int o = i / kHidden, k = i % kHidden;
+    const uint16_t h = f2bf(p[kW2 + i]);
+    w2[i] = h;
+    w2t[k * kHidden + o] = h;
+  }
+  if (i < kOutPad * kHidden) {
+    const int o = i / kHidden, k = i % kHidden;
+    const uint16_t h = f2bf(p[kW3 + i]);
+    w3[i] = h;
+    w3t[k * kOutPad + o] = h;
+  }
+}
+
+// Training target of one camera: A = L^-1 S_cc L^-T with S_cc = B + lambda I - D^-1 (sum_k
+// Jc^T Jp M_p Jp^T Jc) D^-1 the camera's diagonal block of the reduced system and L the
+// Cholesky factor of B + lambda I (one workgroup per camera, packed 45 output).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __restrict__ cam_off,
+                                                           const int* __restrict__ k_pslot,
+                                                           const Vec4<T>* __restrict__ rec_cm, const T* __restrict__ tab,
+                                                           const T* __restrict__ mp, const T* __restrict__ bs,
+                                                           const T* __restrict__ dc, const T* __restrict__ lc,
+                                                           T lambda, float* __restrict__ out) {
+  __shared__ T ct[kCamTab];
+  __shared__ T red[45][kBlock / kWave];
+  const int c = blockIdx.x;
+  if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
+  __syncthreads();
+  T acc[45];
+  for (int i = 0; i < 45; ++i) acc[i] = T(0);
+  const int k0 = cam_off[c], k1 = cam_off[c + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
+    ObsGeom<T> geo;
+    obs_geom(rec_cm[k], ct, geo);
+    T jc[18];
+    jc_block(geo, ct, jc);
+    const int ps = k_pslot[k];
+    const T* m = mp + 6 * ps;
+    // Jp (2x3), K2 = Jp M Jp^T (2x2)
+    T jp[2][3];
+    for (int a = 0; a < 3; ++a) {
+      jp[0][a] = geo.sw * (geo.dp0[0] * ct[a] + geo.dp0[1] * ct[3 + a] + geo.dp0[2] * ct[6 + a]);
+      jp[1][a] = geo.sw * (geo.dp1[0] * ct[a] + geo.dp1[1] * ct[3 + a] + geo.dp1[2] * ct[6 + a]);
+    }
+    T mj[2][3];
+    for (int row = 0; row < 2; ++row)
+      for (int a = 0; a < 3; ++a)
+        mj[row][a] = m[sym3(a, 0)] * jp[row][0] + m[sym3(a, 1)] * jp[row][1] + m[sym3(a, 2)] * jp[row][2];
+    T k2[2][2];
+    for (int u = 0; u < 2; ++u)
+      for (int w = 0; w < 2; ++w) k2[u][w] = jp[u][0] * mj[w][0] + jp[u][1] * mj[w][1] + jp[u][2] * mj[w][2];
+    for (int a = 0; a < 9; ++a) {
+      const T t0 = k2[0][0] * jc[a] + k2[0][1] * jc[9 + a];
+      const T t1 = k2[1][0] * jc[a] + k2[1][1] * jc[9 + a];
+      for (int b = a; b < 9; ++b) acc[sym9(b, a)] += t0 * jc[b] + t1 * jc[9 + b];
+    }
+  }
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int i = 0; i < 45; ++i) {
+    const T v = wave_sum(acc[i]);
+    if (lane == 0) red[i][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T s[45], l[45];
+    const T* d = dc + 9 * c;
+    for (int j = 0; j < 9; ++j)
+      for (int i = j; i < 9; ++i) {
+        T e = T(0);
+        for (int w = 0; w < kBlock / kWave; ++w) e += red[sym9(i, j)][w];
+        s[sym9(i, j)] = bs[45 * c + sym9(i, j)] + (i == j ? lambda : T(0)) - (e / d[i]) / d[j];
+      }
+    for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
+    // A = L^-1 S L^-T: columns of L^-1 S, then rows
+    T X[9][9];
+    for (int j = 0; j < 9; ++j) {
+      T col[9];
+      for (int i = 0; i < 9; ++i) col[i] = s[sym9(i, j)];
+      solve_lower_packed<T, 9>(l, col);
+      for (int i = 0; i < 9; ++i) X[i][j] = col[i];
+    }
+    for (int i = 0; i < 9; ++i) {
+      T row[9];
+      for (int j = 0; j < 9; ++j) row[j] = X[i][j];
+      solve_lower_packed<T, 9>(l, row);  // (L^-1 (L^-1 S)^T)^T = L^-1 S L^-T
+      for (int j = 0; j <= i; ++j) out[(size_t)45 * c + sym9(i, j)] = float(row[j]);
+    }
+  }
+}
+
+}  // namespace fg

@@ -154,6 +154,30 @@ fg_status fg_schur_damped_apply(fg_schur*, double lambda, int scaled, const doub
 fg_status fg_schur_damped_transpose_apply(fg_schur*, double lambda, int scaled, const double* y,
                                           double* x);
 
+/* ---- learned preconditioner (no reference counterpart: the reference's Schur uses plain
+ *      block-Jacobi, src/schur.rs:658-682) ---- */
+
+/* Per-camera network 47 -> 128 -> 128 -> 45 (GELU), BF16 operands, FP32 accumulation. */
+typedef struct fg_mlp fg_mlp;
+size_t fg_mlp_param_count(void);
+fg_status fg_mlp_create(uint64_t seed, fg_mlp** out);
+void fg_mlp_destroy(fg_mlp*);
+fg_status fg_mlp_set_params(fg_mlp*, const float* params);
+fg_status fg_mlp_get_params(fg_mlp*, float* params);
+/* out (n x 45) = network(features n x 47) */
+fg_status fg_mlp_forward(fg_mlp*, size_t n, const float* features, float* out);
+/* Mean loss ||G^T A G - I||_F^2 over n rows (targets n x 45 packed lower) and its gradient. */
+fg_status fg_mlp_loss_grad(fg_mlp*, size_t n, const float* features, const float* targets, double* loss,
+                           float* grads);
+/* Adam epochs on the GPU; epoch_loss receives the mean batch loss of each epoch. */
+fg_status fg_mlp_train(fg_mlp*, size_t n, const float* features, const float* targets, size_t epochs,
+                       size_t batch, double lr, uint64_t seed, double* epoch_loss);
+/* Attach a network to a backend for FG_PRECOND_LEARNED. */
+fg_status fg_schur_set_mlp(fg_schur*, fg_mlp*);
+/* Training data of the prepared model at lambda: per camera the network features (nc x 47),
+ * the target A = L^-1 S_cc L^-T (nc x 45 packed) and the observation count. */
+fg_status fg_schur_capture(fg_schur*, double lambda, float* features, float* targets, int* counts);
+
 /* ---- nonlinear optimizers ---- */
 
 /* OptimizeOptions (src/optimization.rs:39-67) */
