@@ -537,6 +537,8 @@ fg_status alloc_model(fg_schur* s) {
   const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   const size_t rows = std::max({nc, pblocks, (size_t)1024});
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
+  // The memsets above run on the null stream; complete them before stream work starts.
+  FG_TRY(hipDeviceSynchronize());
   return FG_OK;
 }
 
@@ -845,6 +847,7 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
       hipHostMalloc((void**)&b->h_flags, 8 * sizeof(int), 0) != hipSuccess)
     return fail(FG_DEVICE_ERROR);
   if (hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  if (hipDeviceSynchronize() != hipSuccess) return fail(FG_DEVICE_ERROR);
   fg_status st = fg_ba_set_state(b, cameras, points);
   if (st != FG_OK) return fail(st);
   *out = b;
@@ -978,7 +981,7 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
       dalloc(&s->d_flags, 4) != hipSuccess || dalloc(&s->d_scalar, 4) != hipSuccess ||
       dalloc(&s->d_sums, kParts) != hipSuccess)
     return fail();
-  if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess) return fail();
+  if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail();
   if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_sums, 2 * kParts * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_int, 8 * sizeof(int), 0) != hipSuccess)

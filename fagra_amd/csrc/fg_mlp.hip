@@ -47,6 +47,9 @@ fg_status mlp_reserve(fg_mlp* m, size_t n) {
   FG_TRY(hipMalloc((void**)&m->target, np * 45 * 4));
   FG_TRY(hipMalloc((void**)&m->parts, (np / 256 + 1) * sizeof(double)));
   FG_TRY(hipMemset(m->x, 0, np * kInPad * 2));
+  // hipMemset runs on the null stream, which does not order against the non-blocking work
+  // streams: finish it before any later upload can be overtaken.
+  FG_TRY(hipDeviceSynchronize());
   m->cap = np;
   return FG_OK;
 }

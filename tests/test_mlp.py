@@ -103,3 +103,45 @@ def test_gpu_training_reduces_loss():
     losses = net.train(x, t, epochs=8, batch=256, lr=1e-3, seed=0)
     assert np.all(np.isfinite(losses))
     assert losses[-1] < losses[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("force_fallback", [False, True])
+def test_learned_preconditioner_solves_the_same_system(force_fallback):
+    """With any network attached, the learned PCG must reach the same damped step as
+    block-Jacobi (both run to the same tolerance); a forced fallback must restart cleanly."""
+    import fagra_amd as fa
+    from fagra_amd import learned, synthetic
+    p = synthetic.generate("ring16", seed=21, n_cameras=12, n_points=600, huber=1.0)
+    net = learned.Network(params=M.init_params(7))   # nonzero output layer: G != I
+    steps = {}
+    for pc in (fa.PRECOND_BLOCK_JACOBI, fa.PRECOND_LEARNED):
+        ba = fa.BalProblem(p, precision=fa.FP32)
+        s = fa.Schur(precision=fa.FP32, relative_tolerance=1e-6, preconditioner=pc)
+        fa.lib().fg_schur_set_mlp(s.h, net.h)
+        if force_fallback:
+            s.cfg.fallback_factor = 1e-30
+            s.cfg.fallback_iterations = 2
+            s._push()
+        assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+        st, d, pr = s.solve_damped(1e-2)
+        dg = s.diagnostics()
+        assert st == fa.OK and dg.termination == fa.CG_CONVERGED
+        if pc == fa.PRECOND_LEARNED:
+            assert dg.used_fallback == int(force_fallback)
+        steps[pc] = (d, pr)
+    d0, p0 = steps[fa.PRECOND_BLOCK_JACOBI]
+    d1, p1 = steps[fa.PRECOND_LEARNED]
+    assert np.linalg.norm(d1 - d0) < 3e-3 * np.linalg.norm(d0)
+    assert abs(p1 - p0) < 3e-3 * abs(p0)
+
+
+@pytest.mark.gpu
+def test_learned_without_network_is_rejected():
+    import fagra_amd as fa
+    from fagra_amd import synthetic
+    p = synthetic.generate("ring16", seed=22, n_cameras=6, n_points=100)
+    ba = fa.BalProblem(p, precision=fa.FP32)
+    s = fa.Schur(precision=fa.FP32, preconditioner=fa.PRECOND_LEARNED)
+    assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+    assert s.solve_damped(1e-2)[0] == fa.INVALID_OPTIONS
