@@ -80,7 +80,6 @@ struct fg_ba {
   // Reduction scratch.
   double* d_parts = nullptr;
   int* d_flags = nullptr;
-  unsigned* d_counter = nullptr;
   double* d_out = nullptr;
   double* h_pinned = nullptr;  // 8 doubles
   int* h_flags = nullptr;
@@ -104,13 +103,16 @@ struct fg_schur {
   void *rhs = nullptr, *zero = nullptr;
   double *d_delta = nullptr;  // [9nc cameras | 3np points in slot order]
   double *d_rows = nullptr;   // 2no scratch for operator products
-  double *d_parts = nullptr;  // max(nc, blocks) x kParts
+  double *d_parts = nullptr;  // max(nc, blocks) x kParts (per-camera / per-block partials)
+  double *d_cgparts = nullptr;  // PCG update partials, norm partials
+  void* zb = nullptr;           // block-Jacobi z at a possible fallback (T [9nc])
   double *d_sums = nullptr;   // kParts
   DevScalars* d_sc = nullptr;
   int* d_fail = nullptr;
   int* d_flags = nullptr;
   double* d_scalar = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirrors
+  void* h_poll = nullptr;      // two DevScalars, pinned
   double* h_sums = nullptr;
   int* h_int = nullptr;
   std::vector<double> h_delta;  // reference order
@@ -186,7 +188,9 @@ fg_status cost_impl(fg_ba* b, int buf, double* out) {
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL(k_cost<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np, b->d_chunk_base,
                      b->d_chunk_len, b->d_slot_cam, (const Vec2<T>*)b->d_slot_pix, (const T*)b->d_tab[buf],
-                     (const T*)b->d_pts_t[buf], (T)b->huber, b->d_parts, b->d_flags, b->d_counter, b->d_out);
+                     (const T*)b->d_pts_t[buf], (T)b->huber, b->d_parts, b->d_flags);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)b->d_parts, blocks, 0,
+                     b->d_out);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(b->h_pinned, b->d_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -220,23 +224,24 @@ PointArgs point_args(const fg_ba* b) {
 
 template <typename T, int MODE>
 void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, T lambda,
-                   double* delta_out, double* rows_out, double* parts, const DevScalars* sc) {
+                   double* delta_out, double* rows_out, double* parts, const CgState* st) {
   fg_ba* b = s->ba;
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur], fw,
                      (const T*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda, (T*)s->tp,
-                     delta_out, rows_out, parts, sc);
+                     delta_out, rows_out, parts, st);
 }
 
 template <typename T, int MODE>
 void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const T* fw, const T* vin,
-                    const double* delta_c, const T* rhs, T lambda, T* out, double* out_d) {
+                    const double* delta_c, const T* rhs, T lambda, T* out, double* out_d,
+                    const CgState* st = nullptr) {
   fg_ba* b = s->ba;
   hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
                      (const T*)b->d_tab[b->cur], (const T*)s->tp, delta_p, rows_in, fw, (const T*)s->bs,
-                     (const T*)s->dc, (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, s->d_sc);
+                     (const T*)s->dc, (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st);
 }
 
 template <typename T>
@@ -266,8 +271,10 @@ fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m,
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
   const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
-  hipLaunchKernelGGL(k_norm_inf<T>, dim3(blocks), dim3(kBlock), 0, b->stream, (int)n, a, (int)m, c, s->d_parts,
-                     s->d_flags, &s->d_sc->counter_c, s->d_scalar);
+  hipLaunchKernelGGL(k_norm_inf<T>, dim3(blocks), dim3(kBlock), 0, b->stream, (int)n, a, (int)m, c, s->d_cgparts,
+                     s->d_flags);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_cgparts, blocks, 1,
+                     s->d_scalar);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_sums, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -336,74 +343,85 @@ fg_status sum_parts(fg_schur* s, int rows) {
 }
 
 // PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
+// Iteration i launches direction, the two product kernels and the update. The host polls the
+// published done flag of the previous batch while the next batch runs; iterations after the
+// solve ended are no-ops.
 template <typename T>
 fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc;
   const int cblocks = grid_for(nc);
+  const int max_it = (int)s->cfg.max_iterations;
+  const int fallback_at = (int)std::min<size_t>(s->cfg.fallback_iterations, (size_t)std::numeric_limits<int>::max());
   DevScalars init{};
-  init.max_iters = (int)s->cfg.max_iterations;
-  init.lambda = lambda;
-  init.use_learned = learned ? 1 : 0;
-  init.fallback_at = (int)std::min<size_t>(s->cfg.fallback_iterations, (size_t)std::numeric_limits<int>::max());
+  for (int i = 0; i < 2; ++i) {
+    init.st[i].iters = -1;
+    init.st[i].term = -1;
+    init.st[i].use_learned = learned ? 1 : 0;
+  }
+  init.max_iters = max_it;
+  init.fallback_at = fallback_at;
   FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off, (const T*)s->lc,
-                     (const T*)s->gmat, lambda, (const T*)s->rhs, (T*)s->x, (T*)s->r, (T*)s->z, (T*)s->p, s->d_parts,
-                     s->d_sc, s->cfg.relative_tolerance, s->cfg.fallback_factor);
+                     (const T*)s->gmat, lambda, (const T*)s->rhs, (T*)s->x, (T*)s->r, (T*)s->z, (T*)s->p,
+                     s->d_cgparts, (const CgState*)&s->d_sc->st[0]);
   FG_TRY(hipGetLastError());
   const bool timed = s->cfg.collect_timings != 0;
-  const size_t max_it = s->cfg.max_iterations;
-  size_t launched = 0;
-  int batch = 0;
-  const int kBatch = 4;
-  bool stop = false;
-  // Pipelined batches: batch i's done flag is read while batch i+1 is queued; iterations after
-  // convergence are no-ops (every kernel checks the device done flag).
+  auto direction = [&](int i) {
+    hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                       (const T*)s->lc, lambda, (const T*)b->d_tab[b->cur], (const T*)s->dc, (const T*)s->zb,
+                       (const T*)s->z, (T*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
+                       (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], max_it, fallback_at,
+                       s->cfg.relative_tolerance, s->cfg.fallback_factor);
+  };
   hipEvent_t evs[2];
   FG_TRY(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
   FG_TRY(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
-  int* flags = s->h_int;  // [2 slots x 2]
-  // Initial residual may already satisfy the tolerance.
-  FG_TRY(hipMemcpyAsync(flags + 4, &s->d_sc->done, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipStreamSynchronize(b->stream));
-  if (flags[4]) stop = true;
+  DevScalars* polls = (DevScalars*)s->h_poll;  // two pinned slots
+  const int kBatch = 4;
+  int launched = 0, batch = 0;
+  bool stop = false;
   while (!stop && launched < max_it) {
-    const size_t nb = std::min<size_t>(kBatch, max_it - launched);
-    for (size_t it = 0; it < nb; ++it) {
-      hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                         (const T*)s->lc, lambda, (const T*)b->d_tab[b->cur], (const T*)s->dc, (const T*)s->r,
-                         (T*)s->z, (T*)s->p, (T*)s->fw, s->d_parts, s->d_sc);
+    const int nb = std::min(kBatch, max_it - launched);
+    for (int it = 0; it < nb; ++it) {
+      const int i = launched + it;
+      CgState* cur = &s->d_sc->st[(i + 1) & 1];
+      direction(i);
       const bool ev = timed && s->events_used + 2 <= s->events.size();
       if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
-      launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, s->d_sc);
+      launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, cur);
       launch_cameras<T, cSchur>(s, nullptr, nullptr, nullptr, (const T*)s->p, nullptr, nullptr, lambda, (T*)s->q,
-                                nullptr);
+                                nullptr, cur);
       if (ev) {
         (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
         s->events_used += 2;
       }
       hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                          (const T*)s->lc, (const T*)s->gmat, lambda, (const T*)s->p, (const T*)s->q, (T*)s->x,
-                         (T*)s->r, (T*)s->z, s->d_parts, s->d_sc);
+                         (T*)s->r, (T*)s->z, (T*)s->zb, (const double*)s->d_parts, s->d_cgparts, cur, fallback_at);
     }
     FG_TRY(hipGetLastError());
     launched += nb;
     const int slot = batch & 1;
-    FG_TRY(hipMemcpyAsync(flags + 2 * slot, &s->d_sc->done, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    FG_TRY(hipMemcpyAsync(&polls[slot], s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
     FG_TRY(hipEventRecord(evs[slot], b->stream));
     if (batch > 0) {
       FG_TRY(hipEventSynchronize(evs[slot ^ 1]));
-      if (flags[2 * (slot ^ 1)]) stop = true;
+      const DevScalars& h = polls[slot ^ 1];
+      if (h.st[0].done || h.st[1].done) stop = true;
     }
     ++batch;
   }
+  // the last update's convergence / exhaustion decision
+  direction(launched);
+  FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
   (void)hipEventDestroy(evs[0]);
   (void)hipEventDestroy(evs[1]);
-  const DevScalars& h = *s->h_sc;
-  s->iterations += (size_t)h.iters;
-  s->diag.iterations = (size_t)h.iters;
+  const CgState& h = s->h_sc->st[(launched + 1) & 1];
+  s->iterations += (size_t)std::max(h.iters, 0);
+  s->diag.iterations = (size_t)std::max(h.iters, 0);
   s->diag.used_fallback = h.fallback_taken;
   if (h.term == 0) {
     s->diag.termination = FG_CG_CONVERGED;
@@ -413,7 +431,7 @@ fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
   } else {
     s->diag.termination = FG_CG_ITERATION_LIMIT;
   }
-  s->tim.matvecs += (size_t)h.iters;
+  s->tim.matvecs += (size_t)std::max(h.iters, 0);
   return FG_OK;
 }
 
@@ -520,7 +538,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((T**)&s->cs, 6 * np));
   FG_TRY(dalloc((T**)&s->lc, 45 * nc));
   FG_TRY(dalloc((T**)&s->mp, 6 * np));
-  FG_TRY(dalloc((T**)&s->tp, 3 * np));
+  FG_TRY(dalloc((T**)&s->tp, 4 * np));
   FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((T**)v, 9 * nc));
@@ -537,6 +555,8 @@ fg_status alloc_model(fg_schur* s) {
   const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   const size_t rows = std::max({nc, pblocks, (size_t)1024});
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
+  FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc))));
+  FG_TRY(dalloc((T**)&s->zb, 9 * nc));
   // The memsets above run on the null stream; complete them before stream work starts.
   FG_TRY(hipDeviceSynchronize());
   return FG_OK;
@@ -552,6 +572,9 @@ void free_model(fg_schur* s) {
   dfree(s->d_delta);
   dfree(s->d_rows);
   dfree(s->d_parts);
+  dfree(s->d_cgparts);
+  if (s->zb) (void)hipFree(s->zb);
+  s->zb = nullptr;
   dfree(s->mx);
   dfree(s->my);
   dfree(s->mxf);
@@ -839,10 +862,9 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
   }
   const size_t rows = std::max<size_t>(1024, (size_t)grid_for(b->nchunks, kChunksPerBlock));
   if (dalloc(&b->d_parts, rows) != hipSuccess || dalloc(&b->d_flags, 4) != hipSuccess ||
-      dalloc(&b->d_counter, 4) != hipSuccess || dalloc(&b->d_out, 4) != hipSuccess ||
+      dalloc(&b->d_out, 4) != hipSuccess ||
       dalloc(&b->d_delta_stage, 9 * nc + 3 * np) != hipSuccess)
     return fail(FG_DEVICE_ERROR);
-  if (hipMemset(b->d_counter, 0, 4 * sizeof(unsigned)) != hipSuccess) return fail(FG_DEVICE_ERROR);
   if (hipHostMalloc((void**)&b->h_pinned, 8 * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&b->h_flags, 8 * sizeof(int), 0) != hipSuccess)
     return fail(FG_DEVICE_ERROR);
@@ -874,7 +896,6 @@ void fg_ba_destroy(fg_ba* b) {
   }
   dfree(b->d_parts);
   dfree(b->d_flags);
-  dfree(b->d_counter);
   dfree(b->d_out);
   dfree(b->d_delta_stage);
   if (b->h_pinned) (void)hipHostFree(b->h_pinned);
@@ -983,6 +1004,7 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
     return fail();
   if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail();
   if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
+      hipHostMalloc(&s->h_poll, 2 * sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_sums, 2 * kParts * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_int, 8 * sizeof(int), 0) != hipSuccess)
     return fail();
@@ -1001,6 +1023,7 @@ void fg_schur_destroy(fg_schur* s) {
   dfree(s->d_scalar);
   dfree(s->d_sums);
   if (s->h_sc) (void)hipHostFree(s->h_sc);
+  if (s->h_poll) (void)hipHostFree(s->h_poll);
   if (s->h_sums) (void)hipHostFree(s->h_sums);
   if (s->h_int) (void)hipHostFree(s->h_int);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);

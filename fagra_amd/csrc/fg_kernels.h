@@ -18,13 +18,18 @@ constexpr int kBlock = 256;          // threads per workgroup
 constexpr int kParts = 6;            // partial sums per workgroup / camera
 constexpr int kChunksPerBlock = kBlock / kWave;
 
-// Device-side scalars of one linear solve; counters reset by the host before use.
+// PCG state. Kernels of launch i read slot i % 2 (the direction kernel) or (i + 1) % 2 (the
+// product and update kernels); the direction kernel of launch i writes slot (i + 1) % 2 from
+// block 0 only. No workgroup ever reads a slot that another workgroup of the same kernel writes,
+// so no cross-workgroup synchronization is needed inside a kernel.
+struct CgState {
+  double rz, tol2, bb, target, rr, pad_d;
+  int done, iters, term, use_learned;
+  int fallback_taken, pad0, pad1, pad2;
+};
 struct DevScalars {
-  double rz, pq, alpha, beta, rr, bb, tol2, lambda;
-  double learned_target;             // residual^2 the learned preconditioner must reach
-  int done, iters, term, max_iters;
-  int fallback_at, use_learned, fallback_taken, restart;
-  unsigned counter_a, counter_b, counter_c, counter_d;
+  CgState st[2];
+  int max_iters, fallback_at, pad0, pad1;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -43,31 +48,12 @@ __device__ __forceinline__ T block_sum(T v, T* smem /* kBlock/kWave */) {
   return r;  // valid in thread 0
 }
 
-// True in every thread of the last workgroup to finish (after all earlier writes are visible).
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
-}
-
-// Deterministic sum of parts[0..n) by one workgroup (fixed assignment, fixed tree).
-__device__ __forceinline__ double block_sum_parts(const double* parts, int n, double* smem) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[i];
-  return block_sum(s, smem);
-}
-// Same over column col of a (n x kParts_) partial array; result broadcast to all threads.
+// Deterministic sum of column col of a (n x kParts) partial array by one workgroup (fixed
+// assignment, fixed tree); result broadcast to all threads.
 __device__ __forceinline__ double block_sum_parts_col(const double* parts, int n, int col, double* smem) {
   __shared__ double bcast;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[6 * i + col];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[kParts * i + col];
   s = block_sum(s, smem);
   if (threadIdx.x == 0) bcast = s;
   __syncthreads();
@@ -118,15 +104,17 @@ __global__ void k_camera_tables(int nc, const double* __restrict__ cams, T* __re
   const double R[9] = {ww + ii - jj - kk2, ij - wk, wj + ik, wk + ij, ww - ii + jj - kk2,
                        jk - wi, ik - wj, wi + jk, ww - ii - jj + kk2};
   T* o = tab + kCamTab * c;
-  for (int i = 0; i < 9; ++i) o[i] = T(R[i]);
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < kCamTab; ++i) o[i] = T(0);
+  for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) {
+      o[tR(i, j)] = T(R[3 * i + j]);
       double s = 0;
       for (int k = 0; k < 3; ++k) s += R[3 * i + k] * Jr[3 * k + j];
-      o[9 + 3 * i + j] = T(s);
+      o[tRJ(i, j)] = T(s);
     }
-  o[18] = T(cam[3]); o[19] = T(cam[4]); o[20] = T(cam[5]);
-  o[21] = T(cam[6]); o[22] = T(cam[7]); o[23] = T(cam[8]);
+    o[tT(i)] = T(cam[3 + i]);
+  }
+  o[tF] = T(cam[6]); o[tK1] = T(cam[7]); o[tK2] = T(cam[8]);
 }
 
 // Blockwise Huber (src/factors.rs:42-52): cost and sqrt(weight).
@@ -147,14 +135,14 @@ __device__ __forceinline__ void huber_weight(T rx, T ry, T delta, T& cost, T& sw
 template <typename T>
 __device__ __forceinline__ bool project(const T* __restrict__ tab, T x0, T x1, T x2, T& px, T& py, T& qx,
                                         T& qy, T& z) {
-  const T p0 = tab[0] * x0 + tab[1] * x1 + tab[2] * x2 + tab[18];
-  const T p1 = tab[3] * x0 + tab[4] * x1 + tab[5] * x2 + tab[19];
-  const T p2 = tab[6] * x0 + tab[7] * x1 + tab[8] * x2 + tab[20];
+  const T p0 = tab[tR(0, 0)] * x0 + tab[tR(0, 1)] * x1 + tab[tR(0, 2)] * x2 + tab[tT(0)];
+  const T p1 = tab[tR(1, 0)] * x0 + tab[tR(1, 1)] * x1 + tab[tR(1, 2)] * x2 + tab[tT(1)];
+  const T p2 = tab[tR(2, 0)] * x0 + tab[tR(2, 1)] * x1 + tab[tR(2, 2)] * x2 + tab[tT(2)];
   qx = -p0 / p2;
   qy = -p1 / p2;
   z = p2;
   const T r2 = qx * qx + qy * qy;
-  const T s = tab[21] * (T(1) + tab[22] * r2 + tab[23] * r2 * r2);
+  const T s = tab[tF] * (T(1) + tab[tK1] * r2 + tab[tK2] * r2 * r2);
   px = s * qx;
   py = s * qy;
   return p2 != T(0) && isfinite(px) && isfinite(py);
@@ -168,8 +156,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int* __restrict__ chunk_base,
                                                  const int* __restrict__ chunk_len, const int* __restrict__ slot_cam,
                                                  const Vec2<T>* __restrict__ slot_pix, const T* __restrict__ tab,
-                                                 const T* __restrict__ pts, T huber, double* parts, int* flags,
-                                                 unsigned* counter, double* out) {
+                                                 const T* __restrict__ pts, T huber, double* parts, int* flags) {
   __shared__ double smem[kBlock / kWave];
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
@@ -184,8 +171,10 @@ __global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int*
         const int s = base + j * kWave + lane;
         const int c = slot_cam[s];
         if (c < 0) continue;
+        T ct[16];
+        load_cam16(tab, c, ct);
         T px, py, qx, qy, z;
-        bool ok = project(tab + kCamTab * c, x0, x1, x2, px, py, qx, qy, z);
+        bool ok = project(ct, x0, x1, x2, px, py, qx, qy, z);
         const Vec2<T> ob = slot_pix[s];
         T cost, sw;
         huber_weight(px - ob.x, py - ob.y, huber, cost, sw);
@@ -197,13 +186,24 @@ __global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int*
   if (bad) atomicOr(flags, 1);
   const double b = block_sum(acc, smem);
   if (threadIdx.x == 0) parts[blockIdx.x] = b;
-  if (last_block(counter)) {
-    const double total = block_sum_parts(parts, gridDim.x, smem);
-    if (threadIdx.x == 0) {
-      *out = total;
-      *counter = 0;
-    }
+}
+
+// Final reduction of per-workgroup partials by one workgroup: sum (op 0) or maximum (op 1).
+static __global__ void k_reduce_parts(const double* __restrict__ parts, int n, int op, double* __restrict__ out) {
+  __shared__ double smem[kBlock / kWave];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v = op == 0 ? v + parts[i] : fmax(v, parts[i]);
+  if (op == 0) {
+    v = block_sum(v, smem);
+  } else {
+    v = wave_max(v);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) smem[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < (int)(blockDim.x / kWave); ++i) v = fmax(v, smem[i]);
   }
+  if (threadIdx.x == 0) *out = v;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -230,7 +230,8 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
     const int s = base + j * kWave + lane;
     const int c = slot_cam[s];
     if (c < 0) continue;
-    const T* ct = tab + kCamTab * c;
+    T ct[16];
+    load_cam16(tab, c, ct);
     T px, py, qx, qy, z;
     bool ok = project(ct, x0, x1, x2, px, py, qx, qy, z);
     const Vec2<T> ob = slot_pix[s];
@@ -247,8 +248,8 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
     T jp0[3], jp1[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      jp0[k] = sw * (geo.dp0[0] * ct[k] + geo.dp0[1] * ct[3 + k] + geo.dp0[2] * ct[6 + k]);
-      jp1[k] = sw * (geo.dp1[0] * ct[k] + geo.dp1[1] * ct[3 + k] + geo.dp1[2] * ct[6 + k]);
+      jp0[k] = sw * (geo.dp0[0] * ct[tR(0, k)] + geo.dp0[1] * ct[tR(1, k)] + geo.dp0[2] * ct[tR(2, k)]);
+      jp1[k] = sw * (geo.dp1[0] * ct[tR(0, k)] + geo.dp1[1] * ct[tR(1, k)] + geo.dp1[2] * ct[tR(2, k)]);
       ok = ok && isfinite(jp0[k]) && isfinite(jp1[k]);
     }
     ok = ok && isfinite(wrx) && isfinite(wry);
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_norm_inf(int n, const T* __restrict__ a, int m, const T* __restrict__ b,
-                                                     double* parts, int* flags, unsigned* counter, double* out) {
+                                                     double* parts, int* flags) {
   __shared__ double smem[kBlock / kWave];
   double v = 0.0;
   bool bad = false;
@@ -344,14 +345,6 @@ __global__ void __launch_bounds__(kBlock) k_norm_inf(int n, const T* __restrict_
     double r = 0;
     for (int i = 0; i < kBlock / kWave; ++i) r = fmax(r, smem[i]);
     parts[blockIdx.x] = r;
-  }
-  if (last_block(counter)) {
-    if (threadIdx.x == 0) {
-      double r = 0;
-      for (int i = 0; i < (int)gridDim.x; ++i) r = fmax(r, parts[i]);
-      *out = r;
-      *counter = 0;
-    }
   }
 }
 
@@ -473,12 +466,11 @@ __device__ __forceinline__ void forward_table(const T* __restrict__ ct, const S*
                                               bool scaled, T* __restrict__ fw, T z[9]) {
 #pragma unroll
   for (int j = 0; j < 9; ++j) z[j] = scaled ? T(x[j]) / d[j] : T(x[j]);
-  const T* rj = ct + 9;
-  fw[0] = rj[0] * z[0] + rj[1] * z[1] + rj[2] * z[2];
-  fw[1] = rj[3] * z[0] + rj[4] * z[1] + rj[5] * z[2];
-  fw[2] = rj[6] * z[0] + rj[7] * z[1] + rj[8] * z[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) fw[i] = ct[tRJ(i, 0)] * z[0] + ct[tRJ(i, 1)] * z[1] + ct[tRJ(i, 2)] * z[2];
 #pragma unroll
   for (int j = 3; j < 9; ++j) fw[j] = z[j];
+  fw[9] = fw[10] = fw[11] = T(0);
 }
 
 template <typename T, typename S>
@@ -509,14 +501,54 @@ struct PointArgs {
   const int* slot_row;
 };
 
+// One observation of a point-major pass: record, camera table and forward table in registers.
+template <typename T, int MODE>
+__device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& r4,
+                                          const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
+                                          const T* __restrict__ fw, const T dpx[3],
+                                          const double* __restrict__ rows_in, double* __restrict__ rows_out,
+                                          T w[3], double& uu) {
+  T ct[16];
+  load_cam16(tab, c, ct);
+  ObsGeom<T> geo;
+  obs_geom(r4, ct, geo);
+  if (MODE == kSchur || MODE == kRecover) {
+    T f[kFwd];
+    load_fwd(fw, c, f);
+    T u0, u1;
+    jc_apply(geo, f, u0, u1);
+    jpt_accumulate(geo, ct, u0, u1, w);
+  } else if (MODE == kVerify || MODE == kJv) {
+    T f[kFwd];
+    load_fwd(fw, c, f);
+    T u0, u1, v0, v1;
+    jc_apply(geo, f, u0, u1);
+    jp_apply(geo, ct, dpx, v0, v1);
+    u0 += v0;
+    u1 += v1;
+    if (MODE == kJv) {
+      const int row = pa.slot_row[s];
+      rows_out[2 * row] = double(u0);
+      rows_out[2 * row + 1] = double(u1);
+    } else {
+      uu += double(u0) * double(u0) + double(u1) * double(u1);
+      const Vec2<T> r = res[s];
+      jpt_accumulate(geo, ct, u0 + r.x, u1 + r.y, w);
+    }
+  } else if (MODE == kJtu) {
+    const int row = pa.slot_row[s];
+    jpt_accumulate(geo, ct, T(rows_in[2 * row]), T(rows_in[2 * row + 1]), w);
+  }
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_points(
     PointArgs pa, const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
     const T* __restrict__ fw, const T* __restrict__ mp, const T* __restrict__ gp, const T* __restrict__ dpv,
     const double* __restrict__ delta_p_in, const double* __restrict__ rows_in, T lambda, T* __restrict__ t_out,
-    double* __restrict__ delta_out, double* __restrict__ rows_out, double* parts, const DevScalars* sc) {
+    double* __restrict__ delta_out, double* __restrict__ rows_out, double* parts, const CgState* st) {
   __shared__ double smem[kBlock / kWave];
-  if (MODE == kSchur && sc != nullptr && sc->done) return;
+  if (MODE == kSchur && st != nullptr && st->done) return;
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   double uu = 0.0, n2 = 0.0, n2s = 0.0, gd = 0.0, g2 = 0.0, g2s = 0.0;  // verify partials
@@ -532,36 +564,19 @@ __global__ void __launch_bounds__(kBlock) k_points(
       }
       if (MODE != kRhs) {
         const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
-        for (int j = 0; j < len; ++j) {
-          const int s = base + j * kWave + lane;
-          const int c = pa.slot_cam[s];
-          if (c < 0) continue;
-          const T* ct = tab + kCamTab * c;
-          ObsGeom<T> geo;
-          obs_geom(rec[s], ct, geo);
-          if (MODE == kSchur || MODE == kRecover) {
-            T u0, u1;
-            jc_apply(geo, fw + kFwd * c, u0, u1);
-            jpt_accumulate(geo, ct, u0, u1, w);
-          } else if (MODE == kVerify || MODE == kJv) {
-            T u0, u1, v0, v1;
-            jc_apply(geo, fw + kFwd * c, u0, u1);
-            jp_apply(geo, ct, dpx, v0, v1);
-            u0 += v0;
-            u1 += v1;
-            if (MODE == kJv) {
-              const int row = pa.slot_row[s];
-              rows_out[2 * row] = double(u0);
-              rows_out[2 * row + 1] = double(u1);
-            } else {
-              uu += double(u0) * double(u0) + double(u1) * double(u1);
-              const Vec2<T> r = res[s];
-              jpt_accumulate(geo, ct, u0 + r.x, u1 + r.y, w);
-            }
-          } else if (MODE == kJtu) {
-            const int row = pa.slot_row[s];
-            jpt_accumulate(geo, ct, T(rows_in[2 * row]), T(rows_in[2 * row + 1]), w);
-          }
+        int j = 0;
+        // two observations per step: both loads issue before either is used
+        for (; j + 1 < len; j += 2) {
+          const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
+          const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
+          const Vec4<T> r0 = rec[s0], r1 = rec[s1];
+          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, uu);
+          if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, uu);
+        }
+        if (j < len) {
+          const int s0 = base + j * kWave + lane;
+          const int c0 = pa.slot_cam[s0];
+          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, uu);
         }
       }
       if (MODE == kSchur || MODE == kRhs || MODE == kRecover) {
@@ -581,8 +596,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
 #pragma unroll
           for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -double(t[a]);
         } else {
-#pragma unroll
-          for (int a = 0; a < 3; ++a) t_out[3 * i + a] = t[a];
+          reinterpret_cast<Vec4<T>*>(t_out)[i] = Vec4<T>{t[0], t[1], t[2], T(0)};
         }
       } else if (MODE == kVerify) {
 #pragma unroll
@@ -631,12 +645,12 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
     const T* __restrict__ bs, const T* __restrict__ dc, const T* __restrict__ gc, const T* __restrict__ vin,
     const double* __restrict__ delta_c, const T* __restrict__ rhs, T lambda, T* __restrict__ out,
-    double* __restrict__ out_d, double* parts, DevScalars* sc) {
+    double* __restrict__ out_d, double* parts, const CgState* st) {
   __shared__ T ct[kCamTab];
   __shared__ T cfw[kFwd];
   __shared__ T red[9][kBlock / kWave];
   __shared__ double smem[kBlock / kWave];
-  if ((MODE == cSchur) && sc->done) return;
+  if (MODE == cSchur && st != nullptr && st->done) return;
   const int c = blockIdx.x;
   if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
   if (MODE == cVerify && threadIdx.x < kFwd) cfw[threadIdx.x] = fw[kFwd * c + threadIdx.x];
@@ -645,31 +659,42 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
 #pragma unroll
   for (int a = 0; a < 9; ++a) acc[a] = T(0);
   const int k0 = cam_off[c], k1 = cam_off[c + 1];
-  for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
+  auto one = [&](int k, const Vec4<T>& r4, const Vec4<T>& tv) {
     ObsGeom<T> geo;
-    obs_geom(rec_cm[k], ct, geo);
+    obs_geom(r4, ct, geo);
     T v0, v1;
     if (MODE == cJtu) {
       v0 = T(rows_in[2 * k]);
       v1 = T(rows_in[2 * k + 1]);
     } else if (MODE == cVerify) {
-      const int ps = k_pslot[k];
-      const T x[3] = {T(delta_p[3 * ps]), T(delta_p[3 * ps + 1]), T(delta_p[3 * ps + 2])};
+      const T x[3] = {tv.x, tv.y, tv.z};
       T u0, u1, w0, w1;
       jc_apply(geo, cfw, u0, u1);
       jp_apply(geo, ct, x, w0, w1);
-      u0 += w0;
-      u1 += w1;
       const Vec2<T> r = res_cm[k];
-      v0 = u0 + r.x;
-      v1 = u1 + r.y;
+      v0 = u0 + w0 + r.x;
+      v1 = u1 + w1 + r.y;
     } else {
-      const int ps = k_pslot[k];
-      const T x[3] = {tp[3 * ps], tp[3 * ps + 1], tp[3 * ps + 2]};
+      const T x[3] = {tv.x, tv.y, tv.z};
       jp_apply(geo, ct, x, v0, v1);
     }
     jct_factors(geo, v0, v1, acc);
+  };
+  auto gather = [&](int k) -> Vec4<T> {
+    if (MODE == cJtu) return Vec4<T>{T(0), T(0), T(0), T(0)};
+    const int ps = k_pslot[k];
+    if (MODE == cVerify) return Vec4<T>{T(delta_p[3 * ps]), T(delta_p[3 * ps + 1]), T(delta_p[3 * ps + 2]), T(0)};
+    return reinterpret_cast<const Vec4<T>*>(tp)[ps];
+  };
+  int k = k0 + threadIdx.x;
+  // two rows per step: records and gathers of both issue before either is used
+  for (; k + kBlock < k1; k += 2 * kBlock) {
+    const Vec4<T> ra = rec_cm[k], rb = rec_cm[k + kBlock];
+    const Vec4<T> ta = gather(k), tb = gather(k + kBlock);
+    one(k, ra, ta);
+    one(k + kBlock, rb, tb);
   }
+  if (k < k1) one(k, rec_cm[k], gather(k));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
@@ -687,9 +712,8 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     }
     // T(acc): rotation slots through (R Jr)^T
     T y[9];
-    const T* rj = ct + 9;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) y[j] = rj[j] * s[0] + rj[3 + j] * s[1] + rj[6 + j] * s[2];
+    for (int j = 0; j < 3; ++j) y[j] = ct[tRJ(0, j)] * s[0] + ct[tRJ(1, j)] * s[1] + ct[tRJ(2, j)] * s[2];
 #pragma unroll
     for (int j = 3; j < 9; ++j) y[j] = s[j];
     const T* d = dc + 9 * c;
@@ -743,23 +767,6 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
       pc[2] = part2;
       pc[3] = part3;
       pc[4] = part4;
-    }
-  }
-  if (MODE == cSchur) {
-    if (last_block(&sc->counter_a)) {
-      double pq = 0.0;
-      for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += parts[kParts * i];
-      pq = block_sum(pq, smem);
-      if (threadIdx.x == 0) {
-        sc->pq = pq;
-        if (!(pq > 0.0) || !isfinite(pq)) {
-          sc->done = 1;
-          sc->term = 2;  // NotPositiveDefinite
-        } else {
-          sc->alpha = sc->rz / pq;
-        }
-        sc->counter_a = 0;
-      }
     }
   }
 }
@@ -820,14 +827,14 @@ __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_
   }
 }
 
+// x = 0, r = b, z = M r, p = z; per-workgroup partials r.r (col 0) and r.z (col 1).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
                                                     const T* __restrict__ gmat, T lambda, const T* __restrict__ rhs,
                                                     T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                    T* __restrict__ p, double* parts, DevScalars* sc, double rel_tol,
-                                                    double fallback_factor) {
+                                                    T* __restrict__ p, double* parts, const CgState* st) {
   __shared__ double smem[kBlock / kWave];
-  const bool learned = sc->use_learned != 0;
+  const bool learned = st->use_learned != 0;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0;
   if (c < nc) {
@@ -852,94 +859,111 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
   if (threadIdx.x == 0) {
     parts[kParts * blockIdx.x] = b0;
     parts[kParts * blockIdx.x + 1] = b1;
-  }
-  if (last_block(&sc->counter_b)) {
-    const double bb = block_sum_parts_col(parts, gridDim.x, 0, smem);
-    const double rz0 = block_sum_parts_col(parts, gridDim.x, 1, smem);
-    if (threadIdx.x == 0) {
-      sc->bb = bb;
-      sc->rr = bb;
-      sc->rz = rz0;
-      sc->beta = 0.0;
-      sc->iters = 0;
-      sc->tol2 = rel_tol * rel_tol * bb;
-      sc->learned_target = fallback_factor * fallback_factor * bb;
-      if (bb <= sc->tol2) {
-        sc->done = 1;
-        sc->term = 0;  // Converged
-      } else {
-        sc->done = 0;
-        sc->term = -1;
-      }
-      sc->counter_b = 0;
-    }
+    parts[kParts * blockIdx.x + 2] = 0.0;
   }
 }
 
-// p = z + beta p and the forward table of p (scaled coordinates). After a fallback from the
-// learned preconditioner the direction restarts: z = block-Jacobi r, p = z, rz = r.z.
+// Start of PCG iteration i: every workgroup sums the previous update's partials (the same way,
+// so all agree), decides convergence / exhaustion / fallback, and forms p = z + beta p with the
+// forward table of p; workgroup 0 publishes the new state into slot out.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __restrict__ cam_off,
                                                          const T* __restrict__ lc, T lambda, const T* __restrict__ tab,
-                                                         const T* __restrict__ dc, const T* __restrict__ r,
-                                                         T* __restrict__ z, T* __restrict__ p, T* __restrict__ fw,
-                                                         double* parts, DevScalars* sc) {
+                                                         const T* __restrict__ dc, const T* __restrict__ zb,
+                                                         const T* __restrict__ z, T* __restrict__ p,
+                                                         T* __restrict__ fw, const double* __restrict__ parts,
+                                                         int nparts, const CgState* __restrict__ in, CgState* out,
+                                                         int max_iters, int fallback_at, double rel_tol,
+                                                         double fallback_factor) {
   __shared__ double smem[kBlock / kWave];
-  if (sc->done) return;
-  const bool restart = sc->restart != 0;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  double rz = 0.0;
-  if (c < nc) {
-    T pv[9];
-    if (restart) {
-      T rv[9], zv[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) rv[i] = r[9 * c + i];
-      precondition(c, cam_off, lc, (const T*)nullptr, false, lambda, rv, zv);
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        z[9 * c + i] = zv[i];
-        pv[i] = zv[i];
-        p[9 * c + i] = pv[i];
-        rz += double(rv[i]) * double(zv[i]);
-      }
+  CgState s = *in;
+  if (s.done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
+    return;
+  }
+  const double rr = block_sum_parts_col(parts, nparts, 0, smem);
+  const double rz = block_sum_parts_col(parts, nparts, 1, smem);
+  const double rzb = block_sum_parts_col(parts, nparts, 2, smem);
+  double beta = 0.0;
+  bool restart = false;
+  s.rr = rr;
+  if (s.iters < 0) {  // first direction after k_cg_init: p = z already
+    s.bb = rr;
+    s.tol2 = rel_tol * rel_tol * rr;
+    s.target = fallback_factor * fallback_factor * rr;
+    s.rz = rz;
+    s.iters = 0;
+    if (rr <= s.tol2) {
+      s.done = 1;
+      s.term = 0;  // Converged
+    }
+  } else {
+    s.iters += 1;
+    if (rr <= s.tol2) {
+      s.done = 1;
+      s.term = 0;  // Converged
+    } else if (s.iters >= max_iters) {
+      s.done = 1;
+      s.term = 1;  // IterationLimit
+    } else if (s.use_learned && s.iters == fallback_at && rr > s.target) {
+      s.use_learned = 0;  // fall back to block-Jacobi and restart the direction
+      s.fallback_taken = 1;
+      restart = true;
+      s.rz = rzb;
     } else {
-      const T beta = T(sc->beta);
+      beta = rz / s.rz;
+      s.rz = rz;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
+  if (s.done) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const bool first = in->iters < 0;
+  T pv[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        pv[i] = z[9 * c + i] + beta * p[9 * c + i];
-        p[9 * c + i] = pv[i];
-      }
-    }
-    T zz[9];
-    forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
+  for (int i = 0; i < 9; ++i) {
+    if (restart) pv[i] = zb[9 * c + i];
+    else if (first) pv[i] = p[9 * c + i];
+    else pv[i] = z[9 * c + i] + T(beta) * p[9 * c + i];
+    p[9 * c + i] = pv[i];
   }
-  if (restart) {
-    const double b = block_sum(rz, smem);
-    if (threadIdx.x == 0) parts[kParts * blockIdx.x] = b;
-    if (last_block(&sc->counter_d)) {
-      const double t = block_sum_parts_col(parts, gridDim.x, 0, smem);
-      if (threadIdx.x == 0) {
-        sc->rz = t;
-        sc->restart = 0;
-        sc->counter_d = 0;
-      }
-    }
-  }
+  T zz[9];
+  forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
 }
 
-// x += alpha p; r -= alpha q; z = M r; rr, rz; convergence / exhaustion in the last block.
+// x += alpha p; r -= alpha q; z = M r. Every workgroup forms p.q from the camera kernel's
+// per-camera partials (alpha = r.z / p.q; p.q <= 0 ends the solve NotPositiveDefinite).
+// Partials: r.r (col 0), r.z (col 1) and, before a possible fallback, r.z_blockJacobi (col 2).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
                                                       const T* __restrict__ gmat, T lambda, const T* __restrict__ p,
                                                       const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                                                      T* __restrict__ z, double* parts, DevScalars* sc) {
+                                                      T* __restrict__ z, T* __restrict__ zb,
+                                                      const double* __restrict__ pq_parts, double* parts, CgState* st,
+                                                      int fallback_at) {
   __shared__ double smem[kBlock / kWave];
-  if (sc->done) return;
-  const bool learned = sc->use_learned != 0;
-  const T alpha = T(sc->alpha);
+  __shared__ double bcast;
+  const CgState s = *st;
+  if (s.done) return;
+  double pq = 0.0;
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += pq_parts[kParts * i];
+  pq = block_sum(pq, smem);
+  if (threadIdx.x == 0) bcast = pq;
+  __syncthreads();
+  pq = bcast;
+  if (!(pq > 0.0) || !isfinite(pq)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->term = 2;  // NotPositiveDefinite
+    }
+    return;
+  }
+  const T alpha = T(s.rz / pq);
+  const bool learned = s.use_learned != 0;
+  const bool both = learned && s.iters + 1 == fallback_at;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  double rr = 0.0, rz = 0.0;
+  double rr = 0.0, rz = 0.0, rzb = 0.0;
   if (c < nc) {
     T rv[9], zv[9];
 #pragma unroll
@@ -955,34 +979,22 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
       rr += double(rv[i]) * double(rv[i]);
       rz += double(rv[i]) * double(zv[i]);
     }
+    if (both) {
+      precondition(c, cam_off, lc, gmat, false, lambda, rv, zv);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        zb[9 * c + i] = zv[i];
+        rzb += double(rv[i]) * double(zv[i]);
+      }
+    }
   }
   const double b0 = block_sum(rr, smem);
   const double b1 = block_sum(rz, smem);
+  const double b2 = block_sum(rzb, smem);
   if (threadIdx.x == 0) {
     parts[kParts * blockIdx.x] = b0;
     parts[kParts * blockIdx.x + 1] = b1;
-  }
-  if (last_block(&sc->counter_b)) {
-    const double rrt = block_sum_parts_col(parts, gridDim.x, 0, smem);
-    const double rzt = block_sum_parts_col(parts, gridDim.x, 1, smem);
-    if (threadIdx.x == 0) {
-      sc->rr = rrt;
-      sc->iters += 1;
-      if (rrt <= sc->tol2) {
-        sc->done = 1;
-        sc->term = 0;  // Converged
-      } else if (sc->iters >= sc->max_iters) {
-        sc->done = 1;
-        sc->term = 1;  // IterationLimit
-      } else if (learned && sc->iters == sc->fallback_at && rrt > sc->learned_target) {
-        sc->use_learned = 0;  // fall back to block-Jacobi, restart the direction
-        sc->fallback_taken = 1;
-        sc->restart = 1;
-      }
-      sc->beta = rzt / sc->rz;
-      sc->rz = rzt;
-      sc->counter_b = 0;
-    }
+    parts[kParts * blockIdx.x + 2] = b2;
   }
 }
 

@@ -16,8 +16,15 @@
 namespace fg {
 
 constexpr int kWave = 64;
-constexpr int kCamTab = 24;  // R(9) row-major, RJ(9) row-major, t(3), f, k1, k2
-constexpr int kFwd = 9;      // per-camera input table: w' = RJ z_rot (3), z_t (3), z_intr (3)
+// Per-camera table, 32 scalars read as eight 4-vectors: rows of R with t_i in the fourth
+// lane, then (f, k1, k2, 0), then rows of R Jr (fourth lane 0), then padding.
+constexpr int kCamTab = 32;
+__host__ __device__ constexpr int tR(int i, int j) { return 4 * i + j; }
+__host__ __device__ constexpr int tT(int i) { return 4 * i + 3; }
+constexpr int tF = 12, tK1 = 13, tK2 = 14;
+__host__ __device__ constexpr int tRJ(int i, int j) { return 16 + 4 * i + j; }
+// Per-camera input table of a retained vector z: w' = RJ z_rot (3), z_t (3), z_intr (3), pad.
+constexpr int kFwd = 12;
 
 // Packed lower-triangular column-major index for a symmetric n x n block (i >= j).
 __host__ __device__ constexpr int sym_index(int n, int i, int j) {
@@ -28,6 +35,27 @@ __host__ __device__ constexpr int sym3(int i, int j) { return i >= j ? sym_index
 
 template <typename T> struct Vec2 { T x, y; };
 template <typename T> struct alignas(4 * sizeof(T)) Vec4 { T x, y, z, w; };
+
+// First 16 scalars of a camera table (R, t, f, k1, k2) with four 4-vector loads.
+template <typename T>
+__device__ __forceinline__ void load_cam16(const T* __restrict__ tab, int c, T out[16]) {
+  const Vec4<T>* t4 = reinterpret_cast<const Vec4<T>*>(tab) + (kCamTab / 4) * c;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const Vec4<T> v = t4[i];
+    out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
+  }
+}
+// Forward table of a camera (w', z_t, z_intr) with three 4-vector loads.
+template <typename T>
+__device__ __forceinline__ void load_fwd(const T* __restrict__ fw, int c, T out[kFwd]) {
+  const Vec4<T>* f4 = reinterpret_cast<const Vec4<T>*>(fw) + (kFwd / 4) * c;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const Vec4<T> v = f4[i];
+    out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
+  }
+}
 
 // Quantities of one observation re-formed from its record and its camera's table.
 template <typename T>
@@ -42,7 +70,7 @@ struct ObsGeom {
 template <typename T>
 __device__ __forceinline__ void obs_geom(const Vec4<T>& rec, const T* __restrict__ tab, ObsGeom<T>& g) {
   g.qx = rec.x; g.qy = rec.y; g.z = rec.z; g.sw = rec.w;
-  const T f = tab[21], k1 = tab[22], k2 = tab[23];
+  const T f = tab[tF], k1 = tab[tK1], k2 = tab[tK2];
   g.r2 = g.qx * g.qx + g.qy * g.qy;
   g.radial = T(1) + k1 * g.r2 + k2 * g.r2 * g.r2;
   const T c2 = T(2) * k1 + T(4) * k2 * g.r2;
@@ -52,9 +80,9 @@ __device__ __forceinline__ void obs_geom(const Vec4<T>& rec, const T* __restrict
   const T iz = T(-1) / g.z;
   g.dp0[0] = iz * a; g.dp0[1] = iz * b; g.dp0[2] = iz * (a * g.qx + b * g.qy);
   g.dp1[0] = iz * b; g.dp1[1] = iz * d; g.dp1[2] = iz * (b * g.qx + d * g.qy);
-  g.pt[0] = -g.qx * g.z - tab[18];
-  g.pt[1] = -g.qy * g.z - tab[19];
-  g.pt[2] = g.z - tab[20];
+  g.pt[0] = -g.qx * g.z - tab[tT(0)];
+  g.pt[1] = -g.qy * g.z - tab[tT(1)];
+  g.pt[2] = g.z - tab[tT(2)];
   g.f = f;
   g.fr2 = f * g.r2;
   g.fr4 = g.fr2 * g.r2;
@@ -75,9 +103,9 @@ __device__ __forceinline__ void jc_apply(const ObsGeom<T>& g, const T* __restric
 // u = Jp x for a point increment x (3).
 template <typename T>
 __device__ __forceinline__ void jp_apply(const ObsGeom<T>& g, const T* __restrict__ tab, const T x[3], T& u0, T& u1) {
-  const T y0 = tab[0] * x[0] + tab[1] * x[1] + tab[2] * x[2];
-  const T y1 = tab[3] * x[0] + tab[4] * x[1] + tab[5] * x[2];
-  const T y2 = tab[6] * x[0] + tab[7] * x[1] + tab[8] * x[2];
+  const T y0 = tab[tR(0, 0)] * x[0] + tab[tR(0, 1)] * x[1] + tab[tR(0, 2)] * x[2];
+  const T y1 = tab[tR(1, 0)] * x[0] + tab[tR(1, 1)] * x[1] + tab[tR(1, 2)] * x[2];
+  const T y2 = tab[tR(2, 0)] * x[0] + tab[tR(2, 1)] * x[1] + tab[tR(2, 2)] * x[2];
   u0 = g.sw * (g.dp0[0] * y0 + g.dp0[1] * y1 + g.dp0[2] * y2);
   u1 = g.sw * (g.dp1[0] * y0 + g.dp1[1] * y1 + g.dp1[2] * y2);
 }
@@ -88,9 +116,9 @@ __device__ __forceinline__ void jpt_accumulate(const ObsGeom<T>& g, const T* __r
   const T e0 = g.sw * (v0 * g.dp0[0] + v1 * g.dp1[0]);
   const T e1 = g.sw * (v0 * g.dp0[1] + v1 * g.dp1[1]);
   const T e2 = g.sw * (v0 * g.dp0[2] + v1 * g.dp1[2]);
-  w[0] += tab[0] * e0 + tab[3] * e1 + tab[6] * e2;
-  w[1] += tab[1] * e0 + tab[4] * e1 + tab[7] * e2;
-  w[2] += tab[2] * e0 + tab[5] * e1 + tab[8] * e2;
+  w[0] += tab[tR(0, 0)] * e0 + tab[tR(1, 0)] * e1 + tab[tR(2, 0)] * e2;
+  w[1] += tab[tR(0, 1)] * e0 + tab[tR(1, 1)] * e1 + tab[tR(2, 1)] * e2;
+  w[2] += tab[tR(0, 2)] * e0 + tab[tR(1, 2)] * e1 + tab[tR(2, 2)] * e2;
 }
 
 // acc += the camera-frame factors of Jc^T v: rotation slots hold (P - t) x e and are mapped by
@@ -116,7 +144,6 @@ __device__ __forceinline__ void jct_factors(const ObsGeom<T>& g, T v0, T v1, T a
 // Full weighted camera block (2x9, row-major jc[row*9+col]) for Gram assembly.
 template <typename T>
 __device__ __forceinline__ void jc_block(const ObsGeom<T>& g, const T* __restrict__ tab, T jc[18]) {
-  const T* rj = tab + 9;
   const T* dps[2] = {g.dp0, g.dp1};
   const T qs[2] = {g.qx, g.qy};
 #pragma unroll
@@ -126,7 +153,7 @@ __device__ __forceinline__ void jc_block(const ObsGeom<T>& g, const T* __restric
     const T c1 = g.pt[2] * d[0] - g.pt[0] * d[2];
     const T c2 = g.pt[0] * d[1] - g.pt[1] * d[0];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) jc[9 * i + j] = g.sw * (c0 * rj[j] + c1 * rj[3 + j] + c2 * rj[6 + j]);
+    for (int j = 0; j < 3; ++j) jc[9 * i + j] = g.sw * (c0 * tab[tRJ(0, j)] + c1 * tab[tRJ(1, j)] + c2 * tab[tRJ(2, j)]);
     jc[9 * i + 3] = g.sw * d[0];
     jc[9 * i + 4] = g.sw * d[1];
     jc[9 * i + 5] = g.sw * d[2];

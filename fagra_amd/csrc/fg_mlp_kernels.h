@@ -274,8 +274,8 @@ __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __
     // Jp (2x3), K2 = Jp M Jp^T (2x2)
     T jp[2][3];
     for (int a = 0; a < 3; ++a) {
-      jp[0][a] = geo.sw * (geo.dp0[0] * ct[a] + geo.dp0[1] * ct[3 + a] + geo.dp0[2] * ct[6 + a]);
-      jp[1][a] = geo.sw * (geo.dp1[0] * ct[a] + geo.dp1[1] * ct[3 + a] + geo.dp1[2] * ct[6 + a]);
+      jp[0][a] = geo.sw * (geo.dp0[0] * ct[tR(0, a)] + geo.dp0[1] * ct[tR(1, a)] + geo.dp0[2] * ct[tR(2, a)]);
+      jp[1][a] = geo.sw * (geo.dp1[0] * ct[tR(0, a)] + geo.dp1[1] * ct[tR(1, a)] + geo.dp1[2] * ct[tR(2, a)]);
     }
     T mj[2][3];
     for (int row = 0; row < 2; ++row)
