@@ -65,7 +65,10 @@ def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, coll
     hit = {}
     t0 = [0.0]
 
+    trace = []
+
     def on_accept(st):
+        trace.append((time.perf_counter() - t0[0], st.cost))
         if "t" not in hit and st.cost <= target:
             hit["t"] = time.perf_counter() - t0[0]
             hit["steps"] = st.accepted_steps
@@ -86,7 +89,7 @@ def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, coll
            "accepted": out.accepted_steps, "rejected": out.rejected_steps, "linearizations": out.linearizations,
            "cg_iterations": out.linear_iterations, "solves": out.linear_solves,
            "at_target": {k: v for k, v in hit.items() if k != "t"},
-           "matvecs": tim.matvecs, "matvec_kernel_s": tim.matvec_kernel_seconds,
+           "matvecs": tim.matvecs, "matvec_kernel_s": tim.matvec_kernel_seconds, "trace": trace,
            "timings": {f: getattr(tim, f) for f in ("preparation", "factorization", "linear_solve", "products",
                                                     "verification", "linearization", "cost")}}
     backend.close()

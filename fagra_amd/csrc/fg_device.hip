@@ -68,6 +68,11 @@ struct fg_ba {
   int *d_chunk_base = nullptr, *d_chunk_len = nullptr, *d_slot_cam = nullptr, *d_slot_row = nullptr;
   int *d_cam_off = nullptr, *d_k_slot = nullptr, *d_k_pslot = nullptr, *d_pt_len = nullptr;
   void* d_slot_pix = nullptr;  // Vec2<T>[nslots]
+  // f64 observations and camera tables for the linearization residual (the same buffers as the
+  // typed ones in f64 precision).
+  Vec2<double>* d_slot_pix64 = nullptr;
+  double* d_tab64[2] = {nullptr, nullptr};
+  bool fp32 = false;
   // State: two buffers (accepted / trial) of f64 values plus compute-precision points and
   // camera tables.
   double *d_cams[2] = {nullptr, nullptr}, *d_pts[2] = {nullptr, nullptr};
@@ -175,6 +180,9 @@ fg_status ensure_tables(fg_ba* b, int buf) {
   if (b->tab_valid[buf]) return FG_OK;
   hipLaunchKernelGGL(k_camera_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cams[buf], (T*)b->d_tab[buf]);
+  if (b->fp32)
+    hipLaunchKernelGGL(k_camera_tables<double>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                       b->d_cams[buf], b->d_tab64[buf]);
   FG_TRY(hipGetLastError());
   b->tab_valid[buf] = true;
   return FG_OK;
@@ -223,25 +231,27 @@ PointArgs point_args(const fg_ba* b) {
 }
 
 template <typename T, int MODE>
-void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, T lambda,
+void launch_points(fg_schur* s, const double* fw, const double* delta_p_in, const double* rows_in, double lambda,
                    double* delta_out, double* rows_out, double* parts, const CgState* st) {
   fg_ba* b = s->ba;
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur], fw,
-                     (const T*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda, (T*)s->tp,
-                     delta_out, rows_out, parts, st);
+                     (const double*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda,
+                     (double*)s->tp, delta_out, rows_out, parts, st);
 }
 
 template <typename T, int MODE>
-void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const T* fw, const T* vin,
-                    const double* delta_c, const T* rhs, T lambda, T* out, double* out_d,
-                    const CgState* st = nullptr) {
+void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const double* fw,
+                    const double* vin, const double* delta_c, const double* rhs, double lambda, double* out,
+                    double* out_d, const CgState* st = nullptr) {
   fg_ba* b = s->ba;
+  // the Schur, residual and verification modes read the forward table of their input
+  if (fw == nullptr) fw = (const double*)s->fw;
   hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
-                     (const T*)b->d_tab[b->cur], (const T*)s->tp, delta_p, rows_in, fw, (const T*)s->bs,
-                     (const T*)s->dc, (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st);
+                     (const T*)b->d_tab[b->cur], (const double*)s->tp, delta_p, rows_in, fw, (const T*)s->dc,
+                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st);
 }
 
 template <typename T>
@@ -252,13 +262,15 @@ fg_status linearize_impl(fg_schur* s) {
   FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL(k_linearize_points<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np,
-                     b->d_chunk_base, b->d_chunk_len, b->d_slot_cam, (const Vec2<T>*)b->d_slot_pix,
-                     (const T*)b->d_tab[buf], (const T*)b->d_pts_t[buf], (T)b->huber, (Vec4<T>*)s->rec,
-                     (Vec2<T>*)s->res, (T*)s->hpp, (T*)s->gp, s->d_flags);
+                     b->d_chunk_base, b->d_chunk_len, b->d_slot_cam, (const Vec2<double>*)b->d_slot_pix64,
+                     (const T*)b->d_tab[buf], (const double*)b->d_tab64[buf], (const double*)b->d_pts[buf],
+                     b->huber, (Vec4<T>*)s->rec,
+                     (Vec2<T>*)s->res, (double*)s->hpp, (T*)s->gp, s->d_flags);
   FG_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
                      b->d_k_slot, (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[buf],
-                     (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags);
+                     b->d_k_pslot, (const Vec2<double>*)b->d_slot_pix64, (const double*)b->d_tab64[buf],
+                     (const double*)b->d_pts[buf], b->huber, (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
@@ -293,8 +305,8 @@ fg_status prepare_damping_impl(fg_schur* s, double floor) {
   Phase ph(s, &s->tim.preparation);
   FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
   hipLaunchKernelGGL(k_scales<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (int)b->np, (const T*)s->hcc, (const T*)s->hpp, (T)floor, (T*)s->dc, (T*)s->dp, (T*)s->bs,
-                     (T*)s->cs, s->d_flags);
+                     (int)b->np, (const T*)s->hcc, (const double*)s->hpp, (T)floor, (T*)s->dc, (T*)s->dp,
+                     (T*)s->bs, (double*)s->cs, s->d_flags);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
@@ -304,12 +316,12 @@ fg_status prepare_damping_impl(fg_schur* s, double floor) {
 
 // Factor the blocks at lambda; returns the number of failed blocks through *fails.
 template <typename T>
-fg_status factor_impl(fg_schur* s, T lambda, int* fails) {
+fg_status factor_impl(fg_schur* s, double lambda, int* fails) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
   hipLaunchKernelGGL(k_factor<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (int)b->np, b->d_cam_off, b->d_pt_len, (const T*)s->bs, (const T*)s->cs, (const T*)s->dp,
-                     lambda, (T*)s->lc, (T*)s->mp, s->d_fail);
+                     (int)b->np, b->d_cam_off, b->d_pt_len, (const T*)s->bs, (const double*)s->cs, (const T*)s->dp,
+                     lambda, (double*)s->lc, (double*)s->mp, s->d_fail);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
@@ -320,13 +332,13 @@ fg_status factor_impl(fg_schur* s, T lambda, int* fails) {
 // S x for x in scaled retained coordinates (device), into out; also the residual against rhs
 // when resid != nullptr (sums of ||Sx - rhs||^2 and ||rhs||^2 land in s->h_sums[0..1]).
 template <typename T>
-fg_status schur_product(fg_schur* s, const T* x, T lambda, T* out, const T* resid_rhs) {
+fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out, const double* resid_rhs) {
   fg_ba* b = s->ba;
-  hipLaunchKernelGGL((k_forward_tables<T, T>), dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw);
-  launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
-  launch_cameras<T, cResid>(s, nullptr, nullptr, nullptr, x, nullptr, resid_rhs ? resid_rhs : (const T*)s->zero,
-                            lambda, out, nullptr);
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (double*)s->fw);
+  launch_points<T, kSchur>(s, (const double*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
+  launch_cameras<T, cResid>(s, nullptr, nullptr, (const double*)s->fw, x, nullptr,
+                            resid_rhs ? resid_rhs : (const double*)s->zero, lambda, out, nullptr);
   FG_TRY(hipGetLastError());
   s->tim.matvecs += 1;
   return FG_OK;
@@ -347,7 +359,7 @@ fg_status sum_parts(fg_schur* s, int rows) {
 // published done flag of the previous batch while the next batch runs; iterations after the
 // solve ended are no-ops.
 template <typename T>
-fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
+fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc;
   const int cblocks = grid_for(nc);
@@ -362,15 +374,15 @@ fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
   init.max_iters = max_it;
   init.fallback_at = fallback_at;
   FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
-  hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off, (const T*)s->lc,
-                     (const T*)s->gmat, lambda, (const T*)s->rhs, (T*)s->x, (T*)s->r, (T*)s->z, (T*)s->p,
-                     s->d_cgparts, (const CgState*)&s->d_sc->st[0]);
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     (const double*)s->lc, (const T*)s->gmat, lambda, (const double*)s->rhs, (double*)s->x,
+                     (double*)s->r, (double*)s->z, (double*)s->p, s->d_cgparts, (const CgState*)&s->d_sc->st[0]);
   FG_TRY(hipGetLastError());
   const bool timed = s->cfg.collect_timings != 0;
   auto direction = [&](int i) {
     hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                       (const T*)s->lc, lambda, (const T*)b->d_tab[b->cur], (const T*)s->dc, (const T*)s->zb,
-                       (const T*)s->z, (T*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
+                       (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
+                       (double*)s->p, (double*)s->fw, (const double*)s->d_cgparts, cblocks,
                        (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], max_it, fallback_at,
                        s->cfg.relative_tolerance, s->cfg.fallback_factor);
   };
@@ -389,16 +401,17 @@ fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
       direction(i);
       const bool ev = timed && s->events_used + 2 <= s->events.size();
       if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
-      launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, cur);
-      launch_cameras<T, cSchur>(s, nullptr, nullptr, nullptr, (const T*)s->p, nullptr, nullptr, lambda, (T*)s->q,
-                                nullptr, cur);
+      launch_points<T, kSchur>(s, (const double*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, cur);
+      launch_cameras<T, cSchur>(s, nullptr, nullptr, (const double*)s->fw, (const double*)s->p, nullptr, nullptr,
+                                lambda, (double*)s->q, nullptr, cur);
       if (ev) {
         (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
         s->events_used += 2;
       }
       hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                         (const T*)s->lc, (const T*)s->gmat, lambda, (const T*)s->p, (const T*)s->q, (T*)s->x,
-                         (T*)s->r, (T*)s->z, (T*)s->zb, (const double*)s->d_parts, s->d_cgparts, cur, fallback_at);
+                         (const double*)s->lc, (const T*)s->gmat, lambda, (const double*)s->p,
+                         (const double*)s->q, (double*)s->x, (double*)s->r, (double*)s->z, (double*)s->zb,
+                         (const double*)s->d_parts, s->d_cgparts, cur, fallback_at);
     }
     FG_TRY(hipGetLastError());
     launched += nb;
@@ -439,7 +452,7 @@ fg_status cg_impl(fg_schur* s, T lambda, bool learned) {
 template <typename T>
 fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   fg_ba* b = s->ba;
-  const T lambda = (T)lambda_d;
+  const double lambda = lambda_d;
   const int nc = (int)b->nc;
   {
     Phase ph(s, &s->tim.factorization);
@@ -451,7 +464,8 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
     }
     // rhs = Dc^-1 (-gc + sum Jc^T Jp M gp)
     launch_points<T, kRhs>(s, nullptr, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
-    launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (T*)s->rhs, nullptr);
+    launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (double*)s->rhs,
+                            nullptr);
     FG_TRY(hipGetLastError());
   }
   const bool learned = s->cfg.preconditioner == FG_PRECOND_LEARNED;
@@ -472,29 +486,29 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   }
   Phase ph(s, &s->tim.verification);
   // recover_step (schur.rs:312-344)
-  hipLaunchKernelGGL((k_forward_tables<T, T>), dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const T*)s->x, (const T*)s->dc, 1, (T*)s->fw);
-  launch_points<T, kRecover>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr, nullptr,
-                             nullptr);
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (double*)s->fw);
+  launch_points<T, kRecover>(s, (const double*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
+                             nullptr, nullptr);
   hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)(9 * b->nc),
-                     (const T*)s->x, (const T*)s->dc, s->d_delta);
+                     (const double*)s->x, (const T*)s->dc, s->d_delta);
   FG_TRY(hipGetLastError());
   // verify_step (schur.rs:346-397): reduced residual
-  FG_CHECK(schur_product<T>(s, (const T*)s->x, lambda, (T*)s->q, (const T*)s->rhs));
+  FG_CHECK(schur_product<T>(s, (const double*)s->x, lambda, (double*)s->q, (const double*)s->rhs));
   FG_CHECK(sum_parts<T>(s, nc));
   const double resid2 = s->h_sums[0], rhs2 = s->h_sums[1];
   s->diag.reduced_relative_residual = rhs2 == 0.0 ? std::sqrt(resid2) : std::sqrt(resid2 / rhs2);
   // full-model verification in original coordinates (linear_model.rs:254-282)
-  hipLaunchKernelGGL((k_forward_tables<T, double>), dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (double*)s->fw);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
-  launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
+  launch_points<T, kVerify>(s, (const double*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
                             s->d_parts, nullptr);
   FG_CHECK(sum_parts<T>(s, pblocks));
   double pt[kParts];
   std::memcpy(pt, s->h_sums, sizeof(pt));
-  launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, nullptr, s->d_delta, nullptr,
-                             lambda, nullptr, nullptr);
+  launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const double*)s->fw, nullptr, s->d_delta,
+                             nullptr, lambda, nullptr, nullptr);
   FG_CHECK(sum_parts<T>(s, nc));
   const double* cm = s->h_sums;
   // pt: n2, n2s, g.d, g2, g2s, ||J delta||^2 ; cm: n2, n2s, g.d, g2, g2s
@@ -528,21 +542,22 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(hipMemset(s->res, 0, ns * sizeof(Vec2<T>)));
   FG_TRY(dalloc((Vec4<T>**)&s->rec_cm, no));
   FG_TRY(dalloc((Vec2<T>**)&s->res_cm, no));
-  FG_TRY(dalloc((T**)&s->hpp, 6 * np));
+  FG_TRY(dalloc((double**)&s->hpp, 6 * np));
   FG_TRY(dalloc((T**)&s->gp, 3 * np));
   FG_TRY(dalloc((T**)&s->hcc, 45 * nc));
   FG_TRY(dalloc((T**)&s->gc, 9 * nc));
   FG_TRY(dalloc((T**)&s->dc, 9 * nc));
   FG_TRY(dalloc((T**)&s->dp, 3 * np));
   FG_TRY(dalloc((T**)&s->bs, 45 * nc));
-  FG_TRY(dalloc((T**)&s->cs, 6 * np));
-  FG_TRY(dalloc((T**)&s->lc, 45 * nc));
-  FG_TRY(dalloc((T**)&s->mp, 6 * np));
-  FG_TRY(dalloc((T**)&s->tp, 4 * np));
-  FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
+  FG_TRY(dalloc((double**)&s->cs, 6 * np));
+  // f64 in both precisions: factors, per-point solves and every camera-sized vector
+  FG_TRY(dalloc((double**)&s->lc, 45 * nc));
+  FG_TRY(dalloc((double**)&s->mp, 6 * np));
+  FG_TRY(dalloc((double**)&s->tp, 4 * np));
+  FG_TRY(dalloc((double**)&s->fw, kFwd * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
-    FG_TRY(dalloc((T**)v, 9 * nc));
-    FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(T)));
+    FG_TRY(dalloc((double**)v, 9 * nc));
+    FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
   }
   FG_TRY(dalloc((T**)&s->gmat, 45 * nc));
   FG_TRY(dalloc(&s->mx, pad64(nc) * kInPad));
@@ -556,7 +571,7 @@ fg_status alloc_model(fg_schur* s) {
   const size_t rows = std::max({nc, pblocks, (size_t)1024});
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
   FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc))));
-  FG_TRY(dalloc((T**)&s->zb, 9 * nc));
+  FG_TRY(dalloc((double**)&s->zb, 9 * nc));
   // The memsets above run on the null stream; complete them before stream work starts.
   FG_TRY(hipDeviceSynchronize());
   return FG_OK;
@@ -628,14 +643,12 @@ fg_status reduced_apply_impl(fg_schur* s, double lambda, const double* x, double
   fg_ba* b = s->ba;
   const size_t n = 9 * b->nc;
   int fails = 0;
-  FG_CHECK(factor_impl<T>(s, (T)lambda, &fails));
+  FG_CHECK(factor_impl<T>(s, lambda, &fails));
   if (fails) return FG_LINEAR_SOLVE_FAILED;
-  std::vector<T> hx(x, x + n), hy(n);
-  FG_TRY(hipMemcpyAsync(s->p, hx.data(), n * sizeof(T), hipMemcpyHostToDevice, b->stream));
-  FG_CHECK(schur_product<T>(s, (const T*)s->p, (T)lambda, (T*)s->q, nullptr));
-  FG_TRY(hipMemcpyAsync(hy.data(), s->q, n * sizeof(T), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(s->p, x, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  FG_CHECK(schur_product<T>(s, (const double*)s->p, lambda, (double*)s->q, nullptr));
+  FG_TRY(hipMemcpyAsync(y, s->q, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
-  for (size_t i = 0; i < n; ++i) y[i] = (double)hy[i];
   return FG_OK;
 }
 
@@ -643,9 +656,9 @@ template <typename T>
 fg_status jacobian_apply_impl(fg_schur* s, const double* x, double* y) {
   fg_ba* b = s->ba;
   FG_CHECK(upload_vector(b, x, s->d_delta));
-  hipLaunchKernelGGL((k_forward_tables<T, double>), dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
-  launch_points<T, kJv>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, (T)0, nullptr, s->d_rows, nullptr,
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (double*)s->fw);
+  launch_points<T, kJv>(s, (const double*)s->fw, s->d_delta + 9 * b->nc, nullptr, 0.0, nullptr, s->d_rows, nullptr,
                         nullptr);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(y, s->d_rows, 2 * b->no * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -713,13 +726,13 @@ fg_status capture_impl(fg_schur* s, double lambda_d, float* features, float* tar
   const int nc = (int)b->nc;
   const T lambda = (T)lambda_d;
   int fails = 0;
-  FG_CHECK(factor_impl<T>(s, lambda, &fails));
+  FG_CHECK(factor_impl<T>(s, lambda_d, &fails));
   if (fails) return FG_LINEAR_SOLVE_FAILED;
   hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)s->hcc, lambda_d, s->mx, s->mxf);
   hipLaunchKernelGGL(k_schur_diagonal<T>, dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                     b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const T*)b->d_tab[b->cur], (const T*)s->mp,
-                     (const T*)s->bs, (const T*)s->dc, (const T*)s->lc, lambda, s->mtarget);
+                     b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const T*)b->d_tab[b->cur], (const double*)s->mp,
+                     (const T*)s->bs, (const T*)s->dc, (const double*)s->lc, lambda, s->mtarget);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(features, s->mxf, (size_t)nc * kIn * 4, hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(targets, s->mtarget, (size_t)nc * 45 * 4, hipMemcpyDeviceToHost, b->stream));
@@ -846,9 +859,13 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
     return fail(FG_DEVICE_ERROR);
   const size_t ts = precision == FG_FP32 ? sizeof(float) : sizeof(double);
   if (hipMalloc(&b->d_slot_pix, std::max<size_t>(1, 2 * b->nslots) * ts) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  b->fp32 = precision == FG_FP32;
   if (precision == FG_FP32) {
     std::vector<float> f(slot_pix.begin(), slot_pix.end());
-    if (hipMemcpy(b->d_slot_pix, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(b->d_slot_pix, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        dalloc((double**)&b->d_slot_pix64, std::max<size_t>(1, 2 * b->nslots)) != hipSuccess ||
+        hipMemcpy(b->d_slot_pix64, slot_pix.data(), slot_pix.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess)
       return fail(FG_DEVICE_ERROR);
   } else if (hipMemcpy(b->d_slot_pix, slot_pix.data(), slot_pix.size() * sizeof(double), hipMemcpyHostToDevice) !=
              hipSuccess) {
@@ -859,7 +876,13 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
         hipMalloc(&b->d_pts_t[i], 3 * np * ts) != hipSuccess ||
         hipMalloc(&b->d_tab[i], kCamTab * nc * ts) != hipSuccess)
       return fail(FG_DEVICE_ERROR);
+    if (b->fp32) {
+      if (dalloc(&b->d_tab64[i], kCamTab * nc) != hipSuccess) return fail(FG_DEVICE_ERROR);
+    } else {
+      b->d_tab64[i] = (double*)b->d_tab[i];
+    }
   }
+  if (!b->fp32) b->d_slot_pix64 = (Vec2<double>*)b->d_slot_pix;
   const size_t rows = std::max<size_t>(1024, (size_t)grid_for(b->nchunks, kChunksPerBlock));
   if (dalloc(&b->d_parts, rows) != hipSuccess || dalloc(&b->d_flags, 4) != hipSuccess ||
       dalloc(&b->d_out, 4) != hipSuccess ||
@@ -887,8 +910,10 @@ void fg_ba_destroy(fg_ba* b) {
   dfree(b->d_k_slot);
   dfree(b->d_k_pslot);
   dfree(b->d_pt_len);
+  if (b->fp32 && b->d_slot_pix64) (void)hipFree(b->d_slot_pix64);
   if (b->d_slot_pix) (void)hipFree(b->d_slot_pix);
   for (int i = 0; i < 2; ++i) {
+    if (b->fp32 && b->d_tab64[i]) (void)hipFree(b->d_tab64[i]);
     dfree(b->d_cams[i]);
     dfree(b->d_pts[i]);
     if (b->d_pts_t[i]) (void)hipFree(b->d_pts_t[i]);

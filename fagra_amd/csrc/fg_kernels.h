@@ -32,6 +32,19 @@ struct DevScalars {
   int max_iters, fallback_at, pad0, pad1;
 };
 
+// Camera table rows and records of storage type T widened to f64 registers.
+template <typename T>
+__device__ __forceinline__ void load_cam16_d(const T* __restrict__ tab, int c, double out[16]) {
+  T t[16];
+  load_cam16(tab, c, t);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) out[i] = double(t[i]);
+}
+template <typename T>
+__device__ __forceinline__ Vec4<double> to_d(const Vec4<T>& r) {
+  return Vec4<double>{double(r.x), double(r.y), double(r.z), double(r.w)};
+}
+
 // ---------------------------------------------------------------------------------------
 // Small reduction helpers
 // ---------------------------------------------------------------------------------------
@@ -150,7 +163,8 @@ __device__ __forceinline__ bool project(const T* __restrict__ tab, T x0, T x1, T
 
 // ---------------------------------------------------------------------------------------
 // Cost: sum over observations of 0.5||r||^2 or Huber, accumulated in f64 (Frame::cost,
-// model.rs:201-213). Deterministic: per-lane sequential, fixed block tree, last block sums.
+// model.rs:201-213). Deterministic: per-lane sequential, fixed block tree; per-workgroup
+// partials are summed by k_reduce_parts.
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int* __restrict__ chunk_base,
@@ -210,46 +224,58 @@ static __global__ void k_reduce_parts(const double* __restrict__ parts, int n, i
 // Linearization, point-major pass: projection, residual, IRLS weight, the compact record,
 // weighted residual, the point Gram block Hpp = Jp^T Jp and gradient gp = Jp^T r.
 // (Frame::linearize, model.rs:214-236; LinearizedModel::accumulate, linear_model.rs:180-210)
+// The residual is the small difference of two pixel positions of a few hundred pixels, so the
+// projection and the subtraction run in f64 from the f64 state, tables and observations in
+// every precision. The record is stored in T; Jp is re-formed from the stored record and
+// table in f64, the same way the Schur products form it, and Hpp and the gradient accumulate
+// in f64, so the point blocks are the exact Gram blocks of the stored Jacobian.
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_linearize_points(
     int nchunks, int np, const int* __restrict__ chunk_base, const int* __restrict__ chunk_len,
-    const int* __restrict__ slot_cam, const Vec2<T>* __restrict__ slot_pix, const T* __restrict__ tab,
-    const T* __restrict__ pts, T huber, Vec4<T>* __restrict__ rec, Vec2<T>* __restrict__ res,
-    T* __restrict__ hpp, T* __restrict__ gp, int* flags) {
+    const int* __restrict__ slot_cam, const Vec2<double>* __restrict__ slot_pix, const T* __restrict__ tab,
+    const double* __restrict__ tab64, const double* __restrict__ pts, double huber, Vec4<T>* __restrict__ rec,
+    Vec2<T>* __restrict__ res, double* __restrict__ hpp, T* __restrict__ gp, int* flags) {
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   if (chunk >= nchunks) return;
   const int i = chunk * kWave + lane;
   if (i >= np) return;
-  const T x0 = pts[3 * i], x1 = pts[3 * i + 1], x2 = pts[3 * i + 2];
+  const double x0 = pts[3 * i], x1 = pts[3 * i + 1], x2 = pts[3 * i + 2];
   const int base = chunk_base[chunk], len = chunk_len[chunk];
-  T h[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  double h[6] = {0, 0, 0, 0, 0, 0};
+  double g[3] = {0, 0, 0};
   bool bad = false;
   for (int j = 0; j < len; ++j) {
     const int s = base + j * kWave + lane;
     const int c = slot_cam[s];
     if (c < 0) continue;
-    T ct[16];
-    load_cam16(tab, c, ct);
-    T px, py, qx, qy, z;
-    bool ok = project(ct, x0, x1, x2, px, py, qx, qy, z);
-    const Vec2<T> ob = slot_pix[s];
-    const T rx = px - ob.x, ry = py - ob.y;
-    T cost, sw;
+    double cd[16];
+    load_cam16(tab64, c, cd);
+    double px, py, qx, qy, z;
+    bool ok = project(cd, x0, x1, x2, px, py, qx, qy, z);
+    const Vec2<double> ob = slot_pix[s];
+    const double rx = px - ob.x, ry = py - ob.y;
+    double cost, sw;
     huber_weight(rx, ry, huber, cost, sw);
-    Vec4<T> r4 = {qx, qy, z, sw};
+    double ct[16];
+    load_cam16_d(tab, c, ct);
+    const Vec4<T> r4 = {T(qx), T(qy), T(z), T(sw)};
     rec[s] = r4;
-    const T wrx = sw * rx, wry = sw * ry;
-    res[s] = Vec2<T>{wrx, wry};
-    ObsGeom<T> geo;
-    obs_geom(r4, ct, geo);
-    // Jp columns: Jp[:, k] = sw * dp * R[:, k]
-    T jp0[3], jp1[3];
+    const double wrx = sw * rx, wry = sw * ry;
+    res[s] = Vec2<T>{T(wrx), T(wry)};
+    ObsGeom<double> geo, g64;
+    obs_geom(to_d(r4), ct, geo);
+    obs_geom(Vec4<double>{qx, qy, z, sw}, cd, g64);
+    // Jp columns: Jp[:, k] = sw * dp * R[:, k], from the stored record (Hpp) and from the f64
+    // geometry (gradient)
+    double jp0[3], jp1[3], e0[3], e1[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      jp0[k] = sw * (geo.dp0[0] * ct[tR(0, k)] + geo.dp0[1] * ct[tR(1, k)] + geo.dp0[2] * ct[tR(2, k)]);
-      jp1[k] = sw * (geo.dp1[0] * ct[tR(0, k)] + geo.dp1[1] * ct[tR(1, k)] + geo.dp1[2] * ct[tR(2, k)]);
+      jp0[k] = geo.sw * (geo.dp0[0] * ct[tR(0, k)] + geo.dp0[1] * ct[tR(1, k)] + geo.dp0[2] * ct[tR(2, k)]);
+      jp1[k] = geo.sw * (geo.dp1[0] * ct[tR(0, k)] + geo.dp1[1] * ct[tR(1, k)] + geo.dp1[2] * ct[tR(2, k)]);
+      e0[k] = g64.sw * (g64.dp0[0] * cd[tR(0, k)] + g64.dp0[1] * cd[tR(1, k)] + g64.dp0[2] * cd[tR(2, k)]);
+      e1[k] = g64.sw * (g64.dp1[0] * cd[tR(0, k)] + g64.dp1[1] * cd[tR(1, k)] + g64.dp1[2] * cd[tR(2, k)]);
       ok = ok && isfinite(jp0[k]) && isfinite(jp1[k]);
     }
     ok = ok && isfinite(wrx) && isfinite(wry);
@@ -258,14 +284,14 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
       for (int b = a; b < 3; ++b) h[sym3(b, a)] += jp0[a] * jp0[b] + jp1[a] * jp1[b];
-      g[a] += jp0[a] * wrx + jp1[a] * wry;
+      g[a] += e0[a] * wrx + e1[a] * wry;
     }
   }
   if (bad) atomicOr(flags, 1);
 #pragma unroll
   for (int k = 0; k < 6; ++k) hpp[6 * i + k] = h[k];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) gp[3 * i + k] = g[k];
+  for (int k = 0; k < 3; ++k) gp[3 * i + k] = T(g[k]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -275,16 +301,28 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
     const int* __restrict__ cam_off, const int* __restrict__ k_slot, const Vec4<T>* __restrict__ rec,
-    const Vec2<T>* __restrict__ res, const T* __restrict__ tab, Vec4<T>* __restrict__ rec_cm,
-    Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc, T* __restrict__ gc, int* flags) {
+    const Vec2<T>* __restrict__ res, const T* __restrict__ tab, const int* __restrict__ k_pslot,
+    const Vec2<double>* __restrict__ slot_pix, const double* __restrict__ tab64, const double* __restrict__ pts,
+    double huber, Vec4<T>* __restrict__ rec_cm, Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc,
+    T* __restrict__ gc, int* flags) {
+  // A T = f64 record is the exact geometry; a narrower one gives Hcc, while the gradient is
+  // re-formed from the f64 state, whose Jacobian error would otherwise move the minimum.
+  constexpr bool kExact = sizeof(T) == sizeof(double);
   __shared__ T ct[kCamTab];
-  __shared__ T red[54][kBlock / kWave];
+  __shared__ double cd[kCamTab];
+  __shared__ double red[54][kBlock / kWave];
   const int c = blockIdx.x;
-  if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
+  if (threadIdx.x < kCamTab) {
+    ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
+    if (!kExact) cd[threadIdx.x] = tab64[kCamTab * c + threadIdx.x];
+  }
   __syncthreads();
-  T acc[54];
+  T acc[45];
+  double gacc[9];
 #pragma unroll
-  for (int i = 0; i < 54; ++i) acc[i] = T(0);
+  for (int i = 0; i < 45; ++i) acc[i] = T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gacc[i] = 0.0;
   bool bad = false;
   const int k0 = cam_off[c], k1 = cam_off[c + 1];
   for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
@@ -303,22 +341,43 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
     for (int a = 0; a < 9; ++a) {
 #pragma unroll
       for (int b = a; b < 9; ++b) acc[sym9(b, a)] += jc[a] * jc[b] + jc[9 + a] * jc[9 + b];
-      acc[45 + a] += jc[a] * rw.x + jc[9 + a] * rw.y;
+    }
+    if (kExact) {
+#pragma unroll
+      for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
+    } else {
+      const int ps = k_pslot[k];
+      double px, py, qx, qy, z, cost, sw;
+      project(cd, pts[3 * ps], pts[3 * ps + 1], pts[3 * ps + 2], px, py, qx, qy, z);
+      const Vec2<double> ob = slot_pix[s];
+      const double rx = px - ob.x, ry = py - ob.y;
+      huber_weight(rx, ry, huber, cost, sw);
+      ObsGeom<double> g64;
+      obs_geom(Vec4<double>{qx, qy, z, sw}, cd, g64);
+      double j64[18];
+      jc_block(g64, cd, j64);
+#pragma unroll
+      for (int a = 0; a < 9; ++a) gacc[a] += j64[a] * (sw * rx) + j64[9 + a] * (sw * ry);
     }
   }
   if (bad) atomicOr(flags, 1);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 #pragma unroll
-  for (int i = 0; i < 54; ++i) {
+  for (int i = 0; i < 45; ++i) {
     const T v = wave_sum(acc[i]);
-    if (lane == 0) red[i][wave] = v;
+    if (lane == 0) red[i][wave] = double(v);
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const double v = wave_sum(gacc[i]);
+    if (lane == 0) red[45 + i][wave] = v;
   }
   __syncthreads();
   if (threadIdx.x < 54) {
-    T s = T(0);
+    double s = 0.0;
     for (int w = 0; w < kBlock / kWave; ++w) s += red[threadIdx.x][w];
-    if (threadIdx.x < 45) hcc[45 * c + threadIdx.x] = s;
-    else gc[9 * c + threadIdx.x - 45] = s;
+    if (threadIdx.x < 45) hcc[45 * c + threadIdx.x] = T(s);
+    else gc[9 * c + threadIdx.x - 45] = T(s);
   }
 }
 
@@ -353,8 +412,8 @@ __global__ void __launch_bounds__(kBlock) k_norm_inf(int n, const T* __restrict_
 // (block_factors.rs:47-111): D = max(column norm, floor); B = D^-1 Hcc D^-1; C = D^-1 Hpp D^-1.
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const T* __restrict__ hpp, T floor,
-                         T* __restrict__ dc, T* __restrict__ dp, T* __restrict__ bs, T* __restrict__ cs,
+__global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const double* __restrict__ hpp, T floor,
+                         T* __restrict__ dc, T* __restrict__ dp, T* __restrict__ bs, double* __restrict__ cs,
                          int* flags) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < nc) {
@@ -375,15 +434,16 @@ __global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const T* __r
     if (bad) atomicOr(flags, 4);
   } else if (idx < nc + np) {
     const int p = idx - nc;
-    const T* h = hpp + 6 * p;
-    T d[3];
+    const double* h = hpp + 6 * p;
+    double d[3];
     bool bad = false;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const T n = sqrt(h[sym3(j, j)]);
+      const double n = sqrt(h[sym3(j, j)]);
       bad |= !isfinite(n);
-      d[j] = max(n, floor);
-      dp[3 * p + j] = d[j];
+      // the stored (T) scale is the one every later use of Dp reads
+      dp[3 * p + j] = T(max(n, double(floor)));
+      d[j] = double(dp[3 * p + j]);
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j)
@@ -394,40 +454,52 @@ __global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const T* __r
 }
 
 // ---------------------------------------------------------------------------------------
+// Precision split of the solve. T is the storage type of the bulk per-observation data
+// (records, weighted residuals, camera tables) and of the Gram blocks; every product of the
+// reduced system is evaluated in f64 registers from those values, and the camera-sized
+// vectors (PCG state, forward tables) and the per-point solves are f64. The reduced matrix is
+// then the exact Schur complement of the Jacobian the stored records define, which keeps it
+// positive semidefinite down to the smallest damping: with f32 arithmetic, the rounding of
+// the products along the similarity gauge (where S has eigenvalue lambda) exceeds lambda once
+// lambda falls below ~1e-7 and PCG breaks down on its first direction.
+// ---------------------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------------------
 // BlockFactors::factor (block_factors.rs:113-138): Cholesky of B + lambda I per camera and
-// of C + lambda I per point. Point blocks are stored as the unscaled inverse
+// of C + lambda I per point, in f64. Point blocks are stored as the unscaled inverse
 // M = (Hpp + lambda D^2)^-1 = D^-1 (C + lambda I)^-1 D^-1 (packed 6) used by elimination.
 // Unobserved blocks are skipped (no Gram block exists for them in the reference).
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const int* __restrict__ pt_len,
-                         const T* __restrict__ bs, const T* __restrict__ cs, const T* __restrict__ dp, T lambda,
-                         T* __restrict__ lc, T* __restrict__ mp, int* fail_count) {
+                         const T* __restrict__ bs, const double* __restrict__ cs, const T* __restrict__ dp,
+                         double lambda,
+                         double* __restrict__ lc, double* __restrict__ mp, int* fail_count) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < nc) {
     if (cam_off[idx + 1] == cam_off[idx]) return;
-    T a[45];
+    double a[45];
     bool ok = true;
 #pragma unroll
     for (int i = 0; i < 45; ++i) {
-      a[i] = bs[45 * idx + i];
+      a[i] = double(bs[45 * idx + i]);
       ok = ok && isfinite(a[i]);
     }
 #pragma unroll
     for (int j = 0; j < 9; ++j) a[sym9(j, j)] += lambda;
-    ok = ok && chol_packed<T, 9>(a);
+    ok = ok && chol_packed<double, 9>(a);
 #pragma unroll
     for (int i = 0; i < 45; ++i) lc[45 * idx + i] = a[i];
     if (!ok) atomicAdd(fail_count, 1);
   } else if (idx < nc + np) {
     const int p = idx - nc;
-    T* m = mp + 6 * p;
+    double* m = mp + 6 * p;
     if (pt_len[p] == 0) {
 #pragma unroll
-      for (int i = 0; i < 6; ++i) m[i] = T(0);
+      for (int i = 0; i < 6; ++i) m[i] = 0.0;
       return;
     }
-    T a[6];
+    double a[6];
     bool ok = true;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -435,20 +507,19 @@ __global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const 
       ok = ok && isfinite(a[i]);
     }
     a[0] += lambda; a[3] += lambda; a[5] += lambda;
-    ok = ok && chol_packed<T, 3>(a);
+    ok = ok && chol_packed<double, 3>(a);
     // inverse of C + lambda I from its factor: columns of L^-T L^-1
-    T inv[6];
+    double inv[6];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      T e[3] = {T(0), T(0), T(0)};
-      e[j] = T(1);
-      solve_lower_packed<T, 3>(a, e);
-      solve_upper_packed<T, 3>(a, e);
+      double e[3] = {0.0, 0.0, 0.0};
+      e[j] = 1.0;
+      solve_lower_packed<double, 3>(a, e);
+      solve_upper_packed<double, 3>(a, e);
 #pragma unroll
       for (int i = j; i < 3; ++i) inv[sym3(i, j)] = e[i];
     }
-    const T d0 = dp[3 * p], d1 = dp[3 * p + 1], d2 = dp[3 * p + 2];
-    const T d[3] = {d0, d1, d2};
+    const double d[3] = {double(dp[3 * p]), double(dp[3 * p + 1]), double(dp[3 * p + 2])};
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -461,30 +532,33 @@ __global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const 
 // Forward camera tables from a retained-coordinate vector. scaled != 0: z = x / Dc (the
 // reduced system's scaled coordinates); otherwise z = x. fw = (RJ z_rot, z_t, z_intr).
 // ---------------------------------------------------------------------------------------
-template <typename T, typename S>
-__device__ __forceinline__ void forward_table(const T* __restrict__ ct, const S* x, const T* __restrict__ d,
-                                              bool scaled, T* __restrict__ fw, T z[9]) {
+template <typename T>
+__device__ __forceinline__ void forward_table(const T* __restrict__ ct, const double* x, const T* __restrict__ d,
+                                              bool scaled, double* __restrict__ fw, double z[9]) {
 #pragma unroll
-  for (int j = 0; j < 9; ++j) z[j] = scaled ? T(x[j]) / d[j] : T(x[j]);
+  for (int j = 0; j < 9; ++j) z[j] = scaled ? x[j] / double(d[j]) : x[j];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) fw[i] = ct[tRJ(i, 0)] * z[0] + ct[tRJ(i, 1)] * z[1] + ct[tRJ(i, 2)] * z[2];
+  for (int i = 0; i < 3; ++i)
+    fw[i] = double(ct[tRJ(i, 0)]) * z[0] + double(ct[tRJ(i, 1)]) * z[1] + double(ct[tRJ(i, 2)]) * z[2];
 #pragma unroll
   for (int j = 3; j < 9; ++j) fw[j] = z[j];
-  fw[9] = fw[10] = fw[11] = T(0);
+  fw[9] = fw[10] = fw[11] = 0.0;
 }
 
-template <typename T, typename S>
-__global__ void k_forward_tables(int nc, const T* __restrict__ tab, const S* __restrict__ x, const T* __restrict__ dc,
-                                 int scaled, T* __restrict__ fw) {
+template <typename T>
+__global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double* __restrict__ x,
+                                 const T* __restrict__ dc, int scaled, double* __restrict__ fw) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
-  T z[9];
+  double z[9];
   forward_table(tab + kCamTab * c, x + 9 * c, dc + 9 * c, scaled != 0, fw + kFwd * c, z);
 }
 
 // ---------------------------------------------------------------------------------------
 // Point-major half of the Schur products (one lane per point, SELL chunks).
-//   kSchur:   t_p = M_p sum_k Jp^T Jc z        (eliminated-block solve of E^T z)
+//   kSchur:   t_p = (Hpp + lambda Dp^2)^-1 sum_k Jp^T Jc z, with Hpp re-formed from the same
+//             records in f64 and factored in registers, so the elimination is exact for the
+//             stored Jacobian (the eliminated-block solve of E^T z)
 //   kRecover: delta_p = -M_p (gp + sum_k Jp^T Jc z)     (Schur::recover_step, schur.rs:312-344)
 //   kRhs:     t_p = M_p gp                      (factor_and_reduce_rhs, schur.rs:204-247)
 //   kVerify:  normal residual Jp^T (J delta + r) + lambda Dp^2 delta_p and ||J delta||^2
@@ -503,50 +577,68 @@ struct PointArgs {
 
 // One observation of a point-major pass: record, camera table and forward table in registers.
 template <typename T, int MODE>
-__device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& r4,
+__device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& rec,
                                           const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
-                                          const T* __restrict__ fw, const T dpx[3],
+                                          const double* __restrict__ fw, const double dpx[3],
                                           const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                                          T w[3], double& uu) {
-  T ct[16];
-  load_cam16(tab, c, ct);
-  ObsGeom<T> geo;
-  obs_geom(r4, ct, geo);
-  if (MODE == kSchur || MODE == kRecover) {
-    T f[kFwd];
+                                          double w[3], double h[6], double& uu) {
+  double ct[16];
+  load_cam16_d(tab, c, ct);
+  ObsGeom<double> geo;
+  obs_geom(to_d(rec), ct, geo);
+  if (MODE == kSchur) {
+    double f[kFwd];
     load_fwd(fw, c, f);
-    T u0, u1;
+    double u0, u1;
+    jc_apply(geo, f, u0, u1);
+    double j0[3], j1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      j0[k] = geo.sw * (geo.dp0[0] * ct[tR(0, k)] + geo.dp0[1] * ct[tR(1, k)] + geo.dp0[2] * ct[tR(2, k)]);
+      j1[k] = geo.sw * (geo.dp1[0] * ct[tR(0, k)] + geo.dp1[1] * ct[tR(1, k)] + geo.dp1[2] * ct[tR(2, k)]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = a; b < 3; ++b) h[sym3(b, a)] += j0[a] * j0[b] + j1[a] * j1[b];
+      w[a] += j0[a] * u0 + j1[a] * u1;
+    }
+  } else if (MODE == kRecover) {
+    double f[kFwd];
+    load_fwd(fw, c, f);
+    double u0, u1;
     jc_apply(geo, f, u0, u1);
     jpt_accumulate(geo, ct, u0, u1, w);
   } else if (MODE == kVerify || MODE == kJv) {
-    T f[kFwd];
+    double f[kFwd];
     load_fwd(fw, c, f);
-    T u0, u1, v0, v1;
+    double u0, u1, v0, v1;
     jc_apply(geo, f, u0, u1);
     jp_apply(geo, ct, dpx, v0, v1);
     u0 += v0;
     u1 += v1;
     if (MODE == kJv) {
       const int row = pa.slot_row[s];
-      rows_out[2 * row] = double(u0);
-      rows_out[2 * row + 1] = double(u1);
+      rows_out[2 * row] = u0;
+      rows_out[2 * row + 1] = u1;
     } else {
-      uu += double(u0) * double(u0) + double(u1) * double(u1);
+      uu += u0 * u0 + u1 * u1;
       const Vec2<T> r = res[s];
-      jpt_accumulate(geo, ct, u0 + r.x, u1 + r.y, w);
+      jpt_accumulate(geo, ct, u0 + double(r.x), u1 + double(r.y), w);
     }
   } else if (MODE == kJtu) {
     const int row = pa.slot_row[s];
-    jpt_accumulate(geo, ct, T(rows_in[2 * row]), T(rows_in[2 * row + 1]), w);
+    jpt_accumulate(geo, ct, rows_in[2 * row], rows_in[2 * row + 1], w);
   }
 }
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_points(
     PointArgs pa, const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
-    const T* __restrict__ fw, const T* __restrict__ mp, const T* __restrict__ gp, const T* __restrict__ dpv,
-    const double* __restrict__ delta_p_in, const double* __restrict__ rows_in, T lambda, T* __restrict__ t_out,
-    double* __restrict__ delta_out, double* __restrict__ rows_out, double* parts, const CgState* st) {
+    const double* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
+    const T* __restrict__ dpv, const double* __restrict__ delta_p_in, const double* __restrict__ rows_in,
+    double lambda, double* __restrict__ t_out, double* __restrict__ delta_out, double* __restrict__ rows_out,
+    double* parts, const CgState* st) {
   __shared__ double smem[kBlock / kWave];
   if (MODE == kSchur && st != nullptr && st->done) return;
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
@@ -555,12 +647,13 @@ __global__ void __launch_bounds__(kBlock) k_points(
   if (chunk < pa.nchunks) {
     const int i = chunk * kWave + lane;
     if (i < pa.np) {
-      T w[3] = {T(0), T(0), T(0)};
-      T dpx[3] = {T(0), T(0), T(0)};
+      double w[3] = {0.0, 0.0, 0.0};
+      double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      double dpx[3] = {0.0, 0.0, 0.0};
       if (MODE == kVerify || MODE == kJv) {
-        dpx[0] = T(delta_p_in[3 * i]);
-        dpx[1] = T(delta_p_in[3 * i + 1]);
-        dpx[2] = T(delta_p_in[3 * i + 2]);
+        dpx[0] = delta_p_in[3 * i];
+        dpx[1] = delta_p_in[3 * i + 1];
+        dpx[2] = delta_p_in[3 * i + 2];
       }
       if (MODE != kRhs) {
         const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
@@ -570,49 +663,61 @@ __global__ void __launch_bounds__(kBlock) k_points(
           const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
           const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
           const Vec4<T> r0 = rec[s0], r1 = rec[s1];
-          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, uu);
-          if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, uu);
+          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu);
+          if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu);
         }
         if (j < len) {
           const int s0 = base + j * kWave + lane;
           const int c0 = pa.slot_cam[s0];
-          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, uu);
+          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu);
         }
       }
-      if (MODE == kSchur || MODE == kRhs || MODE == kRecover) {
-        const T* m = mp + 6 * i;
-        T v[3];
-        if (MODE == kRhs) {
-          v[0] = gp[3 * i]; v[1] = gp[3 * i + 1]; v[2] = gp[3 * i + 2];
-        } else if (MODE == kRecover) {
-          v[0] = gp[3 * i] + w[0]; v[1] = gp[3 * i + 1] + w[1]; v[2] = gp[3 * i + 2] + w[2];
-        } else {
-          v[0] = w[0]; v[1] = w[1]; v[2] = w[2];
+      if (MODE == kSchur) {
+        double t[3] = {w[0], w[1], w[2]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const double d = double(dpv[3 * i + a]);
+          h[sym3(a, a)] += lambda * d * d;
         }
-        T t[3];
+        if (chol_packed<double, 3>(h)) {
+          solve_lower_packed<double, 3>(h, t);
+          solve_upper_packed<double, 3>(h, t);
+        } else {
+          t[0] = t[1] = t[2] = 0.0;  // unobserved point at lambda = 0: no elimination term
+        }
+        reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
+      } else if (MODE == kRhs || MODE == kRecover) {
+        const double* m = mp + 6 * i;
+        double v[3];
+        if (MODE == kRhs) {
+          v[0] = double(gp[3 * i]); v[1] = double(gp[3 * i + 1]); v[2] = double(gp[3 * i + 2]);
+        } else {
+          v[0] = double(gp[3 * i]) + w[0]; v[1] = double(gp[3 * i + 1]) + w[1]; v[2] = double(gp[3 * i + 2]) + w[2];
+        }
+        double t[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) t[a] = m[sym3(a, 0)] * v[0] + m[sym3(a, 1)] * v[1] + m[sym3(a, 2)] * v[2];
         if (MODE == kRecover) {
 #pragma unroll
-          for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -double(t[a]);
+          for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -t[a];
         } else {
-          reinterpret_cast<Vec4<T>*>(t_out)[i] = Vec4<T>{t[0], t[1], t[2], T(0)};
+          reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
         }
       } else if (MODE == kVerify) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const T d = dpv[3 * i + a];
-          const T nr = w[a] + (lambda * (d * dpx[a])) * d;
-          n2 += double(nr) * double(nr);
-          n2s += double(nr / d) * double(nr / d);
+          const double d = double(dpv[3 * i + a]);
+          const double nr = w[a] + (lambda * (d * dpx[a])) * d;
+          n2 += nr * nr;
+          n2s += (nr / d) * (nr / d);
           const double g = double(gp[3 * i + a]);
           gd += g * delta_p_in[3 * i + a];
           g2 += g * g;
-          g2s += (g / double(d)) * (g / double(d));
+          g2s += (g / d) * (g / d);
         }
       } else if (MODE == kJtu) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = double(w[a]);
+        for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = w[a];
       }
     }
   }
@@ -628,8 +733,10 @@ __global__ void __launch_bounds__(kBlock) k_points(
 // ---------------------------------------------------------------------------------------
 // Camera-major half (one workgroup per camera). Sums the factors of Jc^T v over the camera's
 // rows and finishes the camera block:
-//   kSchur:   q = B p + lambda p - Dc^-1 T(acc),  v = Jp t  (Reduced::apply, schur.rs:606-652),
-//             plus p.q partials and, in the last workgroup, alpha = rz / p.q.
+//   kSchur:   q = lambda p + Dc^-1 T(acc),  v = Jc Dc^-1 p - Jp t, with p.q partials. This is
+//             S p = (B + lambda I) p - E (C + lambda I)^-1 E^T p (Reduced::apply,
+//             schur.rs:606-652) in its factored form Dc^-1 Jc^T (u - Jp t) with u = Jc Dc^-1 p,
+//             which needs no assembled B and cancels per row on 2-vectors.
 //   kResid:   q as kSchur, minus rhs, with ||q - rhs||^2 and ||rhs||^2 partials
 //   kRhs:     rhs = Dc^-1 (-gc + T(acc)),  v = Jp t
 //   kVerify:  nr = T(acc) + lambda Dc^2 delta_c, v = J delta + r; norms and ||J delta||^2
@@ -641,56 +748,63 @@ enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4 };
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_cameras(
     int nc, const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
-    const Vec2<T>* __restrict__ res_cm, const T* __restrict__ tab, const T* __restrict__ tp,
-    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
-    const T* __restrict__ bs, const T* __restrict__ dc, const T* __restrict__ gc, const T* __restrict__ vin,
-    const double* __restrict__ delta_c, const T* __restrict__ rhs, T lambda, T* __restrict__ out,
+    const Vec2<T>* __restrict__ res_cm, const T* __restrict__ tab, const double* __restrict__ tp,
+    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const double* __restrict__ fw,
+    const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
+    const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
     double* __restrict__ out_d, double* parts, const CgState* st) {
-  __shared__ T ct[kCamTab];
-  __shared__ T cfw[kFwd];
-  __shared__ T red[9][kBlock / kWave];
-  __shared__ double smem[kBlock / kWave];
+  __shared__ double ct[kCamTab];
+  __shared__ double cfw[kFwd];
+  __shared__ double red[9][kBlock / kWave];
   if (MODE == cSchur && st != nullptr && st->done) return;
   const int c = blockIdx.x;
-  if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
-  if (MODE == cVerify && threadIdx.x < kFwd) cfw[threadIdx.x] = fw[kFwd * c + threadIdx.x];
+  if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
+  if ((MODE == cVerify || MODE == cSchur || MODE == cResid) && threadIdx.x < kFwd)
+    cfw[threadIdx.x] = fw[kFwd * c + threadIdx.x];
   __syncthreads();
-  T acc[9];
+  double acc[9];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) acc[a] = T(0);
+  for (int a = 0; a < 9; ++a) acc[a] = 0.0;
   const int k0 = cam_off[c], k1 = cam_off[c + 1];
-  auto one = [&](int k, const Vec4<T>& r4, const Vec4<T>& tv) {
-    ObsGeom<T> geo;
-    obs_geom(r4, ct, geo);
-    T v0, v1;
+  auto one = [&](int k, const Vec4<T>& rec, const Vec4<double>& tv) {
+    ObsGeom<double> geo;
+    obs_geom(to_d(rec), ct, geo);
+    double v0, v1;
     if (MODE == cJtu) {
-      v0 = T(rows_in[2 * k]);
-      v1 = T(rows_in[2 * k + 1]);
+      v0 = rows_in[2 * k];
+      v1 = rows_in[2 * k + 1];
     } else if (MODE == cVerify) {
-      const T x[3] = {tv.x, tv.y, tv.z};
-      T u0, u1, w0, w1;
+      const double x[3] = {tv.x, tv.y, tv.z};
+      double u0, u1, w0, w1;
       jc_apply(geo, cfw, u0, u1);
       jp_apply(geo, ct, x, w0, w1);
       const Vec2<T> r = res_cm[k];
-      v0 = u0 + w0 + r.x;
-      v1 = u1 + w1 + r.y;
+      v0 = u0 + w0 + double(r.x);
+      v1 = u1 + w1 + double(r.y);
+    } else if (MODE == cSchur || MODE == cResid) {
+      const double x[3] = {tv.x, tv.y, tv.z};
+      double u0, u1, w0, w1;
+      jc_apply(geo, cfw, u0, u1);
+      jp_apply(geo, ct, x, w0, w1);
+      v0 = u0 - w0;
+      v1 = u1 - w1;
     } else {
-      const T x[3] = {tv.x, tv.y, tv.z};
+      const double x[3] = {tv.x, tv.y, tv.z};
       jp_apply(geo, ct, x, v0, v1);
     }
     jct_factors(geo, v0, v1, acc);
   };
-  auto gather = [&](int k) -> Vec4<T> {
-    if (MODE == cJtu) return Vec4<T>{T(0), T(0), T(0), T(0)};
+  auto gather = [&](int k) -> Vec4<double> {
+    if (MODE == cJtu) return Vec4<double>{0.0, 0.0, 0.0, 0.0};
     const int ps = k_pslot[k];
-    if (MODE == cVerify) return Vec4<T>{T(delta_p[3 * ps]), T(delta_p[3 * ps + 1]), T(delta_p[3 * ps + 2]), T(0)};
-    return reinterpret_cast<const Vec4<T>*>(tp)[ps];
+    if (MODE == cVerify) return Vec4<double>{delta_p[3 * ps], delta_p[3 * ps + 1], delta_p[3 * ps + 2], 0.0};
+    return reinterpret_cast<const Vec4<double>*>(tp)[ps];
   };
   int k = k0 + threadIdx.x;
   // two rows per step: records and gathers of both issue before either is used
   for (; k + kBlock < k1; k += 2 * kBlock) {
     const Vec4<T> ra = rec_cm[k], rb = rec_cm[k + kBlock];
-    const Vec4<T> ta = gather(k), tb = gather(k + kBlock);
+    const Vec4<double> ta = gather(k), tb = gather(k + kBlock);
     one(k, ra, ta);
     one(k + kBlock, rb, tb);
   }
@@ -698,67 +812,63 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
-    const T v = wave_sum(acc[a]);
+    const double v = wave_sum(acc[a]);
     if (lane == 0) red[a][wave] = v;
   }
   __syncthreads();
   double part0 = 0.0, part1 = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
   if (threadIdx.x == 0) {
-    T s[9];
+    double s[9];
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
-      s[a] = T(0);
+      s[a] = 0.0;
       for (int w = 0; w < kBlock / kWave; ++w) s[a] += red[a][w];
     }
     // T(acc): rotation slots through (R Jr)^T
-    T y[9];
+    double y[9];
 #pragma unroll
     for (int j = 0; j < 3; ++j) y[j] = ct[tRJ(0, j)] * s[0] + ct[tRJ(1, j)] * s[1] + ct[tRJ(2, j)] * s[2];
 #pragma unroll
     for (int j = 3; j < 9; ++j) y[j] = s[j];
-    const T* d = dc + 9 * c;
+    double d[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] = double(dc[9 * c + i]);
     if (MODE == cSchur || MODE == cResid) {
-      const T* p = vin + 9 * c;
-      const T* b = bs + 45 * c;
-      const bool observed = k1 > k0;
+      const double* p = vin + 9 * c;
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
-        T q = lambda * p[i] - y[i] / d[i];
-        if (observed) {
-#pragma unroll
-          for (int j = 0; j < 9; ++j) q += b[sym9(i, j)] * p[j];
-        }
+        const double q = lambda * p[i] + y[i] / d[i];
         out[9 * c + i] = q;
         if (MODE == cSchur) {
-          part0 += double(p[i]) * double(q);
+          part0 += p[i] * q;
         } else {
-          const double e = double(q) - double(rhs[9 * c + i]);
+          const double e = q - rhs[9 * c + i];
           part0 += e * e;
-          part1 += double(rhs[9 * c + i]) * double(rhs[9 * c + i]);
+          part1 += rhs[9 * c + i] * rhs[9 * c + i];
         }
       }
     } else if (MODE == cRhs) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
-        const T r = (-gc[9 * c + i] + y[i]) / d[i];
+        const double r = (-double(gc[9 * c + i]) + y[i]) / d[i];
         out[9 * c + i] = r;
-        part0 += double(r) * double(r);
+        part0 += r * r;
       }
     } else if (MODE == cVerify) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
-        const T dd = d[i];
-        const T nr = y[i] + (lambda * (dd * T(delta_c[9 * c + i]))) * dd;
-        part0 += double(nr) * double(nr);
-        part1 += double(nr / dd) * double(nr / dd);
+        const double dd = d[i];
+        const double nr = y[i] + (lambda * (dd * delta_c[9 * c + i])) * dd;
+        part0 += nr * nr;
+        part1 += (nr / dd) * (nr / dd);
         const double g = double(gc[9 * c + i]);
         part2 += g * delta_c[9 * c + i];
         part3 += g * g;
-        part4 += (g / double(dd)) * (g / double(dd));
+        part4 += (g / dd) * (g / dd);
       }
     } else if (MODE == cJtu) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) out_d[9 * c + i] = double(y[i]);
+      for (int i = 0; i < 9; ++i) out_d[9 * c + i] = y[i];
     }
     if (MODE != cJtu) {
       double* pc = parts + kParts * c;
@@ -791,37 +901,38 @@ static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, d
 // ---------------------------------------------------------------------------------------
 // Block-Jacobi z = (B + lambda I)^-1 r per camera (RetainedPreconditioner, schur.rs:670-677;
 // unobserved blocks divide by lambda, block_factors.rs:193-203), or with the learned
-// correction z = L^-T G G^T L^-1 r, G packed lower (9x9) per camera.
+// correction z = L^-T G G^T L^-1 r, G packed lower (9x9, storage type T) per camera.
 template <typename T>
-__device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const T* __restrict__ lc,
-                                             const T* __restrict__ gmat, bool learned, T lambda, const T r[9], T z[9]) {
+__device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const double* __restrict__ lc,
+                                             const T* __restrict__ gmat, bool learned, double lambda,
+                                             const double r[9], double z[9]) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) z[i] = r[i];
   if (cam_off[c + 1] > cam_off[c]) {
-    T l[45];
+    double l[45];
 #pragma unroll
     for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
-    solve_lower_packed<T, 9>(l, z);
+    solve_lower_packed<double, 9>(l, z);
     if (learned) {
       const T* g = gmat + 45 * c;
-      T y[9];
+      double y[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) {  // y = G^T z
-        T s = T(0);
+        double s = 0.0;
 #pragma unroll
-        for (int j = i; j < 9; ++j) s += g[sym_index(9, j, i)] * z[j];
+        for (int j = i; j < 9; ++j) s += double(g[sym_index(9, j, i)]) * z[j];
         y[i] = s;
       }
 #pragma unroll
       for (int i = 0; i < 9; ++i) {  // z = G y
-        T s = T(0);
+        double s = 0.0;
 #pragma unroll
-        for (int j = 0; j <= i; ++j) s += g[sym_index(9, i, j)] * y[j];
+        for (int j = 0; j <= i; ++j) s += double(g[sym_index(9, i, j)]) * y[j];
         z[i] = s;
       }
     }
-    solve_upper_packed<T, 9>(l, z);
-  } else if (lambda != T(0)) {
+    solve_upper_packed<double, 9>(l, z);
+  } else if (lambda != 0.0) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) z[i] = r[i] / lambda;
   }
@@ -829,20 +940,22 @@ __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_
 
 // x = 0, r = b, z = M r, p = z; per-workgroup partials r.r (col 0) and r.z (col 1).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
-                                                    const T* __restrict__ gmat, T lambda, const T* __restrict__ rhs,
-                                                    T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                                    T* __restrict__ p, double* parts, const CgState* st) {
+__global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off,
+                                                    const double* __restrict__ lc, const T* __restrict__ gmat,
+                                                    double lambda, const double* __restrict__ rhs,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ p, double* parts,
+                                                    const CgState* st) {
   __shared__ double smem[kBlock / kWave];
   const bool learned = st->use_learned != 0;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0;
   if (c < nc) {
-    T rv[9], zv[9];
+    double rv[9], zv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       rv[i] = rhs[9 * c + i];
-      x[9 * c + i] = T(0);
+      x[9 * c + i] = 0.0;
       r[9 * c + i] = rv[i];
     }
     precondition(c, cam_off, lc, gmat, learned, lambda, rv, zv);
@@ -850,8 +963,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
       p[9 * c + i] = zv[i];
-      rr += double(rv[i]) * double(rv[i]);
-      rz += double(rv[i]) * double(zv[i]);
+      rr += rv[i] * rv[i];
+      rz += rv[i] * zv[i];
     }
   }
   const double b0 = block_sum(rr, smem);
@@ -868,13 +981,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
 // forward table of p; workgroup 0 publishes the new state into slot out.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __restrict__ cam_off,
-                                                         const T* __restrict__ lc, T lambda, const T* __restrict__ tab,
-                                                         const T* __restrict__ dc, const T* __restrict__ zb,
-                                                         const T* __restrict__ z, T* __restrict__ p,
-                                                         T* __restrict__ fw, const double* __restrict__ parts,
-                                                         int nparts, const CgState* __restrict__ in, CgState* out,
-                                                         int max_iters, int fallback_at, double rel_tol,
-                                                         double fallback_factor) {
+                                                         const T* __restrict__ tab, const T* __restrict__ dc,
+                                                         const double* __restrict__ zb, const double* __restrict__ z,
+                                                         double* __restrict__ p, double* __restrict__ fw,
+                                                         const double* __restrict__ parts, int nparts,
+                                                         const CgState* __restrict__ in, CgState* out, int max_iters,
+                                                         int fallback_at, double rel_tol, double fallback_factor) {
   __shared__ double smem[kBlock / kWave];
   CgState s = *in;
   if (s.done) {
@@ -920,15 +1032,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   const bool first = in->iters < 0;
-  T pv[9];
+  double pv[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     if (restart) pv[i] = zb[9 * c + i];
     else if (first) pv[i] = p[9 * c + i];
-    else pv[i] = z[9 * c + i] + T(beta) * p[9 * c + i];
+    else pv[i] = z[9 * c + i] + beta * p[9 * c + i];
     p[9 * c + i] = pv[i];
   }
-  T zz[9];
+  double zz[9];
   forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
 }
 
@@ -936,12 +1048,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
 // per-camera partials (alpha = r.z / p.q; p.q <= 0 ends the solve NotPositiveDefinite).
 // Partials: r.r (col 0), r.z (col 1) and, before a possible fallback, r.z_blockJacobi (col 2).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off, const T* __restrict__ lc,
-                                                      const T* __restrict__ gmat, T lambda, const T* __restrict__ p,
-                                                      const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                                                      T* __restrict__ z, T* __restrict__ zb,
-                                                      const double* __restrict__ pq_parts, double* parts, CgState* st,
-                                                      int fallback_at) {
+__global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off,
+                                                      const double* __restrict__ lc, const T* __restrict__ gmat,
+                                                      double lambda, const double* __restrict__ p,
+                                                      const double* __restrict__ q, double* __restrict__ x,
+                                                      double* __restrict__ r, double* __restrict__ z,
+                                                      double* __restrict__ zb, const double* __restrict__ pq_parts,
+                                                      double* parts, CgState* st, int fallback_at) {
   __shared__ double smem[kBlock / kWave];
   __shared__ double bcast;
   const CgState s = *st;
@@ -959,13 +1072,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
     }
     return;
   }
-  const T alpha = T(s.rz / pq);
+  const double alpha = s.rz / pq;
   const bool learned = s.use_learned != 0;
   const bool both = learned && s.iters + 1 == fallback_at;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0, rzb = 0.0;
   if (c < nc) {
-    T rv[9], zv[9];
+    double rv[9], zv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       x[9 * c + i] += alpha * p[9 * c + i];
@@ -976,15 +1089,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
-      rr += double(rv[i]) * double(rv[i]);
-      rz += double(rv[i]) * double(zv[i]);
+      rr += rv[i] * rv[i];
+      rz += rv[i] * zv[i];
     }
     if (both) {
       precondition(c, cam_off, lc, gmat, false, lambda, rv, zv);
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         zb[9 * c + i] = zv[i];
-        rzb += double(rv[i]) * double(zv[i]);
+        rzb += rv[i] * zv[i];
       }
     }
   }
@@ -1000,9 +1113,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
 
 // delta_c = x / Dc (original coordinates, f64)
 template <typename T>
-__global__ void k_recover_cameras(int n, const T* __restrict__ x, const T* __restrict__ dc, double* __restrict__ delta) {
+__global__ void k_recover_cameras(int n, const double* __restrict__ x, const T* __restrict__ dc,
+                                  double* __restrict__ delta) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) delta[i] = double(x[i]) / double(dc[i]);
+  if (i < n) delta[i] = x[i] / double(dc[i]);
 }
 
 // State staging: x_trial = x + delta (additive BAL retraction, model.rs:20-28), plus the

@@ -253,8 +253,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __restrict__ cam_off,
                                                            const int* __restrict__ k_pslot,
                                                            const Vec4<T>* __restrict__ rec_cm, const T* __restrict__ tab,
-                                                           const T* __restrict__ mp, const T* __restrict__ bs,
-                                                           const T* __restrict__ dc, const T* __restrict__ lc,
+                                                           const double* __restrict__ mp, const T* __restrict__ bs,
+                                                           const T* __restrict__ dc, const double* __restrict__ lc,
                                                            T lambda, float* __restrict__ out) {
   __shared__ T ct[kCamTab];
   __shared__ T red[45][kBlock / kWave];
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __
     T jc[18];
     jc_block(geo, ct, jc);
     const int ps = k_pslot[k];
-    const T* m = mp + 6 * ps;
+    const double* m = mp + 6 * ps;
     // Jp (2x3), K2 = Jp M Jp^T (2x2)
     T jp[2][3];
     for (int a = 0; a < 3; ++a) {
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __
     T mj[2][3];
     for (int row = 0; row < 2; ++row)
       for (int a = 0; a < 3; ++a)
-        mj[row][a] = m[sym3(a, 0)] * jp[row][0] + m[sym3(a, 1)] * jp[row][1] + m[sym3(a, 2)] * jp[row][2];
+        mj[row][a] = T(m[sym3(a, 0)]) * jp[row][0] + T(m[sym3(a, 1)]) * jp[row][1] + T(m[sym3(a, 2)]) * jp[row][2];
     T k2[2][2];
     for (int u = 0; u < 2; ++u)
       for (int w = 0; w < 2; ++w) k2[u][w] = jp[u][0] * mj[w][0] + jp[u][1] * mj[w][1] + jp[u][2] * mj[w][2];
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __
         for (int w = 0; w < kBlock / kWave; ++w) e += red[sym9(i, j)][w];
         s[sym9(i, j)] = bs[45 * c + sym9(i, j)] + (i == j ? lambda : T(0)) - (e / d[i]) / d[j];
       }
-    for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
+    for (int i = 0; i < 45; ++i) l[i] = T(lc[45 * c + i]);
     // A = L^-1 S L^-T: columns of L^-1 S, then rows
     T X[9][9];
     for (int j = 0; j < 9; ++j) {

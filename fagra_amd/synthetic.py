@@ -11,7 +11,8 @@ Workloads (names carry their sizes):
 * ``ring16``        16 cameras, 2,000 points, tracks U{4..8}, plain least squares
 * ``ring256``       256 cameras, 60,000 points, tracks U{3..8}, Huber delta 1 px
 * ``rings1500``     1,500 cameras on rings of radius 8/10/12, 500,000 points,
-                    tracks U{3..10}, 2% outliers uniform over the image, Huber 1 px
+                    tracks U{3..10}, 2% outliers uniform over the image (only on points
+                    that stay well triangulated by their inliers), Huber 1 px
 * ``zipf4000``      4,000 cameras, 1,000,000 points, Zipf(1.5) tracks clipped to
                     2..200, focal U(400, 800), Huber 1 px
 
@@ -132,6 +133,30 @@ def _tracks(rng, n_points, n_cameras, lengths):
     return point, cam
 
 
+def _triangulable(cams, pts, obs_cam, obs_point, inlier, n_points, min_inliers=3, min_angle_deg=30.0):
+    """Points whose inlier observations pin them down: at least ``min_inliers`` of them, two
+    of which see the point under a parallax angle in [min_angle, 180 - min_angle].
+
+    Under the Huber cost an outlier pulls with a bounded force, and a point held only by two
+    inlier rays of small parallax can lower its cost by sliding to infinity or behind a camera,
+    where the inlier residuals stay bounded and the outlier's shrinks; outliers are only placed
+    on points that this test keeps well triangulated. Observations are grouped by point.
+    """
+    R = rotation_matrices(cams[:, :3])
+    centers = -np.einsum("nji,nj->ni", R, cams[:, 3:6])
+    rays = pts[obs_point] - centers[obs_cam]
+    rays /= np.linalg.norm(rays, axis=1, keepdims=True)
+    idx = np.flatnonzero(inlier)
+    pts_with, first = np.unique(obs_point[idx], return_index=True)
+    ref = np.zeros((n_points, 3))
+    ref[pts_with] = rays[idx[first]]
+    cosang = np.abs(np.einsum("ni,ni->n", rays, ref[obs_point]))
+    wide = inlier & (cosang <= np.cos(np.radians(min_angle_deg)))
+    n_in = np.bincount(obs_point, weights=inlier, minlength=n_points)
+    n_wide = np.bincount(obs_point, weights=wide, minlength=n_points)
+    return (n_in >= min_inliers) & (n_wide >= 1)
+
+
 CONFIGS = {
     "ring16": dict(n_cameras=16, n_points=2000, radii=(10.0,), track=("uniform", 4, 8), huber=0.0,
                    outliers=0.0, focal=("const", 500.0)),
@@ -165,7 +190,13 @@ def generate(name="ring16", seed=0, n_cameras=None, n_points=None, **overrides):
     pixel = project(cams, pts, obs_cam, obs_point)
     pixel += rng.normal(0.0, 0.5, pixel.shape)
     if cfg["outliers"] > 0:
-        bad = rng.random(len(pixel)) < cfg["outliers"]
+        # outliers go to observations of points that stay triangulable without them, at the
+        # rate that puts the configured fraction over all observations
+        everyone = np.ones(len(pixel), bool)
+        eligible = (_triangulable(cams, pts, obs_cam, obs_point, everyone, npts) & (lengths >= 4))[obs_point]
+        rate = min(1.0, cfg["outliers"] * len(pixel) / max(1, int(eligible.sum())))
+        bad = eligible & (rng.random(len(pixel)) < rate)
+        bad &= _triangulable(cams, pts, obs_cam, obs_point, ~bad, npts)[obs_point]
         half = np.abs(pixel).max()
         pixel[bad] = rng.uniform(-half, half, (int(bad.sum()), 2))
     init_c = cams.copy()
