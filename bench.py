@@ -37,20 +37,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def matvec_bytes(nc, npts, no, precision):
-    """Algorithmic HBM bytes of one Schur product (the two matvec kernels).
+def kernel_bytes(nc, npts, no, precision):
+    """Algorithmic HBM bytes of one launch of each Schur-product kernel, every stream counted
+    once (repeated gathers of a camera's table or forward table, or of a point's solve, are
+    cache traffic, not algorithmic bytes).
 
-    Point-major kernel: per observation a compact record (4 scalars) and a camera index;
-    per point the eliminated-block inverse (6 scalars) in and the solve result (3) out.
-    Camera-major kernel: per observation the record again and a point-slot index (the
-    per-point result it gathers is counted once); per camera the normalized Gram block
-    (45), the input and output vectors (9 + 9) and the column scales (9).
+    k_points<kSchur> (one lane per point): per observation the compact record (4 x T) and its
+    camera index (4 B); per point the column scales (3 x T) in and the f64 solve t (one
+    32 B 4-vector) out; per camera the table row it reads (16 x T) and the forward table of p
+    (12 x T).
+    k_cameras<cSchur> (one workgroup per camera): per observation the record copy (4 x T) and
+    the point-slot index (4 B); per point t back in (32 B); per camera the table (32 x T),
+    the forward table (12 x T), the column scales (9 x T) and p in, q out (9 x 8 B each).
     """
     t = 4 if precision == "fp32" else 8
-    per_obs = 2 * (4 * t + 4)
-    per_point = 6 * t + 3 * t + 3 * t
-    per_cam = (45 + 27) * t
-    return no * per_obs + npts * per_point + nc * per_cam
+    points = no * (4 * t + 4) + npts * (3 * t + 32) + nc * (16 + 12) * t
+    cameras = no * (4 * t + 4) + npts * 32 + nc * ((32 + 12 + 9) * t + 2 * 9 * 8)
+    return points, cameras
 
 
 def survey_matvec_bytes(nc, npts, no):
@@ -58,7 +61,7 @@ def survey_matvec_bytes(nc, npts, no):
     return no * 2 * (96 + 4) + npts * (24 + 12 + 12) + nc * (180 + 36 + 36)
 
 
-def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, collect_timings=True):
+def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, collect_timings=False):
     backend = fa.Schur(precision=precision, relative_tolerance=cg_tol, preconditioner=precond,
                        collect_timings=collect_timings)
     lm = fa.LevenbergMarquardt(backend)
@@ -132,6 +135,35 @@ def cpu_baseline(problem, fp64_counts, budget_s):
             "cpu": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
 
 
+def profile_products(args):
+    """Counter-run workload: repeats of each Schur-product kernel at the initial state of the
+    workload in both precisions, then a plain 256 MiB device copy whose FETCH_SIZE and
+    WRITE_SIZE calibrate the counters."""
+    import fagra_amd as fa
+    from fagra_amd import synthetic
+    import torch
+    p = synthetic.generate(args.workload, seed=args.seed)
+    out = {}
+    for prec, name in ((fa.FP32, "fp32"), (fa.FP64, "fp64")):
+        d = fa.BalProblem(p, precision=prec)
+        s = fa.Schur(precision=prec)
+        assert s.prepare(d) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+        x = np.random.default_rng(1).normal(size=9 * p.n_cameras)
+        tp, tc = s.time_products(1e-4, x, repeats=args.kernel_repeats)
+        out[name] = {"k_points_us": tp * 1e6, "k_cameras_us": tc * 1e6,
+                     "bytes": kernel_bytes(p.n_cameras, p.n_points, p.n_observations, name)}
+        log(f"bench: {name} k_points {tp * 1e6:.1f} us, k_cameras {tc * 1e6:.1f} us")
+        s.close()
+        d.close()
+    a = torch.empty(64 * 2**20, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    out["copy_bytes"] = 2 * a.numel() * 4
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,11 +172,18 @@ def main():
     ap.add_argument("--workload", default="rings1500")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cg-tol", type=float, default=1e-3)
-    ap.add_argument("--max-iter", type=int, default=40)
+    ap.add_argument("--max-iter", type=int, default=150)
+    ap.add_argument("--cstar-iter", type=int, default=400)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--kernel-repeats", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32,learned")
+    ap.add_argument("--profile-products", action="store_true",
+                    help="only time the two Schur-product kernels (FP32, FP64) and a 256 MiB device copy, "
+                         "for hardware-counter runs")
     args = ap.parse_args()
+    if args.profile_products:
+        return profile_products(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
                                    "design note only)"}))
@@ -165,13 +204,14 @@ def main():
         dev[prec] = d
         log(f"bench: uploaded precision {prec} in {time.perf_counter() - t:.1f}s")
 
-    # C*: converged FP64 cost on the same input (tight CG tolerance, gradient tolerance 1e-8)
+    # C*: the converged FP64 cost on the same input (the reference's default CG tolerance 1e-10)
     t = time.perf_counter()
-    ref = run_mode(fa, dev[fa.FP64], fa.FP64, fa.PRECOND_BLOCK_JACOBI, 1e-10, -1.0, 60, collect_timings=False)
+    ref = run_mode(fa, dev[fa.FP64], fa.FP64, fa.PRECOND_BLOCK_JACOBI, 1e-10, -1.0, args.cstar_iter,
+                   collect_timings=False)
     c_star = ref["cost"]
     target = c_star * (1 + 1e-5)
-    log(f"bench: C* = {c_star:.10e} (status {ref['status']}, {ref['accepted']} steps, "
-        f"{time.perf_counter() - t:.1f}s)")
+    log(f"bench: C* = {c_star!r} (status {ref['status']}, {ref['accepted']} steps, {ref['cg_iterations']} CG "
+        f"iterations, {time.perf_counter() - t:.1f}s)")
 
     have_model = learned.available()
     modes = []
@@ -191,7 +231,7 @@ def main():
         for k in range(args.steps):
             r = run_mode(fa, dev[prec], prec, pc, args.cg_tol, target, args.max_iter)
             log(f"bench: {name}: time_to_target {r['time_to_target']} total {r['total']:.3f}s "
-                f"cost {r['cost']:.10e} steps {r['accepted']} cg {r['cg_iterations']} matvecs {r['matvecs']}")
+                f"cost {r['cost']!r} steps {r['accepted']} cg {r['cg_iterations']} matvecs {r['matvecs']}")
             runs.append(r)
         results[name] = runs
 
@@ -201,29 +241,49 @@ def main():
     hr = results[head]
     times = [r["time_to_target"] for r in hr if r["time_to_target"] is not None]
     value = statistics.median(times) if times else None
-    f64 = results.get("fp64_block_jacobi", [])
-    f64_times = [r["time_to_target"] for r in f64 if r["time_to_target"] is not None]
-    f64_value = statistics.median(f64_times) if f64_times else None
+    mode_s = {}
+    for name, runs in results.items():
+        ts = [r["time_to_target"] for r in runs if r["time_to_target"] is not None]
+        mode_s[name] = statistics.median(ts) if ts else None
+    f64_value = mode_s.get("fp64_block_jacobi")
 
-    # roofline of the Schur matvec (two kernels), HIP events on the backend stream
-    last = hr[-1]
+    # roofline of the dominant kernel: back-to-back launches of each Schur-product kernel at the
+    # initial state, timed with HIP events on the backend stream
+    hname, hprec, hpc = next(m for m in modes if m[0] == head)
     prec_name = "fp32" if head.startswith("fp32") else "fp64"
-    mv_s = last["matvec_kernel_s"] / max(last["matvecs"] - last["solves"], 1)
-    bytes_mv = matvec_bytes(p.n_cameras, p.n_points, p.n_observations, prec_name)
-    achieved = bytes_mv / mv_s / 1e9 if mv_s > 0 else None
+    d = dev[hprec]
+    d.set_state(*d.initial)
+    sch = fa.Schur(precision=hprec)
+    assert sch.prepare(d) == fa.OK and sch.linearize() == fa.OK and sch.prepare_damping(1e-6) == fa.OK
+    x = np.random.default_rng(1).normal(size=9 * p.n_cameras)
+    t_pts, t_cams = sch.time_products(1e-4, x, repeats=args.kernel_repeats)
+    sch.close()
+    b_pts, b_cams = kernel_bytes(p.n_cameras, p.n_points, p.n_observations, prec_name)
+    if t_pts >= t_cams:
+        kname, kbytes, ksec = "k_points<%s, kSchur>" % ("float" if prec_name == "fp32" else "double"), b_pts, t_pts
+    else:
+        kname, kbytes, ksec = "k_cameras<%s, cSchur>" % ("float" if prec_name == "fp32" else "double"), b_cams, t_cams
+    achieved = kbytes / ksec / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
-                "kernel": "Schur matvec (k_points<kSchur> + k_cameras<cSchur>)",
-                "bytes_per_launch": bytes_mv, "avg_launch_us": mv_s * 1e6,
-                "stored_jacobian_design_bytes": survey_matvec_bytes(p.n_cameras, p.n_points, p.n_observations)}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                "bytes_per_launch": kbytes, "avg_launch_us": ksec * 1e6, "launches_timed": args.kernel_repeats,
+                "schur_product": {"k_points_us": t_pts * 1e6, "k_cameras_us": t_cams * 1e6,
+                                  "bytes": b_pts + b_cams,
+                                  "achieved_GBps": (b_pts + b_cams) / (t_pts + t_cams) / 1e9,
+                                  "stored_jacobian_design_bytes": survey_matvec_bytes(p.n_cameras, p.n_points,
+                                                                                      p.n_observations)}}
+    log(f"bench: roofline {kname}: {ksec * 1e6:.1f} us, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
 
     cpu = None
+    f64 = results.get("fp64_block_jacobi", [])
     if not args.no_cpu and f64:
         cnt = f64[-1]["at_target"] or {"linearizations": f64[-1]["linearizations"],
                                         "iterations": f64[-1]["cg_iterations"], "solves": f64[-1]["solves"],
                                         "cost_evaluations": f64[-1]["accepted"] + f64[-1]["rejected"] + 1}
         cpu = cpu_baseline(p, cnt, args.cpu_budget)
 
+    slim = {k: [{kk: vv for kk, vv in r.items() if kk not in ("timings", "trace")} for r in v]
+            for k, v in results.items()}
     line = {"metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": 1e3 * statistics.mean(r["total"] for r in hr),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": prec_name,
@@ -231,10 +291,9 @@ def main():
             "config": {"workload": f"{args.workload} seed {args.seed}: {p.n_cameras} cameras, {p.n_points} points, "
                                    f"{p.n_observations} observations, Huber {p.huber}", "mode": head,
                        "cg_relative_tolerance": args.cg_tol, "target": "C*(1+1e-5)", "c_star": c_star},
-            "fp64_block_jacobi_s": f64_value,
+            "fp64_block_jacobi_s": f64_value, "mode_seconds": mode_s,
             "speedup_vs_fp64": (f64_value / value) if (value and f64_value) else None,
-            "modes": {k: [{kk: vv for kk, vv in r.items() if kk != "timings"} for r in v] for k, v in results.items()},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "modes": slim, "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line))
 
 

@@ -130,7 +130,7 @@ class Schur:
     """GPU Schur backend with the reference's public fields (src/schur.rs:116-125)."""
 
     def __init__(self, precision=FP64, max_iterations=None, relative_tolerance=None,
-                 preconditioner=PRECOND_BLOCK_JACOBI, collect_timings=False, on_solve=None):
+                 preconditioner=PRECOND_BLOCK_JACOBI, collect_timings=False, on_solve=None, network=None):
         cfg = _lib.SchurConfig()
         check(lib().fg_schur_config_default(ctypes.byref(cfg), int(precision)), "fg_schur_config_default")
         if max_iterations is not None:
@@ -148,6 +148,18 @@ class Schur:
         check(lib().fg_schur_create(ctypes.byref(cfg), ctypes.byref(handle)), "fg_schur_create")
         self.h = handle.value
         self.problem = None
+        self._net = None
+        if network is None and preconditioner == PRECOND_LEARNED:
+            from . import learned
+            if learned.available():
+                network = learned.Network.load()
+        if network is not None:
+            self.set_network(network)
+
+    def set_network(self, network):
+        """Attach a learned.Network for PRECOND_LEARNED (kept alive with this backend)."""
+        check(lib().fg_schur_set_mlp(self.h, network.h), "fg_schur_set_mlp")
+        self._net = network
 
     # public fields mirrored into the config on every prepare
     @property
@@ -251,6 +263,14 @@ class Schur:
         out = np.zeros((self.problem.nc, 9, 9))
         check(lib().fg_schur_camera_gram(self.h, _dptr(out)), "fg_schur_camera_gram")
         return out
+
+    def time_products(self, lam, x, repeats=50):
+        """Mean seconds per launch of the point-major and camera-major product kernels."""
+        x = _f64(x)
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().fg_schur_time_products(self.h, float(lam), _dptr(x), int(repeats), ctypes.byref(a),
+                                           ctypes.byref(b)), "fg_schur_time_products")
+        return a.value, b.value
 
     def reduced_apply(self, lam, x):
         x = _f64(x)

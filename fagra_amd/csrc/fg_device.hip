@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -100,6 +101,7 @@ struct fg_schur {
   fg_ba* ba = nullptr;
   size_t dimension = 0;
   bool prepared = false, linearized = false, ready = false;
+  bool use_graphs = true;  // FAGRA_PCG_GRAPHS=0 launches PCG batches kernel by kernel
   // Model (typed device arrays stored untyped).
   void *rec = nullptr, *res = nullptr, *rec_cm = nullptr, *res_cm = nullptr;
   void *hpp = nullptr, *gp = nullptr, *hcc = nullptr, *gc = nullptr;
@@ -125,6 +127,9 @@ struct fg_schur {
   size_t solves = 0, iterations = 0;
   fg_schur_diagnostics diag;
   fg_schur_timings tim;
+  hipGraphExec_t cg_graph[2] = {nullptr, nullptr};  // PCG batch per accepted-state buffer
+  int cg_graph_size[2] = {0, 0};                      // sizeof(T) it was captured for
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> events;  // matvec kernel timing pool (pairs)
   size_t events_used = 0;
   hipEvent_t ph_a = nullptr, ph_b = nullptr;
@@ -132,7 +137,8 @@ struct fg_schur {
   bool step_valid = false;
   // learned preconditioner
   fg_mlp* mlp = nullptr;
-  void* gmat = nullptr;        // T [nc][45]
+  void* mbj = nullptr;         // f64 [nc][45] explicit (B + lambda I)^-1
+  void* gmat = nullptr;        // f64 [nc][45] explicit learned inverse L^-T G G^T L^-1
   uint16_t* mx = nullptr;      // bf16 [ncpad][64] features
   float* my = nullptr;         // f32 [ncpad][48] network output
   float* mxf = nullptr;        // f32 [nc][47] features (capture)
@@ -231,7 +237,7 @@ PointArgs point_args(const fg_ba* b) {
 }
 
 template <typename T, int MODE>
-void launch_points(fg_schur* s, const double* fw, const double* delta_p_in, const double* rows_in, double lambda,
+void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, double lambda,
                    double* delta_out, double* rows_out, double* parts, const CgState* st) {
   fg_ba* b = s->ba;
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
@@ -242,12 +248,12 @@ void launch_points(fg_schur* s, const double* fw, const double* delta_p_in, cons
 }
 
 template <typename T, int MODE>
-void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const double* fw,
+void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const T* fw,
                     const double* vin, const double* delta_c, const double* rhs, double lambda, double* out,
                     double* out_d, const CgState* st = nullptr) {
   fg_ba* b = s->ba;
   // the Schur, residual and verification modes read the forward table of their input
-  if (fw == nullptr) fw = (const double*)s->fw;
+  if (fw == nullptr) fw = (const T*)s->fw;
   hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
                      (const T*)b->d_tab[b->cur], (const double*)s->tp, delta_p, rows_in, fw, (const T*)s->dc,
@@ -321,7 +327,7 @@ fg_status factor_impl(fg_schur* s, double lambda, int* fails) {
   FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
   hipLaunchKernelGGL(k_factor<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
                      (int)b->np, b->d_cam_off, b->d_pt_len, (const T*)s->bs, (const double*)s->cs, (const T*)s->dp,
-                     lambda, (double*)s->lc, (double*)s->mp, s->d_fail);
+                     lambda, (double*)s->lc, (double*)s->mbj, (double*)s->mp, s->d_fail);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
@@ -335,9 +341,9 @@ template <typename T>
 fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out, const double* resid_rhs) {
   fg_ba* b = s->ba;
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (double*)s->fw);
-  launch_points<T, kSchur>(s, (const double*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
-  launch_cameras<T, cResid>(s, nullptr, nullptr, (const double*)s->fw, x, nullptr,
+                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw);
+  launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
+  launch_cameras<T, cResid>(s, nullptr, nullptr, (const T*)s->fw, x, nullptr,
                             resid_rhs ? resid_rhs : (const double*)s->zero, lambda, out, nullptr);
   FG_TRY(hipGetLastError());
   s->tim.matvecs += 1;
@@ -355,9 +361,67 @@ fg_status sum_parts(fg_schur* s, int rows) {
 }
 
 // PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
-// Iteration i launches direction, the two product kernels and the update. The host polls the
-// published done flag of the previous batch while the next batch runs; iterations after the
-// solve ended are no-ops.
+// Iteration i launches direction, the two product kernels and the update. Iterations go out in
+// batches of kBatch (even, so every batch starts on the same state slot); the host polls the
+// published done flag of the previous batch while the next one runs, and iterations after the
+// solve ended are no-ops. The damping and the PCG settings travel in device state, so one
+// captured HIP graph per accepted-state buffer serves every batch of every solve; with
+// collect_timings the batch is launched kernel by kernel between timing events instead.
+constexpr int kBatch = 4;
+
+template <typename T>
+void launch_cg_iteration(fg_schur* s, int i, bool timed) {
+  fg_ba* b = s->ba;
+  const int nc = (int)b->nc;
+  const int cblocks = grid_for(nc);
+  CgState* cur = &s->d_sc->st[(i + 1) & 1];
+  hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
+                     (double*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
+                     (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], (const DevScalars*)s->d_sc);
+  const bool ev = timed && s->events_used + 2 <= s->events.size();
+  if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
+  launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, 0.0, nullptr, nullptr, nullptr, cur);
+  launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, 0.0,
+                            (double*)s->q, nullptr, cur);
+  if (ev) {
+    (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
+    s->events_used += 2;
+  }
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     (const double*)s->mbj, (const double*)s->gmat, (const double*)s->p, (const double*)s->q,
+                     (double*)s->x, (double*)s->r, (double*)s->z, (double*)s->zb, (const double*)s->d_parts,
+                     s->d_cgparts, cur, (const DevScalars*)s->d_sc);
+}
+
+template <typename T>
+fg_status cg_batch_graph(fg_schur* s, hipGraphExec_t* out) {
+  fg_ba* b = s->ba;
+  const int slot = b->cur;
+  if (s->cg_graph[slot] && s->cg_graph_size[slot] == (int)sizeof(T)) {
+    *out = s->cg_graph[slot];
+    return FG_OK;
+  }
+  if (s->cg_graph[slot]) (void)hipGraphExecDestroy(s->cg_graph[slot]);
+  s->cg_graph[slot] = nullptr;
+  hipGraph_t g = nullptr;
+  FG_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < kBatch; ++i) launch_cg_iteration<T>(s, i, false);
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return FG_DEVICE_ERROR;
+  }
+  hipGraphExec_t ge = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) return FG_DEVICE_ERROR;
+  s->cg_graph[slot] = ge;
+  s->cg_graph_size[slot] = (int)sizeof(T);
+  *out = ge;
+  return FG_OK;
+}
+
 template <typename T>
 fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   fg_ba* b = s->ba;
@@ -370,68 +434,50 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
     init.st[i].iters = -1;
     init.st[i].term = -1;
     init.st[i].use_learned = learned ? 1 : 0;
+    init.st[i].lambda = lambda;
   }
   init.max_iters = max_it;
   init.fallback_at = fallback_at;
+  init.rel_tol = s->cfg.relative_tolerance;
+  init.fallback_factor = s->cfg.fallback_factor;
   FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                     (const double*)s->lc, (const T*)s->gmat, lambda, (const double*)s->rhs, (double*)s->x,
+                     (const double*)s->mbj, (const double*)s->gmat, (const double*)s->rhs, (double*)s->x,
                      (double*)s->r, (double*)s->z, (double*)s->p, s->d_cgparts, (const CgState*)&s->d_sc->st[0]);
   FG_TRY(hipGetLastError());
   const bool timed = s->cfg.collect_timings != 0;
-  auto direction = [&](int i) {
-    hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                       (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
-                       (double*)s->p, (double*)s->fw, (const double*)s->d_cgparts, cblocks,
-                       (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], max_it, fallback_at,
-                       s->cfg.relative_tolerance, s->cfg.fallback_factor);
-  };
-  hipEvent_t evs[2];
-  FG_TRY(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
-  FG_TRY(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+  hipGraphExec_t graph = nullptr;
+  if (!timed && s->use_graphs) FG_CHECK(cg_batch_graph<T>(s, &graph));
   DevScalars* polls = (DevScalars*)s->h_poll;  // two pinned slots
-  const int kBatch = 4;
   int launched = 0, batch = 0;
   bool stop = false;
   while (!stop && launched < max_it) {
-    const int nb = std::min(kBatch, max_it - launched);
-    for (int it = 0; it < nb; ++it) {
-      const int i = launched + it;
-      CgState* cur = &s->d_sc->st[(i + 1) & 1];
-      direction(i);
-      const bool ev = timed && s->events_used + 2 <= s->events.size();
-      if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
-      launch_points<T, kSchur>(s, (const double*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, cur);
-      launch_cameras<T, cSchur>(s, nullptr, nullptr, (const double*)s->fw, (const double*)s->p, nullptr, nullptr,
-                                lambda, (double*)s->q, nullptr, cur);
-      if (ev) {
-        (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
-        s->events_used += 2;
-      }
-      hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                         (const double*)s->lc, (const T*)s->gmat, lambda, (const double*)s->p,
-                         (const double*)s->q, (double*)s->x, (double*)s->r, (double*)s->z, (double*)s->zb,
-                         (const double*)s->d_parts, s->d_cgparts, cur, fallback_at);
+    if (graph) {
+      FG_TRY(hipGraphLaunch(graph, b->stream));
+    } else {
+      for (int it = 0; it < kBatch; ++it) launch_cg_iteration<T>(s, launched + it, timed);
+      FG_TRY(hipGetLastError());
     }
-    FG_TRY(hipGetLastError());
-    launched += nb;
+    launched += kBatch;
     const int slot = batch & 1;
     FG_TRY(hipMemcpyAsync(&polls[slot], s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
-    FG_TRY(hipEventRecord(evs[slot], b->stream));
+    FG_TRY(hipEventRecord(s->poll_ev[slot], b->stream));
     if (batch > 0) {
-      FG_TRY(hipEventSynchronize(evs[slot ^ 1]));
+      FG_TRY(hipEventSynchronize(s->poll_ev[slot ^ 1]));
       const DevScalars& h = polls[slot ^ 1];
       if (h.st[0].done || h.st[1].done) stop = true;
     }
     ++batch;
   }
   // the last update's convergence / exhaustion decision
-  direction(launched);
+  hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
+                     (double*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
+                     (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
+                     (const DevScalars*)s->d_sc);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
-  (void)hipEventDestroy(evs[0]);
-  (void)hipEventDestroy(evs[1]);
   const CgState& h = s->h_sc->st[(launched + 1) & 1];
   s->iterations += (size_t)std::max(h.iters, 0);
   s->diag.iterations = (size_t)std::max(h.iters, 0);
@@ -472,12 +518,12 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   if (learned) {
     Phase ph(s, &s->tim.preconditioner);
     if (!s->mlp) return FG_INVALID_OPTIONS;
-    // per-camera features -> network -> G = I + s tril(y)
+    // per-camera features -> network -> G = I + s tril(y) -> explicit L^-T G G^T L^-1
     hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                        (const T*)s->hcc, lambda_d, s->mx, (float*)nullptr);
     mlp_forward(s->mlp, b->stream, pad64(nc), s->mx, s->my);
-    hipLaunchKernelGGL(k_mlp_build_g<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, (const float*)s->my,
-                       (T*)s->gmat);
+    hipLaunchKernelGGL(k_mlp_build_g, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                       (const float*)s->my, (const double*)s->lc, (double*)s->gmat);
     FG_TRY(hipGetLastError());
   }
   {
@@ -487,8 +533,8 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   Phase ph(s, &s->tim.verification);
   // recover_step (schur.rs:312-344)
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (double*)s->fw);
-  launch_points<T, kRecover>(s, (const double*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
+                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw);
+  launch_points<T, kRecover>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
                              nullptr, nullptr);
   hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)(9 * b->nc),
                      (const double*)s->x, (const T*)s->dc, s->d_delta);
@@ -500,14 +546,14 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   s->diag.reduced_relative_residual = rhs2 == 0.0 ? std::sqrt(resid2) : std::sqrt(resid2 / rhs2);
   // full-model verification in original coordinates (linear_model.rs:254-282)
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (double*)s->fw);
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
-  launch_points<T, kVerify>(s, (const double*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
+  launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
                             s->d_parts, nullptr);
   FG_CHECK(sum_parts<T>(s, pblocks));
   double pt[kParts];
   std::memcpy(pt, s->h_sums, sizeof(pt));
-  launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const double*)s->fw, nullptr, s->d_delta,
+  launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, nullptr, s->d_delta,
                              nullptr, lambda, nullptr, nullptr);
   FG_CHECK(sum_parts<T>(s, nc));
   const double* cm = s->h_sums;
@@ -550,16 +596,18 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((T**)&s->dp, 3 * np));
   FG_TRY(dalloc((T**)&s->bs, 45 * nc));
   FG_TRY(dalloc((double**)&s->cs, 6 * np));
-  // f64 in both precisions: factors, per-point solves and every camera-sized vector
+  // f64 in both precisions: factors, per-point solves and the PCG vectors; the forward table
+  // of the PCG direction is stored in T
   FG_TRY(dalloc((double**)&s->lc, 45 * nc));
   FG_TRY(dalloc((double**)&s->mp, 6 * np));
   FG_TRY(dalloc((double**)&s->tp, 4 * np));
-  FG_TRY(dalloc((double**)&s->fw, kFwd * nc));
+  FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((double**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
   }
-  FG_TRY(dalloc((T**)&s->gmat, 45 * nc));
+  FG_TRY(dalloc((double**)&s->gmat, 45 * nc));
+  FG_TRY(dalloc((double**)&s->mbj, 45 * nc));
   FG_TRY(dalloc(&s->mx, pad64(nc) * kInPad));
   FG_TRY(hipMemset(s->mx, 0, pad64(nc) * kInPad * sizeof(uint16_t)));
   FG_TRY(dalloc(&s->my, pad64(nc) * kOutPad));
@@ -578,9 +626,15 @@ fg_status alloc_model(fg_schur* s) {
 }
 
 void free_model(fg_schur* s) {
+  // captured PCG batches hold the model's device pointers
+  for (int i = 0; i < 2; ++i) {
+    if (s->cg_graph[i]) (void)hipGraphExecDestroy(s->cg_graph[i]);
+    s->cg_graph[i] = nullptr;
+    s->cg_graph_size[i] = 0;
+  }
   for (void** v : {&s->rec, &s->res, &s->rec_cm, &s->res_cm, &s->hpp, &s->gp, &s->hcc, &s->gc, &s->dc, &s->dp,
                    &s->bs, &s->cs, &s->lc, &s->mp, &s->tp, &s->fw, &s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs,
-                   &s->zero, &s->gmat}) {
+                   &s->zero, &s->gmat, &s->mbj}) {
     if (*v) (void)hipFree(*v);
     *v = nullptr;
   }
@@ -657,8 +711,8 @@ fg_status jacobian_apply_impl(fg_schur* s, const double* x, double* y) {
   fg_ba* b = s->ba;
   FG_CHECK(upload_vector(b, x, s->d_delta));
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (double*)s->fw);
-  launch_points<T, kJv>(s, (const double*)s->fw, s->d_delta + 9 * b->nc, nullptr, 0.0, nullptr, s->d_rows, nullptr,
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+  launch_points<T, kJv>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, 0.0, nullptr, s->d_rows, nullptr,
                         nullptr);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(y, s->d_rows, 2 * b->no * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -739,6 +793,44 @@ fg_status capture_impl(fg_schur* s, double lambda_d, float* features, float* tar
   FG_TRY(hipStreamSynchronize(b->stream));
   if (counts)
     for (int c = 0; c < nc; ++c) counts[c] = (int)(b->cam_off[c + 1] - b->cam_off[c]);
+  return FG_OK;
+}
+
+// Back-to-back launches of the two Schur-product kernels on the backend stream, each group
+// bracketed by HIP events; reports the mean duration of one launch of each.
+template <typename T>
+fg_status time_products_impl(fg_schur* s, double lambda, const double* x, int repeats, double* sec_points,
+                             double* sec_cameras) {
+  fg_ba* b = s->ba;
+  const size_t n = 9 * b->nc;
+  int fails = 0;
+  FG_CHECK(factor_impl<T>(s, lambda, &fails));
+  if (fails) return FG_LINEAR_SOLVE_FAILED;
+  FG_TRY(hipMemcpyAsync(s->p, x, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->p, (const T*)s->dc, 1, (T*)s->fw);
+  // one untimed pass of each warms the caches the timed passes see
+  launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
+  launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, lambda,
+                            (double*)s->q, nullptr);
+  hipEvent_t ev[3];
+  for (auto& e : ev) FG_TRY(hipEventCreate(&e));
+  FG_TRY(hipEventRecord(ev[0], b->stream));
+  for (int i = 0; i < repeats; ++i)
+    launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
+  FG_TRY(hipEventRecord(ev[1], b->stream));
+  for (int i = 0; i < repeats; ++i)
+    launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, lambda,
+                              (double*)s->q, nullptr);
+  FG_TRY(hipEventRecord(ev[2], b->stream));
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipEventSynchronize(ev[2]));
+  float m1 = 0.f, m2 = 0.f;
+  FG_TRY(hipEventElapsedTime(&m1, ev[0], ev[1]));
+  FG_TRY(hipEventElapsedTime(&m2, ev[1], ev[2]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *sec_points = 1e-3 * m1 / repeats;
+  *sec_cameras = 1e-3 * m2 / repeats;
   return FG_OK;
 }
 
@@ -1017,6 +1109,8 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
   if (cfg->precision != FG_FP64 && cfg->precision != FG_FP32) return FG_INVALID_OPTIONS;
   fg_schur* s = new fg_schur();
   s->cfg = *cfg;
+  // Kernel-tracing profilers that cannot follow stream capture can switch the graphs off.
+  if (const char* e = std::getenv("FAGRA_PCG_GRAPHS")) s->use_graphs = std::strcmp(e, "0") != 0;
   reset_diag(s, 0.0);
   std::memset(&s->tim, 0, sizeof(s->tim));
   auto fail = [&]() {
@@ -1033,7 +1127,10 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
       hipHostMalloc((void**)&s->h_sums, 2 * kParts * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_int, 8 * sizeof(int), 0) != hipSuccess)
     return fail();
-  if (hipEventCreate(&s->ph_a) != hipSuccess || hipEventCreate(&s->ph_b) != hipSuccess) return fail();
+  if (hipEventCreate(&s->ph_a) != hipSuccess || hipEventCreate(&s->ph_b) != hipSuccess ||
+      hipEventCreateWithFlags(&s->poll_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->poll_ev[1], hipEventDisableTiming) != hipSuccess)
+    return fail();
   *out = s;
   return FG_OK;
 }
@@ -1052,6 +1149,8 @@ void fg_schur_destroy(fg_schur* s) {
   if (s->h_sums) (void)hipHostFree(s->h_sums);
   if (s->h_int) (void)hipHostFree(s->h_int);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->poll_ev)
+    if (e) (void)hipEventDestroy(e);
   if (s->ph_a) (void)hipEventDestroy(s->ph_a);
   if (s->ph_b) (void)hipEventDestroy(s->ph_b);
   delete s;
@@ -1151,7 +1250,9 @@ static fg_status solve_damped_device(fg_schur* s, double lambda, double* predict
                                              : attempt_impl<double>(s, lambda, &pred);
   if (s->cfg.collect_timings && s->events_used > 0) {
     (void)hipEventSynchronize(s->events[s->events_used - 1]);
-    for (size_t i = 0; i + 1 < s->events_used; i += 2) {
+    // the first diag.iterations pairs bracket real products; later ones are the no-op tail
+    const size_t pairs = std::min(s->events_used / 2, s->diag.iterations);
+    for (size_t i = 0; i + 1 < 2 * pairs; i += 2) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, s->events[i], s->events[i + 1]) == hipSuccess) {
         s->tim.matvec_kernel_seconds += 1e-3 * ms;
@@ -1210,6 +1311,13 @@ fg_status fg_schur_capture(fg_schur* s, double lambda, float* features, float* t
   if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
   return s->cfg.precision == FG_FP32 ? capture_impl<float>(s, lambda, features, targets, counts)
                                      : capture_impl<double>(s, lambda, features, targets, counts);
+}
+
+fg_status fg_schur_time_products(fg_schur* s, double lambda, const double* x, int repeats, double* sec_points,
+                                 double* sec_cameras) {
+  if (!s->ready || !std::isfinite(lambda) || lambda < 0.0 || repeats <= 0) return FG_INVALID_OPTIONS;
+  return s->cfg.precision == FG_FP32 ? time_products_impl<float>(s, lambda, x, repeats, sec_points, sec_cameras)
+                                     : time_products_impl<double>(s, lambda, x, repeats, sec_points, sec_cameras);
 }
 
 fg_status fg_schur_reduced_apply(fg_schur* s, double lambda, const double* x, double* y) {

@@ -23,13 +23,14 @@ constexpr int kChunksPerBlock = kBlock / kWave;
 // block 0 only. No workgroup ever reads a slot that another workgroup of the same kernel writes,
 // so no cross-workgroup synchronization is needed inside a kernel.
 struct CgState {
-  double rz, tol2, bb, target, rr, pad_d;
+  double rz, tol2, bb, target, rr, lambda;
   int done, iters, term, use_learned;
   int fallback_taken, pad0, pad1, pad2;
 };
 struct DevScalars {
   CgState st[2];
   int max_iters, fallback_at, pad0, pad1;
+  double rel_tol, fallback_factor;
 };
 
 // Camera table rows and records of storage type T widened to f64 registers.
@@ -39,6 +40,13 @@ __device__ __forceinline__ void load_cam16_d(const T* __restrict__ tab, int c, d
   load_cam16(tab, c, t);
 #pragma unroll
   for (int i = 0; i < 16; ++i) out[i] = double(t[i]);
+}
+template <typename T>
+__device__ __forceinline__ void load_fwd_d(const T* __restrict__ fw, int c, double out[kFwd]) {
+  T t[kFwd];
+  load_fwd(fw, c, t);
+#pragma unroll
+  for (int i = 0; i < kFwd; ++i) out[i] = double(t[i]);
 }
 template <typename T>
 __device__ __forceinline__ Vec4<double> to_d(const Vec4<T>& r) {
@@ -461,7 +469,11 @@ __global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const double
 // then the exact Schur complement of the Jacobian the stored records define, which keeps it
 // positive semidefinite down to the smallest damping: with f32 arithmetic, the rounding of
 // the products along the similarity gauge (where S has eigenvalue lambda) exceeds lambda once
-// lambda falls below ~1e-7 and PCG breaks down on its first direction.
+// lambda falls below ~1e-7 and PCG breaks down on its first direction. Two rounding points
+// remain, chosen so that their error in p.Sp is first order in the eliminated residual
+// e = u - Jp t (whose square p.Sp is) rather than in u: the forward table of p is stored in T
+// (both passes then see the same rounded direction), while the per-point solve t stays f64
+// (a rounded t would leave an error of order eps |u|^2 in p.Sp).
 // ---------------------------------------------------------------------------------------
 
 // ---------------------------------------------------------------------------------------
@@ -474,7 +486,8 @@ template <typename T>
 __global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const int* __restrict__ pt_len,
                          const T* __restrict__ bs, const double* __restrict__ cs, const T* __restrict__ dp,
                          double lambda,
-                         double* __restrict__ lc, double* __restrict__ mp, int* fail_count) {
+                         double* __restrict__ lc, double* __restrict__ mbj, double* __restrict__ mp,
+                         int* fail_count) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < nc) {
     if (cam_off[idx + 1] == cam_off[idx]) return;
@@ -490,6 +503,16 @@ __global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const 
     ok = ok && chol_packed<double, 9>(a);
 #pragma unroll
     for (int i = 0; i < 45; ++i) lc[45 * idx + i] = a[i];
+    // explicit (B + lambda I)^-1 = L^-T L^-1 (packed), so that applying the block-Jacobi
+    // preconditioner in PCG is one symmetric 9x9 product instead of two dependent solves
+    for (int j = 0; j < 9; ++j) {
+      double e[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) e[i] = i == j ? 1.0 : 0.0;
+      solve_lower_packed<double, 9>(a, e);
+      solve_upper_packed<double, 9>(a, e);
+      for (int i = j; i < 9; ++i) mbj[45 * idx + sym9(i, j)] = e[i];
+    }
     if (!ok) atomicAdd(fail_count, 1);
   } else if (idx < nc + np) {
     const int p = idx - nc;
@@ -534,20 +557,20 @@ __global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const 
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void forward_table(const T* __restrict__ ct, const double* x, const T* __restrict__ d,
-                                              bool scaled, double* __restrict__ fw, double z[9]) {
+                                              bool scaled, T* __restrict__ fw, double z[9]) {
 #pragma unroll
   for (int j = 0; j < 9; ++j) z[j] = scaled ? x[j] / double(d[j]) : x[j];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-    fw[i] = double(ct[tRJ(i, 0)]) * z[0] + double(ct[tRJ(i, 1)]) * z[1] + double(ct[tRJ(i, 2)]) * z[2];
+    fw[i] = T(double(ct[tRJ(i, 0)]) * z[0] + double(ct[tRJ(i, 1)]) * z[1] + double(ct[tRJ(i, 2)]) * z[2]);
 #pragma unroll
-  for (int j = 3; j < 9; ++j) fw[j] = z[j];
-  fw[9] = fw[10] = fw[11] = 0.0;
+  for (int j = 3; j < 9; ++j) fw[j] = T(z[j]);
+  fw[9] = fw[10] = fw[11] = T(0);
 }
 
 template <typename T>
 __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double* __restrict__ x,
-                                 const T* __restrict__ dc, int scaled, double* __restrict__ fw) {
+                                 const T* __restrict__ dc, int scaled, T* __restrict__ fw) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   double z[9];
@@ -579,7 +602,7 @@ struct PointArgs {
 template <typename T, int MODE>
 __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& rec,
                                           const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
-                                          const double* __restrict__ fw, const double dpx[3],
+                                          const T* __restrict__ fw, const double dpx[3],
                                           const double* __restrict__ rows_in, double* __restrict__ rows_out,
                                           double w[3], double h[6], double& uu) {
   double ct[16];
@@ -588,7 +611,7 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
   obs_geom(to_d(rec), ct, geo);
   if (MODE == kSchur) {
     double f[kFwd];
-    load_fwd(fw, c, f);
+    load_fwd_d(fw, c, f);
     double u0, u1;
     jc_apply(geo, f, u0, u1);
     double j0[3], j1[3];
@@ -605,13 +628,13 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
     }
   } else if (MODE == kRecover) {
     double f[kFwd];
-    load_fwd(fw, c, f);
+    load_fwd_d(fw, c, f);
     double u0, u1;
     jc_apply(geo, f, u0, u1);
     jpt_accumulate(geo, ct, u0, u1, w);
   } else if (MODE == kVerify || MODE == kJv) {
     double f[kFwd];
-    load_fwd(fw, c, f);
+    load_fwd_d(fw, c, f);
     double u0, u1, v0, v1;
     jc_apply(geo, f, u0, u1);
     jp_apply(geo, ct, dpx, v0, v1);
@@ -635,12 +658,15 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_points(
     PointArgs pa, const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
-    const double* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
+    const T* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
     const T* __restrict__ dpv, const double* __restrict__ delta_p_in, const double* __restrict__ rows_in,
     double lambda, double* __restrict__ t_out, double* __restrict__ delta_out, double* __restrict__ rows_out,
     double* parts, const CgState* st) {
   __shared__ double smem[kBlock / kWave];
-  if (MODE == kSchur && st != nullptr && st->done) return;
+  if (MODE == kSchur && st != nullptr) {
+    if (st->done) return;
+    lambda = st->lambda;  // inside PCG the damping travels with the state
+  }
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   double uu = 0.0, n2 = 0.0, n2s = 0.0, gd = 0.0, g2 = 0.0, g2s = 0.0;  // verify partials
@@ -749,18 +775,21 @@ template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_cameras(
     int nc, const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
     const Vec2<T>* __restrict__ res_cm, const T* __restrict__ tab, const double* __restrict__ tp,
-    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const double* __restrict__ fw,
+    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
     const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
     double* __restrict__ out_d, double* parts, const CgState* st) {
   __shared__ double ct[kCamTab];
   __shared__ double cfw[kFwd];
   __shared__ double red[9][kBlock / kWave];
-  if (MODE == cSchur && st != nullptr && st->done) return;
+  if (MODE == cSchur && st != nullptr) {
+    if (st->done) return;
+    lambda = st->lambda;
+  }
   const int c = blockIdx.x;
   if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
   if ((MODE == cVerify || MODE == cSchur || MODE == cResid) && threadIdx.x < kFwd)
-    cfw[threadIdx.x] = fw[kFwd * c + threadIdx.x];
+    cfw[threadIdx.x] = double(fw[kFwd * c + threadIdx.x]);
   __syncthreads();
   double acc[9];
 #pragma unroll
@@ -901,53 +930,38 @@ static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, d
 // ---------------------------------------------------------------------------------------
 // Block-Jacobi z = (B + lambda I)^-1 r per camera (RetainedPreconditioner, schur.rs:670-677;
 // unobserved blocks divide by lambda, block_factors.rs:193-203), or with the learned
-// correction z = L^-T G G^T L^-1 r, G packed lower (9x9, storage type T) per camera.
-template <typename T>
-__device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const double* __restrict__ lc,
-                                             const T* __restrict__ gmat, bool learned, double lambda,
-                                             const double r[9], double z[9]) {
-#pragma unroll
-  for (int i = 0; i < 9; ++i) z[i] = r[i];
+// correction z = L^-T G G^T L^-1 r. Both inverses are formed explicitly per camera (packed
+// symmetric 45) once per solve, so one application is a symmetric 9x9 product.
+__device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const double* __restrict__ minv,
+                                             double lambda, const double r[9], double z[9]) {
   if (cam_off[c + 1] > cam_off[c]) {
-    double l[45];
+    double m[45];
 #pragma unroll
-    for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
-    solve_lower_packed<double, 9>(l, z);
-    if (learned) {
-      const T* g = gmat + 45 * c;
-      double y[9];
+    for (int i = 0; i < 45; ++i) m[i] = minv[45 * c + i];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) {  // y = G^T z
-        double s = 0.0;
+    for (int i = 0; i < 9; ++i) {
+      double s = 0.0;
 #pragma unroll
-        for (int j = i; j < 9; ++j) s += double(g[sym_index(9, j, i)]) * z[j];
-        y[i] = s;
-      }
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {  // z = G y
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j <= i; ++j) s += double(g[sym_index(9, i, j)]) * y[j];
-        z[i] = s;
-      }
+      for (int j = 0; j < 9; ++j) s += m[sym9(i, j)] * r[j];
+      z[i] = s;
     }
-    solve_upper_packed<double, 9>(l, z);
-  } else if (lambda != 0.0) {
+  } else {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) z[i] = r[i] / lambda;
+    for (int i = 0; i < 9; ++i) z[i] = lambda != 0.0 ? r[i] / lambda : r[i];
   }
 }
 
 // x = 0, r = b, z = M r, p = z; per-workgroup partials r.r (col 0) and r.z (col 1).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off,
-                                                    const double* __restrict__ lc, const T* __restrict__ gmat,
-                                                    double lambda, const double* __restrict__ rhs,
-                                                    double* __restrict__ x, double* __restrict__ r,
-                                                    double* __restrict__ z, double* __restrict__ p, double* parts,
-                                                    const CgState* st) {
+                                                    const double* __restrict__ mbj,
+                                                    const double* __restrict__ mlearn,
+                                                    const double* __restrict__ rhs, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ z,
+                                                    double* __restrict__ p, double* parts, const CgState* st) {
   __shared__ double smem[kBlock / kWave];
   const bool learned = st->use_learned != 0;
+  const double lambda = st->lambda;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0;
   if (c < nc) {
@@ -958,7 +972,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restric
       x[9 * c + i] = 0.0;
       r[9 * c + i] = rv[i];
     }
-    precondition(c, cam_off, lc, gmat, learned, lambda, rv, zv);
+    precondition(c, cam_off, learned ? mlearn : mbj, lambda, rv, zv);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
@@ -983,11 +997,13 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __restrict__ cam_off,
                                                          const T* __restrict__ tab, const T* __restrict__ dc,
                                                          const double* __restrict__ zb, const double* __restrict__ z,
-                                                         double* __restrict__ p, double* __restrict__ fw,
+                                                         double* __restrict__ p, T* __restrict__ fw,
                                                          const double* __restrict__ parts, int nparts,
-                                                         const CgState* __restrict__ in, CgState* out, int max_iters,
-                                                         int fallback_at, double rel_tol, double fallback_factor) {
+                                                         const CgState* __restrict__ in, CgState* out,
+                                                         const DevScalars* __restrict__ sc) {
   __shared__ double smem[kBlock / kWave];
+  const int max_iters = sc->max_iters, fallback_at = sc->fallback_at;
+  const double rel_tol = sc->rel_tol, fallback_factor = sc->fallback_factor;
   CgState s = *in;
   if (s.done) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
@@ -1049,16 +1065,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
 // Partials: r.r (col 0), r.z (col 1) and, before a possible fallback, r.z_blockJacobi (col 2).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off,
-                                                      const double* __restrict__ lc, const T* __restrict__ gmat,
-                                                      double lambda, const double* __restrict__ p,
+                                                      const double* __restrict__ mbj,
+                                                      const double* __restrict__ mlearn,
+                                                      const double* __restrict__ p,
                                                       const double* __restrict__ q, double* __restrict__ x,
                                                       double* __restrict__ r, double* __restrict__ z,
                                                       double* __restrict__ zb, const double* __restrict__ pq_parts,
-                                                      double* parts, CgState* st, int fallback_at) {
+                                                      double* parts, CgState* st, const DevScalars* __restrict__ sc) {
   __shared__ double smem[kBlock / kWave];
   __shared__ double bcast;
   const CgState s = *st;
   if (s.done) return;
+  const double lambda = s.lambda;
+  const int fallback_at = sc->fallback_at;
   double pq = 0.0;
   for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += pq_parts[kParts * i];
   pq = block_sum(pq, smem);
@@ -1085,7 +1104,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
       rv[i] = r[9 * c + i] - alpha * q[9 * c + i];
       r[9 * c + i] = rv[i];
     }
-    precondition(c, cam_off, lc, gmat, learned, lambda, rv, zv);
+    precondition(c, cam_off, learned ? mlearn : mbj, lambda, rv, zv);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
@@ -1093,7 +1112,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
       rz += rv[i] * zv[i];
     }
     if (both) {
-      precondition(c, cam_off, lc, gmat, false, lambda, rv, zv);
+      precondition(c, cam_off, mbj, lambda, rv, zv);
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         zb[9 * c + i] = zv[i];

@@ -104,13 +104,30 @@ __device__ __forceinline__ void build_g(const float* yrow, float g[45]) {
   for (int j = 0; j < 9; ++j) g[sym_index(9, j, j)] += 1.f;
 }
 
-template <typename T>
-__global__ void k_mlp_build_g(int nc, const float* __restrict__ y, T* __restrict__ gout) {
+// The learned inverse per observed camera, M = L^-T G G^T L^-1 (packed symmetric, f64), from
+// the network output y and the block-Jacobi factor L of B + lambda I: W = L^-T G column by
+// column, then M = W W^T.
+static __global__ void k_mlp_build_g(int nc, const int* __restrict__ cam_off, const float* __restrict__ y,
+                                     const double* __restrict__ lc, double* __restrict__ mout) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+  if (c >= nc || cam_off[c + 1] == cam_off[c]) return;
   float g[45];
   build_g(y + (size_t)kOutPad * c, g);
-  for (int k = 0; k < 45; ++k) gout[45 * c + k] = T(g[k]);
+  double l[45];
+  for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
+  double w[9][9];
+  for (int j = 0; j < 9; ++j) {
+    double col[9];
+    for (int i = 0; i < 9; ++i) col[i] = i >= j ? double(g[sym_index(9, i, j)]) : 0.0;
+    solve_upper_packed<double, 9>(l, col);
+    for (int i = 0; i < 9; ++i) w[i][j] = col[i];
+  }
+  for (int j = 0; j < 9; ++j)
+    for (int i = j; i < 9; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < 9; ++k) s += w[i][k] * w[j][k];
+      mout[45 * c + sym9(i, j)] = s;
+    }
 }
 
 // Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).

@@ -146,6 +146,12 @@ fg_status fg_schur_model(fg_schur*, double* gradient, double* scales, double* ne
 fg_status fg_schur_camera_gram(fg_schur*, double* gram /* nc x 81, row-major full */);
 /* Reduced::apply (src/schur.rs:606-652): y = S x on the 9nc retained coordinates. */
 fg_status fg_schur_reduced_apply(fg_schur*, double lambda, const double* x, double* y);
+/* Measurement hook, no reference counterpart: `repeats` back-to-back launches of each of the two
+ * Schur-product kernels (point-major elimination, camera-major accumulation) for the scaled
+ * camera vector x at lambda, timed with HIP events on the backend stream; mean seconds per
+ * launch of each. */
+fg_status fg_schur_time_products(fg_schur*, double lambda, const double* x, int repeats, double* sec_points,
+                                 double* sec_cameras);
 /* Jacobian::apply / transpose_apply (src/linear_model.rs:39-87). */
 fg_status fg_schur_jacobian_apply(fg_schur*, const double* x, double* y);
 fg_status fg_schur_jacobian_transpose_apply(fg_schur*, const double* y, double* x);

@@ -183,3 +183,26 @@ def test_gauss_newton_first_step_matches_oracle():
     out = fa.GaussNewton(fa.Schur()).solve_batch(ba2, fa.OptimizeOptions(5, 1e-8, 1e-10, 1e-12))
     assert out.status in (fa.OK, fa.LINEAR_SOLVE_FAILED, fa.NO_CONVERGENCE)
     assert out.cost < out.initial_cost
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_graph_batches_match_eager_launches(precision):
+    """PCG batches replayed from the captured HIP graph (the default) and launched kernel by
+    kernel (collect_timings) produce the same steps, over several solves and damping values
+    with eager work between the replays."""
+    p = small(seed=9, nc=10, npts=500, huber=1.0)
+    runs = {}
+    for timed in (False, True):
+        ba = fa.BalProblem(p, precision=precision)
+        s = fa.Schur(precision=precision, relative_tolerance=1e-8, collect_timings=timed)
+        assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+        out = []
+        for lam in [1e-3, 1.0, 1e-6, 1e-3, 100.0, 1e-12]:
+            st, d, pr = s.solve_damped(lam)
+            assert st == fa.OK, (timed, lam, s.diagnostics().termination)
+            out.append((d, pr, s.diagnostics().iterations))
+            ba.cost()  # eager work between replays
+        runs[timed] = out
+    for (d0, p0, i0), (d1, p1, i1) in zip(runs[False], runs[True]):
+        assert i0 == i1
+        assert np.array_equal(d0, d1) and p0 == p1

@@ -142,6 +142,7 @@ def test_learned_without_network_is_rejected():
     from fagra_amd import synthetic
     p = synthetic.generate("ring16", seed=22, n_cameras=6, n_points=100)
     ba = fa.BalProblem(p, precision=fa.FP32)
-    s = fa.Schur(precision=fa.FP32, preconditioner=fa.PRECOND_LEARNED)
+    s = fa.Schur(precision=fa.FP32)  # no network attached (the constructor would attach one)
+    s.preconditioner = fa.PRECOND_LEARNED
     assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
     assert s.solve_damped(1e-2)[0] == fa.INVALID_OPTIONS
