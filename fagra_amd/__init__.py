@@ -5,6 +5,7 @@ Python mirror of the reference interfaces over the C ABI in ``include/fagra_gpu.
 * :class:`BalProblem` -- the device-resident BAL Frame batch evaluator
   (examples/bal/model.rs:194-237) with trial/accept state buffers (src/storage.rs:282-318).
 * :class:`Schur` -- the GPU Schur-PCG ``LeastSquaresBackend`` (src/schur.rs:116-546).
+* :class:`Lsmr` -- the GPU LSMR ``LeastSquaresBackend`` (src/lsmr.rs:273-595).
 * :class:`LevenbergMarquardt` / :class:`GaussNewton` -- ``solve_batch``
   (src/optimization/levenberg_marquardt.rs:252-298, src/optimization/gauss_newton.rs:63-74).
 
@@ -20,9 +21,10 @@ from ._lib import (CG_CONVERGED, CG_ITERATION_LIMIT, CG_NON_FINITE, CG_NONE, CG_
                    INVALID_OPTIONS, LINEAR_SOLVE_FAILED, NO_CONVERGENCE, NO_PROGRESS, OK, PRECOND_BLOCK_JACOBI,
                    PRECOND_LEARNED, REJ_INVALID_EVALUATION, REJ_LINEAR_SOLVE, REJ_NONE, REJ_POOR_AGREEMENT,
                    REJ_PREDICTION, TERM_COST, TERM_GRADIENT, TERM_NO_VARIABLES, TERM_NONE, TERM_STEP, FagraError,
-                   LmOptions, LmOutcome, OptimizeOptions, SchurDiagnostics, SchurTimings, check)
+                   LSMR_CONVERGED, LSMR_ITERATION_LIMIT, LSMR_NON_FINITE, LSMR_NONE, LmOptions, LmOutcome,
+                   LsmrDiagnostics, LsmrTimings, OptimizeOptions, SchurDiagnostics, SchurTimings, check)
 
-__all__ = ["BalProblem", "Schur", "LevenbergMarquardt", "GaussNewton", "OptimizeOptions", "LmOptions",
+__all__ = ["BalProblem", "Schur", "Lsmr", "LevenbergMarquardt", "GaussNewton", "OptimizeOptions", "LmOptions",
            "LmOutcome", "FagraError", "FP32", "FP64", "device_info", "lm_optimize", "gn_optimize"]
 
 
@@ -304,8 +306,117 @@ class Schur:
         return x
 
 
+class Lsmr:
+    """GPU LSMR backend with the reference's public fields (src/lsmr.rs:273-295).
+
+    Damped solves run right-preconditioned LSMR on [J D^-1; sqrt(lambda) I] (diagonal or
+    block-Jacobi preconditioning) or [J; sqrt(lambda) D] (identity); the undamped solve runs
+    identity-preconditioned LSMR on J.
+    """
+
+    _lm_entry = "fg_lm_solve_batch_lsmr"
+    _gn_entry = "fg_gn_solve_batch_lsmr"
+
+    def __init__(self, precision=FP64, max_iterations=None, relative_tolerance=None,
+                 diagonal_preconditioning=True, block_preconditioning=False, collect_timings=False,
+                 on_solve=None):
+        cfg = _lib.LsmrConfig()
+        check(lib().fg_lsmr_config_default(ctypes.byref(cfg), int(precision)), "fg_lsmr_config_default")
+        if max_iterations is not None:
+            cfg.max_iterations = max_iterations
+        if relative_tolerance is not None:
+            cfg.relative_tolerance = relative_tolerance
+        cfg.diagonal_preconditioning = int(diagonal_preconditioning)
+        cfg.block_preconditioning = int(block_preconditioning)
+        cfg.collect_timings = int(collect_timings)
+        self._on_solve = None
+        if on_solve is not None:
+            self._on_solve = _lib.ON_LSMR_SOLVE(lambda d, u: on_solve(d.contents))
+            cfg.on_solve = self._on_solve
+        self.cfg = cfg
+        handle = ctypes.c_void_p()
+        check(lib().fg_lsmr_create(ctypes.byref(cfg), ctypes.byref(handle)), "fg_lsmr_create")
+        self.h = handle.value
+        self.problem = None
+
+    def _field(name):
+        def get(self):
+            return getattr(self.cfg, name)
+
+        def put(self, v):
+            setattr(self.cfg, name, v)
+            check(lib().fg_lsmr_set_config(self.h, ctypes.byref(self.cfg)), "fg_lsmr_set_config")
+        return property(get, put)
+
+    max_iterations = _field("max_iterations")
+    relative_tolerance = _field("relative_tolerance")
+    diagonal_preconditioning = _field("diagonal_preconditioning")
+    block_preconditioning = _field("block_preconditioning")
+    del _field
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fg_lsmr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # LeastSquaresBackend mirror
+    def prepare(self, problem, dimension=None):
+        self.problem = problem
+        return lib().fg_lsmr_prepare(self.h, problem.h, problem.dimension if dimension is None else dimension)
+
+    def clear(self):
+        lib().fg_lsmr_clear(self.h)
+
+    def linearize(self):
+        return lib().fg_lsmr_linearize(self.h)
+
+    def gradient_norm(self):
+        out = ctypes.c_double()
+        st = lib().fg_lsmr_gradient_norm(self.h, ctypes.byref(out))
+        return st, out.value
+
+    def prepare_damping(self, min_column_norm):
+        return lib().fg_lsmr_prepare_damping(self.h, float(min_column_norm))
+
+    def solve_damped(self, lam):
+        ptr = ctypes.POINTER(ctypes.c_double)()
+        pred = ctypes.c_double()
+        st = lib().fg_lsmr_solve_damped(self.h, float(lam), ctypes.byref(ptr), ctypes.byref(pred))
+        if st != OK:
+            return st, None, None
+        return st, np.ctypeslib.as_array(ptr, shape=(self.problem.dimension,)).copy(), pred.value
+
+    def solve(self):
+        ptr = ctypes.POINTER(ctypes.c_double)()
+        st = lib().fg_lsmr_solve(self.h, ctypes.byref(ptr))
+        if st != OK:
+            return st, None
+        return st, np.ctypeslib.as_array(ptr, shape=(self.problem.dimension,)).copy()
+
+    def statistics(self):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        lib().fg_lsmr_statistics(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def diagnostics(self):
+        d = LsmrDiagnostics()
+        lib().fg_lsmr_diagnostics_get(self.h, ctypes.byref(d))
+        return d
+
+    def timings(self):
+        t = LsmrTimings()
+        lib().fg_lsmr_timings_get(self.h, ctypes.byref(t))
+        return t
+
+
 class LevenbergMarquardt:
-    """LevenbergMarquardt<Schur> (src/optimization/levenberg_marquardt.rs:205-317)."""
+    """LevenbergMarquardt<Schur | Lsmr> (src/optimization/levenberg_marquardt.rs:205-317)."""
 
     def __init__(self, backend):
         self.backend = backend
@@ -319,8 +430,9 @@ class LevenbergMarquardt:
         out = LmOutcome()
         cb = _lib.ON_ACCEPT(lambda st, u: self.on_accept(st.contents)) if self.on_accept else _lib.ON_ACCEPT()
         self.backend.problem = problem
-        lib().fg_lm_solve_batch(self.backend.h, problem.h, ctypes.byref(options), ctypes.byref(self.options), cb,
-                                None, ctypes.byref(out))
+        entry = getattr(lib(), getattr(self.backend, "_lm_entry", "fg_lm_solve_batch"))
+        entry(self.backend.h, problem.h, ctypes.byref(options), ctypes.byref(self.options), cb, None,
+              ctypes.byref(out))
         self._statistics = out
         return out
 
@@ -329,7 +441,7 @@ class LevenbergMarquardt:
 
 
 class GaussNewton:
-    """GaussNewton<Schur> (src/optimization/gauss_newton.rs:21-177)."""
+    """GaussNewton<Schur | Lsmr> (src/optimization/gauss_newton.rs:21-177)."""
 
     def __init__(self, backend):
         self.backend = backend
@@ -338,7 +450,8 @@ class GaussNewton:
         options = options or default_optimize_options()
         out = LmOutcome()
         self.backend.problem = problem
-        lib().fg_gn_solve_batch(self.backend.h, problem.h, ctypes.byref(options), ctypes.byref(out))
+        entry = getattr(lib(), getattr(self.backend, "_gn_entry", "fg_gn_solve_batch"))
+        entry(self.backend.h, problem.h, ctypes.byref(options), ctypes.byref(out))
         return out
 
 

@@ -51,6 +51,31 @@ class SchurConfig(ctypes.Structure):
                 ("fallback_iterations", _sz), ("on_solve", ON_SOLVE), ("on_solve_user", _VP)]
 
 
+LSMR_NONE, LSMR_CONVERGED, LSMR_ITERATION_LIMIT, LSMR_NON_FINITE = -1, 0, 1, 2
+
+
+class LsmrDiagnostics(ctypes.Structure):
+    _fields_ = [("termination", ctypes.c_int), ("iterations", _sz), ("finite_step", ctypes.c_int),
+                ("estimated_absolute_residual", _d), ("estimated_relative_residual", _d),
+                ("original_absolute_residual", _d), ("original_relative_residual", _d),
+                ("scaled_absolute_residual", _d), ("scaled_relative_residual", _d),
+                ("min_scale", _d), ("max_scale", _d), ("damping", _d), ("block_fallbacks", _sz)]
+
+
+class LsmrTimings(ctypes.Structure):
+    _fields_ = [("preparation", _d), ("factorization", _d), ("linear_solve", _d), ("forward", _d),
+                ("transpose", _d), ("preconditioner", _d), ("residual_checks", _d)]
+
+
+ON_LSMR_SOLVE = ctypes.CFUNCTYPE(None, ctypes.POINTER(LsmrDiagnostics), _VP)
+
+
+class LsmrConfig(ctypes.Structure):
+    _fields_ = [("max_iterations", _sz), ("relative_tolerance", _d), ("diagonal_preconditioning", ctypes.c_int),
+                ("block_preconditioning", ctypes.c_int), ("collect_timings", ctypes.c_int),
+                ("precision", ctypes.c_int), ("on_solve", ON_LSMR_SOLVE), ("on_solve_user", _VP)]
+
+
 class OptimizeOptions(ctypes.Structure):
     _fields_ = [("max_iterations", _sz), ("gradient_tolerance", _d), ("step_tolerance", _d),
                 ("cost_tolerance", _d)]
@@ -125,6 +150,24 @@ SIGNATURES = {
     "fg_schur_damped_transpose_apply": (ctypes.c_int, [_VP, _d, ctypes.c_int, _dp, _dp]),
     "fg_optimize_options_default": (ctypes.c_int, [ctypes.POINTER(OptimizeOptions)]),
     "fg_lm_options_default": (ctypes.c_int, [ctypes.POINTER(LmOptions)]),
+    "fg_lsmr_config_default": (ctypes.c_int, [ctypes.POINTER(LsmrConfig), ctypes.c_int]),
+    "fg_lsmr_create": (ctypes.c_int, [ctypes.POINTER(LsmrConfig), ctypes.POINTER(_VP)]),
+    "fg_lsmr_destroy": (None, [_VP]),
+    "fg_lsmr_set_config": (ctypes.c_int, [_VP, ctypes.POINTER(LsmrConfig)]),
+    "fg_lsmr_prepare": (ctypes.c_int, [_VP, _VP, _sz]),
+    "fg_lsmr_clear": (None, [_VP]),
+    "fg_lsmr_linearize": (ctypes.c_int, [_VP]),
+    "fg_lsmr_gradient_norm": (ctypes.c_int, [_VP, _dp]),
+    "fg_lsmr_prepare_damping": (ctypes.c_int, [_VP, _d]),
+    "fg_lsmr_solve_damped": (ctypes.c_int, [_VP, _d, ctypes.POINTER(_dp), _dp]),
+    "fg_lsmr_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_dp)]),
+    "fg_lsmr_diagnostics_get": (None, [_VP, ctypes.POINTER(LsmrDiagnostics)]),
+    "fg_lsmr_statistics": (None, [_VP, _szp, _szp]),
+    "fg_lsmr_timings_get": (None, [_VP, ctypes.POINTER(LsmrTimings)]),
+    "fg_lm_solve_batch_lsmr": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions), ctypes.POINTER(LmOptions),
+                                              ON_ACCEPT, _VP, ctypes.POINTER(LmOutcome)]),
+    "fg_gn_solve_batch_lsmr": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions),
+                                              ctypes.POINTER(LmOutcome)]),
     "fg_lm_solve_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions), ctypes.POINTER(LmOptions),
                                          ON_ACCEPT, _VP, ctypes.POINTER(LmOutcome)]),
     "fg_gn_solve_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions), ctypes.POINTER(LmOutcome)]),

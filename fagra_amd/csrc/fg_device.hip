@@ -18,6 +18,7 @@
 #include "fg_internal.h"
 #include "fg_kernels.h"
 #include "fg_mlp_kernels.h"
+#include "fg_lsmr_kernels.h"
 
 using namespace fg;
 
@@ -49,6 +50,9 @@ void dfree(P*& p) {
   p = nullptr;
 }
 inline int grid_for(size_t n, int block = kBlock) { return (int)((n + block - 1) / block); }
+// The per-camera PCG vector kernels run one-wave workgroups: at 1,500 cameras that is 24
+// workgroups instead of 6, and their partial-sum rows stay few enough to re-sum everywhere.
+constexpr int kCgBlock = 64;
 
 }  // namespace
 
@@ -101,7 +105,8 @@ struct fg_schur {
   fg_ba* ba = nullptr;
   size_t dimension = 0;
   bool prepared = false, linearized = false, ready = false;
-  bool use_graphs = true;  // FAGRA_PCG_GRAPHS=0 launches PCG batches kernel by kernel
+  bool use_graphs = true;
+  bool factor_failed = false;  // the last solve's block factorization failed  // FAGRA_PCG_GRAPHS=0 launches PCG batches kernel by kernel
   // Model (typed device arrays stored untyped).
   void *rec = nullptr, *res = nullptr, *rec_cm = nullptr, *res_cm = nullptr;
   void *hpp = nullptr, *gp = nullptr, *hcc = nullptr, *gc = nullptr;
@@ -113,14 +118,15 @@ struct fg_schur {
   double *d_parts = nullptr;  // max(nc, blocks) x kParts (per-camera / per-block partials)
   double *d_cgparts = nullptr;  // PCG update partials, norm partials
   void* zb = nullptr;           // block-Jacobi z at a possible fallback (T [9nc])
-  double *d_sums = nullptr;   // kParts
+  double *d_sums = nullptr;   // 4 x kParts column sums (launch_sum_parts slots)
   DevScalars* d_sc = nullptr;
   int* d_fail = nullptr;
   int* d_flags = nullptr;
   double* d_scalar = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirrors
   void* h_poll = nullptr;      // two DevScalars, pinned
-  double* h_sums = nullptr;
+  double* h_sums = nullptr;   // 4 x kParts
+  double* h_norm = nullptr;
   int* h_int = nullptr;
   std::vector<double> h_delta;  // reference order
   // statistics / diagnostics / timings
@@ -275,8 +281,7 @@ fg_status linearize_impl(fg_schur* s) {
   FG_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
                      b->d_k_slot, (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[buf],
-                     b->d_k_pslot, (const Vec2<double>*)b->d_slot_pix64, (const double*)b->d_tab64[buf],
-                     (const double*)b->d_pts[buf], b->huber, (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags);
+                     (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
@@ -285,7 +290,7 @@ fg_status linearize_impl(fg_schur* s) {
 }
 
 template <typename T>
-fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m, double* out) {
+fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m, double* out, int sums = 0) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
   const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
@@ -294,14 +299,17 @@ fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m,
   hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_cgparts, blocks, 1,
                      s->d_scalar);
   FG_TRY(hipGetLastError());
-  FG_TRY(hipMemcpyAsync(s->h_sums, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  // sums > 0: the first `sums` values of d_sums come back with the same synchronization
+  if (sums > 0)
+    FG_TRY(hipMemcpyAsync(s->h_sums, s->d_sums, sums * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(s->h_norm, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
   if (s->h_int[0] != 0) {
     *out = kNaN;
     return FG_INVALID_EVALUATION;
   }
-  *out = s->h_sums[0];
+  *out = s->h_norm[0];
   return FG_OK;
 }
 
@@ -322,21 +330,25 @@ fg_status prepare_damping_impl(fg_schur* s, double floor) {
 
 // Factor the blocks at lambda; returns the number of failed blocks through *fails.
 template <typename T>
+// fails == nullptr: the failure count stays on the device (d_fail), where the PCG's first
+// direction reads it.
 fg_status factor_impl(fg_schur* s, double lambda, int* fails) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
-  hipLaunchKernelGGL(k_factor<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (int)b->np, b->d_cam_off, b->d_pt_len, (const T*)s->bs, (const double*)s->cs, (const T*)s->dp,
-                     lambda, (double*)s->lc, (double*)s->mbj, (double*)s->mp, s->d_fail);
+  hipLaunchKernelGGL(k_factor_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
+  hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
+                     b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail);
   FG_TRY(hipGetLastError());
+  if (fails == nullptr) return FG_OK;
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
   *fails = s->h_int[0];
   return FG_OK;
 }
 
-// S x for x in scaled retained coordinates (device), into out; also the residual against rhs
-// when resid != nullptr (sums of ||Sx - rhs||^2 and ||rhs||^2 land in s->h_sums[0..1]).
+// S x for x in scaled retained coordinates (device), into out, with per-camera partials of
+// ||Sx - resid_rhs||^2 and ||resid_rhs||^2 in d_parts (resid_rhs == nullptr: against zero).
 template <typename T>
 fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out, const double* resid_rhs) {
   fg_ba* b = s->ba;
@@ -350,14 +362,10 @@ fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out
   return FG_OK;
 }
 
-template <typename T>
-fg_status sum_parts(fg_schur* s, int rows) {
-  fg_ba* b = s->ba;
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_parts, rows, s->d_sums);
-  FG_TRY(hipGetLastError());
-  FG_TRY(hipMemcpyAsync(s->h_sums, s->d_sums, kParts * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipStreamSynchronize(b->stream));
-  return FG_OK;
+// Column sums of the (rows x kParts) partials in d_parts into d_sums[slot * kParts ...].
+void launch_sum_parts(fg_schur* s, int rows, int slot) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s->ba->stream, (const double*)s->d_parts, rows,
+                     s->d_sums + slot * kParts);
 }
 
 // PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
@@ -373,12 +381,13 @@ template <typename T>
 void launch_cg_iteration(fg_schur* s, int i, bool timed) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc;
-  const int cblocks = grid_for(nc);
+  const int cblocks = grid_for(nc, kCgBlock);
   CgState* cur = &s->d_sc->st[(i + 1) & 1];
-  hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+  hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
                      (double*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
-                     (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], (const DevScalars*)s->d_sc);
+                     (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], (const DevScalars*)s->d_sc,
+                     (const int*)s->d_fail);
   const bool ev = timed && s->events_used + 2 <= s->events.size();
   if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, 0.0, nullptr, nullptr, nullptr, cur);
@@ -388,7 +397,7 @@ void launch_cg_iteration(fg_schur* s, int i, bool timed) {
     (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
     s->events_used += 2;
   }
-  hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const double*)s->mbj, (const double*)s->gmat, (const double*)s->p, (const double*)s->q,
                      (double*)s->x, (double*)s->r, (double*)s->z, (double*)s->zb, (const double*)s->d_parts,
                      s->d_cgparts, cur, (const DevScalars*)s->d_sc);
@@ -426,7 +435,7 @@ template <typename T>
 fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc;
-  const int cblocks = grid_for(nc);
+  const int cblocks = grid_for(nc, kCgBlock);
   const int max_it = (int)s->cfg.max_iterations;
   const int fallback_at = (int)std::min<size_t>(s->cfg.fallback_iterations, (size_t)std::numeric_limits<int>::max());
   DevScalars init{};
@@ -441,7 +450,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   init.rel_tol = s->cfg.relative_tolerance;
   init.fallback_factor = s->cfg.fallback_factor;
   FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
-  hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const double*)s->mbj, (const double*)s->gmat, (const double*)s->rhs, (double*)s->x,
                      (double*)s->r, (double*)s->z, (double*)s->p, s->d_cgparts, (const CgState*)&s->d_sc->st[0]);
   FG_TRY(hipGetLastError());
@@ -470,14 +479,16 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
     ++batch;
   }
   // the last update's convergence / exhaustion decision
-  hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+  hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
                      (double*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
-                     (const DevScalars*)s->d_sc);
+                     (const DevScalars*)s->d_sc, (const int*)s->d_fail);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(s->h_int + 1, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
+  s->factor_failed = s->h_int[1] != 0;
   const CgState& h = s->h_sc->st[(launched + 1) & 1];
   s->iterations += (size_t)std::max(h.iters, 0);
   s->diag.iterations = (size_t)std::max(h.iters, 0);
@@ -502,12 +513,8 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   const int nc = (int)b->nc;
   {
     Phase ph(s, &s->tim.factorization);
-    int fails = 0;
-    FG_CHECK(factor_impl<T>(s, lambda, &fails));
-    if (fails != 0) {
-      s->diag.termination = FG_CG_NOT_POSITIVE_DEFINITE;
-      return FG_LINEAR_SOLVE_FAILED;
-    }
+    // a failed block is reported by the PCG (NotPositiveDefinite before its first iteration)
+    FG_CHECK(factor_impl<T>(s, lambda, nullptr));
     // rhs = Dc^-1 (-gc + sum Jc^T Jp M gp)
     launch_points<T, kRhs>(s, nullptr, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
     launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (double*)s->rhs,
@@ -519,10 +526,10 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
     Phase ph(s, &s->tim.preconditioner);
     if (!s->mlp) return FG_INVALID_OPTIONS;
     // per-camera features -> network -> G = I + s tril(y) -> explicit L^-T G G^T L^-1
-    hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                       (const T*)s->hcc, lambda_d, s->mx, (float*)nullptr);
+    hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc * kInPad)), dim3(kBlock), 0, b->stream, nc,
+                       b->d_cam_off, (const T*)s->hcc, lambda_d, s->mx, (float*)nullptr);
     mlp_forward(s->mlp, b->stream, pad64(nc), s->mx, s->my);
-    hipLaunchKernelGGL(k_mlp_build_g, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+    hipLaunchKernelGGL(k_mlp_build_g, dim3((nc + 3) / 4), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                        (const float*)s->my, (const double*)s->lc, (double*)s->gmat);
     FG_TRY(hipGetLastError());
   }
@@ -530,6 +537,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
     Phase ph(s, &s->tim.linear_solve);
     FG_CHECK(cg_impl<T>(s, lambda, learned));
   }
+  if (s->factor_failed) return FG_LINEAR_SOLVE_FAILED;  // NotPositiveDefinite, no iteration
   Phase ph(s, &s->tim.verification);
   // recover_step (schur.rs:312-344)
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
@@ -539,31 +547,33 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)(9 * b->nc),
                      (const double*)s->x, (const T*)s->dc, s->d_delta);
   FG_TRY(hipGetLastError());
-  // verify_step (schur.rs:346-397): reduced residual
+  // verify_step (schur.rs:346-397), every reduction launched first and read back once:
+  // reduced residual ||S x - rhs||, then the full-model verification in original coordinates
+  // (linear_model.rs:254-282), then ||delta||_inf
   FG_CHECK(schur_product<T>(s, (const double*)s->x, lambda, (double*)s->q, (const double*)s->rhs));
-  FG_CHECK(sum_parts<T>(s, nc));
-  const double resid2 = s->h_sums[0], rhs2 = s->h_sums[1];
-  s->diag.reduced_relative_residual = rhs2 == 0.0 ? std::sqrt(resid2) : std::sqrt(resid2 / rhs2);
-  // full-model verification in original coordinates (linear_model.rs:254-282)
+  launch_sum_parts(s, nc, 0);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
                      (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
                             s->d_parts, nullptr);
-  FG_CHECK(sum_parts<T>(s, pblocks));
-  double pt[kParts];
-  std::memcpy(pt, s->h_sums, sizeof(pt));
+  launch_sum_parts(s, pblocks, 1);
   launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, nullptr, s->d_delta,
                              nullptr, lambda, nullptr, nullptr);
-  FG_CHECK(sum_parts<T>(s, nc));
-  const double* cm = s->h_sums;
-  // pt: n2, n2s, g.d, g2, g2s, ||J delta||^2 ; cm: n2, n2s, g.d, g2, g2s
+  launch_sum_parts(s, nc, 2);
+  FG_TRY(hipGetLastError());
+  double step_norm = 0.0;
+  const fg_status nst = norm_inf_impl<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0, &step_norm,
+                                              3 * kParts);
+  if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
+  const double* rs = s->h_sums;                // ||Sx - rhs||^2, ||rhs||^2
+  const double* pt = s->h_sums + kParts;       // n2, n2s, g.d, g2, g2s, ||J delta||^2 (points)
+  const double* cm = s->h_sums + 2 * kParts;   // n2, n2s, g.d, g2, g2s (cameras)
+  const double resid2 = rs[0], rhs2 = rs[1];
+  s->diag.reduced_relative_residual = rhs2 == 0.0 ? std::sqrt(resid2) : std::sqrt(resid2 / rhs2);
   const double nr2 = pt[0] + cm[0], nr2s = pt[1] + cm[1], gd = pt[2] + cm[2], g2 = pt[3] + cm[3],
                g2s = pt[4] + cm[4], jd2 = pt[5];
   const double pred = -gd - 0.5 * jd2;
-  double step_norm = 0.0;
-  const fg_status nst = norm_inf_impl<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0, &step_norm);
-  if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
   const bool finite_step = nst == FG_OK;
   s->diag.original_relative_residual = g2 == 0.0 ? std::sqrt(nr2) : std::sqrt(nr2 / g2);
   s->diag.scaled_relative_residual = g2s == 0.0 ? std::sqrt(nr2s) : std::sqrt(nr2s / g2s);
@@ -618,7 +628,7 @@ fg_status alloc_model(fg_schur* s) {
   const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   const size_t rows = std::max({nc, pblocks, (size_t)1024});
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
-  FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc))));
+  FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc, kCgBlock))));
   FG_TRY(dalloc((double**)&s->zb, 9 * nc));
   // The memsets above run on the null stream; complete them before stream work starts.
   FG_TRY(hipDeviceSynchronize());
@@ -782,11 +792,11 @@ fg_status capture_impl(fg_schur* s, double lambda_d, float* features, float* tar
   int fails = 0;
   FG_CHECK(factor_impl<T>(s, lambda_d, &fails));
   if (fails) return FG_LINEAR_SOLVE_FAILED;
-  hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                     (const T*)s->hcc, lambda_d, s->mx, s->mxf);
+  hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc * kInPad)), dim3(kBlock), 0, b->stream, nc,
+                     b->d_cam_off, (const T*)s->hcc, lambda_d, s->mx, s->mxf);
   hipLaunchKernelGGL(k_schur_diagonal<T>, dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                      b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const T*)b->d_tab[b->cur], (const double*)s->mp,
-                     (const T*)s->bs, (const T*)s->dc, (const double*)s->lc, lambda, s->mtarget);
+                     (const T*)s->dp, (const T*)s->bs, (const T*)s->dc, (const double*)s->lc, lambda, s->mtarget);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(features, s->mxf, (size_t)nc * kIn * 4, hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(targets, s->mtarget, (size_t)nc * 45 * 4, hipMemcpyDeviceToHost, b->stream));
@@ -1119,12 +1129,13 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
   };
   if (dalloc(&s->d_sc, 1) != hipSuccess || dalloc(&s->d_fail, 4) != hipSuccess ||
       dalloc(&s->d_flags, 4) != hipSuccess || dalloc(&s->d_scalar, 4) != hipSuccess ||
-      dalloc(&s->d_sums, kParts) != hipSuccess)
+      dalloc(&s->d_sums, 4 * kParts) != hipSuccess)
     return fail();
   if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail();
   if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc(&s->h_poll, 2 * sizeof(DevScalars), 0) != hipSuccess ||
-      hipHostMalloc((void**)&s->h_sums, 2 * kParts * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_sums, 4 * kParts * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_norm, 2 * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_int, 8 * sizeof(int), 0) != hipSuccess)
     return fail();
   if (hipEventCreate(&s->ph_a) != hipSuccess || hipEventCreate(&s->ph_b) != hipSuccess ||
@@ -1147,6 +1158,7 @@ void fg_schur_destroy(fg_schur* s) {
   if (s->h_sc) (void)hipHostFree(s->h_sc);
   if (s->h_poll) (void)hipHostFree(s->h_poll);
   if (s->h_sums) (void)hipHostFree(s->h_sums);
+  if (s->h_norm) (void)hipHostFree(s->h_norm);
   if (s->h_int) (void)hipHostFree(s->h_int);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : s->poll_ev)
@@ -1363,6 +1375,450 @@ fg_status fg_schur_damped_transpose_apply(fg_schur* s, double lambda, int scaled
   return FG_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// LSMR backend (src/lsmr.rs). The model, scales and block factors are those of an owned
+// fg_schur over the same problem; the LSMR iteration runs on the device.
+// ------------------------------------------------------------------------------------------
+struct fg_lsmr {
+  fg_lsmr_config cfg;
+  fg_schur* s = nullptr;
+  int prepared_mode = -1;  // 0 identity, 1 diagonal, 2 block-Jacobi; -1 none
+  fg_lsmr_diagnostics diag;
+  size_t solves = 0, iterations = 0;
+  fg_lsmr_timings tim;
+  size_t n = 0, rows = 0;  // coordinates, Jacobian rows (2 no)
+  double *d_u = nullptr, *d_v = nullptr, *d_ws = nullptr, *d_zt = nullptr, *d_h = nullptr, *d_hbar = nullptr,
+         *d_y = nullptr, *d_up = nullptr, *d_vp = nullptr;
+  LsmrState* d_st = nullptr;
+  LsmrState* h_st = nullptr;
+  std::vector<double> h_delta;
+};
+
+namespace {
+
+void lsmr_reset_diag(fg_lsmr* l) {
+  l->diag = fg_lsmr_diagnostics{FG_LSMR_NONE, 0, 0, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, 0};
+}
+
+int lsmr_requested_mode(const fg_lsmr* l) {
+  return l->cfg.block_preconditioning ? 2 : (l->cfg.diagonal_preconditioning ? 1 : 0);
+}
+
+void lsmr_free_vectors(fg_lsmr* l) {
+  for (double** v : {&l->d_u, &l->d_v, &l->d_ws, &l->d_zt, &l->d_h, &l->d_hbar, &l->d_y, &l->d_up, &l->d_vp}) {
+    dfree(*v);
+    *v = nullptr;
+  }
+  l->n = l->rows = 0;
+}
+
+fg_status lsmr_alloc_vectors(fg_lsmr* l) {
+  fg_ba* b = l->s->ba;
+  const size_t n = 9 * b->nc + 3 * b->np, rows = 2 * b->no;
+  if (l->n == n && l->rows == rows && l->d_u) return FG_OK;
+  lsmr_free_vectors(l);
+  const size_t m = rows + n;
+  const size_t ublocks = (size_t)std::min(4096, grid_for(m));
+  const size_t vblocks = (size_t)grid_for(b->nc + b->np);
+  FG_TRY(dalloc(&l->d_u, m));
+  FG_TRY(dalloc(&l->d_v, n));
+  FG_TRY(dalloc(&l->d_ws, n));
+  FG_TRY(dalloc(&l->d_zt, n));
+  FG_TRY(dalloc(&l->d_h, n));
+  FG_TRY(dalloc(&l->d_hbar, n));
+  FG_TRY(dalloc(&l->d_y, n));
+  FG_TRY(dalloc(&l->d_up, std::max<size_t>(ublocks, 1024)));
+  FG_TRY(dalloc(&l->d_vp, std::max<size_t>(vblocks, 1024)));
+  l->n = n;
+  l->rows = rows;
+  l->h_delta.assign(n, 0.0);
+  return FG_OK;
+}
+
+// A J^T u product into zt (cameras then point slots) from the top rows of U.
+template <typename T>
+void lsmr_transpose_rows(fg_lsmr* l) {
+  fg_schur* s = l->s;
+  fg_ba* b = s->ba;
+  launch_points<T, kJtu>(s, (const T*)s->fw, nullptr, l->d_u, 0.0, l->d_zt + 9 * b->nc, nullptr, nullptr, nullptr);
+  launch_cameras<T, cJtu>(s, nullptr, l->d_u, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, l->d_zt);
+}
+
+// One LSMR iteration: ws = P v / (alpha D); U = A ws - (alpha/beta) U; beta; V = P^T A^T U /
+// beta - (beta/alpha) V; alpha; recurrences; h, hbar, y.
+template <typename T>
+void lsmr_iteration(fg_lsmr* l, int mode, bool damped) {
+  fg_schur* s = l->s;
+  fg_ba* b = s->ba;
+  const int nc = (int)b->nc, np = (int)b->np;
+  const int nblk = nc + np;
+  const int n_bottom = damped ? (int)l->n : 0;
+  const int m = (int)l->rows + n_bottom;
+  const int ublocks = std::min(4096, grid_for(m));
+  const int vblocks = grid_for(nblk);
+  const bool scaled = mode != 0;
+  hipLaunchKernelGGL(k_lsmr_pv<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_v,
+                     (const double*)s->lc, (const double*)s->mp, (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0,
+                     scaled ? 1 : 0, 0, (const LsmrState*)l->d_st, l->d_ws);
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
+                     (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw);
+  launch_points<T, kJv>(s, (const T*)s->fw, l->d_ws + 9 * nc, nullptr, 0.0, nullptr, s->d_rows, nullptr, nullptr);
+  hipLaunchKernelGGL(k_lsmr_u<T>, dim3(ublocks), dim3(kBlock), 0, b->stream, (int)l->rows, nc, n_bottom,
+                     (const double*)s->d_rows, (const double*)l->d_ws, (const T*)s->dc, (const T*)s->dp,
+                     (const LsmrState*)l->d_st, l->d_u, l->d_up);
+  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, ublocks, 0, l->d_st);
+  lsmr_transpose_rows<T>(l);
+  hipLaunchKernelGGL(k_lsmr_v<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_zt,
+                     (const double*)(l->d_u + l->rows), (const double*)s->lc, (const double*)s->mp,
+                     (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0, scaled ? 1 : 0, damped ? 1 : 0, 0,
+                     (const LsmrState*)l->d_st, l->d_v, l->d_vp);
+  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_vp, vblocks, 1, l->d_st);
+  hipLaunchKernelGGL(k_lsmr_scalar, dim3(1), dim3(64), 0, b->stream, l->d_st, 0);
+  hipLaunchKernelGGL(k_lsmr_h, dim3(grid_for(l->n)), dim3(kBlock), 0, b->stream, (int)l->n, (const double*)l->d_v,
+                     l->d_st, 0, l->d_h, l->d_hbar, l->d_y);
+  hipLaunchKernelGGL(k_lsmr_close, dim3(1), dim3(64), 0, b->stream, l->d_st);
+}
+
+// The LSMR solve of src/lsmr.rs:215-280 on the device: x = P y into s->d_delta in solver
+// coordinates (the caller recovers original coordinates). Returns the run's facts.
+template <typename T>
+fg_status lsmr_run(fg_lsmr* l, int mode, bool damped, double lambda, bool* converged, size_t* iters,
+                   double* abs_res, double* rel_res) {
+  fg_schur* s = l->s;
+  fg_ba* b = s->ba;
+  const int nc = (int)b->nc, np = (int)b->np;
+  const int nblk = nc + np;
+  const int n_bottom = damped ? (int)l->n : 0;
+  const int m = (int)l->rows + n_bottom;
+  const int ublocks = std::min(4096, grid_for(m));
+  const int vblocks = grid_for(nblk);
+  LsmrState init{};
+  init.max_iters = (int)std::min<size_t>(l->cfg.max_iterations, (size_t)std::numeric_limits<int>::max());
+  init.rel_tol = l->cfg.relative_tolerance;
+  init.root = std::sqrt(lambda);
+  FG_TRY(hipMemcpyAsync(l->d_st, &init, sizeof(LsmrState), hipMemcpyHostToDevice, b->stream));
+  // beta_1 u_1 = b; alpha_1 v_1 = P^T A^T u_1
+  hipLaunchKernelGGL(k_lsmr_init_u<T>, dim3(ublocks), dim3(kBlock), 0, b->stream, (int)b->no, n_bottom,
+                     (const Vec2<T>*)s->res_cm, l->d_u, l->d_up);
+  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, ublocks, 0, l->d_st);
+  lsmr_transpose_rows<T>(l);
+  hipLaunchKernelGGL(k_lsmr_v<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_zt,
+                     (const double*)(l->d_u + l->rows), (const double*)s->lc, (const double*)s->mp,
+                     (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0, mode != 0 ? 1 : 0, damped ? 1 : 0, 1,
+                     (const LsmrState*)l->d_st, l->d_v, l->d_vp);
+  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_vp, vblocks, 1, l->d_st);
+  hipLaunchKernelGGL(k_lsmr_scalar, dim3(1), dim3(64), 0, b->stream, l->d_st, 1);
+  hipLaunchKernelGGL(k_lsmr_h, dim3(grid_for(l->n)), dim3(kBlock), 0, b->stream, (int)l->n, (const double*)l->d_v,
+                     l->d_st, 1, l->d_h, l->d_hbar, l->d_y);
+  FG_TRY(hipGetLastError());
+  // iterations in batches; the host reads the previous batch's state while the next runs
+  const int kLsmrBatch = 8;
+  int launched = 0, batch = 0;
+  bool stop = false;
+  const int max_it = init.max_iters;
+  while (!stop && launched < max_it) {
+    for (int i = 0; i < kLsmrBatch; ++i) lsmr_iteration<T>(l, mode, damped);
+    FG_TRY(hipGetLastError());
+    launched += kLsmrBatch;
+    const int slot = batch & 1;
+    FG_TRY(hipMemcpyAsync(&l->h_st[1 + slot], l->d_st, sizeof(LsmrState), hipMemcpyDeviceToHost, b->stream));
+    FG_TRY(hipEventRecord(s->poll_ev[slot], b->stream));
+    if (batch > 0) {
+      FG_TRY(hipEventSynchronize(s->poll_ev[slot ^ 1]));
+      if (l->h_st[1 + (slot ^ 1)].done) stop = true;
+    }
+    ++batch;
+  }
+  // x = P y (solver coordinates)
+  hipLaunchKernelGGL(k_lsmr_pv<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_y,
+                     (const double*)s->lc, (const double*)s->mp, (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0,
+                     0, 1, (const LsmrState*)l->d_st, s->d_delta);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(&l->h_st[0], l->d_st, sizeof(LsmrState), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  const LsmrState& h = l->h_st[0];
+  *converged = h.converged != 0;
+  *iters = (size_t)h.iters;
+  *abs_res = h.abs_res;
+  *rel_res = h.norm_atb > 0.0 ? h.abs_res / h.norm_atb : 0.0;
+  return FG_OK;
+}
+
+// solve (src/lsmr.rs:215-280) + recover_and_verify (:430-468) for a damped attempt.
+template <typename T>
+fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted) {
+  fg_schur* s = l->s;
+  fg_ba* b = s->ba;
+  const int mode = l->prepared_mode;
+  const int nc = (int)b->nc, np = (int)b->np;
+  const int nblk = nc + np;
+  l->diag.damping = lambda;
+  // min / max scale
+  {
+    const int rb = std::min(1024, grid_for(l->n));
+    hipLaunchKernelGGL(k_scale_range<T>, dim3(rb), dim3(kBlock), 0, b->stream, 9 * nc, 3 * np, (const T*)s->dc,
+                       (const T*)s->dp, s->d_parts);
+    std::vector<double> parts(kParts * rb);
+    FG_TRY(hipMemcpyAsync(parts.data(), s->d_parts, parts.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          b->stream));
+    FG_TRY(hipStreamSynchronize(b->stream));
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    for (int i = 0; i < rb; ++i) {
+      lo = std::min(lo, parts[kParts * i]);
+      hi = std::max(hi, parts[kParts * i + 1]);
+    }
+    l->diag.min_scale = lo;
+    l->diag.max_scale = hi;
+    const double root = std::sqrt(lambda);
+    if (!std::isfinite(mode != 0 ? root : root * hi)) {
+      l->diag.termination = FG_LSMR_NON_FINITE;
+      return FG_LINEAR_SOLVE_FAILED;
+    }
+  }
+  if (mode == 2) {
+    Phase ph(s, &l->tim.factorization);
+    int fails = 0;
+    FG_CHECK(factor_impl<T>(s, lambda, &fails));
+    l->diag.block_fallbacks = (size_t)fails;
+  }
+  bool converged = false;
+  size_t iters = 0;
+  double abs_res = kNaN, rel_res = kNaN;
+  {
+    Phase ph(s, &l->tim.linear_solve);
+    l->solves += 1;
+    FG_CHECK(lsmr_run<T>(l, mode, true, lambda, &converged, &iters, &abs_res, &rel_res));
+  }
+  l->iterations += iters;
+  l->diag.iterations = iters;
+  l->diag.estimated_absolute_residual = abs_res;
+  l->diag.estimated_relative_residual = rel_res;
+  Phase ph(s, &l->tim.residual_checks);
+  // recover: step = x / D (scaled modes) in s->d_delta, then verify_step in original
+  // coordinates with the normal residual written out for the block-coordinate norm
+  if (mode != 0)
+    hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(l->n)), dim3(kBlock), 0, b->stream, (int)(9 * nc),
+                       (const double*)s->d_delta, (const T*)s->dc, s->d_delta);
+  if (mode != 0)
+    hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(3 * np)), dim3(kBlock), 0, b->stream, 3 * np,
+                       (const double*)(s->d_delta + 9 * nc), (const T*)s->dp, s->d_delta + 9 * nc);
+  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+  const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * nc, nullptr, lambda, l->d_zt + 9 * nc, nullptr,
+                            s->d_parts, nullptr);
+  launch_sum_parts(s, pblocks, 1);
+  launch_cameras<T, cVerify>(s, s->d_delta + 9 * nc, nullptr, (const T*)s->fw, nullptr, s->d_delta, nullptr, lambda,
+                             nullptr, l->d_zt);
+  launch_sum_parts(s, nc, 2);
+  if (mode == 2) {
+    const int sb = std::min(1024, grid_for(nblk));
+    hipLaunchKernelGGL(k_lsmr_scaled_residual<T>, dim3(sb), dim3(kBlock), 0, b->stream, nc, np,
+                       (const double*)l->d_zt, (const T*)s->gc, (const T*)s->gp, (const double*)s->lc,
+                       (const double*)s->mp, (const T*)s->dc, (const T*)s->dp, s->d_parts);
+    launch_sum_parts(s, sb, 3);
+  }
+  FG_TRY(hipGetLastError());
+  double step_norm = 0.0;
+  const fg_status nst = norm_inf_impl<double>(s, s->d_delta, l->n, nullptr, 0, &step_norm, 4 * kParts);
+  if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
+  const bool finite_step = nst == FG_OK;
+  const double* pt = s->h_sums + kParts;      // points: n2, n2s, g.d, g2, g2s, ||J delta||^2
+  const double* cm = s->h_sums + 2 * kParts;  // cameras: n2, n2s, g.d, g2, g2s
+  const double* bk = s->h_sums + 3 * kParts;  // block coordinates: ||L^-1 D^-1 nr||^2, ||L^-1 D^-1 g||^2
+  const double nr2 = pt[0] + cm[0], gd = pt[2] + cm[2], g2 = pt[3] + cm[3], jd2 = pt[5];
+  double sn2, sg2;
+  if (mode == 0) {
+    sn2 = nr2;
+    sg2 = g2;
+  } else if (mode == 1) {
+    sn2 = pt[1] + cm[1];
+    sg2 = pt[4] + cm[4];
+  } else {
+    sn2 = bk[0];
+    sg2 = bk[1];
+  }
+  auto relative = [](double a, double r) { return r == 0.0 ? a : a / r; };  // linear_model.rs relative()
+  const double rn = std::sqrt(nr2), gn = std::sqrt(g2), sn = std::sqrt(sn2), sg = std::sqrt(sg2);
+  l->diag.finite_step = finite_step ? 1 : 0;
+  l->diag.original_absolute_residual = rn;
+  l->diag.original_relative_residual = relative(rn, gn);
+  l->diag.scaled_absolute_residual = sn;
+  l->diag.scaled_relative_residual = relative(sn, sg);
+  const bool finite = finite_step && std::isfinite(abs_res) && std::isfinite(rel_res);
+  l->diag.termination = finite ? (converged ? FG_LSMR_CONVERGED : FG_LSMR_ITERATION_LIMIT) : FG_LSMR_NON_FINITE;
+  s->last_step_norm = finite_step ? step_norm : kNaN;
+  if (!finite_step || !std::isfinite(rn) || !std::isfinite(sn)) {
+    l->diag.termination = FG_LSMR_NON_FINITE;
+    return FG_LINEAR_SOLVE_FAILED;
+  }
+  *predicted = -gd - 0.5 * jd2;
+  return l->diag.termination == FG_LSMR_CONVERGED ? FG_OK : FG_LINEAR_SOLVE_FAILED;
+}
+
+// Undamped solve (src/lsmr.rs:535-552): identity LSMR on J, rhs -r, step in s->d_delta.
+template <typename T>
+fg_status lsmr_solve_impl(fg_lsmr* l) {
+  fg_schur* s = l->s;
+  bool converged = false;
+  size_t iters = 0;
+  double abs_res = kNaN, rel_res = kNaN;
+  l->solves += 1;
+  FG_CHECK(lsmr_run<T>(l, 0, false, 0.0, &converged, &iters, &abs_res, &rel_res));
+  l->iterations += iters;
+  l->diag.iterations = iters;
+  l->diag.estimated_absolute_residual = abs_res;
+  l->diag.estimated_relative_residual = rel_res;
+  double step_norm = 0.0;
+  const fg_status nst = norm_inf_impl<double>(s, s->d_delta, l->n, nullptr, 0, &step_norm);
+  if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
+  l->diag.finite_step = nst == FG_OK ? 1 : 0;
+  const bool finite = l->diag.finite_step && std::isfinite(abs_res) && std::isfinite(rel_res);
+  l->diag.termination = finite ? (converged ? FG_LSMR_CONVERGED : FG_LSMR_ITERATION_LIMIT) : FG_LSMR_NON_FINITE;
+  s->last_step_norm = l->diag.finite_step ? step_norm : kNaN;
+  return l->diag.termination == FG_LSMR_CONVERGED ? FG_OK : FG_LINEAR_SOLVE_FAILED;
+}
+
+// solve_damped (src/lsmr.rs:560-595) with the step left on the device.
+fg_status lsmr_solve_damped_device(fg_lsmr* l, double lambda, double* pred) {
+  lsmr_reset_diag(l);
+  l->s->step_valid = false;
+  const int mode = l->prepared_mode;
+  if (mode < 0 || mode != lsmr_requested_mode(l) || !std::isfinite(lambda) || lambda < 0.0)
+    return FG_INVALID_OPTIONS;
+  const fg_status st = l->cfg.precision == FG_FP32 ? lsmr_solve_damped_impl<float>(l, lambda, pred)
+                                                   : lsmr_solve_damped_impl<double>(l, lambda, pred);
+  if (l->cfg.on_solve && l->diag.termination != FG_LSMR_NONE) l->cfg.on_solve(&l->diag, l->cfg.on_solve_user);
+  if (st != FG_OK) return st;
+  l->s->step_valid = true;
+  return FG_OK;
+}
+
+// solve (src/lsmr.rs:513-534) with the step left on the device.
+fg_status lsmr_solve_device(fg_lsmr* l) {
+  lsmr_reset_diag(l);
+  l->s->step_valid = false;
+  if (!l->s->linearized) return FG_INVALID_OPTIONS;
+  const fg_status st = l->cfg.precision == FG_FP32 ? lsmr_solve_impl<float>(l) : lsmr_solve_impl<double>(l);
+  if (l->cfg.on_solve && l->diag.termination != FG_LSMR_NONE) l->cfg.on_solve(&l->diag, l->cfg.on_solve_user);
+  if (st != FG_OK) return st;
+  l->s->step_valid = true;
+  return FG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fg_status fg_lsmr_config_default(fg_lsmr_config* c, int precision) {
+  if (precision != FG_FP64 && precision != FG_FP32) return FG_INVALID_OPTIONS;
+  c->max_iterations = 1000;
+  // max(1e-10, 128 eps) of the scalar (src/lsmr.rs:299-300); the solve runs in f64
+  c->relative_tolerance = 1e-10;
+  c->diagonal_preconditioning = 1;
+  c->block_preconditioning = 0;
+  c->collect_timings = 0;
+  c->precision = precision;
+  c->on_solve = nullptr;
+  c->on_solve_user = nullptr;
+  return FG_OK;
+}
+
+fg_status fg_lsmr_create(const fg_lsmr_config* cfg, fg_lsmr** out) {
+  *out = nullptr;
+  if (cfg->precision != FG_FP64 && cfg->precision != FG_FP32) return FG_INVALID_OPTIONS;
+  fg_lsmr* l = new fg_lsmr();
+  l->cfg = *cfg;
+  lsmr_reset_diag(l);
+  std::memset(&l->tim, 0, sizeof(l->tim));
+  fg_schur_config sc;
+  fg_schur_config_default(&sc, cfg->precision);
+  sc.collect_timings = cfg->collect_timings;
+  fg_status st = fg_schur_create(&sc, &l->s);
+  if (st != FG_OK || dalloc(&l->d_st, 1) != hipSuccess ||
+      hipHostMalloc((void**)&l->h_st, 3 * sizeof(LsmrState), 0) != hipSuccess) {
+    fg_lsmr_destroy(l);
+    return st != FG_OK ? st : FG_DEVICE_ERROR;
+  }
+  *out = l;
+  return FG_OK;
+}
+
+void fg_lsmr_destroy(fg_lsmr* l) {
+  if (!l) return;
+  if (l->s && l->s->ba) (void)hipStreamSynchronize(l->s->ba->stream);
+  lsmr_free_vectors(l);
+  dfree(l->d_st);
+  if (l->h_st) (void)hipHostFree(l->h_st);
+  if (l->s) fg_schur_destroy(l->s);
+  delete l;
+}
+
+fg_status fg_lsmr_set_config(fg_lsmr* l, const fg_lsmr_config* cfg) {
+  if (cfg->precision != l->cfg.precision) return FG_INVALID_OPTIONS;
+  l->cfg = *cfg;
+  l->s->cfg.collect_timings = cfg->collect_timings;
+  return FG_OK;
+}
+
+fg_status fg_lsmr_prepare(fg_lsmr* l, fg_ba* b, size_t dimension) {
+  l->solves = l->iterations = 0;
+  std::memset(&l->tim, 0, sizeof(l->tim));
+  lsmr_reset_diag(l);
+  l->prepared_mode = -1;
+  if (l->cfg.max_iterations == 0 || !(l->cfg.relative_tolerance >= 0.0 && l->cfg.relative_tolerance < 1.0))
+    return FG_INVALID_OPTIONS;
+  FG_CHECK(fg_schur_prepare(l->s, b, dimension));
+  return lsmr_alloc_vectors(l);
+}
+
+void fg_lsmr_clear(fg_lsmr* l) {
+  l->prepared_mode = -1;
+  fg_schur_clear(l->s);
+}
+
+fg_status fg_lsmr_linearize(fg_lsmr* l) {
+  l->prepared_mode = -1;
+  return fg_schur_linearize(l->s);
+}
+
+fg_status fg_lsmr_gradient_norm(fg_lsmr* l, double* out) { return fg_schur_gradient_norm(l->s, out); }
+
+fg_status fg_lsmr_prepare_damping(fg_lsmr* l, double min_column_norm) {
+  l->prepared_mode = -1;
+  {
+    Phase ph(l->s, &l->tim.preparation);
+    FG_CHECK(fg_schur_prepare_damping(l->s, min_column_norm));
+  }
+  l->prepared_mode = lsmr_requested_mode(l);
+  return FG_OK;
+}
+
+fg_status fg_lsmr_solve_damped(fg_lsmr* l, double lambda, const double** delta, double* predicted) {
+  double pred = kNaN;
+  FG_CHECK(lsmr_solve_damped_device(l, lambda, &pred));
+  FG_CHECK(download_delta(l->s, l->h_delta.data()));
+  if (delta) *delta = l->h_delta.data();
+  if (predicted) *predicted = pred;
+  return FG_OK;
+}
+
+fg_status fg_lsmr_solve(fg_lsmr* l, const double** delta) {
+  FG_CHECK(lsmr_solve_device(l));
+  FG_CHECK(download_delta(l->s, l->h_delta.data()));
+  if (delta) *delta = l->h_delta.data();
+  return FG_OK;
+}
+
+void fg_lsmr_diagnostics_get(const fg_lsmr* l, fg_lsmr_diagnostics* d) { *d = l->diag; }
+void fg_lsmr_statistics(const fg_lsmr* l, size_t* solves, size_t* iterations) {
+  if (solves) *solves = l->solves;
+  if (iterations) *iterations = l->iterations;
+}
+void fg_lsmr_timings_get(const fg_lsmr* l, fg_lsmr_timings* t) { *t = l->tim; }
+
 // ------------------------------------------------------------------------------------------
 // LM / GN on the device problem
 // ------------------------------------------------------------------------------------------
@@ -1434,6 +1890,75 @@ fg_status fg_gn_solve_batch(fg_schur* s, fg_ba* b, const fg_optimize_options* op
   if (!b || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
   FG_CHECK(bind(s, b));
   fg_problem pb = device_problem(s);
+  fg_gn_optimize(&pb, options, out);
+  return (fg_status)out->status;
+}
+
+// The same optimizers over the LSMR backend: cost, staging and acceptance are those of the
+// bound problem; the linear model and damped steps are the LSMR backend's.
+static int lcb_prepare(void* c) {
+  fg_lsmr* l = (fg_lsmr*)c;
+  l->s->ba->trial_active = false;
+  return fg_lsmr_prepare(l, l->s->ba, 9 * l->s->ba->nc + 3 * l->s->ba->np);
+}
+static int lcb_cost(void* c, int trial, double* out) { return fg_ba_cost(((fg_lsmr*)c)->s->ba, trial, out); }
+static int lcb_linearize(void* c) { return fg_lsmr_linearize((fg_lsmr*)c); }
+static int lcb_gradient_norm(void* c, double* out) { return fg_lsmr_gradient_norm((fg_lsmr*)c, out); }
+static int lcb_prepare_damping(void* c, double f) { return fg_lsmr_prepare_damping((fg_lsmr*)c, f); }
+static int lcb_solve_damped(void* c, double lambda, double* norm, double* pred) {
+  fg_lsmr* l = (fg_lsmr*)c;
+  fg_status st = lsmr_solve_damped_device(l, lambda, pred);
+  if (st == FG_OK) *norm = l->s->last_step_norm;
+  return st;
+}
+static int lcb_solve(void* c, double* norm) {
+  fg_lsmr* l = (fg_lsmr*)c;
+  FG_CHECK(lsmr_solve_device(l));
+  *norm = l->s->last_step_norm;
+  return FG_OK;
+}
+static int lcb_stage(void* c) { return cb_stage(((fg_lsmr*)c)->s); }
+static void lcb_accept(void* c) {
+  fg_lsmr* l = (fg_lsmr*)c;
+  l->prepared_mode = -1;
+  cb_accept(l->s);
+}
+static void lcb_reject(void* c) { cb_reject(((fg_lsmr*)c)->s); }
+static void lcb_work(void* c, size_t* solves, size_t* iterations) { fg_lsmr_statistics((fg_lsmr*)c, solves, iterations); }
+
+static fg_problem lsmr_problem(fg_lsmr* l) {
+  fg_problem pb;
+  pb.ctx = l;
+  pb.dimension = 9 * l->s->ba->nc + 3 * l->s->ba->np;
+  pb.prepare = lcb_prepare;
+  pb.cost = lcb_cost;
+  pb.linearize = lcb_linearize;
+  pb.gradient_norm = lcb_gradient_norm;
+  pb.prepare_damping = lcb_prepare_damping;
+  pb.solve_damped = lcb_solve_damped;
+  pb.solve = lcb_solve;
+  pb.stage = lcb_stage;
+  pb.accept = lcb_accept;
+  pb.reject = lcb_reject;
+  pb.linear_work = lcb_work;
+  return pb;
+}
+
+fg_status fg_lm_solve_batch_lsmr(fg_lsmr* l, fg_ba* b, const fg_optimize_options* options, const fg_lm_options* lm,
+                                 fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out) {
+  if (!b || b->precision != l->cfg.precision) return FG_INVALID_OPTIONS;
+  FG_CHECK(bind(l->s, b));
+  FG_CHECK(lsmr_alloc_vectors(l));
+  fg_problem pb = lsmr_problem(l);
+  fg_lm_optimize(&pb, options, lm, on_accept, user, out);
+  return (fg_status)out->status;
+}
+
+fg_status fg_gn_solve_batch_lsmr(fg_lsmr* l, fg_ba* b, const fg_optimize_options* options, fg_lm_outcome* out) {
+  if (!b || b->precision != l->cfg.precision) return FG_INVALID_OPTIONS;
+  FG_CHECK(bind(l->s, b));
+  FG_CHECK(lsmr_alloc_vectors(l));
+  fg_problem pb = lsmr_problem(l);
   fg_gn_optimize(&pb, options, out);
   return (fg_status)out->status;
 }

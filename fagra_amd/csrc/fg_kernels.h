@@ -309,21 +309,14 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
     const int* __restrict__ cam_off, const int* __restrict__ k_slot, const Vec4<T>* __restrict__ rec,
-    const Vec2<T>* __restrict__ res, const T* __restrict__ tab, const int* __restrict__ k_pslot,
-    const Vec2<double>* __restrict__ slot_pix, const double* __restrict__ tab64, const double* __restrict__ pts,
-    double huber, Vec4<T>* __restrict__ rec_cm, Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc,
-    T* __restrict__ gc, int* flags) {
-  // A T = f64 record is the exact geometry; a narrower one gives Hcc, while the gradient is
-  // re-formed from the f64 state, whose Jacobian error would otherwise move the minimum.
-  constexpr bool kExact = sizeof(T) == sizeof(double);
+    const Vec2<T>* __restrict__ res, const T* __restrict__ tab, Vec4<T>* __restrict__ rec_cm,
+    Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc, T* __restrict__ gc, int* flags) {
+  // The weighted residual was formed in f64 and rounded once to T (relative error eps of the
+  // residual itself), so the gradient needs only f64 accumulation here.
   __shared__ T ct[kCamTab];
-  __shared__ double cd[kCamTab];
   __shared__ double red[54][kBlock / kWave];
   const int c = blockIdx.x;
-  if (threadIdx.x < kCamTab) {
-    ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
-    if (!kExact) cd[threadIdx.x] = tab64[kCamTab * c + threadIdx.x];
-  }
+  if (threadIdx.x < kCamTab) ct[threadIdx.x] = tab[kCamTab * c + threadIdx.x];
   __syncthreads();
   T acc[45];
   double gacc[9];
@@ -350,23 +343,8 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
 #pragma unroll
       for (int b = a; b < 9; ++b) acc[sym9(b, a)] += jc[a] * jc[b] + jc[9 + a] * jc[9 + b];
     }
-    if (kExact) {
 #pragma unroll
-      for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
-    } else {
-      const int ps = k_pslot[k];
-      double px, py, qx, qy, z, cost, sw;
-      project(cd, pts[3 * ps], pts[3 * ps + 1], pts[3 * ps + 2], px, py, qx, qy, z);
-      const Vec2<double> ob = slot_pix[s];
-      const double rx = px - ob.x, ry = py - ob.y;
-      huber_weight(rx, ry, huber, cost, sw);
-      ObsGeom<double> g64;
-      obs_geom(Vec4<double>{qx, qy, z, sw}, cd, g64);
-      double j64[18];
-      jc_block(g64, cd, j64);
-#pragma unroll
-      for (int a = 0; a < 9; ++a) gacc[a] += j64[a] * (sw * rx) + j64[9 + a] * (sw * ry);
-    }
+    for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
   }
   if (bad) atomicOr(flags, 1);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -477,78 +455,98 @@ __global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const double
 // ---------------------------------------------------------------------------------------
 
 // ---------------------------------------------------------------------------------------
-// BlockFactors::factor (block_factors.rs:113-138): Cholesky of B + lambda I per camera and
-// of C + lambda I per point, in f64. Point blocks are stored as the unscaled inverse
-// M = (Hpp + lambda D^2)^-1 = D^-1 (C + lambda I)^-1 D^-1 (packed 6) used by elimination.
-// Unobserved blocks are skipped (no Gram block exists for them in the reference).
+// BlockFactors::factor (block_factors.rs:113-138), point blocks: the f64 Cholesky factor of
+// C + lambda I (packed 6), applied by substitution (point_solve). Solving with the factor is
+// backward stable, so the eliminated projection I - Jp (Hpp + lambda D^2)^-1 Jp^T stays
+// positive semidefinite to rounding; an explicit inverse is off by eps * cond(Hpp), which at
+// lambda = 1e-12 is enough to stall a 1e-10 PCG on the gauge directions. Unobserved blocks
+// are skipped (no Gram block exists for them in the reference): a zero factor, zero solve.
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__global__ void k_factor(int nc, int np, const int* __restrict__ cam_off, const int* __restrict__ pt_len,
-                         const T* __restrict__ bs, const double* __restrict__ cs, const T* __restrict__ dp,
-                         double lambda,
-                         double* __restrict__ lc, double* __restrict__ mbj, double* __restrict__ mp,
-                         int* fail_count) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < nc) {
-    if (cam_off[idx + 1] == cam_off[idx]) return;
-    double a[45];
-    bool ok = true;
+__global__ void k_factor_points(int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
+                                double lambda, double* __restrict__ lp, int* fail_count) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double* l = lp + 6 * p;
+  if (pt_len[p] == 0) {
 #pragma unroll
-    for (int i = 0; i < 45; ++i) {
-      a[i] = double(bs[45 * idx + i]);
-      ok = ok && isfinite(a[i]);
-    }
+    for (int i = 0; i < 6; ++i) l[i] = 0.0;
+    return;
+  }
+  double a[6];
+  bool ok = true;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) a[sym9(j, j)] += lambda;
-    ok = ok && chol_packed<double, 9>(a);
+  for (int i = 0; i < 6; ++i) {
+    a[i] = cs[6 * p + i];
+    ok = ok && isfinite(a[i]);
+  }
+  a[0] += lambda; a[3] += lambda; a[5] += lambda;
+  ok = ok && chol_packed<double, 3>(a);
 #pragma unroll
-    for (int i = 0; i < 45; ++i) lc[45 * idx + i] = a[i];
-    // explicit (B + lambda I)^-1 = L^-T L^-1 (packed), so that applying the block-Jacobi
-    // preconditioner in PCG is one symmetric 9x9 product instead of two dependent solves
-    for (int j = 0; j < 9; ++j) {
-      double e[9];
+  for (int i = 0; i < 6; ++i) l[i] = ok ? a[i] : 0.0;  // zero factor: failed (see cameras)
+  if (!ok) atomicAdd(fail_count, 1);
+}
+
+// t = (Hpp + lambda D^2)^-1 v = D^-1 (C + lambda I)^-1 D^-1 v with the stored factor of point i.
+template <typename T>
+__device__ __forceinline__ void point_solve(const double* __restrict__ lp, const T* __restrict__ dpv, int i,
+                                            const double v[3], double t[3]) {
+  double l[6];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) e[i] = i == j ? 1.0 : 0.0;
-      solve_lower_packed<double, 9>(a, e);
-      solve_upper_packed<double, 9>(a, e);
-      for (int i = j; i < 9; ++i) mbj[45 * idx + sym9(i, j)] = e[i];
-    }
-    if (!ok) atomicAdd(fail_count, 1);
-  } else if (idx < nc + np) {
-    const int p = idx - nc;
-    double* m = mp + 6 * p;
-    if (pt_len[p] == 0) {
+  for (int k = 0; k < 6; ++k) l[k] = lp[6 * i + k];
+  if (l[0] == 0.0) {  // unobserved point
+    t[0] = t[1] = t[2] = 0.0;
+    return;
+  }
+  double d[3];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) m[i] = 0.0;
-      return;
-    }
-    double a[6];
-    bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    d[a] = double(dpv[3 * i + a]);
+    t[a] = v[a] / d[a];
+  }
+  solve_lower_packed<double, 3>(l, t);
+  solve_upper_packed<double, 3>(l, t);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      a[i] = cs[6 * p + i];
-      ok = ok && isfinite(a[i]);
-    }
-    a[0] += lambda; a[3] += lambda; a[5] += lambda;
-    ok = ok && chol_packed<double, 3>(a);
-    // inverse of C + lambda I from its factor: columns of L^-T L^-1
-    double inv[6];
+  for (int a = 0; a < 3; ++a) t[a] /= d[a];
+}
+
+// Camera blocks of BlockFactors::factor: nine lanes per camera each factor B + lambda I in
+// registers (the same arithmetic, so the same factor) and solve one column of its explicit
+// inverse; lane 0 stores the factor and counts a failure.
+template <typename T>
+__global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const T* __restrict__ bs, double lambda,
+                                 double* __restrict__ lc, double* __restrict__ mbj, int* fail_count) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = gid / 9, j = gid % 9;
+  if (c >= nc) return;
+  if (cam_off[c + 1] == cam_off[c]) {  // unobserved: no block; a zero factor marks it
+    if (j == 0)
+      for (int i = 0; i < 45; ++i) lc[45 * c + i] = 0.0;
+    return;
+  }
+  double a[45];
+  bool ok = true;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double e[3] = {0.0, 0.0, 0.0};
-      e[j] = 1.0;
-      solve_lower_packed<double, 3>(a, e);
-      solve_upper_packed<double, 3>(a, e);
+  for (int i = 0; i < 45; ++i) {
+    a[i] = double(bs[45 * c + i]);
+    ok = ok && isfinite(a[i]);
+  }
 #pragma unroll
-      for (int i = j; i < 3; ++i) inv[sym3(i, j)] = e[i];
-    }
-    const double d[3] = {double(dp[3 * p]), double(dp[3 * p + 1]), double(dp[3 * p + 2])};
+  for (int k = 0; k < 9; ++k) a[sym9(k, k)] += lambda;
+  ok = ok && chol_packed<double, 9>(a);
+  if (j == 0) {
+    // a failed block stores a zero factor: Schur reports it, LSMR keeps diagonal treatment
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int i = j; i < 3; ++i) m[sym3(i, j)] = (inv[sym3(i, j)] / d[i]) / d[j];
+    for (int i = 0; i < 45; ++i) lc[45 * c + i] = ok ? a[i] : 0.0;
     if (!ok) atomicAdd(fail_count, 1);
   }
+  // column j of (B + lambda I)^-1 = L^-T L^-1 e_j, lower part only
+  double e[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) e[i] = i == j ? 1.0 : 0.0;
+  solve_lower_packed<double, 9>(a, e);
+  solve_upper_packed<double, 9>(a, e);
+  for (int i = j; i < 9; ++i) mbj[45 * c + sym9(i, j)] = e[i];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -579,11 +577,11 @@ __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double
 
 // ---------------------------------------------------------------------------------------
 // Point-major half of the Schur products (one lane per point, SELL chunks).
-//   kSchur:   t_p = (Hpp + lambda Dp^2)^-1 sum_k Jp^T Jc z, with Hpp re-formed from the same
-//             records in f64 and factored in registers, so the elimination is exact for the
-//             stored Jacobian (the eliminated-block solve of E^T z)
-//   kRecover: delta_p = -M_p (gp + sum_k Jp^T Jc z)     (Schur::recover_step, schur.rs:312-344)
-//   kRhs:     t_p = M_p gp                      (factor_and_reduce_rhs, schur.rs:204-247)
+//   kSchur:   t_p = (Hpp + lambda Dp^2)^-1 sum_k Jp^T Jc z (the eliminated-block solve of
+//             E^T z), with Hpp re-formed in f64 from the same records and factored in registers
+//   kRecover: delta_p = -(Hpp + lambda Dp^2)^-1 (gp + sum_k Jp^T Jc z)   (Schur::recover_step,
+//             schur.rs:312-344)
+//   kRhs:     t_p = (Hpp + lambda Dp^2)^-1 gp   (factor_and_reduce_rhs, schur.rs:204-247)
 //   kVerify:  normal residual Jp^T (J delta + r) + lambda Dp^2 delta_p and ||J delta||^2
 //   kJv:      rows 2k,2k+1 of J x               (Jacobian::apply, linear_model.rs:39-55)
 //   kJtu:     x_p = sum_k Jp^T u_k              (Jacobian::transpose_apply, :64-87)
@@ -699,6 +697,9 @@ __global__ void __launch_bounds__(kBlock) k_points(
         }
       }
       if (MODE == kSchur) {
+        // Hpp + lambda Dp^2 re-formed from the same rows and factored in registers: the
+        // eliminated projection is then exact for the Jp of this pass (the stored factor of
+        // the same block, scaled and rounded differently, slowed tight PCG solves 2x)
         double t[3] = {w[0], w[1], w[2]};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -713,7 +714,6 @@ __global__ void __launch_bounds__(kBlock) k_points(
         }
         reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
       } else if (MODE == kRhs || MODE == kRecover) {
-        const double* m = mp + 6 * i;
         double v[3];
         if (MODE == kRhs) {
           v[0] = double(gp[3 * i]); v[1] = double(gp[3 * i + 1]); v[2] = double(gp[3 * i + 2]);
@@ -721,8 +721,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
           v[0] = double(gp[3 * i]) + w[0]; v[1] = double(gp[3 * i + 1]) + w[1]; v[2] = double(gp[3 * i + 2]) + w[2];
         }
         double t[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) t[a] = m[sym3(a, 0)] * v[0] + m[sym3(a, 1)] * v[1] + m[sym3(a, 2)] * v[2];
+        point_solve(mp, dpv, i, v, t);
         if (MODE == kRecover) {
 #pragma unroll
           for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -t[a];
@@ -734,6 +733,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
         for (int a = 0; a < 3; ++a) {
           const double d = double(dpv[3 * i + a]);
           const double nr = w[a] + (lambda * (d * dpx[a])) * d;
+          if (delta_out) delta_out[3 * i + a] = nr;  // the normal residual itself, on request
           n2 += nr * nr;
           n2s += (nr / d) * (nr / d);
           const double g = double(gp[3 * i + a]);
@@ -888,6 +888,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
       for (int i = 0; i < 9; ++i) {
         const double dd = d[i];
         const double nr = y[i] + (lambda * (dd * delta_c[9 * c + i])) * dd;
+        if (out_d) out_d[9 * c + i] = nr;  // the normal residual itself, on request
         part0 += nr * nr;
         part1 += (nr / dd) * (nr / dd);
         const double g = double(gc[9 * c + i]);
@@ -914,13 +915,24 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
 // Column sums of a (rows x kParts) partial array in fixed order (deterministic).
 // ---------------------------------------------------------------------------------------
 static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* __restrict__ out) {
-  __shared__ double smem[kBlock / kWave];
+  __shared__ double smem[kParts][kBlock / kWave];
+  double s[kParts];
+#pragma unroll
+  for (int col = 0; col < kParts; ++col) s[col] = 0.0;
+  for (int i = threadIdx.x; i < rows; i += blockDim.x)
+#pragma unroll
+    for (int col = 0; col < kParts; ++col) s[col] += parts[kParts * i + col];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
   for (int col = 0; col < kParts; ++col) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < rows; i += blockDim.x) s += parts[kParts * i + col];
-    s = block_sum(s, smem);
-    if (threadIdx.x == 0) out[col] = s;
-    __syncthreads();
+    const double v = wave_sum(s[col]);
+    if (lane == 0) smem[col][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kParts) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += smem[threadIdx.x][w];
+    out[threadIdx.x] = t;
   }
 }
 
@@ -1000,7 +1012,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
                                                          double* __restrict__ p, T* __restrict__ fw,
                                                          const double* __restrict__ parts, int nparts,
                                                          const CgState* __restrict__ in, CgState* out,
-                                                         const DevScalars* __restrict__ sc) {
+                                                         const DevScalars* __restrict__ sc,
+                                                         const int* __restrict__ factor_fails) {
   __shared__ double smem[kBlock / kWave];
   const int max_iters = sc->max_iters, fallback_at = sc->fallback_at;
   const double rel_tol = sc->rel_tol, fallback_factor = sc->fallback_factor;
@@ -1021,7 +1034,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
     s.target = fallback_factor * fallback_factor * rr;
     s.rz = rz;
     s.iters = 0;
-    if (rr <= s.tol2) {
+    if (*factor_fails != 0) {  // a block of B + lambda I or C + lambda I did not factor
+      s.done = 1;
+      s.term = 2;  // NotPositiveDefinite
+    } else if (rr <= s.tol2) {
       s.done = 1;
       s.term = 0;  // Converged
     }

@@ -20,35 +20,37 @@ __host__ __device__ inline void upper_index(int k, int& i, int& j) {
 
 // Features of one camera block: for the 45 upper-triangle entries of the unnormalized Gram
 // block, log(H_ii) on the diagonal and the correlation H_ij / sqrt(H_ii H_jj) off it; then
-// log(observations + 1) and log10(lambda), all scaled to O(1).
+// log(observations + 1) and log10(lambda), all scaled to O(1). One lane per (camera, feature).
 template <typename T>
 __global__ void k_mlp_features(int nc, const int* __restrict__ cam_off, const T* __restrict__ hcc, double lambda,
                                uint16_t* __restrict__ x, float* __restrict__ xf) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = gid / kInPad, k = gid % kInPad;
   if (c >= nc) return;
   const T* h = hcc + 45 * c;
-  float f[kInPad];
-  for (int k = 0; k < kInPad; ++k) f[k] = 0.f;
-  for (int k = 0; k < 45; ++k) {
+  float f = 0.f;
+  if (k < 45) {
     int i, j;
     upper_index(k, i, j);
     const double hii = double(h[sym9(i, i)]), hjj = double(h[sym9(j, j)]);
     if (i == j) {
-      f[k] = float(log(fmax(hii, 1e-30)) / 8.0);
+      f = float(log(fmax(hii, 1e-30)) / 8.0);
     } else {
       const double d = sqrt(hii * hjj);
-      f[k] = d > 0.0 ? float(double(h[sym9(i, j)]) / d) : 0.f;
+      f = d > 0.0 ? float(double(h[sym9(i, j)]) / d) : 0.f;
     }
+  } else if (k == 45) {
+    f = float(log(double(cam_off[c + 1] - cam_off[c]) + 1.0) / 8.0);
+  } else if (k == 46) {
+    f = float(log10(fmax(lambda, 1e-30)) / 8.0);
   }
-  f[45] = float(log(double(cam_off[c + 1] - cam_off[c]) + 1.0) / 8.0);
-  f[46] = float(log10(fmax(lambda, 1e-30)) / 8.0);
-  for (int k = 0; k < kInPad; ++k) x[(size_t)kInPad * c + k] = f2bf(f[k]);
-  if (xf)
-    for (int k = 0; k < kIn; ++k) xf[(size_t)kIn * c + k] = f[k];
+  x[(size_t)kInPad * c + k] = f2bf(f);
+  if (xf && k < kIn) xf[(size_t)kIn * c + k] = f;
 }
 
-// Fused three-layer forward for 16 rows per wave. Z1/H1/Z2/H2 are stored when non-null
-// (training); Y receives the f32 output rows (kOutPad wide).
+// Fused three-layer forward for 16 rows per wave, one to four waves per workgroup (small
+// inference batches launch one-wave workgroups to spread over more CUs). Z1/H1/Z2/H2 are
+// stored when non-null (training); Y receives the f32 output rows (kOutPad wide).
 static __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w1,
                                                      const float* __restrict__ b1, const uint16_t* __restrict__ w2,
                                                      const float* __restrict__ b2, const uint16_t* __restrict__ w3,
@@ -58,7 +60,7 @@ static __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* __re
   __shared__ uint16_t stage[2][4][16][kHidden];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
-  const size_t m0 = ((size_t)blockIdx.x * 4 + wave) * 16;
+  const size_t m0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + wave) * 16;
   // layer 1
   for (int nt = 0; nt < kHidden / 16; ++nt) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -105,29 +107,39 @@ __device__ __forceinline__ void build_g(const float* yrow, float g[45]) {
 }
 
 // The learned inverse per observed camera, M = L^-T G G^T L^-1 (packed symmetric, f64), from
-// the network output y and the block-Jacobi factor L of B + lambda I: W = L^-T G column by
-// column, then M = W W^T.
-static __global__ void k_mlp_build_g(int nc, const int* __restrict__ cam_off, const float* __restrict__ y,
-                                     const double* __restrict__ lc, double* __restrict__ mout) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc || cam_off[c + 1] == cam_off[c]) return;
-  float g[45];
-  build_g(y + (size_t)kOutPad * c, g);
-  double l[45];
-  for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
-  double w[9][9];
-  for (int j = 0; j < 9; ++j) {
-    double col[9];
-    for (int i = 0; i < 9; ++i) col[i] = i >= j ? double(g[sym_index(9, i, j)]) : 0.0;
+// the network output y and the block-Jacobi factor L of B + lambda I. One 64-lane group per
+// camera: lanes 0-8 form column j of W = L^-T G (a triangular solve each), lanes 0-44 then
+// form one packed entry of M = W W^T.
+static __global__ void __launch_bounds__(256) k_mlp_build_g(int nc, const int* __restrict__ cam_off,
+                                                            const float* __restrict__ y,
+                                                            const double* __restrict__ lc,
+                                                            double* __restrict__ mout) {
+  __shared__ double w[4][9][9];
+  const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + grp;
+  const bool active = c < nc && cam_off[c + 1] > cam_off[c];
+  if (active && lane < 9) {
+    const int j = lane;
+    const float* yr = y + (size_t)kOutPad * c;
+    double l[45], col[9];
+    for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
+    for (int i = 0; i < 9; ++i)  // column j of G = I + s tril(y)
+      col[i] = i < j ? 0.0 : double(kOutScale * yr[sym_index(9, i, j)]) + (i == j ? 1.0 : 0.0);
     solve_upper_packed<double, 9>(l, col);
-    for (int i = 0; i < 9; ++i) w[i][j] = col[i];
+    for (int i = 0; i < 9; ++i) w[grp][i][j] = col[i];
   }
-  for (int j = 0; j < 9; ++j)
-    for (int i = j; i < 9; ++i) {
-      double s = 0.0;
-      for (int k = 0; k < 9; ++k) s += w[i][k] * w[j][k];
-      mout[45 * c + sym9(i, j)] = s;
+  __syncthreads();
+  if (active && lane < 45) {
+    int j = 0, k = lane;
+    while (k >= 9 - j) {  // packed lower, column-major: (i, j) with i >= j
+      k -= 9 - j;
+      ++j;
     }
+    const int i = j + k;
+    double s = 0.0;
+    for (int t = 0; t < 9; ++t) s += w[grp][i][t] * w[grp][j][t];
+    mout[45 * c + sym9(i, j)] = s;
+  }
 }
 
 // Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).
@@ -270,7 +282,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __restrict__ cam_off,
                                                            const int* __restrict__ k_pslot,
                                                            const Vec4<T>* __restrict__ rec_cm, const T* __restrict__ tab,
-                                                           const double* __restrict__ mp, const T* __restrict__ bs,
+                                                           const double* __restrict__ mp, const T* __restrict__ dpv,
+                                                           const T* __restrict__ bs,
                                                            const T* __restrict__ dc, const double* __restrict__ lc,
                                                            T lambda, float* __restrict__ out) {
   __shared__ T ct[kCamTab];
@@ -287,17 +300,19 @@ __global__ void __launch_bounds__(kBlock) k_schur_diagonal(int nc, const int* __
     T jc[18];
     jc_block(geo, ct, jc);
     const int ps = k_pslot[k];
-    const double* m = mp + 6 * ps;
-    // Jp (2x3), K2 = Jp M Jp^T (2x2)
+    // Jp (2x3), K2 = Jp M Jp^T (2x2) with M = (Hpp + lambda D^2)^-1 from the point's factor
     T jp[2][3];
     for (int a = 0; a < 3; ++a) {
       jp[0][a] = geo.sw * (geo.dp0[0] * ct[tR(0, a)] + geo.dp0[1] * ct[tR(1, a)] + geo.dp0[2] * ct[tR(2, a)]);
       jp[1][a] = geo.sw * (geo.dp1[0] * ct[tR(0, a)] + geo.dp1[1] * ct[tR(1, a)] + geo.dp1[2] * ct[tR(2, a)]);
     }
     T mj[2][3];
-    for (int row = 0; row < 2; ++row)
-      for (int a = 0; a < 3; ++a)
-        mj[row][a] = T(m[sym3(a, 0)]) * jp[row][0] + T(m[sym3(a, 1)]) * jp[row][1] + T(m[sym3(a, 2)]) * jp[row][2];
+    for (int row = 0; row < 2; ++row) {
+      const double v[3] = {double(jp[row][0]), double(jp[row][1]), double(jp[row][2])};
+      double t[3];
+      point_solve(mp, dpv, ps, v, t);
+      for (int a = 0; a < 3; ++a) mj[row][a] = T(t[a]);
+    }
     T k2[2][2];
     for (int u = 0; u < 2; ++u)
       for (int w = 0; w < 2; ++w) k2[u][w] = jp[u][0] * mj[w][0] + jp[u][1] * mj[w][1] + jp[u][2] * mj[w][2];

@@ -184,6 +184,62 @@ fg_status fg_schur_set_mlp(fg_schur*, fg_mlp*);
  * the target A = L^-1 S_cc L^-T (nc x 45 packed) and the observation count. */
 fg_status fg_schur_capture(fg_schur*, double lambda, float* features, float* targets, int* counts);
 
+/* ---- LSMR backend (the reference's second drop-in, src/lsmr.rs:301-595) ----
+ * Same device model and kernels as the Schur backend; the LSMR iteration (faer 0.24.4's
+ * matrix_free::lsmr, Fong & Saunders 2011) runs on the device with its scalar recurrences in
+ * device state. Vectors are f64; the Jacobian products use the backend's precision. */
+typedef struct fg_lsmr fg_lsmr;
+/* LsmrTermination (src/lsmr.rs:60-68) */
+typedef enum {
+  FG_LSMR_NONE = -1,
+  FG_LSMR_CONVERGED = 0,
+  FG_LSMR_ITERATION_LIMIT = 1,
+  FG_LSMR_NON_FINITE = 2
+} fg_lsmr_termination;
+/* LsmrDiagnostics (src/lsmr.rs:72-105); NaN = None. */
+typedef struct {
+  int termination; /* fg_lsmr_termination */
+  size_t iterations;
+  int finite_step;
+  double estimated_absolute_residual, estimated_relative_residual;
+  double original_absolute_residual, original_relative_residual;
+  double scaled_absolute_residual, scaled_relative_residual;
+  double min_scale, max_scale, damping;
+  size_t block_fallbacks;
+} fg_lsmr_diagnostics;
+/* LsmrTimings (src/lsmr.rs:40-57), seconds. */
+typedef struct {
+  double preparation, factorization, linear_solve, forward, transpose, preconditioner, residual_checks;
+} fg_lsmr_timings;
+typedef void (*fg_lsmr_observer_fn)(const fg_lsmr_diagnostics*, void* user);
+/* Lsmr public fields (src/lsmr.rs:273-295) + precision. */
+typedef struct {
+  size_t max_iterations;          /* default 1000 */
+  double relative_tolerance;      /* default max(1e-10, 128 eps) */
+  int diagonal_preconditioning;   /* default 1 */
+  int block_preconditioning;      /* default 0 */
+  int collect_timings;            /* default 0 */
+  int precision;                  /* fg_precision of the Jacobian products */
+  fg_lsmr_observer_fn on_solve;
+  void* on_solve_user;
+} fg_lsmr_config;
+
+fg_status fg_lsmr_config_default(fg_lsmr_config*, int precision);
+fg_status fg_lsmr_create(const fg_lsmr_config*, fg_lsmr** out);
+void fg_lsmr_destroy(fg_lsmr*);
+fg_status fg_lsmr_set_config(fg_lsmr*, const fg_lsmr_config*);
+/* LeastSquaresBackend for Lsmr (src/lsmr.rs:471-595) over a device problem. */
+fg_status fg_lsmr_prepare(fg_lsmr*, fg_ba*, size_t dimension);
+void fg_lsmr_clear(fg_lsmr*);
+fg_status fg_lsmr_linearize(fg_lsmr*);
+fg_status fg_lsmr_gradient_norm(fg_lsmr*, double* out);
+fg_status fg_lsmr_prepare_damping(fg_lsmr*, double min_column_norm);
+fg_status fg_lsmr_solve_damped(fg_lsmr*, double lambda, const double** delta, double* predicted);
+fg_status fg_lsmr_solve(fg_lsmr*, const double** delta);
+void fg_lsmr_diagnostics_get(const fg_lsmr*, fg_lsmr_diagnostics*);
+void fg_lsmr_statistics(const fg_lsmr*, size_t* solves, size_t* iterations);
+void fg_lsmr_timings_get(const fg_lsmr*, fg_lsmr_timings*);
+
 /* ---- nonlinear optimizers ---- */
 
 /* OptimizeOptions (src/optimization.rs:39-67) */
@@ -224,6 +280,10 @@ fg_status fg_lm_solve_batch(fg_schur*, fg_ba*, const fg_optimize_options*, const
                             fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out);
 /* GaussNewton::<Schur>::solve_batch on the device problem. */
 fg_status fg_gn_solve_batch(fg_schur*, fg_ba*, const fg_optimize_options*, fg_lm_outcome* out);
+/* The same loops with the LSMR backend (LevenbergMarquardt::<Lsmr> / GaussNewton::<Lsmr>). */
+fg_status fg_lm_solve_batch_lsmr(fg_lsmr*, fg_ba*, const fg_optimize_options*, const fg_lm_options*,
+                                 fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out);
+fg_status fg_gn_solve_batch_lsmr(fg_lsmr*, fg_ba*, const fg_optimize_options*, fg_lm_outcome* out);
 
 /* The same LM/GN loops over caller-supplied state and backend callbacks. */
 typedef struct {

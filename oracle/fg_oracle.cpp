@@ -810,6 +810,205 @@ double orc_schur_verify(orc_schur* s, const double* step, double lambda, double*
 }  // extern "C"
 
 // ---------------------------------------------------------------------------------------
+// LSMR backend over the same linearized model: src/lsmr.rs:215-280 (solve), :430-468
+// (recover_and_verify), :560-595 (solve_damped), :535-552 (undamped solve). The iteration is
+// faer 0.24.4's `matrix_free::lsmr` (a dependency absent from the reference tree), restated
+// from its published algorithm: Fong & Saunders, "LSMR: an iterative algorithm for sparse
+// least-squares problems", SIAM J. Sci. Comput. 33(5), 2011, Algorithm LSMR without local
+// reorthogonalization, x0 = 0, right preconditioner P (solver coordinates y, x = P y), stopped
+// when the normal-residual estimate |zeta_bar| <= tol * ||(A P)^T b||, which is the relative
+// estimate the reference reports (src/lsmr.rs:82-85).
+// ---------------------------------------------------------------------------------------
+namespace {
+double nrm2(const std::vector<double>& v) {
+  double s = 0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+// mode: 0 identity [J; sqrt(l) D], 1 diagonal [J D^-1; sqrt(l) I], 2 block (1 plus P = L^-T);
+// undamped: operator J alone, rhs of length rows.
+struct LsmrOperator {
+  const orc_schur* s;
+  int mode;
+  bool undamped;
+  double lambda;
+  size_t n() const { return s->model.jacobian.cols; }
+  size_t m() const { return s->model.jacobian.rows + (undamped ? 0 : n()); }
+  void precond(double* x, bool transpose) const {
+    if (mode == 2) s->blocks.apply_factor_inverse(x, transpose);
+  }
+  // y = A P x
+  void apply(const double* x, double* y) const {
+    std::vector<double> px(x, x + n());
+    precond(px.data(), false);
+    if (undamped) s->model.jacobian.apply(px.data(), y);
+    else orc_schur_damped_apply(s, lambda, mode != 0, px.data(), y);
+  }
+  // x = P^T A^T y
+  void transpose_apply(const double* y, double* x) const {
+    if (undamped) s->model.jacobian.transpose_apply(y, x);
+    else orc_schur_damped_transpose_apply(s, lambda, mode != 0, y, x);
+    precond(x, true);
+  }
+};
+struct LsmrResult {
+  bool converged;
+  size_t iterations;
+  double abs_residual, rel_residual;
+};
+LsmrResult lsmr(const LsmrOperator& op, const double* b, double* x_out, double tol, size_t max_iter) {
+  const size_t m = op.m(), n = op.n();
+  std::vector<double> u(b, b + m), v(n), y(n, 0.0), h(n), hbar(n, 0.0), tmp_m(m), tmp_n(n);
+  double beta = nrm2(u);
+  if (beta > 0) for (double& e : u) e /= beta;
+  op.transpose_apply(u.data(), v.data());
+  double alpha = nrm2(v);
+  if (alpha > 0) for (double& e : v) e /= alpha;
+  double alphabar = alpha, zetabar = alpha * beta, rho = 1, rhobar = 1, cbar = 1, sbar = 0;
+  h = v;
+  const double norm_atb = alpha * beta;
+  LsmrResult r{false, 0, std::fabs(zetabar), 0.0};
+  if (!(norm_atb > 0)) {  // A^T b = 0: the zero step is exact
+    r.converged = true;
+    r.abs_residual = 0;
+    std::fill(x_out, x_out + n, 0.0);
+    return r;
+  }
+  for (size_t k = 1; k <= max_iter; ++k) {
+    op.apply(v.data(), tmp_m.data());
+    for (size_t i = 0; i < m; ++i) u[i] = tmp_m[i] - alpha * u[i];
+    beta = nrm2(u);
+    if (beta > 0) for (double& e : u) e /= beta;
+    op.transpose_apply(u.data(), tmp_n.data());
+    for (size_t j = 0; j < n; ++j) v[j] = tmp_n[j] - beta * v[j];
+    alpha = nrm2(v);
+    if (alpha > 0) for (double& e : v) e /= alpha;
+    const double rho_old = rho, rhobar_old = rhobar;
+    rho = std::hypot(alphabar, beta);
+    const double c = alphabar / rho, sn = beta / rho;
+    const double theta_new = sn * alpha;
+    alphabar = c * alpha;
+    const double thetabar = sbar * rho;
+    rhobar = std::hypot(cbar * rho, theta_new);
+    cbar = cbar * rho / rhobar;
+    sbar = theta_new / rhobar;
+    const double zeta = cbar * zetabar;
+    zetabar = -sbar * zetabar;
+    const double f1 = thetabar * rho / (rho_old * rhobar_old), f2 = zeta / (rho * rhobar), f3 = theta_new / rho;
+    for (size_t j = 0; j < n; ++j) {
+      hbar[j] = h[j] - f1 * hbar[j];
+      y[j] += f2 * hbar[j];
+      h[j] = v[j] - f3 * h[j];
+    }
+    r.iterations = k;
+    r.abs_residual = std::fabs(zetabar);
+    r.rel_residual = r.abs_residual / norm_atb;
+    if (r.abs_residual <= tol * norm_atb) {
+      r.converged = true;
+      break;
+    }
+  }
+  op.precond(y.data(), false);
+  std::copy(y.begin(), y.end(), x_out);
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
+// src/lsmr.rs:560-595 with recover_and_verify (:430-468); mode as LsmrOperator.
+int orc_lsmr_solve_damped(orc_schur* s, double lambda, int mode, size_t max_iterations, double tol,
+                          double* delta, double* predicted, orc_lsmr_diagnostics* d) {
+  *d = orc_lsmr_diagnostics{ORC_LSMR_NONE, 0, 0, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, 0};
+  if (mode < 0 || mode > 2 || max_iterations == 0 || !(tol >= 0.0 && tol < 1.0)) return ORC_INVALID_OPTIONS;
+  if (!std::isfinite(lambda) || lambda < 0.0) return ORC_INVALID_OPTIONS;
+  const LinearizedModel& m = s->model;
+  const size_t n = m.jacobian.cols, rows = m.jacobian.rows;
+  d->damping = lambda;
+  d->min_scale = *std::min_element(m.scales.begin(), m.scales.end());
+  d->max_scale = *std::max_element(m.scales.begin(), m.scales.end());
+  const double root = std::sqrt(lambda);
+  for (size_t j = 0; j < n; ++j)
+    if (!std::isfinite(mode != 0 ? root : root * m.scales[j])) {
+      d->termination = ORC_LSMR_NON_FINITE;
+      return ORC_LINEAR_SOLVE_FAILED;
+    }
+  if (mode == 2) d->block_fallbacks = s->blocks.factor(lambda);
+  std::vector<double> rhs(rows + n, 0.0);
+  std::copy(m.negative_residual.begin(), m.negative_residual.end(), rhs.begin());
+  LsmrOperator op{s, mode, false, lambda};
+  std::vector<double> step(n, 0.0);
+  bool ok_solve = true;
+  if (rows + n > 0 && n > 0) {
+    s->solves += 1;
+    LsmrResult r = lsmr(op, rhs.data(), step.data(), tol, max_iterations);
+    s->iterations += r.iterations;
+    d->iterations = r.iterations;
+    d->estimated_absolute_residual = r.abs_residual;
+    d->estimated_relative_residual = r.rel_residual;
+    d->finite_step = norm_inf(step.data(), n, nullptr) == ORC_OK;
+    const bool finite = d->finite_step && std::isfinite(r.abs_residual) && std::isfinite(r.rel_residual);
+    d->termination = finite ? (r.converged ? ORC_LSMR_CONVERGED : ORC_LSMR_ITERATION_LIMIT) : ORC_LSMR_NON_FINITE;
+    ok_solve = d->termination == ORC_LSMR_CONVERGED;
+  } else {
+    d->termination = ORC_LSMR_CONVERGED;
+    d->finite_step = 1;
+  }
+  // recover_and_verify
+  if (mode != 0)
+    for (size_t j = 0; j < n; ++j) step[j] /= m.scales[j];
+  s->work.step = step;
+  s->work.prepare_verification(rows, n);
+  StepVerification v = m.verify_step(s->work, lambda);
+  d->finite_step = v.finite_step;
+  if (mode != 0) s->work.scale_residual(m.scales);
+  if (mode == 2) {
+    s->blocks.apply_factor_inverse(s->work.normal_residual.data(), true);
+    s->blocks.apply_factor_inverse(s->work.reference.data(), true);
+  }
+  const double scaled_norm = nrm2(s->work.normal_residual), scaled_reference = nrm2(s->work.reference);
+  auto relative = [](double a, double b) { return b == 0.0 ? a : a / b; };  // linear_model.rs relative()
+  d->original_absolute_residual = v.residual_norm;
+  d->original_relative_residual = relative(v.residual_norm, v.reference_norm);
+  d->scaled_absolute_residual = scaled_norm;
+  d->scaled_relative_residual = relative(scaled_norm, scaled_reference);
+  std::copy(step.begin(), step.end(), delta);
+  if (!v.finite_step || !std::isfinite(v.residual_norm) || !std::isfinite(scaled_norm)) {
+    d->termination = ORC_LSMR_NON_FINITE;
+    return ORC_LINEAR_SOLVE_FAILED;
+  }
+  if (predicted) *predicted = v.predicted_reduction;
+  return ok_solve ? ORC_OK : ORC_LINEAR_SOLVE_FAILED;
+}
+
+// src/lsmr.rs:535-552: identity-preconditioned LSMR on J itself, rhs = -r, no verification.
+int orc_lsmr_solve(orc_schur* s, size_t max_iterations, double tol, double* delta, orc_lsmr_diagnostics* d) {
+  *d = orc_lsmr_diagnostics{ORC_LSMR_NONE, 0, 0, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, 0};
+  if (max_iterations == 0 || !(tol >= 0.0 && tol < 1.0)) return ORC_INVALID_OPTIONS;
+  const LinearizedModel& m = s->model;
+  const size_t n = m.jacobian.cols, rows = m.jacobian.rows;
+  std::fill(delta, delta + n, 0.0);
+  if (rows == 0 || n == 0) {
+    d->termination = ORC_LSMR_CONVERGED;
+    d->finite_step = 1;
+    return ORC_OK;
+  }
+  s->solves += 1;
+  LsmrOperator op{s, 0, true, 0.0};
+  LsmrResult r = lsmr(op, m.negative_residual.data(), delta, tol, max_iterations);
+  s->iterations += r.iterations;
+  d->iterations = r.iterations;
+  d->estimated_absolute_residual = r.abs_residual;
+  d->estimated_relative_residual = r.rel_residual;
+  d->finite_step = norm_inf(delta, n, nullptr) == ORC_OK;
+  const bool finite = d->finite_step && std::isfinite(r.abs_residual) && std::isfinite(r.rel_residual);
+  d->termination = finite ? (r.converged ? ORC_LSMR_CONVERGED : ORC_LSMR_ITERATION_LIMIT) : ORC_LSMR_NON_FINITE;
+  return d->termination == ORC_LSMR_CONVERGED ? ORC_OK : ORC_LINEAR_SOLVE_FAILED;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
 // BA problem and batch evaluator: examples/bal/model.rs:194-237, examples/bal.rs:176-201
 // ---------------------------------------------------------------------------------------
 struct orc_ba {

@@ -42,6 +42,21 @@ typedef struct {
   double original_relative_residual;
 } orc_schur_diagnostics;
 
+/* LsmrTermination (src/lsmr.rs:60-68); -1 = none. */
+enum { ORC_LSMR_NONE = -1, ORC_LSMR_CONVERGED = 0, ORC_LSMR_ITERATION_LIMIT = 1, ORC_LSMR_NON_FINITE = 2 };
+
+/* LsmrDiagnostics (src/lsmr.rs:72-105); NaN = None. */
+typedef struct {
+  int termination;
+  size_t iterations;
+  int finite_step;
+  double estimated_absolute_residual, estimated_relative_residual;
+  double original_absolute_residual, original_relative_residual;
+  double scaled_absolute_residual, scaled_relative_residual;
+  double min_scale, max_scale, damping;
+  size_t block_fallbacks;
+} orc_lsmr_diagnostics;
+
 /* ---- projection (examples/bal/model.rs:126-184) ---- */
 int orc_projection(const double cam[9], double rotation[9], double right[9]);
 int orc_project(const double cam[9], const double point[3], double pixel[2]);
@@ -92,6 +107,13 @@ void orc_schur_gram_set(orc_schur*, size_t index, double value);
 int orc_schur_prepare_scales(orc_schur*, double floor);
 double orc_schur_verify(orc_schur*, const double* step, double lambda, double* normal_residual,
                         double* reference, double* residual_norm, double* reference_norm);
+
+/* ---- LSMR backend over an orc_schur's linearized model (src/lsmr.rs); mode 0 identity,
+ *      1 diagonal scaling, 2 block-Jacobi (blocks prepared by orc_schur_prepare_damping) ---- */
+int orc_lsmr_solve_damped(orc_schur*, double lambda, int mode, size_t max_iterations, double relative_tolerance,
+                          double* delta, double* predicted, orc_lsmr_diagnostics* diagnostics);
+int orc_lsmr_solve(orc_schur*, size_t max_iterations, double relative_tolerance, double* delta,
+                   orc_lsmr_diagnostics* diagnostics);
 
 /* ---- BA problem + batch evaluator (examples/bal/model.rs, examples/bal.rs:176-201) ---- */
 typedef struct orc_ba orc_ba;

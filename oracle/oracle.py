@@ -31,6 +31,20 @@ class SchurDiagnostics(ctypes.Structure):
                 ("original_relative_residual", ctypes.c_double)]
 
 
+class LsmrDiagnostics(ctypes.Structure):
+    """LsmrDiagnostics (src/lsmr.rs:72-105); NaN = None."""
+    _fields_ = [("termination", ctypes.c_int), ("iterations", _size_t), ("finite_step", ctypes.c_int),
+                ("estimated_absolute_residual", ctypes.c_double), ("estimated_relative_residual", ctypes.c_double),
+                ("original_absolute_residual", ctypes.c_double), ("original_relative_residual", ctypes.c_double),
+                ("scaled_absolute_residual", ctypes.c_double), ("scaled_relative_residual", ctypes.c_double),
+                ("min_scale", ctypes.c_double), ("max_scale", ctypes.c_double), ("damping", ctypes.c_double),
+                ("block_fallbacks", _size_t)]
+
+
+LSMR_IDENTITY, LSMR_DIAGONAL, LSMR_BLOCK = 0, 1, 2
+LSMR_CONVERGED, LSMR_ITERATION_LIMIT, LSMR_NON_FINITE = 0, 1, 2
+
+
 class OptimizeOptions(ctypes.Structure):
     _fields_ = [("max_iterations", _size_t), ("gradient_tolerance", ctypes.c_double),
                 ("step_tolerance", ctypes.c_double), ("cost_tolerance", ctypes.c_double)]
@@ -134,6 +148,9 @@ def lib():
         "orc_schur_gram_set": (None, [_VP, _size_t, ctypes.c_double]),
         "orc_schur_prepare_scales": (ctypes.c_int, [_VP, ctypes.c_double]),
         "orc_schur_verify": (ctypes.c_double, [_VP, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]),
+        "orc_lsmr_solve_damped": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, _size_t, ctypes.c_double,
+                                                 _dp, _dp, ctypes.POINTER(LsmrDiagnostics)]),
+        "orc_lsmr_solve": (ctypes.c_int, [_VP, _size_t, ctypes.c_double, _dp, ctypes.POINTER(LsmrDiagnostics)]),
         "orc_ba_new": (_VP, [_size_t, _size_t, _size_t, _dp, _dp, _i64p, _i64p, _dp, ctypes.c_double]),
         "orc_ba_free": (None, [_VP]),
         "orc_ba_layout": (None, [_VP, _i64p, _i64p, _i64p, _i64p]),
@@ -257,6 +274,21 @@ class Schur:
         d, pd = _out(self.dimension())
         st = lib().orc_schur_solve(self.h, pd)
         return st, d
+
+    # LSMR backend over this model (src/lsmr.rs); blocks come from prepare_damping
+    def lsmr_solve_damped(self, lam, mode=LSMR_DIAGONAL, max_iterations=1000, tol=1e-10):
+        d, pd = _out(self.dimension())
+        pred = ctypes.c_double(float("nan"))
+        diag = LsmrDiagnostics()
+        st = lib().orc_lsmr_solve_damped(self.h, lam, mode, max_iterations, tol, pd, ctypes.byref(pred),
+                                         ctypes.byref(diag))
+        return st, d, pred.value, diag
+
+    def lsmr_solve(self, max_iterations=1000, tol=1e-10):
+        d, pd = _out(self.dimension())
+        diag = LsmrDiagnostics()
+        st = lib().orc_lsmr_solve(self.h, max_iterations, tol, pd, ctypes.byref(diag))
+        return st, d, diag
 
     def diagnostics(self):
         d = SchurDiagnostics()
