@@ -7,7 +7,7 @@ LIB_PATH = os.path.join(HERE, "libfagra_gpu.so")
 
 OK, INVALID_OPTIONS, LINEAR_SOLVE_FAILED, NO_CONVERGENCE, NO_PROGRESS = 0, 1, 2, 3, 4
 INVALID_EVALUATION, DIMENSION_MISMATCH, INVALID_EMISSION, UNSUPPORTED_STRUCTURE = 5, 6, 7, 8
-DEVICE_ERROR = 100
+INVALID_INPUT, DEVICE_ERROR = 9, 100
 FP64, FP32 = 0, 1
 PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1
 CG_NONE, CG_CONVERGED, CG_ITERATION_LIMIT, CG_NOT_POSITIVE_DEFINITE, CG_NON_FINITE = -1, 0, 1, 2, 3
@@ -18,7 +18,7 @@ STATUS_NAMES = {OK: "Ok", INVALID_OPTIONS: "InvalidOptions", LINEAR_SOLVE_FAILED
                 NO_CONVERGENCE: "NoConvergence", NO_PROGRESS: "NoProgress",
                 INVALID_EVALUATION: "InvalidEvaluation", DIMENSION_MISMATCH: "DimensionMismatch",
                 INVALID_EMISSION: "InvalidEmission", UNSUPPORTED_STRUCTURE: "UnsupportedStructure",
-                DEVICE_ERROR: "DeviceError"}
+                INVALID_INPUT: "InvalidInput", DEVICE_ERROR: "DeviceError"}
 
 _sz = ctypes.c_size_t
 _d = ctypes.c_double
@@ -74,6 +74,10 @@ class LsmrConfig(ctypes.Structure):
     _fields_ = [("max_iterations", _sz), ("relative_tolerance", _d), ("diagonal_preconditioning", ctypes.c_int),
                 ("block_preconditioning", ctypes.c_int), ("collect_timings", ctypes.c_int),
                 ("precision", ctypes.c_int), ("on_solve", ON_LSMR_SOLVE), ("on_solve_user", _VP)]
+
+
+class BalSizes(ctypes.Structure):
+    _fields_ = [("n_cameras", _sz), ("n_points", _sz), ("n_observations", _sz)]
 
 
 class OptimizeOptions(ctypes.Structure):
@@ -168,6 +172,11 @@ SIGNATURES = {
                                               ON_ACCEPT, _VP, ctypes.POINTER(LmOutcome)]),
     "fg_gn_solve_batch_lsmr": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions),
                                               ctypes.POINTER(LmOutcome)]),
+    "fg_bal_scan": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), ctypes.c_char_p, _sz]),
+    "fg_bal_parse": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), _dp, _dp, _i64p, _i64p, _dp,
+                                    ctypes.c_char_p, _sz]),
+    "fg_bal_format": (ctypes.c_int, [ctypes.POINTER(BalSizes), _dp, _dp, _i64p, _i64p, _dp, ctypes.c_char_p, _sz,
+                                     _szp]),
     "fg_lm_solve_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions), ctypes.POINTER(LmOptions),
                                          ON_ACCEPT, _VP, ctypes.POINTER(LmOutcome)]),
     "fg_gn_solve_batch": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions), ctypes.POINTER(LmOutcome)]),

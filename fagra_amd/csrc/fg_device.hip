@@ -244,26 +244,27 @@ PointArgs point_args(const fg_ba* b) {
 
 template <typename T, int MODE>
 void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, double lambda,
-                   double* delta_out, double* rows_out, double* parts, const CgState* st) {
+                   double* delta_out, double* rows_out, double* parts, const CgState* st,
+                   const int* halt = nullptr) {
   fg_ba* b = s->ba;
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur], fw,
                      (const double*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda,
-                     (double*)s->tp, delta_out, rows_out, parts, st);
+                     (double*)s->tp, delta_out, rows_out, parts, st, halt);
 }
 
 template <typename T, int MODE>
 void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const T* fw,
                     const double* vin, const double* delta_c, const double* rhs, double lambda, double* out,
-                    double* out_d, const CgState* st = nullptr) {
+                    double* out_d, const CgState* st = nullptr, const int* halt = nullptr) {
   fg_ba* b = s->ba;
   // the Schur, residual and verification modes read the forward table of their input
   if (fw == nullptr) fw = (const T*)s->fw;
   hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
                      (const T*)b->d_tab[b->cur], (const double*)s->tp, delta_p, rows_in, fw, (const T*)s->dc,
-                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st);
+                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, halt);
 }
 
 template <typename T>
@@ -1442,8 +1443,11 @@ template <typename T>
 void lsmr_transpose_rows(fg_lsmr* l) {
   fg_schur* s = l->s;
   fg_ba* b = s->ba;
-  launch_points<T, kJtu>(s, (const T*)s->fw, nullptr, l->d_u, 0.0, l->d_zt + 9 * b->nc, nullptr, nullptr, nullptr);
-  launch_cameras<T, cJtu>(s, nullptr, l->d_u, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, l->d_zt);
+  const int* halt = &l->d_st->done;
+  launch_points<T, kJtu>(s, (const T*)s->fw, nullptr, l->d_u, 0.0, l->d_zt + 9 * b->nc, nullptr, nullptr, nullptr,
+                         halt);
+  launch_cameras<T, cJtu>(s, nullptr, l->d_u, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, l->d_zt, nullptr,
+                          halt);
 }
 
 // One LSMR iteration: ws = P v / (alpha D); U = A ws - (alpha/beta) U; beta; V = P^T A^T U /
@@ -1464,7 +1468,8 @@ void lsmr_iteration(fg_lsmr* l, int mode, bool damped) {
                      scaled ? 1 : 0, 0, (const LsmrState*)l->d_st, l->d_ws);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
                      (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw);
-  launch_points<T, kJv>(s, (const T*)s->fw, l->d_ws + 9 * nc, nullptr, 0.0, nullptr, s->d_rows, nullptr, nullptr);
+  launch_points<T, kJv>(s, (const T*)s->fw, l->d_ws + 9 * nc, nullptr, 0.0, nullptr, s->d_rows, nullptr, nullptr,
+                        &l->d_st->done);
   hipLaunchKernelGGL(k_lsmr_u<T>, dim3(ublocks), dim3(kBlock), 0, b->stream, (int)l->rows, nc, n_bottom,
                      (const double*)s->d_rows, (const double*)l->d_ws, (const T*)s->dc, (const T*)s->dp,
                      (const LsmrState*)l->d_st, l->d_u, l->d_up);
@@ -1513,8 +1518,9 @@ fg_status lsmr_run(fg_lsmr* l, int mode, bool damped, double lambda, bool* conve
   hipLaunchKernelGGL(k_lsmr_h, dim3(grid_for(l->n)), dim3(kBlock), 0, b->stream, (int)l->n, (const double*)l->d_v,
                      l->d_st, 1, l->d_h, l->d_hbar, l->d_y);
   FG_TRY(hipGetLastError());
-  // iterations in batches; the host reads the previous batch's state while the next runs
-  const int kLsmrBatch = 8;
+  // iterations in batches; the host reads the previous batch's state while the next runs,
+  // so at most two batches run past convergence, as kernels that return at once
+  const int kLsmrBatch = 2;
   int launched = 0, batch = 0;
   bool stop = false;
   const int max_it = init.max_iters;

@@ -659,12 +659,13 @@ __global__ void __launch_bounds__(kBlock) k_points(
     const T* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
     const T* __restrict__ dpv, const double* __restrict__ delta_p_in, const double* __restrict__ rows_in,
     double lambda, double* __restrict__ t_out, double* __restrict__ delta_out, double* __restrict__ rows_out,
-    double* parts, const CgState* st) {
+    double* parts, const CgState* st, const int* halt) {
   __shared__ double smem[kBlock / kWave];
   if (MODE == kSchur && st != nullptr) {
     if (st->done) return;
     lambda = st->lambda;  // inside PCG the damping travels with the state
   }
+  if (halt != nullptr && *halt) return;  // a finished iterative solve's queued tail
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   double uu = 0.0, n2 = 0.0, n2s = 0.0, gd = 0.0, g2 = 0.0, g2s = 0.0;  // verify partials
@@ -778,7 +779,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
     const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
-    double* __restrict__ out_d, double* parts, const CgState* st) {
+    double* __restrict__ out_d, double* parts, const CgState* st, const int* halt) {
   __shared__ double ct[kCamTab];
   __shared__ double cfw[kFwd];
   __shared__ double red[9][kBlock / kWave];
@@ -786,6 +787,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     if (st->done) return;
     lambda = st->lambda;
   }
+  if (halt != nullptr && *halt) return;
   const int c = blockIdx.x;
   if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
   if ((MODE == cVerify || MODE == cSchur || MODE == cResid) && threadIdx.x < kFwd)

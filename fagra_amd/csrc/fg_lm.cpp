@@ -284,6 +284,7 @@ const char* fg_status_string(int status) {
     case FG_DIMENSION_MISMATCH: return "residual or Jacobian dimensions do not match";
     case FG_INVALID_EMISSION: return "invalid factor scope or emitted variable identity";
     case FG_UNSUPPORTED_STRUCTURE: return "residual structure is unsupported by the selected backend";
+    case FG_INVALID_INPUT: return "invalid BAL input";
     case FG_DEVICE_ERROR: return "HIP runtime error";
     default: return "unknown status";
   }

@@ -49,13 +49,10 @@ class BalProblem:
         return self.obs_camera.shape[0]
 
     def to_bal_text(self):
-        """BAL text (examples/bal/model.rs:59-113): header, observations, cameras, points."""
-        lines = [f"{self.n_cameras} {self.n_points} {self.n_observations}"]
-        for c, p, (u, v) in zip(self.obs_camera, self.obs_point, self.obs_pixel):
-            lines.append(f"{c} {p} {u!r} {v!r}")
-        lines.extend(repr(float(x)) for x in self.cameras.ravel())
-        lines.extend(repr(float(x)) for x in self.points.ravel())
-        return "\n".join(lines) + "\n"
+        """BAL text (read back bit for bit by fagra_amd.bal.parse)."""
+        from . import bal
+
+        return bal.format(self)
 
 
 def rotation_matrices(w):
@@ -209,37 +206,7 @@ def generate(name="ring16", seed=0, n_cameras=None, n_points=None, **overrides):
 
 
 def parse_bal(text):
-    """Parse BAL text with the reference's validation (examples/bal/model.rs:59-113)."""
-    it = iter(text.split())
+    """Problem::parse (examples/bal/model.rs:59-113) via the native reader."""
+    from . import bal
 
-    def nxt(kind):
-        try:
-            tok = next(it)
-        except StopIteration:
-            raise ValueError("truncated BAL input") from None
-        try:
-            return kind(tok)
-        except ValueError:
-            raise ValueError("invalid BAL token") from None
-
-    nc, npts, no = nxt(int), nxt(int), nxt(int)
-    if nc <= 0 or npts <= 0 or no <= 0:
-        raise ValueError("empty BAL problem")
-    oc = np.zeros(no, np.int64)
-    op = np.zeros(no, np.int64)
-    px = np.zeros((no, 2))
-    for k in range(no):
-        c, p = nxt(int), nxt(int)
-        u, v = nxt(float), nxt(float)
-        if c < 0 or c >= nc or p < 0 or p >= npts or not (np.isfinite(u) and np.isfinite(v)):
-            raise ValueError("invalid observation index or value")
-        oc[k], op[k], px[k] = c, p, (u, v)
-    vals = np.zeros(9 * nc + 3 * npts)
-    for i in range(len(vals)):
-        vals[i] = nxt(float)
-        if not np.isfinite(vals[i]):
-            raise ValueError("nonfinite parameter")
-    if next(it, None) is not None:
-        raise ValueError("trailing BAL tokens")
-    return BalProblem(name="bal", cameras=vals[:9 * nc].reshape(nc, 9), points=vals[9 * nc:].reshape(npts, 3),
-                      obs_camera=oc, obs_point=op, obs_pixel=px)
+    return bal.parse(text)

@@ -35,6 +35,7 @@ typedef enum {
   FG_DIMENSION_MISMATCH = 6,     /* EvaluationError::DimensionMismatch */
   FG_INVALID_EMISSION = 7,       /* EvaluationError::InvalidEmission */
   FG_UNSUPPORTED_STRUCTURE = 8,  /* EvaluationError::UnsupportedStructure */
+  FG_INVALID_INPUT = 9,          /* BAL text rejected (Problem::parse's Err(String)) */
   FG_DEVICE_ERROR = 100          /* HIP runtime failure; no reference counterpart */
 } fg_status;
 
@@ -305,6 +306,25 @@ typedef struct {
 void fg_lm_optimize(const fg_problem*, const fg_optimize_options*, const fg_lm_options*,
                     fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out);
 void fg_gn_optimize(const fg_problem*, const fg_optimize_options*, fg_lm_outcome* out);
+
+/* ---- BAL problem text (examples/bal/model.rs:59-113) ---- */
+typedef struct {
+  size_t n_cameras, n_points, n_observations;
+} fg_bal_sizes;
+/* Problem::parse's validation over the whole text (no values stored): FG_OK with the sizes,
+ * or FG_INVALID_INPUT with the reference's message in err. */
+fg_status fg_bal_scan(const char* text, size_t len, fg_bal_sizes* sizes, char* err, size_t err_len);
+/* Problem::parse into caller arrays sized from fg_bal_scan: cameras 9 n_c, points 3 n_p,
+ * observation camera / point indices n_o, pixels 2 n_o, observations in input order. */
+fg_status fg_bal_parse(const char* text, size_t len, const fg_bal_sizes* sizes, double* cameras, double* points,
+                       int64_t* obs_camera, int64_t* obs_point, double* pixels, char* err, size_t err_len);
+/* BAL text that fg_bal_parse reads back bit for bit (shortest round-trip decimals); no
+ * reference counterpart (the reference only reads). *written = bytes needed; FG_OK once
+ * they fit in cap, FG_DIMENSION_MISMATCH when buf is null or too small, FG_INVALID_OPTIONS
+ * for out-of-range indices or nonfinite values. */
+fg_status fg_bal_format(const fg_bal_sizes* sizes, const double* cameras, const double* points,
+                        const int64_t* obs_camera, const int64_t* obs_point, const double* pixels, char* buf,
+                        size_t cap, size_t* written);
 
 /* ---- device information ---- */
 fg_status fg_device_info(char* name, size_t name_len, int* cus, size_t* lds_bytes);
