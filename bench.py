@@ -62,8 +62,14 @@ def survey_matvec_bytes(nc, npts, no):
 
 
 def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, collect_timings=False):
-    backend = fa.Schur(precision=precision, relative_tolerance=cg_tol, preconditioner=precond,
-                       collect_timings=collect_timings)
+    """One LM solve from the initial state; precond names the Schur preconditioner or, as
+    "lsmr_block" / "lsmr_diagonal", the LSMR backend with that right preconditioning."""
+    if precond in ("lsmr_block", "lsmr_diagonal"):
+        backend = fa.Lsmr(precision=precision, relative_tolerance=cg_tol, diagonal_preconditioning=True,
+                          block_preconditioning=precond == "lsmr_block", collect_timings=collect_timings)
+    else:
+        backend = fa.Schur(precision=precision, relative_tolerance=cg_tol, preconditioner=precond,
+                           collect_timings=collect_timings)
     lm = fa.LevenbergMarquardt(backend)
     hit = {}
     t0 = [0.0]
@@ -92,9 +98,8 @@ def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, coll
            "accepted": out.accepted_steps, "rejected": out.rejected_steps, "linearizations": out.linearizations,
            "cg_iterations": out.linear_iterations, "solves": out.linear_solves,
            "at_target": {k: v for k, v in hit.items() if k != "t"},
-           "matvecs": tim.matvecs, "matvec_kernel_s": tim.matvec_kernel_seconds, "trace": trace,
-           "timings": {f: getattr(tim, f) for f in ("preparation", "factorization", "linear_solve", "products",
-                                                    "verification", "linearization", "cost")}}
+           "matvecs": getattr(tim, "matvecs", None), "matvec_kernel_s": getattr(tim, "matvec_kernel_seconds", None),
+           "trace": trace, "timings": {f: getattr(tim, f) for f, _ in tim._fields_}}
     backend.close()
     return res
 
@@ -177,7 +182,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--kernel-repeats", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="fp64,fp32,learned")
+    ap.add_argument("--modes", default="fp64,fp32,learned,lsmr")
     ap.add_argument("--profile-products", action="store_true",
                     help="only time the two Schur-product kernels (FP32, FP64) and a 256 MiB device copy, "
                          "for hardware-counter runs")
@@ -222,6 +227,11 @@ def main():
             modes.append(("fp32_block_jacobi", fa.FP32, fa.PRECOND_BLOCK_JACOBI))
         elif m == "learned" and have_model:
             modes.append(("fp32_learned", fa.FP32, fa.PRECOND_LEARNED))
+        elif m == "lsmr":  # the LSMR backend (block-Jacobi right preconditioning), both precisions
+            modes.append(("fp64_lsmr_block", fa.FP64, "lsmr_block"))
+            modes.append(("fp32_lsmr_block", fa.FP32, "lsmr_block"))
+        elif m == "lsmr_diagonal":
+            modes.append(("fp64_lsmr_diagonal", fa.FP64, "lsmr_diagonal"))
     results = {}
     for name, prec, pc in modes:
         for w in range(args.warmup):
@@ -237,7 +247,7 @@ def main():
 
     head = "fp32_learned" if "fp32_learned" in results else "fp32_block_jacobi"
     if head not in results:
-        head = list(results)[0]
+        head = next((k for k in results if not k.endswith(("lsmr_block", "lsmr_diagonal"))), list(results)[0])
     hr = results[head]
     times = [r["time_to_target"] for r in hr if r["time_to_target"] is not None]
     value = statistics.median(times) if times else None

@@ -244,27 +244,26 @@ PointArgs point_args(const fg_ba* b) {
 
 template <typename T, int MODE>
 void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, double lambda,
-                   double* delta_out, double* rows_out, double* parts, const CgState* st,
-                   const int* halt = nullptr) {
+                   double* delta_out, double* rows_out, double* parts, const CgState* st) {
   fg_ba* b = s->ba;
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur], fw,
                      (const double*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda,
-                     (double*)s->tp, delta_out, rows_out, parts, st, halt);
+                     (double*)s->tp, delta_out, rows_out, parts, st, nullptr);
 }
 
 template <typename T, int MODE>
 void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, const T* fw,
                     const double* vin, const double* delta_c, const double* rhs, double lambda, double* out,
-                    double* out_d, const CgState* st = nullptr, const int* halt = nullptr) {
+                    double* out_d, const CgState* st = nullptr) {
   fg_ba* b = s->ba;
   // the Schur, residual and verification modes read the forward table of their input
   if (fw == nullptr) fw = (const T*)s->fw;
   hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
                      (const T*)b->d_tab[b->cur], (const double*)s->tp, delta_p, rows_in, fw, (const T*)s->dc,
-                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, halt);
+                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, nullptr);
 }
 
 template <typename T>
@@ -1389,9 +1388,11 @@ struct fg_lsmr {
   fg_lsmr_diagnostics diag;
   size_t solves = 0, iterations = 0;
   fg_lsmr_timings tim;
-  size_t n = 0, rows = 0;  // coordinates, Jacobian rows (2 no)
-  double *d_u = nullptr, *d_v = nullptr, *d_ws = nullptr, *d_zt = nullptr, *d_h = nullptr, *d_hbar = nullptr,
-         *d_y = nullptr, *d_up = nullptr, *d_vp = nullptr;
+  size_t n = 0, nslots = 0;  // coordinates, point-major observation slots
+  // d_u: Jacobian rows of U in slot order (2 per slot); d_ub: its n damping rows
+  double *d_u = nullptr, *d_ub = nullptr, *d_v = nullptr, *d_ws = nullptr, *d_zt = nullptr, *d_h = nullptr,
+         *d_hbar = nullptr, *d_y = nullptr, *d_up = nullptr, *d_vp = nullptr;
+  int pblocks = 0;  // workgroups of the point pass (its partials precede the cameras')
   LsmrState* d_st = nullptr;
   LsmrState* h_st = nullptr;
   std::vector<double> h_delta;
@@ -1408,22 +1409,24 @@ int lsmr_requested_mode(const fg_lsmr* l) {
 }
 
 void lsmr_free_vectors(fg_lsmr* l) {
-  for (double** v : {&l->d_u, &l->d_v, &l->d_ws, &l->d_zt, &l->d_h, &l->d_hbar, &l->d_y, &l->d_up, &l->d_vp}) {
+  for (double** v : {&l->d_u, &l->d_ub, &l->d_v, &l->d_ws, &l->d_zt, &l->d_h, &l->d_hbar, &l->d_y, &l->d_up,
+                     &l->d_vp}) {
     dfree(*v);
     *v = nullptr;
   }
-  l->n = l->rows = 0;
+  l->n = l->nslots = 0;
 }
 
 fg_status lsmr_alloc_vectors(fg_lsmr* l) {
   fg_ba* b = l->s->ba;
-  const size_t n = 9 * b->nc + 3 * b->np, rows = 2 * b->no;
-  if (l->n == n && l->rows == rows && l->d_u) return FG_OK;
+  const size_t n = 9 * b->nc + 3 * b->np;
+  if (l->n == n && l->nslots == b->nslots && l->d_u) return FG_OK;
   lsmr_free_vectors(l);
-  const size_t m = rows + n;
-  const size_t ublocks = (size_t)std::min(4096, grid_for(m));
+  l->pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  const size_t ublocks = (size_t)l->pblocks + b->nc;
   const size_t vblocks = (size_t)grid_for(b->nc + b->np);
-  FG_TRY(dalloc(&l->d_u, m));
+  FG_TRY(dalloc(&l->d_u, 2 * std::max<size_t>(1, b->nslots)));
+  FG_TRY(dalloc(&l->d_ub, n));
   FG_TRY(dalloc(&l->d_v, n));
   FG_TRY(dalloc(&l->d_ws, n));
   FG_TRY(dalloc(&l->d_zt, n));
@@ -1433,21 +1436,36 @@ fg_status lsmr_alloc_vectors(fg_lsmr* l) {
   FG_TRY(dalloc(&l->d_up, std::max<size_t>(ublocks, 1024)));
   FG_TRY(dalloc(&l->d_vp, std::max<size_t>(vblocks, 1024)));
   l->n = n;
-  l->rows = rows;
+  l->nslots = b->nslots;
   l->h_delta.assign(n, 0.0);
   return FG_OK;
 }
 
-// A J^T u product into zt (cameras then point slots) from the top rows of U.
+// The two passes over the observations of one LSMR step (k_points<kLsmr> then
+// k_cameras<cLsmr>): U = A ws - (alpha/beta) U with ||U||^2 partials and zt = J^T U. With
+// ws == nullptr only zt and ||U||^2 of the current U (the start vector b).
 template <typename T>
-void lsmr_transpose_rows(fg_lsmr* l) {
+void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
   fg_schur* s = l->s;
   fg_ba* b = s->ba;
-  const int* halt = &l->d_st->done;
-  launch_points<T, kJtu>(s, (const T*)s->fw, nullptr, l->d_u, 0.0, l->d_zt + 9 * b->nc, nullptr, nullptr, nullptr,
-                         halt);
-  launch_cameras<T, cJtu>(s, nullptr, l->d_u, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr, l->d_zt, nullptr,
-                          halt);
+  const int nc = (int)b->nc;
+  const bool step = ws != nullptr;
+  const LsmrState* st = l->d_st;
+  hipLaunchKernelGGL((k_points<T, kLsmr>), dim3(l->pblocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
+                     (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur],
+                     step ? (const T*)s->fw : nullptr, (const double*)s->mp, (const T*)s->gp, (const T*)s->dp,
+                     step ? ws + 9 * nc : nullptr, (const double*)l->d_u, 0.0,
+                     damped && step ? l->d_ub + 9 * nc : nullptr, l->d_zt + 9 * nc, l->d_u, l->d_up,
+                     (const CgState*)nullptr, st);
+  // the camera pass gathers U by observation slot (k_slot in the gather-index argument)
+  hipLaunchKernelGGL((k_cameras<T, cLsmr>), dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
+                     b->d_k_slot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm, (const T*)b->d_tab[b->cur],
+                     (const double*)s->tp, (const double*)nullptr, (const double*)l->d_u, (const T*)s->fw,
+                     (const T*)s->dc, (const T*)s->gc, damped && step ? ws : nullptr, (const double*)nullptr,
+                     (const double*)nullptr, 0.0, damped && step ? l->d_ub : nullptr, l->d_zt,
+                     l->d_up + l->pblocks, (const CgState*)nullptr, st);
+  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, l->pblocks + nc, 0,
+                     l->d_st);
 }
 
 // One LSMR iteration: ws = P v / (alpha D); U = A ws - (alpha/beta) U; beta; V = P^T A^T U /
@@ -1458,9 +1476,6 @@ void lsmr_iteration(fg_lsmr* l, int mode, bool damped) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc, np = (int)b->np;
   const int nblk = nc + np;
-  const int n_bottom = damped ? (int)l->n : 0;
-  const int m = (int)l->rows + n_bottom;
-  const int ublocks = std::min(4096, grid_for(m));
   const int vblocks = grid_for(nblk);
   const bool scaled = mode != 0;
   hipLaunchKernelGGL(k_lsmr_pv<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_v,
@@ -1468,16 +1483,10 @@ void lsmr_iteration(fg_lsmr* l, int mode, bool damped) {
                      scaled ? 1 : 0, 0, (const LsmrState*)l->d_st, l->d_ws);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
                      (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw);
-  launch_points<T, kJv>(s, (const T*)s->fw, l->d_ws + 9 * nc, nullptr, 0.0, nullptr, s->d_rows, nullptr, nullptr,
-                        &l->d_st->done);
-  hipLaunchKernelGGL(k_lsmr_u<T>, dim3(ublocks), dim3(kBlock), 0, b->stream, (int)l->rows, nc, n_bottom,
-                     (const double*)s->d_rows, (const double*)l->d_ws, (const T*)s->dc, (const T*)s->dp,
-                     (const LsmrState*)l->d_st, l->d_u, l->d_up);
-  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, ublocks, 0, l->d_st);
-  lsmr_transpose_rows<T>(l);
+  lsmr_passes<T>(l, l->d_ws, damped);
   hipLaunchKernelGGL(k_lsmr_v<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_zt,
-                     (const double*)(l->d_u + l->rows), (const double*)s->lc, (const double*)s->mp,
-                     (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0, scaled ? 1 : 0, damped ? 1 : 0, 0,
+                     (const double*)l->d_ub, (const double*)s->lc, (const double*)s->mp, (const T*)s->dc,
+                     (const T*)s->dp, mode == 2 ? 1 : 0, scaled ? 1 : 0, damped ? 1 : 0, 0,
                      (const LsmrState*)l->d_st, l->d_v, l->d_vp);
   hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_vp, vblocks, 1, l->d_st);
   hipLaunchKernelGGL(k_lsmr_scalar, dim3(1), dim3(64), 0, b->stream, l->d_st, 0);
@@ -1495,9 +1504,6 @@ fg_status lsmr_run(fg_lsmr* l, int mode, bool damped, double lambda, bool* conve
   fg_ba* b = s->ba;
   const int nc = (int)b->nc, np = (int)b->np;
   const int nblk = nc + np;
-  const int n_bottom = damped ? (int)l->n : 0;
-  const int m = (int)l->rows + n_bottom;
-  const int ublocks = std::min(4096, grid_for(m));
   const int vblocks = grid_for(nblk);
   LsmrState init{};
   init.max_iters = (int)std::min<size_t>(l->cfg.max_iterations, (size_t)std::numeric_limits<int>::max());
@@ -1505,12 +1511,12 @@ fg_status lsmr_run(fg_lsmr* l, int mode, bool damped, double lambda, bool* conve
   init.root = std::sqrt(lambda);
   FG_TRY(hipMemcpyAsync(l->d_st, &init, sizeof(LsmrState), hipMemcpyHostToDevice, b->stream));
   // beta_1 u_1 = b; alpha_1 v_1 = P^T A^T u_1
-  hipLaunchKernelGGL(k_lsmr_init_u<T>, dim3(ublocks), dim3(kBlock), 0, b->stream, (int)b->no, n_bottom,
-                     (const Vec2<T>*)s->res_cm, l->d_u, l->d_up);
-  hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, ublocks, 0, l->d_st);
-  lsmr_transpose_rows<T>(l);
+  hipLaunchKernelGGL(k_lsmr_init_u<T>, dim3(grid_for(std::max(b->nslots, l->n))), dim3(kBlock), 0, b->stream,
+                     (int)b->nslots, (int)l->n, (const int*)b->d_slot_cam, (const Vec2<T>*)s->res,
+                     (Vec2<double>*)l->d_u, l->d_ub);
+  lsmr_passes<T>(l, nullptr, damped);
   hipLaunchKernelGGL(k_lsmr_v<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_zt,
-                     (const double*)(l->d_u + l->rows), (const double*)s->lc, (const double*)s->mp,
+                     (const double*)l->d_ub, (const double*)s->lc, (const double*)s->mp,
                      (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0, mode != 0 ? 1 : 0, damped ? 1 : 0, 1,
                      (const LsmrState*)l->d_st, l->d_v, l->d_vp);
   hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_vp, vblocks, 1, l->d_st);

@@ -27,6 +27,17 @@ struct CgState {
   int done, iters, term, use_learned;
   int fallback_taken, pad0, pad1, pad2;
 };
+// LSMR state (the scalar recurrences of Fong & Saunders 2011, norms of the unnormalized
+// vectors U = beta u and V = alpha v); declared here so the product kernels can read the
+// update coefficients and the finished flag.
+struct LsmrState {
+  double alpha, beta, alphabar, zetabar, rho, rhobar, cbar, sbar;
+  double f1, f2, f3;        // hbar, y and h update factors of the current iteration
+  double norm_atb, abs_res;
+  int done, converged, iters, max_iters;
+  double rel_tol, root;             // sqrt(lambda)
+  double beta_new, alpha_new;        // norms summed this iteration, adopted by k_lsmr_scalar
+};
 struct DevScalars {
   CgState st[2];
   int max_iters, fallback_at, pad0, pad1;
@@ -585,8 +596,12 @@ __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double
 //   kVerify:  normal residual Jp^T (J delta + r) + lambda Dp^2 delta_p and ||J delta||^2
 //   kJv:      rows 2k,2k+1 of J x               (Jacobian::apply, linear_model.rs:39-55)
 //   kJtu:     x_p = sum_k Jp^T u_k              (Jacobian::transpose_apply, :64-87)
+//   kLsmr:    one LSMR step over rows kept in slot order: U_s = J ws - (alpha/beta) U_s for
+//             each observation, then x_p = sum_k Jp^T U_k, the point damping rows
+//             U_p = sqrt(lambda) Dp ws_p - (alpha/beta) U_p, and ||U||^2 partials; with no
+//             forward table it only forms x_p and ||U||^2 of the current U (the start vector)
 // ---------------------------------------------------------------------------------------
-enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu = 5 };
+enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu = 5, kLsmr = 6 };
 
 struct PointArgs {
   int nchunks, np;
@@ -602,7 +617,7 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
                                           const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
                                           const T* __restrict__ fw, const double dpx[3],
                                           const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                                          double w[3], double h[6], double& uu) {
+                                          double w[3], double h[6], double& uu, double coef) {
   double ct[16];
   load_cam16_d(tab, c, ct);
   ObsGeom<double> geo;
@@ -650,6 +665,21 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
   } else if (MODE == kJtu) {
     const int row = pa.slot_row[s];
     jpt_accumulate(geo, ct, rows_in[2 * row], rows_in[2 * row + 1], w);
+  } else if (MODE == kLsmr) {
+    const Vec2<double> old = reinterpret_cast<const Vec2<double>*>(rows_in)[s];
+    double u0 = old.x, u1 = old.y;
+    if (fw != nullptr) {
+      double f[kFwd];
+      load_fwd_d(fw, c, f);
+      double v0, v1, x0, x1;
+      jc_apply(geo, f, v0, v1);
+      jp_apply(geo, ct, dpx, x0, x1);
+      u0 = (v0 + x0) - coef * u0;
+      u1 = (v1 + x1) - coef * u1;
+      reinterpret_cast<Vec2<double>*>(rows_out)[s] = Vec2<double>{u0, u1};
+    }
+    uu += u0 * u0 + u1 * u1;
+    jpt_accumulate(geo, ct, u0, u1, w);
   }
 }
 
@@ -659,13 +689,16 @@ __global__ void __launch_bounds__(kBlock) k_points(
     const T* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
     const T* __restrict__ dpv, const double* __restrict__ delta_p_in, const double* __restrict__ rows_in,
     double lambda, double* __restrict__ t_out, double* __restrict__ delta_out, double* __restrict__ rows_out,
-    double* parts, const CgState* st, const int* halt) {
+    double* parts, const CgState* st, const LsmrState* ls) {
   __shared__ double smem[kBlock / kWave];
   if (MODE == kSchur && st != nullptr) {
     if (st->done) return;
     lambda = st->lambda;  // inside PCG the damping travels with the state
   }
-  if (halt != nullptr && *halt) return;  // a finished iterative solve's queued tail
+  if (ls != nullptr && ls->done) return;  // a finished LSMR solve's queued tail
+  // kLsmr: U update coefficient alpha/beta and sqrt(lambda) of the damping rows
+  const double coef = MODE == kLsmr ? ls->alpha / ls->beta : 0.0;
+  const double root = MODE == kLsmr ? ls->root : 0.0;
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   double uu = 0.0, n2 = 0.0, n2s = 0.0, gd = 0.0, g2 = 0.0, g2s = 0.0;  // verify partials
@@ -675,7 +708,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
       double w[3] = {0.0, 0.0, 0.0};
       double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       double dpx[3] = {0.0, 0.0, 0.0};
-      if (MODE == kVerify || MODE == kJv) {
+      if (MODE == kVerify || MODE == kJv || (MODE == kLsmr && fw != nullptr)) {
         dpx[0] = delta_p_in[3 * i];
         dpx[1] = delta_p_in[3 * i + 1];
         dpx[2] = delta_p_in[3 * i + 2];
@@ -688,13 +721,14 @@ __global__ void __launch_bounds__(kBlock) k_points(
           const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
           const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
           const Vec4<T> r0 = rec[s0], r1 = rec[s1];
-          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu);
-          if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu);
+          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+          if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
         }
         if (j < len) {
           const int s0 = base + j * kWave + lane;
           const int c0 = pa.slot_cam[s0];
-          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu);
+          if (c0 >= 0)
+            point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
         }
       }
       if (MODE == kSchur) {
@@ -745,8 +779,22 @@ __global__ void __launch_bounds__(kBlock) k_points(
       } else if (MODE == kJtu) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = w[a];
+      } else if (MODE == kLsmr) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          delta_out[3 * i + a] = w[a];
+          if (t_out != nullptr && fw != nullptr) {  // damping rows of this point
+            const double ub = root * double(dpv[3 * i + a]) * dpx[a] - coef * t_out[3 * i + a];
+            t_out[3 * i + a] = ub;
+            uu += ub * ub;
+          }
+        }
       }
     }
+  }
+  if (MODE == kLsmr) {
+    const double b = block_sum(uu, smem);
+    if (threadIdx.x == 0) parts[blockIdx.x] = b;
   }
   if (MODE == kVerify) {
     const double v[6] = {n2, n2s, gd, g2, g2s, uu};
@@ -768,9 +816,12 @@ __global__ void __launch_bounds__(kBlock) k_points(
 //   kRhs:     rhs = Dc^-1 (-gc + T(acc)),  v = Jp t
 //   kVerify:  nr = T(acc) + lambda Dc^2 delta_c, v = J delta + r; norms and ||J delta||^2
 //   kJtu:     x_c = T(acc), v = u_k
+//   kLsmr:    x_c = T(acc) with v = U gathered from slot order (k_pslot holds the
+//             observation slots), then the camera damping rows
+//             U_c = sqrt(lambda) Dc ws_c - (alpha/beta) U_c and their ||.||^2 (no ws: x_c only)
 // T(acc) maps the rotation slots by (R Jr)^T.
 // ---------------------------------------------------------------------------------------
-enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4 };
+enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5 };
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_cameras(
@@ -779,7 +830,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
     const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
-    double* __restrict__ out_d, double* parts, const CgState* st, const int* halt) {
+    double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls) {
   __shared__ double ct[kCamTab];
   __shared__ double cfw[kFwd];
   __shared__ double red[9][kBlock / kWave];
@@ -787,7 +838,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     if (st->done) return;
     lambda = st->lambda;
   }
-  if (halt != nullptr && *halt) return;
+  if (ls != nullptr && ls->done) return;
   const int c = blockIdx.x;
   if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
   if ((MODE == cVerify || MODE == cSchur || MODE == cResid) && threadIdx.x < kFwd)
@@ -804,6 +855,9 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     if (MODE == cJtu) {
       v0 = rows_in[2 * k];
       v1 = rows_in[2 * k + 1];
+    } else if (MODE == cLsmr) {
+      v0 = tv.x;
+      v1 = tv.y;
     } else if (MODE == cVerify) {
       const double x[3] = {tv.x, tv.y, tv.z};
       double u0, u1, w0, w1;
@@ -828,6 +882,10 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
   auto gather = [&](int k) -> Vec4<double> {
     if (MODE == cJtu) return Vec4<double>{0.0, 0.0, 0.0, 0.0};
     const int ps = k_pslot[k];
+    if (MODE == cLsmr) {
+      const Vec2<double> u = reinterpret_cast<const Vec2<double>*>(rows_in)[ps];
+      return Vec4<double>{u.x, u.y, 0.0, 0.0};
+    }
     if (MODE == cVerify) return Vec4<double>{delta_p[3 * ps], delta_p[3 * ps + 1], delta_p[3 * ps + 2], 0.0};
     return reinterpret_cast<const Vec4<double>*>(tp)[ps];
   };
@@ -901,8 +959,20 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     } else if (MODE == cJtu) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) out_d[9 * c + i] = y[i];
+    } else if (MODE == cLsmr) {
+      const double coef = ls->alpha / ls->beta, root = ls->root;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        out_d[9 * c + i] = y[i];
+        if (vin != nullptr && out != nullptr) {  // damping rows of this camera
+          const double ub = root * d[i] * vin[9 * c + i] - coef * out[9 * c + i];
+          out[9 * c + i] = ub;
+          part0 += ub * ub;
+        }
+      }
+      parts[c] = part0;
     }
-    if (MODE != cJtu) {
+    if (MODE != cJtu && MODE != cLsmr) {
       double* pc = parts + kParts * c;
       pc[0] = part0;
       pc[1] = part1;

@@ -1,25 +1,21 @@
 // fg_lsmr_kernels.h -- device side of the LSMR backend (src/lsmr.rs): the right block
 // preconditioner, the vector updates and the scalar recurrences of LSMR (faer 0.24.4's
 // matrix_free::lsmr, i.e. Fong & Saunders 2011 without reorthogonalization). The operator
-// products reuse the Jacobian kernels (k_points<kJv>, k_points<kJtu>, k_cameras<cJtu>).
+// products are the fused point-major and camera-major passes k_points<kLsmr> and
+// k_cameras<cLsmr>.
 //
 // Vectors live in the library's internal order: cameras (9 nc, reference order) then points
-// (3 np, slot order); rows in reference row order, then the n damping rows. LSMR's u and v
-// are kept unnormalized (U, V) with their norms beta, alpha in device state, so no kernel
-// ever waits for a norm it has not itself summed.
+// (3 np, slot order). The Jacobian rows of U are kept in the point-major slot order (the
+// order of the records, so the point pass reads and writes them coalesced and the camera
+// pass gathers them), the n damping rows in coordinate order. Row order does not change
+// LSMR: only norms and J / J^T products of U enter. LSMR's u and v are kept unnormalized
+// (U, V) with their norms beta, alpha in device state, so no kernel ever waits for a norm it
+// has not itself summed.
 #pragma once
 #include "fg_kernels.h"
 
 namespace fg {
 
-struct LsmrState {
-  double alpha, beta, alphabar, zetabar, rho, rhobar, cbar, sbar;
-  double f1, f2, f3;        // hbar, y and h update factors of the current iteration
-  double norm_atb, abs_res;
-  int done, converged, iters, max_iters;
-  double rel_tol, root;             // sqrt(lambda)
-  double beta_new, alpha_new;        // norms summed this iteration, adopted by k_lsmr_scalar
-};
 
 // One block of P = L^-T (transpose = 0) or P^T = L^-1 (transpose = 1) in place on x: camera
 // blocks (9, factor lc) for c < nc, point blocks (3, factor lp) after. A zero factor
@@ -65,34 +61,6 @@ __global__ void k_lsmr_pv(int nc, int np, const double* __restrict__ v, const do
     const double d = blk < nc ? double(dc[off + i]) : double(dp[3 * (blk - nc) + i]);
     ws[off + i] = scaled ? x[i] * inv_alpha / d : x[i] * inv_alpha;
   }
-}
-
-// U = [J ws ; sqrt(lambda) D ws] - (alpha / beta) U with per-workgroup partials of ||U||^2.
-// The damping rows are absent for the undamped solve (n_bottom = 0).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_lsmr_u(int n_top, int nc, int n_bottom, const double* __restrict__ rows,
-                                                   const double* __restrict__ ws, const T* __restrict__ dc,
-                                                   const T* __restrict__ dp, const LsmrState* __restrict__ st,
-                                                   double* __restrict__ u, double* __restrict__ parts) {
-  __shared__ double smem[kBlock / kWave];
-  if (st->done) return;
-  const double a = st->alpha / st->beta, root = st->root;
-  double ss = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_top + n_bottom; i += gridDim.x * blockDim.x) {
-    double t;
-    if (i < n_top) {
-      t = rows[i];
-    } else {
-      const int j = i - n_top;
-      const double d = j < 9 * nc ? double(dc[j]) : double(dp[j - 9 * nc]);
-      t = root * d * ws[j];
-    }
-    const double un = t - a * u[i];
-    u[i] = un;
-    ss += un * un;
-  }
-  const double b = block_sum(ss, smem);
-  if (threadIdx.x == 0) parts[blockIdx.x] = b;
 }
 
 // beta (or alpha) = sqrt(sum of partials) into the state; which = 0 beta, 1 alpha.
@@ -227,24 +195,18 @@ static __global__ void k_lsmr_close(LsmrState* st) {
   if (threadIdx.x == 0 && blockIdx.x == 0 && st->done == 2) st->done = 1;
 }
 
-// U = b: the negative weighted residual in reference row order (camera-major copy), zero
-// damping rows; partials of ||U||^2.
+// U = b: the negative weighted residual in slot order (zero on padding slots) and zero
+// damping rows. Its norm comes from the first transpose pass.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_lsmr_init_u(int no, int n_bottom, const Vec2<T>* __restrict__ res_cm,
-                                                        double* __restrict__ u, double* __restrict__ parts) {
-  __shared__ double smem[kBlock / kWave];
-  double ss = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * no + n_bottom; i += gridDim.x * blockDim.x) {
-    double t = 0.0;
-    if (i < 2 * no) {
-      const Vec2<T> r = res_cm[i >> 1];
-      t = -double((i & 1) ? r.y : r.x);
-    }
-    u[i] = t;
-    ss += t * t;
+__global__ void k_lsmr_init_u(int nslots, int n_bottom, const int* __restrict__ slot_cam,
+                              const Vec2<T>* __restrict__ res, Vec2<double>* __restrict__ u,
+                              double* __restrict__ u_bottom) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) {
+    const Vec2<T> r = res[i];
+    u[i] = slot_cam[i] >= 0 ? Vec2<double>{-double(r.x), -double(r.y)} : Vec2<double>{0.0, 0.0};
   }
-  const double b = block_sum(ss, smem);
-  if (threadIdx.x == 0) parts[blockIdx.x] = b;
+  if (i < n_bottom) u_bottom[i] = 0.0;
 }
 
 // The scaled normal residual in block coordinates: per block, L^-1 (nr / D) and L^-1 (g / D)
