@@ -20,7 +20,8 @@ from ._lib import (CG_CONVERGED, CG_ITERATION_LIMIT, CG_NON_FINITE, CG_NONE, CG_
                    DEVICE_ERROR, DIMENSION_MISMATCH, FP32, FP64, INVALID_EMISSION, INVALID_EVALUATION,
                    INVALID_OPTIONS, LINEAR_SOLVE_FAILED, NO_CONVERGENCE, NO_PROGRESS, OK, PRECOND_BLOCK_JACOBI,
                    PRECOND_LEARNED, REJ_INVALID_EVALUATION, REJ_LINEAR_SOLVE, REJ_NONE, REJ_POOR_AGREEMENT,
-                   REJ_PREDICTION, TERM_COST, TERM_GRADIENT, TERM_NO_VARIABLES, TERM_NONE, TERM_STEP, FagraError,
+                   REJ_PREDICTION, TERM_COST, TERM_GRADIENT, TERM_NO_VARIABLES, TERM_NONE, TERM_STEP,
+                   UNSUPPORTED_STRUCTURE, INVALID_INPUT, FagraError,
                    LSMR_CONVERGED, LSMR_ITERATION_LIMIT, LSMR_NON_FINITE, LSMR_NONE, LmOptions, LmOutcome,
                    LsmrDiagnostics, LsmrTimings, OptimizeOptions, SchurDiagnostics, SchurTimings, check)
 

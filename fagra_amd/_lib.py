@@ -172,6 +172,19 @@ SIGNATURES = {
                                               ON_ACCEPT, _VP, ctypes.POINTER(LmOutcome)]),
     "fg_gn_solve_batch_lsmr": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(OptimizeOptions),
                                               ctypes.POINTER(LmOutcome)]),
+    "fg_gschur_create": (ctypes.c_int, [ctypes.POINTER(SchurConfig), _sz, _sz, _sz, ctypes.POINTER(_VP)]),
+    "fg_gschur_destroy": (None, [_VP]),
+    "fg_gschur_set_config": (ctypes.c_int, [_VP, ctypes.POINTER(SchurConfig)]),
+    "fg_gschur_prepare": (ctypes.c_int, [_VP, _sz]),
+    "fg_gschur_clear": (None, [_VP]),
+    "fg_gschur_accumulate": (ctypes.c_int, [_VP, _sz, _sz, _dp, _sz, _szp, _szp, _dp]),
+    "fg_gschur_gradient_norm": (ctypes.c_int, [_VP, _dp]),
+    "fg_gschur_prepare_damping": (ctypes.c_int, [_VP, _d]),
+    "fg_gschur_solve_damped": (ctypes.c_int, [_VP, _d, ctypes.POINTER(_dp), _dp]),
+    "fg_gschur_solve": (ctypes.c_int, [_VP, ctypes.POINTER(_dp)]),
+    "fg_gschur_diagnostics_get": (None, [_VP, ctypes.POINTER(SchurDiagnostics)]),
+    "fg_gschur_statistics": (None, [_VP, _szp, _szp]),
+    "fg_gschur_last_step_norm": (_d, [_VP]),
     "fg_bal_scan": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), ctypes.c_char_p, _sz]),
     "fg_bal_parse": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), _dp, _dp, _i64p, _i64p, _dp,
                                     ctypes.c_char_p, _sz]),

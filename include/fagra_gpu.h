@@ -307,6 +307,36 @@ void fg_lm_optimize(const fg_problem*, const fg_optimize_options*, const fg_lm_o
                     fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out);
 void fg_gn_optimize(const fg_problem*, const fg_optimize_options*, fg_lm_outcome* out);
 
+/* ---- Schur backend for emissions of any factor family ----
+ * Schur::new(retained, retained_width, eliminated_width) (src/schur.rs:116-178) over explicit
+ * Jacobian blocks that the caller's own model emits on the host (its FactorBatch::linearize),
+ * accumulated in bulk; the damped systems are solved on the device (f64, block-Jacobi PCG).
+ * Block widths up to 16 (wider: FG_UNSUPPORTED_STRUCTURE at create). The config must be FP64
+ * with block-Jacobi preconditioning. */
+typedef struct fg_gschur fg_gschur;
+fg_status fg_gschur_create(const fg_schur_config*, size_t retained, size_t retained_width,
+                           size_t eliminated_width, fg_gschur** out);
+void fg_gschur_destroy(fg_gschur*);
+fg_status fg_gschur_set_config(fg_gschur*, const fg_schur_config*);
+/* LeastSquaresBackend::prepare / clear (src/schur.rs:409-437) */
+fg_status fg_gschur_prepare(fg_gschur*, size_t dimension);
+void fg_gschur_clear(fg_gschur*);
+/* LeastSquaresBackend::accumulate (src/schur.rs:438-482, src/linear_model.rs:180-210) for n
+ * emissions of `rows` rows and n_blocks blocks each: residual[n*rows], cols[n*n_blocks] (block
+ * column offsets), widths[n_blocks], values: per emission, its blocks in order, each
+ * column-major rows x width. The batch is validated (DimensionMismatch, UnsupportedStructure)
+ * before any of it is appended. */
+fg_status fg_gschur_accumulate(fg_gschur*, size_t n, size_t rows, const double* residual, size_t n_blocks,
+                               const size_t* cols, const size_t* widths, const double* values);
+fg_status fg_gschur_gradient_norm(fg_gschur*, double* out);
+fg_status fg_gschur_prepare_damping(fg_gschur*, double min_column_norm);
+fg_status fg_gschur_solve_damped(fg_gschur*, double lambda, const double** delta, double* predicted);
+fg_status fg_gschur_solve(fg_gschur*, const double** delta);
+void fg_gschur_diagnostics_get(const fg_gschur*, fg_schur_diagnostics*);
+void fg_gschur_statistics(const fg_gschur*, size_t* solves, size_t* iterations);
+/* infinity norm of the last solved step (NaN when nonfinite), for fg_problem.solve_damped */
+double fg_gschur_last_step_norm(const fg_gschur*);
+
 /* ---- BAL problem text (examples/bal/model.rs:59-113) ---- */
 typedef struct {
   size_t n_cameras, n_points, n_observations;
