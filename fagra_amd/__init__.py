@@ -21,7 +21,8 @@ from ._lib import (CG_CONVERGED, CG_ITERATION_LIMIT, CG_NON_FINITE, CG_NONE, CG_
                    INVALID_OPTIONS, LINEAR_SOLVE_FAILED, NO_CONVERGENCE, NO_PROGRESS, OK, PRECOND_BLOCK_JACOBI,
                    PRECOND_LEARNED, REJ_INVALID_EVALUATION, REJ_LINEAR_SOLVE, REJ_NONE, REJ_POOR_AGREEMENT,
                    REJ_PREDICTION, TERM_COST, TERM_GRADIENT, TERM_NO_VARIABLES, TERM_NONE, TERM_STEP,
-                   UNSUPPORTED_STRUCTURE, INVALID_INPUT, FagraError,
+                   UNSUPPORTED_STRUCTURE, INVALID_INPUT, DUPLICATE_COVARIANCE_BLOCK, SINGULAR_INFORMATION,
+                   INVALID_INFORMATION, INVALID_RANK_TOLERANCE, UNKNOWN_KEY, FagraError,
                    LSMR_CONVERGED, LSMR_ITERATION_LIMIT, LSMR_NON_FINITE, LSMR_NONE, LmOptions, LmOutcome,
                    LsmrDiagnostics, LsmrTimings, OptimizeOptions, SchurDiagnostics, SchurTimings, check)
 
@@ -51,6 +52,45 @@ def default_lm_options():
     o = LmOptions()
     lib().fg_lm_options_default(ctypes.byref(o))
     return o
+
+
+def covariance_options(relative_pivot_tolerance=None):
+    o = _lib.CovarianceOptions()
+    lib().fg_covariance_options_default(ctypes.byref(o))
+    if relative_pivot_tolerance is not None:
+        o.relative_pivot_tolerance = relative_pivot_tolerance
+    return o
+
+
+def _selection(blocks):
+    offs = np.array([int(o) for o, _ in blocks], dtype=np.uintp)
+    wids = np.array([int(w) for _, w in blocks], dtype=np.uintp)
+    return offs, wids, int(wids.sum()) if len(wids) else 0
+
+
+def _covariance(fn, handle, blocks, tolerance):
+    """Selected covariance through fn (fg_schur_covariance / fg_gschur_covariance): blocks are
+    (offset, width) variable blocks in selection order; returns (status, k x k matrix)."""
+    offs, wids, k = _selection(blocks)
+    out = np.zeros((k, k), order="F")
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    st = fn(handle, offs.ctypes.data_as(szp), wids.ctypes.data_as(szp), len(offs),
+            ctypes.byref(covariance_options(tolerance)), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return st, (out if st == OK else None)
+
+
+def dense_covariance(information, columns, tolerance=None):
+    """CovarianceBackend for DenseNormalCholesky (src/covariance.rs:275-283) on the device:
+    the inverse of a dense information matrix at the selected coordinates."""
+    h = np.asfortranarray(information, dtype=np.float64)
+    n = h.shape[0]
+    cols = np.ascontiguousarray(columns, dtype=np.uintp)
+    out = np.zeros((len(cols), len(cols)), order="F")
+    st = lib().fg_dense_covariance(h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n,
+                                   cols.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), len(cols),
+                                   ctypes.byref(covariance_options(tolerance)),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return st, (out if st == OK else None)
 
 
 def device_info():
@@ -254,6 +294,10 @@ class Schur:
         t = SchurTimings()
         lib().fg_schur_timings_get(self.h, ctypes.byref(t))
         return t
+
+    def covariance(self, blocks, tolerance=None):
+        """Selected covariance of the current linearization (src/schur.rs:548-556)."""
+        return _covariance(lib().fg_schur_covariance, self.h, blocks, tolerance)
 
     # parity access
     def model(self):

@@ -8,6 +8,8 @@ LIB_PATH = os.path.join(HERE, "libfagra_gpu.so")
 OK, INVALID_OPTIONS, LINEAR_SOLVE_FAILED, NO_CONVERGENCE, NO_PROGRESS = 0, 1, 2, 3, 4
 INVALID_EVALUATION, DIMENSION_MISMATCH, INVALID_EMISSION, UNSUPPORTED_STRUCTURE = 5, 6, 7, 8
 INVALID_INPUT, DEVICE_ERROR = 9, 100
+DUPLICATE_COVARIANCE_BLOCK, SINGULAR_INFORMATION, INVALID_INFORMATION, INVALID_RANK_TOLERANCE = 10, 11, 12, 13
+UNKNOWN_KEY = 14
 FP64, FP32 = 0, 1
 PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1
 CG_NONE, CG_CONVERGED, CG_ITERATION_LIMIT, CG_NOT_POSITIVE_DEFINITE, CG_NON_FINITE = -1, 0, 1, 2, 3
@@ -18,7 +20,9 @@ STATUS_NAMES = {OK: "Ok", INVALID_OPTIONS: "InvalidOptions", LINEAR_SOLVE_FAILED
                 NO_CONVERGENCE: "NoConvergence", NO_PROGRESS: "NoProgress",
                 INVALID_EVALUATION: "InvalidEvaluation", DIMENSION_MISMATCH: "DimensionMismatch",
                 INVALID_EMISSION: "InvalidEmission", UNSUPPORTED_STRUCTURE: "UnsupportedStructure",
-                INVALID_INPUT: "InvalidInput", DEVICE_ERROR: "DeviceError"}
+                INVALID_INPUT: "InvalidInput", DUPLICATE_COVARIANCE_BLOCK: "DuplicateCovarianceBlock",
+                SINGULAR_INFORMATION: "SingularInformation", INVALID_INFORMATION: "InvalidInformation",
+                INVALID_RANK_TOLERANCE: "InvalidRankTolerance", UNKNOWN_KEY: "Unknown", DEVICE_ERROR: "DeviceError"}
 
 _sz = ctypes.c_size_t
 _d = ctypes.c_double
@@ -74,6 +78,10 @@ class LsmrConfig(ctypes.Structure):
     _fields_ = [("max_iterations", _sz), ("relative_tolerance", _d), ("diagonal_preconditioning", ctypes.c_int),
                 ("block_preconditioning", ctypes.c_int), ("collect_timings", ctypes.c_int),
                 ("precision", ctypes.c_int), ("on_solve", ON_LSMR_SOLVE), ("on_solve_user", _VP)]
+
+
+class CovarianceOptions(ctypes.Structure):
+    _fields_ = [("relative_pivot_tolerance", _d)]
 
 
 class BalSizes(ctypes.Structure):
@@ -185,6 +193,10 @@ SIGNATURES = {
     "fg_gschur_diagnostics_get": (None, [_VP, ctypes.POINTER(SchurDiagnostics)]),
     "fg_gschur_statistics": (None, [_VP, _szp, _szp]),
     "fg_gschur_last_step_norm": (_d, [_VP]),
+    "fg_covariance_options_default": (ctypes.c_int, [ctypes.POINTER(CovarianceOptions)]),
+    "fg_dense_covariance": (ctypes.c_int, [_dp, _sz, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
+    "fg_schur_covariance": (ctypes.c_int, [_VP, _szp, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
+    "fg_gschur_covariance": (ctypes.c_int, [_VP, _szp, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
     "fg_bal_scan": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), ctypes.c_char_p, _sz]),
     "fg_bal_parse": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), _dp, _dp, _i64p, _i64p, _dp,
                                     ctypes.c_char_p, _sz]),

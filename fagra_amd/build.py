@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfagra_gpu.so")
-SOURCES = ["fg_device.hip", "fg_mlp.hip", "fg_generic.hip", "fg_lm.cpp", "fg_bal.cpp"]
+SOURCES = ["fg_device.hip", "fg_mlp.hip", "fg_generic.hip", "fg_covariance.hip", "fg_lm.cpp", "fg_bal.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FAGRA_ARCH", "gfx950")
 

@@ -1,0 +1,368 @@
+// fg_covariance.hip -- selected covariance from undamped information on the device
+// (SelectedCovariance, src/covariance.rs:163-262; DenseNormalCholesky::information_factor,
+// src/normal.rs:341-392).
+//
+// The information H (n x n, column-major, lower triangle significant) is factored in place by
+// a right-looking blocked Cholesky in the reference's coordinate order (64-column panels: the
+// diagonal tile factored in LDS by one workgroup, the panel below solved row by row against
+// that tile, the trailing lower triangle updated by 64 x 64 LDS-tiled f64 products). The
+// reference's rank screen follows (pivot L_ii against sqrt(H_ii)), then Y = L^-1 E for the
+// selected unit columns E by blocked forward substitution, and the covariance Y^T Y.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "fg_internal.h"
+#include "fg_kernels.h"
+
+namespace {
+
+using namespace fg;
+
+constexpr int kNB = 64;  // panel width
+
+#define CV_TRY(x)                                  \
+  do {                                             \
+    if ((x) != hipSuccess) return FG_DEVICE_ERROR; \
+  } while (0)
+
+// diagonal copy and finiteness of the lower triangle (normal.rs:350-358)
+__global__ void k_cov_check(const double* __restrict__ h, size_t n, double* __restrict__ diag, int* flags) {
+  const size_t total = n * n;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t col = idx / n, row = idx % n;
+    if (row < col) continue;
+    const double v = h[idx];
+    if (!isfinite(v)) atomicOr(flags, 1);
+    if (row == col) diag[col] = v;
+  }
+}
+
+// Cholesky of the diagonal tile [k0, k0 + kb) in LDS; a nonpositive or nonfinite pivot sets
+// flags[1] (faer's LLT error, SingularInformation)
+__global__ void __launch_bounds__(kBlock) k_cov_potrf(double* __restrict__ a, int n, int k0, int kb, int* flags) {
+  __shared__ double t[kNB][kNB + 1];
+  __shared__ int fail;
+  if (flags[1]) return;
+  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+    const int r = e % kNB, c = e / kNB;
+    t[r][c] = (r < kb && c < kb && r >= c) ? a[(size_t)(k0 + c) * n + k0 + r] : 0.0;
+  }
+  if (threadIdx.x == 0) fail = 0;
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    if (threadIdx.x == 0) {
+      const double d = t[j][j];
+      if (!(d > 0.0) || !isfinite(d)) fail = 1;
+      else t[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    if (fail) break;
+    const double piv = t[j][j];
+    for (int i = j + 1 + threadIdx.x; i < kb; i += blockDim.x) t[i][j] /= piv;
+    __syncthreads();
+    const int m = kb - j - 1;
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+      const int i = j + 1 + e / m, l = j + 1 + e % m;
+      if (i >= l) t[i][l] -= t[i][j] * t[l][j];
+    }
+    __syncthreads();
+  }
+  if (fail) {
+    if (threadIdx.x == 0) flags[1] = 1;
+    return;
+  }
+  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+    const int r = e % kNB, c = e / kNB;
+    if (r < kb && c < kb && r >= c) a[(size_t)(k0 + c) * n + k0 + r] = t[r][c];
+  }
+}
+
+// Load the factored diagonal tile, padded with the identity beyond kb.
+__device__ __forceinline__ void load_tile(const double* __restrict__ a, int n, int k0, int kb,
+                                          double (*l)[kNB + 1]) {
+  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+    const int r = e % kNB, c = e / kNB;
+    double v = 0.0;
+    if (r < kb && c < kb) {
+      if (r >= c) v = a[(size_t)(k0 + c) * n + k0 + r];
+    } else if (r == c) {
+      v = 1.0;
+    }
+    l[r][c] = v;
+  }
+}
+
+// Panel rows below the tile: x L^T = a for each row (one thread per row)
+__global__ void __launch_bounds__(kBlock) k_cov_trsm(double* __restrict__ a, int n, int k0, int kb, const int* flags) {
+  __shared__ double l[kNB][kNB + 1];
+  if (flags[1]) return;
+  load_tile(a, n, k0, kb, l);
+  __syncthreads();
+  const int i = k0 + kb + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[kNB];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) x[j] = j < kb ? a[(size_t)(k0 + j) * n + i] : 0.0;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    double v = x[j];
+#pragma unroll
+    for (int t = 0; t < j; ++t) v -= x[t] * l[j][t];
+    x[j] = v / l[j][j];
+  }
+#pragma unroll
+  for (int j = 0; j < kNB; ++j)
+    if (j < kb) a[(size_t)(k0 + j) * n + i] = x[j];
+}
+
+// Trailing update A[i, j] -= sum_t P[i, t] P[j, t] for m0 <= j <= i < n over 64 x 64 lower tiles;
+// each thread owns a 4 x 4 strided block of its tile.
+__global__ void __launch_bounds__(kBlock) k_cov_syrk(double* __restrict__ a, int n, int k0, int kb, int m0,
+                                                     const int* flags) {
+  __shared__ double p[kNB][kNB + 1];
+  __shared__ double q[kNB][kNB + 1];
+  if (flags[1]) return;
+  const long long qi = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * (double)qi + 1.0) - 1.0) / 2.0);
+  while ((long long)(ti + 1) * (ti + 2) / 2 <= qi) ++ti;
+  while ((long long)ti * (ti + 1) / 2 > qi) --ti;
+  const int tj = (int)(qi - (long long)ti * (ti + 1) / 2);
+  const int r0 = m0 + kNB * ti, c0 = m0 + kNB * tj;
+  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+    const int r = e % kNB, t = e / kNB;
+    p[r][t] = (t < kb && r0 + r < n) ? a[(size_t)(k0 + t) * n + r0 + r] : 0.0;
+    q[r][t] = (t < kb && c0 + r < n) ? a[(size_t)(k0 + t) * n + c0 + r] : 0.0;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int t = 0; t < kb; ++t) {
+    double pv[4], qv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pv[u] = p[ty + 16 * u][t];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qv[v] = q[tx + 16 * v][t];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += pv[u] * qv[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = r0 + ty + 16 * u, col = c0 + tx + 16 * v;
+      if (row < n && col < n && row >= col) a[(size_t)col * n + row] -= acc[u][v];
+    }
+}
+
+// rank screen (normal.rs:380-390): the first failing coordinate decides, as in the reference's
+// sequential loop; code 2 = nonfinite (InvalidInformation), 1 = SingularInformation
+__global__ void k_cov_screen(const double* __restrict__ l, int n, const double* __restrict__ diag, double tol,
+                             int* first) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double pivot = l[(size_t)i * n + i], d = diag[i];
+  int code = 0;
+  if (!isfinite(pivot) || !isfinite(d)) {
+    code = 2;
+  } else {
+    const double ratio = pivot / sqrt(d);
+    if (d <= 0.0 || pivot <= 0.0 || ratio * ratio <= tol) code = 1;
+  }
+  if (code) atomicMin(first, 4 * i + code);
+}
+
+// E: unit columns at the selected coordinates
+__global__ void k_cov_unit(double* __restrict__ y, int n, int k, const int* __restrict__ cols) {
+  const size_t total = (size_t)n * k;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+    const int col = (int)(idx / n), row = (int)(idx % n);
+    y[idx] = row == cols[col] ? 1.0 : 0.0;
+  }
+}
+
+// Y[k0 : k0 + kb, a] = L_kk^-1 Y[k0 : k0 + kb, a], one thread per selected column
+__global__ void __launch_bounds__(kBlock) k_cov_fwd_tile(const double* __restrict__ a, int n, int k0, int kb,
+                                                         double* __restrict__ y, int k) {
+  __shared__ double l[kNB][kNB + 1];
+  load_tile(a, n, k0, kb, l);
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double x[kNB];
+  double* yc = y + (size_t)c * n;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) x[j] = j < kb ? yc[k0 + j] : 0.0;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    double v = x[j];
+#pragma unroll
+    for (int t = 0; t < j; ++t) v -= l[j][t] * x[t];
+    x[j] = v / l[j][j];
+  }
+#pragma unroll
+  for (int j = 0; j < kNB; ++j)
+    if (j < kb) yc[k0 + j] = x[j];
+}
+
+// Y[i, a] -= sum_t L[i, k0 + t] Y[k0 + t, a] for rows below the tile
+__global__ void __launch_bounds__(kBlock) k_cov_fwd_update(const double* __restrict__ a, int n, int k0, int kb,
+                                                           double* __restrict__ y, int k) {
+  __shared__ double yt[kNB];
+  const int c = blockIdx.y;
+  double* yc = y + (size_t)c * n;
+  if (threadIdx.x < kNB) yt[threadIdx.x] = threadIdx.x < kb ? yc[k0 + threadIdx.x] : 0.0;
+  __syncthreads();
+  const int i = k0 + kb + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = yc[i];
+  for (int t = 0; t < kb; ++t) v -= a[(size_t)(k0 + t) * n + i] * yt[t];
+  yc[i] = v;
+}
+
+// C[a, b] = Y[:, a] . Y[:, b] for b >= a (one workgroup each), mirrored; nonfinite -> flag
+__global__ void __launch_bounds__(kBlock) k_cov_gram(const double* __restrict__ y, int n, int k,
+                                                     double* __restrict__ c, int* flags) {
+  __shared__ double smem[kBlock / kWave];
+  const int qi = blockIdx.x;
+  int b = (int)((sqrt(8.0 * (double)qi + 1.0) - 1.0) / 2.0);
+  while ((b + 1) * (b + 2) / 2 <= qi) ++b;
+  while (b * (b + 1) / 2 > qi) --b;
+  const int a2 = qi - b * (b + 1) / 2;  // a2 <= b
+  const double* ya = y + (size_t)a2 * n;
+  const double* yb = y + (size_t)b * n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += ya[i] * yb[i];
+  s = block_sum(s, smem);
+  if (threadIdx.x == 0) {
+    if (!isfinite(s)) atomicOr(flags, 4);
+    c[(size_t)b * k + a2] = s;
+    c[(size_t)a2 * k + b] = s;
+  }
+}
+
+int grid_of(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kBlock - 1) / kBlock, 65535)); }
+
+}  // namespace
+
+namespace fg {
+
+fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* columns, size_t k, double tolerance,
+                           double* out) {
+  if (!std::isfinite(tolerance) || tolerance < 0.0 || tolerance >= 1.0) return FG_INVALID_RANK_TOLERANCE;
+  for (size_t a = 0; a < k; ++a)
+    if (columns[a] >= n) return FG_UNKNOWN_KEY;
+  if (n > (size_t)INT_MAX / 2 || k > (size_t)INT_MAX / 2 || (k && n > SIZE_MAX / 8 / k)) return FG_INVALID_INFORMATION;
+  if (n == 0) return FG_OK;  // k == 0 here: the empty selection of an empty model
+  double *diag = nullptr, *y = nullptr, *c = nullptr;
+  int *flags = nullptr, *cols = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)diag, (void*)y, (void*)c, (void*)flags, (void*)cols})
+      if (p) (void)hipFree(p);
+  };
+  auto fail = [&](fg_status st) {
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    return st;
+  };
+  if (hipMalloc(&diag, n * sizeof(double)) != hipSuccess || hipMalloc(&flags, 4 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&y, std::max<size_t>(1, n * k) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c, std::max<size_t>(1, k * k) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&cols, std::max<size_t>(1, k) * sizeof(int)) != hipSuccess)
+    return fail(FG_DEVICE_ERROR);
+  const int ni = (int)n, ki = (int)k;
+  std::vector<int> hc(columns, columns + k);
+  int init[4] = {0, 0, INT_MAX, 0};
+  if (hipMemcpyAsync(flags, init, sizeof(init), hipMemcpyHostToDevice, s) != hipSuccess ||
+      (k && hipMemcpyAsync(cols, hc.data(), k * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess))
+    return fail(FG_DEVICE_ERROR);
+  hipLaunchKernelGGL(k_cov_check, dim3(grid_of(n * n)), dim3(kBlock), 0, s, (const double*)h, n, diag, flags);
+  int hf[4];
+  if (hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(FG_DEVICE_ERROR);
+  if (hf[0]) return fail(FG_INVALID_INFORMATION);  // nonfinite information
+  for (int k0 = 0; k0 < ni; k0 += kNB) {
+    const int kb = std::min(kNB, ni - k0), m0 = k0 + kb, rest = ni - m0;
+    hipLaunchKernelGGL(k_cov_potrf, dim3(1), dim3(kBlock), 0, s, h, ni, k0, kb, flags);
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_cov_trsm, dim3(grid_of(rest)), dim3(kBlock), 0, s, h, ni, k0, kb, (const int*)flags);
+      const long long m = (rest + kNB - 1) / kNB;
+      hipLaunchKernelGGL(k_cov_syrk, dim3((unsigned)(m * (m + 1) / 2)), dim3(kBlock), 0, s, h, ni, k0, kb, m0,
+                         (const int*)flags);
+    }
+  }
+  hipLaunchKernelGGL(k_cov_screen, dim3(grid_of(n)), dim3(kBlock), 0, s, (const double*)h, ni, (const double*)diag,
+                     tolerance, flags + 2);
+  if (hipGetLastError() != hipSuccess) return fail(FG_DEVICE_ERROR);
+  if (hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(FG_DEVICE_ERROR);
+  if (hf[1]) return fail(FG_SINGULAR_INFORMATION);  // the factorization failed
+  if (hf[2] != INT_MAX) return fail((hf[2] & 3) == 2 ? FG_INVALID_INFORMATION : FG_SINGULAR_INFORMATION);
+  if (k == 0) {
+    cleanup();
+    return FG_OK;
+  }
+  hipLaunchKernelGGL(k_cov_unit, dim3(grid_of(n * k)), dim3(kBlock), 0, s, y, ni, ki, (const int*)cols);
+  for (int k0 = 0; k0 < ni; k0 += kNB) {
+    const int kb = std::min(kNB, ni - k0), rest = ni - k0 - kb;
+    hipLaunchKernelGGL(k_cov_fwd_tile, dim3(grid_of(k)), dim3(kBlock), 0, s, (const double*)h, ni, k0, kb, y, ki);
+    if (rest > 0)
+      hipLaunchKernelGGL(k_cov_fwd_update, dim3(grid_of(rest), ki), dim3(kBlock), 0, s, (const double*)h, ni, k0,
+                         kb, y, ki);
+  }
+  hipLaunchKernelGGL(k_cov_gram, dim3((unsigned)(k * (k + 1) / 2)), dim3(kBlock), 0, s, (const double*)y, ni, ki, c,
+                     flags);
+  if (hipGetLastError() != hipSuccess) return fail(FG_DEVICE_ERROR);
+  if (hipMemcpyAsync(out, c, k * k * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(FG_DEVICE_ERROR);
+  cleanup();
+  return (hf[0] & 4) ? FG_INVALID_INFORMATION : FG_OK;
+}
+
+}  // namespace fg
+
+extern "C" {
+
+fg_status fg_covariance_options_default(fg_covariance_options* o) {
+  o->relative_pivot_tolerance = 64.0 * 2.220446049250313e-16;  // 64 epsilon (covariance.rs:145-151)
+  return FG_OK;
+}
+
+fg_status fg_dense_covariance(const double* information, size_t n, const size_t* columns, size_t k,
+                              const fg_covariance_options* options, double* out) {
+  if (!std::isfinite(options->relative_pivot_tolerance) || options->relative_pivot_tolerance < 0.0 ||
+      options->relative_pivot_tolerance >= 1.0)
+    return FG_INVALID_RANK_TOLERANCE;
+  for (size_t a = 0; a < k; ++a) {
+    if (columns[a] >= n) return FG_UNKNOWN_KEY;
+    for (size_t b = 0; b < a; ++b)
+      if (columns[b] == columns[a]) return FG_DUPLICATE_COVARIANCE_BLOCK;
+  }
+  if (n == 0) return FG_OK;
+  hipStream_t s = nullptr;
+  double* h = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return FG_DEVICE_ERROR;
+  fg_status st = FG_DEVICE_ERROR;
+  if (hipMalloc(&h, n * n * sizeof(double)) == hipSuccess &&
+      hipMemcpyAsync(h, information, n * n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess)
+    st = fg::dense_covariance(s, h, n, columns, k, options->relative_pivot_tolerance, out);
+  (void)hipStreamSynchronize(s);
+  if (h) (void)hipFree(h);
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+}  // extern "C"

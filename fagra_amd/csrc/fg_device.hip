@@ -67,6 +67,7 @@ struct fg_ba {
   std::vector<int64_t> cam_order, cam_off, pt_perm, pt_off;
   // Point slots: points by track length, longest first (stable).
   std::vector<int32_t> point_of_slot, slot_of_point;
+  int* d_point_of_slot = nullptr;  // uploaded on the first covariance query
   int nchunks = 0;
   size_t nslots = 0;
   // Device static data.
@@ -1012,6 +1013,7 @@ void fg_ba_destroy(fg_ba* b) {
   dfree(b->d_k_slot);
   dfree(b->d_k_pslot);
   dfree(b->d_pt_len);
+  dfree(b->d_point_of_slot);
   if (b->fp32 && b->d_slot_pix64) (void)hipFree(b->d_slot_pix64);
   if (b->d_slot_pix) (void)hipFree(b->d_slot_pix);
   for (int i = 0; i < 2; ++i) {
@@ -1376,6 +1378,119 @@ fg_status fg_schur_damped_transpose_apply(fg_schur* s, double lambda, int scaled
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Covariance (CovarianceBackend for Schur, src/schur.rs:548-556, via LinearizedModel::
+// covariance, src/linear_model.rs:122-167): the undamped information J^T J of the current
+// linearization in reference coordinates, assembled densely on the device from the camera
+// Grams, the point Grams and per-observation J_p^T J_c cross blocks.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+__global__ void k_info_cameras(int nc, size_t n, const T* __restrict__ hcc, double* __restrict__ h) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nc * 81) return;
+  const int c = t / 81, i = (t % 81) / 9, j = t % 9;
+  if (i < j) return;
+  h[(size_t)(9 * c + j) * n + 9 * c + i] = double(hcc[45 * c + sym9(i, j)]);
+}
+
+__global__ void k_info_points(int np, size_t n, size_t base, const double* __restrict__ hpp,
+                              const int* __restrict__ point_of_slot, double* __restrict__ h) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np * 9) return;
+  const int sl = t / 9, a = (t % 9) / 3, b = t % 3;
+  if (a < b) return;
+  const size_t p = base + 3 * (size_t)point_of_slot[sl];
+  h[(p + b) * n + p + a] = hpp[6 * sl + sym3(a, b)];
+}
+
+// One workgroup per camera: rows of its observations, cross blocks J_p^T J_c into the lower
+// triangle (point rows, camera columns); a point seen twice by a camera adds twice.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_info_cross(int nc, size_t n, size_t base, const int* __restrict__ cam_off,
+                                                       const int* __restrict__ k_pslot,
+                                                       const int* __restrict__ point_of_slot,
+                                                       const Vec4<T>* __restrict__ rec_cm, const T* __restrict__ tab,
+                                                       double* __restrict__ h) {
+  __shared__ double ct[kCamTab];
+  const int c = blockIdx.x;
+  if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
+  __syncthreads();
+  for (int k = cam_off[c] + threadIdx.x; k < cam_off[c + 1]; k += blockDim.x) {
+    ObsGeom<double> geo;
+    obs_geom(to_d(rec_cm[k]), ct, geo);
+    double jc[18], jp[2][3];
+    jc_block(geo, ct, jc);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      jp[0][a] = geo.sw * (geo.dp0[0] * ct[tR(0, a)] + geo.dp0[1] * ct[tR(1, a)] + geo.dp0[2] * ct[tR(2, a)]);
+      jp[1][a] = geo.sw * (geo.dp1[0] * ct[tR(0, a)] + geo.dp1[1] * ct[tR(1, a)] + geo.dp1[2] * ct[tR(2, a)]);
+    }
+    const size_t p = base + 3 * (size_t)point_of_slot[k_pslot[k]];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int i = 0; i < 9; ++i)
+        atomicAdd(&h[(size_t)(9 * c + i) * n + p + a], jp[0][a] * jc[i] + jp[1][a] * jc[9 + i]);
+  }
+}
+
+template <typename T>
+fg_status covariance_impl(fg_schur* s, const std::vector<size_t>& cols, double tol, double* out) {
+  fg_ba* b = s->ba;
+  const size_t nc = b->nc, np = b->np, n = 9 * nc + 3 * np;
+  if (!b->d_point_of_slot) {
+    std::vector<int> pos(b->point_of_slot.begin(), b->point_of_slot.end());
+    pos.resize(std::max<size_t>(1, np), 0);
+    FG_TRY(dalloc(&b->d_point_of_slot, pos.size()));
+    FG_TRY(hipMemcpy(b->d_point_of_slot, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  if (n > 0 && n > SIZE_MAX / 8 / n) return FG_INVALID_INFORMATION;
+  double* h = nullptr;
+  FG_TRY(hipMalloc(&h, std::max<size_t>(1, n * n) * sizeof(double)));
+  fg_status st = FG_DEVICE_ERROR;
+  if (hipMemsetAsync(h, 0, n * n * sizeof(double), b->stream) == hipSuccess) {
+    if (nc)
+      hipLaunchKernelGGL(k_info_cameras<T>, dim3(grid_for(nc * 81)), dim3(kBlock), 0, b->stream, (int)nc, n,
+                         (const T*)s->hcc, h);
+    if (np)
+      hipLaunchKernelGGL(k_info_points, dim3(grid_for(np * 9)), dim3(kBlock), 0, b->stream, (int)np, n, 9 * nc,
+                         (const double*)s->hpp, (const int*)b->d_point_of_slot, h);
+    if (nc)
+      hipLaunchKernelGGL(k_info_cross<T>, dim3((unsigned)nc), dim3(kBlock), 0, b->stream, (int)nc, n, 9 * nc,
+                         b->d_cam_off, b->d_k_pslot, (const int*)b->d_point_of_slot, (const Vec4<T>*)s->rec_cm,
+                         (const T*)b->d_tab[b->cur], h);
+    if (hipGetLastError() == hipSuccess) st = fg::dense_covariance(b->stream, h, n, cols.data(), cols.size(), tol, out);
+  }
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(h);
+  return st;
+}
+
+}  // namespace
+
+extern "C" fg_status fg_schur_covariance(fg_schur* s, const size_t* offsets, const size_t* widths, size_t n_blocks,
+                                         const fg_covariance_options* options, double* out) {
+  const double tol = options->relative_pivot_tolerance;
+  if (!std::isfinite(tol) || tol < 0.0 || tol >= 1.0) return FG_INVALID_RANK_TOLERANCE;
+  if (!s->ba || !s->prepared) return FG_INVALID_OPTIONS;
+  const size_t nc = s->ba->nc, n = 9 * nc + 3 * s->ba->np;
+  std::vector<size_t> cols;
+  for (size_t i = 0; i < n_blocks; ++i) {  // SelectedCovariance::prepare (src/covariance.rs:180-207)
+    const size_t o = offsets[i], w = widths[i];
+    for (size_t j = 0; j < i; ++j)
+      if (offsets[j] == o) return FG_DUPLICATE_COVARIANCE_BLOCK;
+    const bool cam = o < 9 * nc && o % 9 == 0 && w == 9;
+    const bool pt = o >= 9 * nc && o < n && (o - 9 * nc) % 3 == 0 && w == 3;
+    if (!cam && !pt) return FG_UNKNOWN_KEY;
+    for (size_t c = o; c < o + w; ++c) cols.push_back(c);
+  }
+  if (!s->linearized) FG_CHECK(fg_schur_linearize(s));  // the current state's model (optimization.rs:269-283)
+  return s->cfg.precision == FG_FP32 ? covariance_impl<float>(s, cols, tol, out)
+                                     : covariance_impl<double>(s, cols, tol, out);
+}
 
 // ------------------------------------------------------------------------------------------
 // LSMR backend (src/lsmr.rs). The model, scales and block factors are those of an owned

@@ -535,6 +535,37 @@ __global__ void __launch_bounds__(kBlock) k_gs_absmax(int n, const double* __res
   }
 }
 
+// Dense undamped information J^T J (LinearizedModel::covariance, linear_model.rs:122-167):
+// one thread per emission adds its block products into the lower triangle.
+__global__ void k_gs_info(int ne, int rw, int ew, int retained, size_t n, const Em* __restrict__ em,
+                          const double* __restrict__ vals, double* __restrict__ h) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const Em m = em[e];
+  const int nb = (m.rblk >= 0) + (m.eblk >= 0);
+  int col[2], w[2], off[2], k = 0;
+  if (m.rblk >= 0) {
+    col[k] = m.rblk * rw;
+    w[k] = rw;
+    off[k++] = m.roff;
+  }
+  if (m.eblk >= 0) {
+    col[k] = retained + m.eblk * ew;
+    w[k] = ew;
+    off[k++] = m.eoff;
+  }
+  for (int x = 0; x < nb; ++x)
+    for (int y = 0; y <= x; ++y)  // block x has the larger columns (retained before eliminated)
+      for (int ca = 0; ca < w[x]; ++ca)
+        for (int cb = 0; cb < w[y]; ++cb) {
+          const size_t r = (size_t)col[x] + ca, c = (size_t)col[y] + cb;
+          if (r < c) continue;
+          double v = 0.0;
+          for (int i = 0; i < m.rows; ++i) v += vals[off[x] + ca * m.rows + i] * vals[off[y] + cb * m.rows + i];
+          atomicAdd(&h[c * n + r], v);
+        }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -1090,5 +1121,39 @@ void fg_gschur_statistics(const fg_gschur* g, size_t* solves, size_t* iterations
   if (iterations) *iterations = g->iterations;
 }
 double fg_gschur_last_step_norm(const fg_gschur* g) { return g->last_step_norm; }
+
+// CovarianceBackend for Schur (src/schur.rs:548-556) over the accumulated emissions.
+fg_status fg_gschur_covariance(fg_gschur* g, const size_t* offsets, const size_t* widths, size_t n_blocks,
+                               const fg_covariance_options* options, double* out) {
+  const double tol = options->relative_pivot_tolerance;
+  if (!std::isfinite(tol) || tol < 0.0 || tol >= 1.0) return FG_INVALID_RANK_TOLERANCE;
+  if (!g->prepared) return FG_INVALID_OPTIONS;
+  const size_t n = g->dim;
+  std::vector<size_t> cols;
+  for (size_t i = 0; i < n_blocks; ++i) {  // SelectedCovariance::prepare (src/covariance.rs:180-207)
+    const size_t o = offsets[i], w = widths[i];
+    for (size_t j = 0; j < i; ++j)
+      if (offsets[j] == o) return FG_DUPLICATE_COVARIANCE_BLOCK;
+    const bool ret = o < g->retained && o % g->rw == 0 && w == g->rw;
+    const bool elim = o >= g->retained && o < n && (o - g->retained) % g->ew == 0 && w == g->ew;
+    if (!ret && !elim) return FG_UNKNOWN_KEY;
+    for (size_t c = o; c < o + w; ++c) cols.push_back(c);
+  }
+  GS_CHECK(gs_finalize(g));
+  if (n > 0 && n > SIZE_MAX / 8 / n) return FG_INVALID_INFORMATION;
+  double* h = nullptr;
+  GS_TRY(hipMalloc(&h, std::max<size_t>(1, n * n) * sizeof(double)));
+  fg_status st = FG_DEVICE_ERROR;
+  const int ne = (int)g->h_em.size();
+  if (hipMemsetAsync(h, 0, n * n * sizeof(double), g->stream) == hipSuccess) {
+    if (ne)
+      hipLaunchKernelGGL(k_gs_info, dim3(blocks_for(ne)), dim3(kBlock), 0, g->stream, ne, (int)g->rw, (int)g->ew,
+                         (int)g->retained, n, (const Em*)g->d_em, (const double*)g->d_vals, h);
+    if (hipGetLastError() == hipSuccess) st = fg::dense_covariance(g->stream, h, n, cols.data(), cols.size(), tol, out);
+  }
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(h);
+  return st;
+}
 
 }  // extern "C"

@@ -34,4 +34,9 @@ fg_status mlp_reserve(fg_mlp* m, size_t n);
 // y[n][48] = network(x[n][64]) on the given stream; x and y are device buffers of >= npad rows.
 void mlp_forward(fg_mlp* m, hipStream_t stream, size_t npad, const uint16_t* x, float* y);
 inline size_t pad64(size_t n) { return (n + 63) / 64 * 64; }
+// Selected covariance of the dense information h (device, n x n column-major, lower triangle
+// significant; overwritten by its Cholesky factor) at the given coordinates, k x k into the
+// host array out, on stream s. Status codes of SelectedCovariance::compute.
+fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* columns, size_t k, double tolerance,
+                           double* out);
 }  // namespace fg

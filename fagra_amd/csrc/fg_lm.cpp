@@ -285,6 +285,11 @@ const char* fg_status_string(int status) {
     case FG_INVALID_EMISSION: return "invalid factor scope or emitted variable identity";
     case FG_UNSUPPORTED_STRUCTURE: return "residual structure is unsupported by the selected backend";
     case FG_INVALID_INPUT: return "invalid BAL input";
+    case FG_DUPLICATE_COVARIANCE_BLOCK: return "duplicate variable in covariance selection";
+    case FG_SINGULAR_INFORMATION: return "undamped information is singular or numerically rank deficient";
+    case FG_INVALID_INFORMATION: return "invalid or nonfinite covariance information";
+    case FG_INVALID_RANK_TOLERANCE: return "relative rank tolerance must be finite and in [0, 1)";
+    case FG_UNKNOWN_KEY: return "unknown handle";
     case FG_DEVICE_ERROR: return "HIP runtime error";
     default: return "unknown status";
   }

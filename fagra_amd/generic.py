@@ -148,6 +148,11 @@ class GenericSchur:
     def last_step_norm(self):
         return _lib_handle().fg_gschur_last_step_norm(self.h)
 
+    def covariance(self, blocks, tolerance=None):
+        """Selected covariance (offset, width blocks in selection order) of the accumulated model."""
+        from . import _covariance
+        return _covariance(_lib_handle().fg_gschur_covariance, self.h, blocks, tolerance)
+
 
 class _Callbacks:
     """fg_problem callbacks joining a host model and a GenericSchur backend."""

@@ -36,6 +36,11 @@ typedef enum {
   FG_INVALID_EMISSION = 7,       /* EvaluationError::InvalidEmission */
   FG_UNSUPPORTED_STRUCTURE = 8,  /* EvaluationError::UnsupportedStructure */
   FG_INVALID_INPUT = 9,          /* BAL text rejected (Problem::parse's Err(String)) */
+  FG_DUPLICATE_COVARIANCE_BLOCK = 10, /* SolverError::DuplicateCovarianceBlock */
+  FG_SINGULAR_INFORMATION = 11,  /* SolverError::SingularInformation */
+  FG_INVALID_INFORMATION = 12,   /* SolverError::InvalidInformation */
+  FG_INVALID_RANK_TOLERANCE = 13, /* SolverError::InvalidRankTolerance */
+  FG_UNKNOWN_KEY = 14,           /* KeyError::Unknown (a selection that is not a variable block) */
   FG_DEVICE_ERROR = 100          /* HIP runtime failure; no reference counterpart */
 } fg_status;
 
@@ -336,6 +341,27 @@ void fg_gschur_diagnostics_get(const fg_gschur*, fg_schur_diagnostics*);
 void fg_gschur_statistics(const fg_gschur*, size_t* solves, size_t* iterations);
 /* infinity norm of the last solved step (NaN when nonfinite), for fg_problem.solve_damped */
 double fg_gschur_last_step_norm(const fg_gschur*);
+
+/* ---- selected covariance (src/covariance.rs) ---- */
+/* CovarianceOptions (src/covariance.rs:136-160): default 64 epsilon, finite and in [0, 1). */
+typedef struct {
+  double relative_pivot_tolerance;
+} fg_covariance_options;
+fg_status fg_covariance_options_default(fg_covariance_options*);
+/* CovarianceBackend for DenseNormalCholesky (src/covariance.rs:275-283): the inverse of a dense
+ * information matrix (n x n column-major, lower triangle read) at the selected coordinates,
+ * k x k column-major into out, factored on the device with the reference's rank screen
+ * (src/normal.rs:372-392). */
+fg_status fg_dense_covariance(const double* information, size_t n, const size_t* columns, size_t k,
+                              const fg_covariance_options*, double* out);
+/* CovarianceBackend for Schur (src/schur.rs:548-556): undamped information J^T J of the current
+ * linearization, selected variable blocks given by (offset, width) in reference coordinates
+ * (cameras 9c/9, points 9 nc + 3p/3; generic: the block grid of the partition), in selection
+ * order; out is (sum of widths)^2 column-major. Dense: n^2 doubles of device memory. */
+fg_status fg_schur_covariance(fg_schur*, const size_t* offsets, const size_t* widths, size_t n_blocks,
+                              const fg_covariance_options*, double* out);
+fg_status fg_gschur_covariance(fg_gschur*, const size_t* offsets, const size_t* widths, size_t n_blocks,
+                               const fg_covariance_options*, double* out);
 
 /* ---- BAL problem text (examples/bal/model.rs:59-113) ---- */
 typedef struct {

@@ -982,6 +982,110 @@ int orc_lsmr_solve_damped(orc_schur* s, double lambda, int mode, size_t max_iter
 }
 
 // src/lsmr.rs:535-552: identity-preconditioned LSMR on J itself, rhs = -r, no verification.
+// DenseNormalCholesky::information_factor (src/normal.rs:341-392) and
+// SelectedCovariance::compute_factor (src/covariance.rs:214-249) on a dense information
+// matrix (column-major n x n, lower triangle significant; overwritten by its factor).
+int orc_dense_covariance(const double* information, size_t n, const size_t* columns, size_t k, double tolerance,
+                         double* out) {
+  if (!is_finite(tolerance) || tolerance < 0.0 || tolerance >= 1.0) return ORC_INVALID_RANK_TOLERANCE;
+  for (size_t a = 0; a < k; ++a)
+    if (columns[a] >= n) return ORC_UNKNOWN_KEY;
+  // row-major lower copy: row i of L is contiguous, so every Cholesky dot product streams
+  std::vector<double> l(n * n, 0.0), diagonal(n);
+  // factorize: a nonfinite lower entry is an evaluation error, mapped to InvalidInformation
+  for (size_t c = 0; c < n; ++c) {
+    diagonal[c] = information[c * n + c];
+    for (size_t r = c; r < n; ++r) {
+      const double v = information[c * n + r];
+      if (!is_finite(v)) return ORC_INVALID_INFORMATION;
+      l[r * n + c] = v;
+    }
+  }
+  // column Cholesky (the cholesky_lower recurrence, same summation order), row-major storage
+  for (size_t j = 0; j < n; ++j) {
+    const double* lj = l.data() + j * n;
+    double d = lj[j];
+    for (size_t k = 0; k < j; ++k) d -= lj[k] * lj[k];
+    if (!(d > 0.0) || !is_finite(d)) return ORC_SINGULAR_INFORMATION;
+    d = std::sqrt(d);
+    l[j * n + j] = d;
+    for (size_t i = j + 1; i < n; ++i) {
+      double* li = l.data() + i * n;
+      double s = li[j];
+      for (size_t k = 0; k < j; ++k) s -= li[k] * lj[k];
+      li[j] = s / d;
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const double pivot = l[i * n + i], d = diagonal[i];
+    if (!is_finite(pivot) || !is_finite(d)) return ORC_INVALID_INFORMATION;
+    const double ratio = pivot / std::sqrt(d);
+    if (d <= 0.0 || pivot <= 0.0 || ratio * ratio <= tolerance) return ORC_SINGULAR_INFORMATION;
+  }
+  // L Y = E, covariance = Y^T Y
+  std::vector<double> y(n * k, 0.0);
+  for (size_t a = 0; a < k; ++a) {
+    double* ya = y.data() + a * n;
+    ya[columns[a]] = 1.0;
+    for (size_t i = 0; i < n; ++i) {  // forward substitution with row i of L
+      const double* li = l.data() + i * n;
+      double s = ya[i];
+      for (size_t t = 0; t < i; ++t) s -= li[t] * ya[t];
+      ya[i] = s / li[i];
+    }
+  }
+  for (size_t b = 0; b < k; ++b)
+    for (size_t a = 0; a < k; ++a) {
+      double s = 0.0;
+      for (size_t i = 0; i < n; ++i) s += y[a * n + i] * y[b * n + i];
+      if (!is_finite(s)) return ORC_INVALID_INFORMATION;
+      out[b * k + a] = s;
+    }
+  return ORC_OK;
+}
+
+int orc_schur_covariance(const orc_schur* s, const size_t* offsets, const size_t* widths, size_t nblocks,
+                         double tolerance, double* out) {
+  if (!is_finite(tolerance) || tolerance < 0.0 || tolerance >= 1.0) return ORC_INVALID_RANK_TOLERANCE;
+  const Jacobian& jac = s->model.jacobian;
+  const size_t n = jac.cols;
+  // selection: each (offset, width) must be a variable block of the model (the grid of the
+  // Schur partition); SelectedCovariance::prepare (src/covariance.rs:180-207)
+  std::vector<size_t> cols;
+  for (size_t i = 0; i < nblocks; ++i) {
+    const size_t o = offsets[i], w = widths[i];
+    for (size_t j = 0; j < i; ++j)
+      if (offsets[j] == o) return ORC_DUPLICATE_COVARIANCE_BLOCK;
+    const bool ret = o < s->retained && o % s->rw == 0 && w == s->rw && o + w <= s->retained;
+    const bool elim = o >= s->retained && (o - s->retained) % s->ew == 0 && w == s->ew && o + w <= n;
+    if (!ret && !elim) return ORC_UNKNOWN_KEY;
+    for (size_t c = o; c < o + w; ++c) cols.push_back(c);
+  }
+  // H = J^T J over the cached blocks, grouped by rows (src/linear_model.rs:122-167; only the
+  // lower triangle is read)
+  std::vector<double> h(n * n, 0.0);
+  // blocks of one emission are contiguous (same row and row count)
+  for (size_t a = 0; a < jac.blocks.size(); ++a) {
+    const Block& A = jac.blocks[a];
+    size_t lo = a, hi = a + 1;
+    while (lo > 0 && jac.blocks[lo - 1].row == A.row && jac.blocks[lo - 1].rows == A.rows) --lo;
+    while (hi < jac.blocks.size() && jac.blocks[hi].row == A.row && jac.blocks[hi].rows == A.rows) ++hi;
+    for (size_t b = lo; b < hi; ++b) {
+      const Block& B = jac.blocks[b];
+      for (size_t ca = 0; ca < A.cols; ++ca)
+        for (size_t cb = 0; cb < B.cols; ++cb) {
+          const size_t r = A.col + ca, c = B.col + cb;
+          if (r < c) continue;
+          double v = 0.0;
+          for (size_t i = 0; i < A.rows; ++i)
+            v += jac.values[A.start + ca * A.rows + i] * jac.values[B.start + cb * B.rows + i];
+          h[c * n + r] += v;
+        }
+    }
+  }
+  return orc_dense_covariance(h.data(), n, cols.data(), cols.size(), tolerance, out);
+}
+
 int orc_lsmr_solve(orc_schur* s, size_t max_iterations, double tol, double* delta, orc_lsmr_diagnostics* d) {
   *d = orc_lsmr_diagnostics{ORC_LSMR_NONE, 0, 0, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, kNaN, 0};
   if (max_iterations == 0 || !(tol >= 0.0 && tol < 1.0)) return ORC_INVALID_OPTIONS;

@@ -25,7 +25,13 @@ enum {
   ORC_INVALID_EVALUATION = 5,
   ORC_DIMENSION_MISMATCH = 6,
   ORC_INVALID_EMISSION = 7,
-  ORC_UNSUPPORTED_STRUCTURE = 8
+  ORC_UNSUPPORTED_STRUCTURE = 8,
+  /* SolverError variants of the covariance query (src/error.rs:43-58) and KeyError::Unknown */
+  ORC_DUPLICATE_COVARIANCE_BLOCK = 10,
+  ORC_SINGULAR_INFORMATION = 11,
+  ORC_INVALID_INFORMATION = 12,
+  ORC_INVALID_RANK_TOLERANCE = 13,
+  ORC_UNKNOWN_KEY = 14
 };
 
 /* Schur termination (src/schur.rs:32-41); -1 = none. */
@@ -181,6 +187,16 @@ typedef struct {
 void orc_lm_optimize(const orc_problem*, const orc_optimize_options*, const orc_lm_options*,
                      orc_lm_outcome*);
 void orc_gn_optimize(const orc_problem*, const orc_optimize_options*, orc_lm_outcome*);
+
+/* Selected covariance (src/covariance.rs:163-262, src/normal.rs:341-392): the inverse of the
+ * information H (n x n, column-major, lower triangle read) at the selected coordinates, k x k
+ * column-major into out. */
+int orc_dense_covariance(const double* information, size_t n, const size_t* columns, size_t k, double tolerance,
+                         double* out);
+/* LinearizedModel::covariance (src/linear_model.rs:122-167) of a Schur model: H = J^T J
+ * replayed from the cached blocks; selected variable blocks by (offset, width). */
+int orc_schur_covariance(const orc_schur* s, const size_t* offsets, const size_t* widths, size_t nblocks,
+                         double tolerance, double* out);
 
 /* BA convenience: LM (or GN) with the Schur backend on a BA problem. */
 void orc_ba_lm(orc_ba*, size_t max_iterations_cg, double relative_tolerance,

@@ -16,6 +16,9 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 OK, INVALID_OPTIONS, LINEAR_SOLVE_FAILED, NO_CONVERGENCE, NO_PROGRESS = 0, 1, 2, 3, 4
 INVALID_EVALUATION, DIMENSION_MISMATCH, INVALID_EMISSION, UNSUPPORTED_STRUCTURE = 5, 6, 7, 8
 CG_NONE, CG_CONVERGED, CG_ITERATION_LIMIT, CG_NOT_POSITIVE_DEFINITE, CG_NON_FINITE = -1, 0, 1, 2, 3
+DUPLICATE_COVARIANCE_BLOCK, SINGULAR_INFORMATION, INVALID_INFORMATION, INVALID_RANK_TOLERANCE = 10, 11, 12, 13
+UNKNOWN_KEY = 14
+COVARIANCE_TOLERANCE = 64 * np.finfo(np.float64).eps  # CovarianceOptions default (src/covariance.rs:145-151)
 
 _size_t = ctypes.c_size_t
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -151,6 +154,8 @@ def lib():
         "orc_lsmr_solve_damped": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, _size_t, ctypes.c_double,
                                                  _dp, _dp, ctypes.POINTER(LsmrDiagnostics)]),
         "orc_lsmr_solve": (ctypes.c_int, [_VP, _size_t, ctypes.c_double, _dp, ctypes.POINTER(LsmrDiagnostics)]),
+        "orc_dense_covariance": (ctypes.c_int, [_dp, _size_t, _szp, _size_t, ctypes.c_double, _dp]),
+        "orc_schur_covariance": (ctypes.c_int, [_VP, _szp, _szp, _size_t, ctypes.c_double, _dp]),
         "orc_ba_new": (_VP, [_size_t, _size_t, _size_t, _dp, _dp, _i64p, _i64p, _dp, ctypes.c_double]),
         "orc_ba_free": (None, [_VP]),
         "orc_ba_layout": (None, [_VP, _i64p, _i64p, _i64p, _i64p]),
@@ -210,6 +215,16 @@ def jacobians(cam, point):
     jp, pjp = _out(6)
     lib().orc_jacobians(pc, pp, pjc, pjp)
     return jc.reshape(9, 2).T.copy(), jp.reshape(3, 2).T.copy()
+
+
+def dense_covariance(information, columns, tolerance=COVARIANCE_TOLERANCE):
+    """SelectedCovariance on a dense information matrix (src/covariance.rs:163-262)."""
+    h = np.asfortranarray(information, dtype=np.float64)
+    cols = (ctypes.c_size_t * max(1, len(columns)))(*[int(c) for c in columns])
+    out = np.zeros((len(columns), len(columns)), order="F")
+    st = lib().orc_dense_covariance(h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), h.shape[0], cols,
+                                    len(columns), tolerance, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return st, (out if st == OK else None)
 
 
 class Schur:
@@ -283,6 +298,16 @@ class Schur:
         st = lib().orc_lsmr_solve_damped(self.h, lam, mode, max_iterations, tol, pd, ctypes.byref(pred),
                                          ctypes.byref(diag))
         return st, d, pred.value, diag
+
+    def covariance(self, blocks, tolerance=COVARIANCE_TOLERANCE):
+        """LinearizedModel::covariance (src/linear_model.rs:122-167) for (offset, width) blocks."""
+        offs = (ctypes.c_size_t * max(1, len(blocks)))(*[int(o) for o, _ in blocks])
+        wids = (ctypes.c_size_t * max(1, len(blocks)))(*[int(w) for _, w in blocks])
+        k = sum(int(w) for _, w in blocks)
+        out = np.zeros((k, k), order="F")
+        st = lib().orc_schur_covariance(self.h, offs, wids, len(blocks), tolerance,
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return st, (out if st == OK else None)
 
     def lsmr_solve(self, max_iterations=1000, tol=1e-10):
         d, pd = _out(self.dimension())
