@@ -140,6 +140,71 @@ def cpu_baseline(problem, fp64_counts, budget_s):
             "cpu": platform.processor() or platform.machine(), "host_nproc": os.cpu_count()}
 
 
+def widened_rows(no_cpu):
+    """Measurements of the rows beyond the BAL Schur path, each against the oracle on a bounded
+    sample: the generic-emission Schur backend (a planar pose / landmark model, one damped solve)
+    and the selected covariance (a dense 3,072-coordinate information matrix)."""
+    import fagra_amd as fa
+    from fagra_amd import generic
+    from fagra_amd.planar import Planar
+    out = {}
+    m = Planar(n_poses=2000, n_landmarks=50000, n_obs=400000, seed=7, noise=0.01)
+    g = generic.GenericSchur(3 * m.np_, 3, 2, relative_tolerance=1e-6)
+    t = time.perf_counter()
+    assert g.prepare(m.dimension) == fa.OK and m.linearize(g) == fa.OK
+    st, _ = g.gradient_norm()
+    t_acc = time.perf_counter() - t
+    t = time.perf_counter()
+    assert g.prepare_damping(1e-6) == fa.OK
+    t_prep = time.perf_counter() - t
+    g.solve_damped(1e-3)  # warm
+    reps, t = 5, time.perf_counter()
+    for _ in range(reps):
+        st, d, pred = g.solve_damped(1e-3)
+    t_solve = (time.perf_counter() - t) / reps
+    gen = {"model": "planar: 2000 poses (3), 50000 landmarks (2), 400000 observations (2 rows) + prior",
+           "dimension": m.dimension, "host_accumulate_s": t_acc, "prepare_damping_s": t_prep,
+           "solve_damped_s": t_solve, "pcg_iterations": int(g.diagnostics().iterations), "status": st}
+    if not no_cpu:
+        from oracle import oracle as O
+        o = O.Schur(3 * m.np_, 3, 2)
+        o.relative_tolerance = 1e-6
+        assert o.prepare(m.dimension) == O.OK
+        r, pr = m.residuals(m.x)
+        jp, jl = m.jacobians(m.x)
+        for e in range(m.pi.size):
+            o.accumulate(r[e], [(3 * int(m.pi[e]), jp[e]), (3 * m.np_ + 2 * int(m.li[e]), jl[e])])
+        o.accumulate(pr, [(0, np.eye(3))])
+        t = time.perf_counter()
+        o.prepare_damping(1e-6)
+        t_oprep = time.perf_counter() - t
+        t = time.perf_counter()
+        st_o, d_o, _ = o.solve_damped(1e-3)
+        t_osolve = time.perf_counter() - t
+        gen["cpu_oracle"] = {"prepare_damping_s": t_oprep, "solve_damped_s": t_osolve, "cores": 1,
+                             "step_rel_diff": float(np.linalg.norm(d - d_o) / np.linalg.norm(d_o))}
+    out["generic_schur"] = gen
+    n, k = 3072, 64
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(n + 64, n))
+    H = np.asfortranarray(A.T @ A)  # column-major, as the ABI takes it
+    cols = rng.permutation(n)[:k]
+    fa.dense_covariance(H[:128, :128], cols[:4] % 128)  # warm
+    t = time.perf_counter()
+    st, C = fa.dense_covariance(H, cols)
+    t_cov = time.perf_counter() - t
+    cov = {"n": n, "selected": k, "seconds": t_cov, "status": st,
+           "tflops_cholesky": (n ** 3 / 3) / t_cov / 1e12}
+    if not no_cpu:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        st_o, C_o = O.dense_covariance(H, cols)
+        cov["cpu_oracle"] = {"seconds": time.perf_counter() - t, "cores": 1,
+                             "max_abs_diff": float(np.max(np.abs(C - C_o)))}
+    out["covariance"] = cov
+    return out
+
+
 def profile_products(args):
     """Counter-run workload: repeats of each Schur-product kernel at the initial state of the
     workload in both precisions, then a plain 256 MiB device copy whose FETCH_SIZE and
@@ -182,6 +247,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--kernel-repeats", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-widened", action="store_true",
+                    help="skip the generic-emission and covariance measurements")
     ap.add_argument("--modes", default="fp64,fp32,learned,lsmr")
     ap.add_argument("--profile-products", action="store_true",
                     help="only time the two Schur-product kernels (FP32, FP64) and a 256 MiB device copy, "
@@ -284,6 +351,14 @@ def main():
                                                                                       p.n_observations)}}
     log(f"bench: roofline {kname}: {ksec * 1e6:.1f} us, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
 
+    widened = None
+    if not args.no_widened:
+        t = time.perf_counter()
+        widened = widened_rows(args.no_cpu)
+        log(f"bench: widened rows measured in {time.perf_counter() - t:.1f}s: "
+            f"generic solve {widened['generic_schur']['solve_damped_s'] * 1e3:.1f} ms, "
+            f"covariance {widened['covariance']['seconds'] * 1e3:.1f} ms")
+
     cpu = None
     f64 = results.get("fp64_block_jacobi", [])
     if not args.no_cpu and f64:
@@ -303,7 +378,7 @@ def main():
                        "cg_relative_tolerance": args.cg_tol, "target": "C*(1+1e-5)", "c_star": c_star},
             "fp64_block_jacobi_s": f64_value, "mode_seconds": mode_s,
             "speedup_vs_fp64": (f64_value / value) if (value and f64_value) else None,
-            "modes": slim, "roofline": roofline, "cpu_baseline": cpu}
+            "modes": slim, "roofline": roofline, "cpu_baseline": cpu, "widened": widened}
     print(json.dumps(line))
 
 

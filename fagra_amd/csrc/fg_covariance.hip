@@ -41,82 +41,91 @@ __global__ void k_cov_check(const double* __restrict__ h, size_t n, double* __re
   }
 }
 
-// Cholesky of the diagonal tile [k0, k0 + kb) in LDS; a nonpositive or nonfinite pivot sets
-// flags[1] (faer's LLT error, SingularInformation)
-__global__ void __launch_bounds__(kBlock) k_cov_potrf(double* __restrict__ a, int n, int k0, int kb, int* flags) {
+// Cholesky of the diagonal tile [k0, k0 + kb) by one wavefront (lane i owns row i of the tile
+// in LDS), then the inverse of its factor into linv (64 x 64, row-major, identity-padded beyond
+// kb) for the panel and selected-column solves. A nonpositive or nonfinite pivot sets flags[1]
+// (faer's LLT error, SingularInformation).
+__global__ void __launch_bounds__(kWave) k_cov_potrf(double* __restrict__ a, int n, int k0, int kb,
+                                                     double* __restrict__ linv, int* flags) {
   __shared__ double t[kNB][kNB + 1];
+  __shared__ double v[kNB][kNB + 1];
   __shared__ int fail;
   if (flags[1]) return;
-  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
-    const int r = e % kNB, c = e / kNB;
-    t[r][c] = (r < kb && c < kb && r >= c) ? a[(size_t)(k0 + c) * n + k0 + r] : 0.0;
-  }
-  if (threadIdx.x == 0) fail = 0;
+  const int i = threadIdx.x;
+  for (int c = 0; c < kNB; ++c) t[i][c] = (i < kb && c <= i) ? a[(size_t)(k0 + c) * n + k0 + i] : (i == c ? 1.0 : 0.0);
+  if (i == 0) fail = 0;
   __syncthreads();
   for (int j = 0; j < kb; ++j) {
-    if (threadIdx.x == 0) {
+    if (i == j) {
       const double d = t[j][j];
       if (!(d > 0.0) || !isfinite(d)) fail = 1;
       else t[j][j] = sqrt(d);
     }
     __syncthreads();
     if (fail) break;
-    const double piv = t[j][j];
-    for (int i = j + 1 + threadIdx.x; i < kb; i += blockDim.x) t[i][j] /= piv;
+    if (i > j && i < kb) t[i][j] /= t[j][j];
     __syncthreads();
-    const int m = kb - j - 1;
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-      const int i = j + 1 + e / m, l = j + 1 + e % m;
-      if (i >= l) t[i][l] -= t[i][j] * t[l][j];
+    if (i > j && i < kb) {
+      const double f = t[i][j];
+      for (int l = j + 1; l <= i; ++l) t[i][l] -= f * t[l][j];
     }
     __syncthreads();
   }
   if (fail) {
-    if (threadIdx.x == 0) flags[1] = 1;
+    if (i == 0) flags[1] = 1;
     return;
   }
-  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
-    const int r = e % kNB, c = e / kNB;
-    if (r < kb && c < kb && r >= c) a[(size_t)(k0 + c) * n + k0 + r] = t[r][c];
+  for (int c = 0; c <= i && i < kb; ++c) a[(size_t)(k0 + c) * n + k0 + i] = t[i][c];
+  // column i of L^-1 (forward substitution on e_i); rows above i are zero
+  for (int r = 0; r < kNB; ++r) v[r][i] = 0.0;
+  v[i][i] = 1.0 / t[i][i];
+  for (int r = i + 1; r < kNB; ++r) {
+    double s = 0.0;
+    for (int q = i; q < r; ++q) s += t[r][q] * v[q][i];
+    v[r][i] = -s / t[r][r];
   }
+  __syncthreads();
+  for (int c = 0; c < kNB; ++c) linv[i * kNB + c] = v[i][c];
 }
 
-// Load the factored diagonal tile, padded with the identity beyond kb.
-__device__ __forceinline__ void load_tile(const double* __restrict__ a, int n, int k0, int kb,
-                                          double (*l)[kNB + 1]) {
-  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
-    const int r = e % kNB, c = e / kNB;
-    double v = 0.0;
-    if (r < kb && c < kb) {
-      if (r >= c) v = a[(size_t)(k0 + c) * n + k0 + r];
-    } else if (r == c) {
-      v = 1.0;
-    }
-    l[r][c] = v;
-  }
-}
-
-// Panel rows below the tile: x L^T = a for each row (one thread per row)
-__global__ void __launch_bounds__(kBlock) k_cov_trsm(double* __restrict__ a, int n, int k0, int kb, const int* flags) {
+// Panel rows below the tile, X = A_panel L^-T as a product with the tile inverse: 64 rows per
+// workgroup, each thread a 4 x 4 strided block (out[r][j] = sum_t A[r][t] Linv[j][t]).
+__global__ void __launch_bounds__(kBlock) k_cov_trsm(double* __restrict__ a, int n, int k0, int kb,
+                                                     const double* __restrict__ linv, const int* flags) {
+  __shared__ double p[kNB][kNB + 1];
   __shared__ double l[kNB][kNB + 1];
   if (flags[1]) return;
-  load_tile(a, n, k0, kb, l);
+  const int r0 = k0 + kb + kNB * blockIdx.x;
+  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+    const int r = e % kNB, c = e / kNB;
+    p[r][c] = (c < kb && r0 + r < n) ? a[(size_t)(k0 + c) * n + r0 + r] : 0.0;
+    l[e / kNB][e % kNB] = linv[e];
+  }
   __syncthreads();
-  const int i = k0 + kb + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double x[kNB];
+  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  double acc[4][4];
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) x[j] = j < kb ? a[(size_t)(k0 + j) * n + i] : 0.0;
+  for (int u = 0; u < 4; ++u)
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    double v = x[j];
+    for (int w = 0; w < 4; ++w) acc[u][w] = 0.0;
+  for (int t = 0; t < kb; ++t) {
+    double pv[4], lv[4];
 #pragma unroll
-    for (int t = 0; t < j; ++t) v -= x[t] * l[j][t];
-    x[j] = v / l[j][j];
+    for (int u = 0; u < 4; ++u) pv[u] = p[ty + 16 * u][t];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) lv[w] = l[tx + 16 * w][t];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) acc[u][w] += pv[u] * lv[w];
   }
 #pragma unroll
-  for (int j = 0; j < kNB; ++j)
-    if (j < kb) a[(size_t)(k0 + j) * n + i] = x[j];
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int row = r0 + ty + 16 * u, col = tx + 16 * w;
+      if (row < n && col < kb) a[(size_t)(k0 + col) * n + row] = acc[u][w];
+    }
 }
 
 // Trailing update A[i, j] -= sum_t P[i, t] P[j, t] for m0 <= j <= i < n over 64 x 64 lower tiles;
@@ -190,28 +199,17 @@ __global__ void k_cov_unit(double* __restrict__ y, int n, int k, const int* __re
   }
 }
 
-// Y[k0 : k0 + kb, a] = L_kk^-1 Y[k0 : k0 + kb, a], one thread per selected column
-__global__ void __launch_bounds__(kBlock) k_cov_fwd_tile(const double* __restrict__ a, int n, int k0, int kb,
-                                                         double* __restrict__ y, int k) {
-  __shared__ double l[kNB][kNB + 1];
-  load_tile(a, n, k0, kb, l);
+// Y[k0 : k0 + kb, a] = L_kk^-1 Y[k0 : k0 + kb, a]: one workgroup of 64 per selected column
+__global__ void __launch_bounds__(kWave) k_cov_fwd_tile(const double* __restrict__ linv, int n, int k0, int kb,
+                                                        double* __restrict__ y) {
+  __shared__ double yt[kNB];
+  double* yc = y + (size_t)blockIdx.x * n;
+  const int j = threadIdx.x;
+  yt[j] = j < kb ? yc[k0 + j] : 0.0;
   __syncthreads();
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= k) return;
-  double x[kNB];
-  double* yc = y + (size_t)c * n;
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) x[j] = j < kb ? yc[k0 + j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    double v = x[j];
-#pragma unroll
-    for (int t = 0; t < j; ++t) v -= l[j][t] * x[t];
-    x[j] = v / l[j][j];
-  }
-#pragma unroll
-  for (int j = 0; j < kNB; ++j)
-    if (j < kb) yc[k0 + j] = x[j];
+  double s = 0.0;
+  for (int t = 0; t <= j; ++t) s += linv[j * kNB + t] * yt[t];
+  if (j < kb) yc[k0 + j] = s;
 }
 
 // Y[i, a] -= sum_t L[i, k0 + t] Y[k0 + t, a] for rows below the tile
@@ -280,6 +278,13 @@ fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* col
       hipMalloc(&cols, std::max<size_t>(1, k) * sizeof(int)) != hipSuccess)
     return fail(FG_DEVICE_ERROR);
   const int ni = (int)n, ki = (int)k;
+  const int panels = (ni + kNB - 1) / kNB;
+  double* linv = nullptr;  // per panel, the inverse of its diagonal factor tile
+  if (hipMalloc(&linv, (size_t)panels * kNB * kNB * sizeof(double)) != hipSuccess) return fail(FG_DEVICE_ERROR);
+  struct Free {
+    double* p;
+    ~Free() { (void)hipFree(p); }
+  } free_linv{linv};
   std::vector<int> hc(columns, columns + k);
   int init[4] = {0, 0, INT_MAX, 0};
   if (hipMemcpyAsync(flags, init, sizeof(init), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -293,9 +298,11 @@ fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* col
   if (hf[0]) return fail(FG_INVALID_INFORMATION);  // nonfinite information
   for (int k0 = 0; k0 < ni; k0 += kNB) {
     const int kb = std::min(kNB, ni - k0), m0 = k0 + kb, rest = ni - m0;
-    hipLaunchKernelGGL(k_cov_potrf, dim3(1), dim3(kBlock), 0, s, h, ni, k0, kb, flags);
+    double* li = linv + (size_t)(k0 / kNB) * kNB * kNB;
+    hipLaunchKernelGGL(k_cov_potrf, dim3(1), dim3(kWave), 0, s, h, ni, k0, kb, li, flags);
     if (rest > 0) {
-      hipLaunchKernelGGL(k_cov_trsm, dim3(grid_of(rest)), dim3(kBlock), 0, s, h, ni, k0, kb, (const int*)flags);
+      hipLaunchKernelGGL(k_cov_trsm, dim3((rest + kNB - 1) / kNB), dim3(kBlock), 0, s, h, ni, k0, kb,
+                         (const double*)li, (const int*)flags);
       const long long m = (rest + kNB - 1) / kNB;
       hipLaunchKernelGGL(k_cov_syrk, dim3((unsigned)(m * (m + 1) / 2)), dim3(kBlock), 0, s, h, ni, k0, kb, m0,
                          (const int*)flags);
@@ -316,7 +323,8 @@ fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* col
   hipLaunchKernelGGL(k_cov_unit, dim3(grid_of(n * k)), dim3(kBlock), 0, s, y, ni, ki, (const int*)cols);
   for (int k0 = 0; k0 < ni; k0 += kNB) {
     const int kb = std::min(kNB, ni - k0), rest = ni - k0 - kb;
-    hipLaunchKernelGGL(k_cov_fwd_tile, dim3(grid_of(k)), dim3(kBlock), 0, s, (const double*)h, ni, k0, kb, y, ki);
+    hipLaunchKernelGGL(k_cov_fwd_tile, dim3(ki), dim3(kWave), 0, s,
+                       (const double*)(linv + (size_t)(k0 / kNB) * kNB * kNB), ni, k0, kb, y);
     if (rest > 0)
       hipLaunchKernelGGL(k_cov_fwd_update, dim3(grid_of(rest), ki), dim3(kBlock), 0, s, (const double*)h, ni, k0,
                          kb, y, ki);
