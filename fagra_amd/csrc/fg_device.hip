@@ -137,6 +137,7 @@ struct fg_schur {
   hipGraphExec_t cg_graph[2] = {nullptr, nullptr};  // PCG batch per accepted-state buffer
   int cg_graph_size[2] = {0, 0};                      // sizeof(T) it was captured for
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
+  int cg_launched = 0;  // PCG iterations launched by the last cg_impl (state slot parity)
   std::vector<hipEvent_t> events;  // matvec kernel timing pool (pairs)
   size_t events_used = 0;
   hipEvent_t ph_a = nullptr, ph_b = nullptr;
@@ -364,9 +365,9 @@ fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out
 }
 
 // Column sums of the (rows x kParts) partials in d_parts into d_sums[slot * kParts ...].
-void launch_sum_parts(fg_schur* s, int rows, int slot) {
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s->ba->stream, (const double*)s->d_parts, rows,
-                     s->d_sums + slot * kParts);
+void launch_sum_parts(fg_schur* s, int rows, int slot, int first_row = 0) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s->ba->stream,
+                     (const double*)(s->d_parts + (size_t)kParts * first_row), rows, s->d_sums + slot * kParts);
 }
 
 // PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
@@ -486,9 +487,16 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
                      (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
                      (const DevScalars*)s->d_sc, (const int*)s->d_fail);
   FG_TRY(hipGetLastError());
+  // read back with the verification's synchronization (cg_collect)
   FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(s->h_int + 1, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipStreamSynchronize(b->stream));
+  s->cg_launched = launched;
+  return FG_OK;
+}
+
+// PCG statistics and diagnostics from the state read back by cg_impl (after a synchronization).
+void cg_collect(fg_schur* s) {
+  const int launched = s->cg_launched;
   s->factor_failed = s->h_int[1] != 0;
   const CgState& h = s->h_sc->st[(launched + 1) & 1];
   s->iterations += (size_t)std::max(h.iters, 0);
@@ -503,7 +511,6 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
     s->diag.termination = FG_CG_ITERATION_LIMIT;
   }
   s->tim.matvecs += (size_t)std::max(h.iters, 0);
-  return FG_OK;
 }
 
 // Schur::attempt (schur.rs:192-202): factor + reduced rhs, CG, recover, verify.
@@ -538,35 +545,30 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
     Phase ph(s, &s->tim.linear_solve);
     FG_CHECK(cg_impl<T>(s, lambda, learned));
   }
-  if (s->factor_failed) return FG_LINEAR_SOLVE_FAILED;  // NotPositiveDefinite, no iteration
   Phase ph(s, &s->tim.verification);
-  // recover_step (schur.rs:312-344)
+  // recover_step (schur.rs:312-344) and verify_step (schur.rs:346-397) in one point pass and
+  // one camera pass (k_points<kCheck>, k_cameras<cCheck>), launched right behind the PCG and
+  // read back with its state in one synchronization: the reduced residual ||S x - rhs||, the
+  // full-model verification of delta in original coordinates (linear_model.rs:254-282), and
+  // ||delta||_inf. With a failed block factorization the PCG did not iterate and these results
+  // are discarded (NotPositiveDefinite).
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
                      (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw);
-  launch_points<T, kRecover>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
-                             nullptr, nullptr);
-  hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)(9 * b->nc),
-                     (const double*)s->x, (const T*)s->dc, s->d_delta);
-  FG_TRY(hipGetLastError());
-  // verify_step (schur.rs:346-397), every reduction launched first and read back once:
-  // reduced residual ||S x - rhs||, then the full-model verification in original coordinates
-  // (linear_model.rs:254-282), then ||delta||_inf
-  FG_CHECK(schur_product<T>(s, (const double*)s->x, lambda, (double*)s->q, (const double*)s->rhs));
-  launch_sum_parts(s, nc, 0);
-  hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
-  launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, lambda, nullptr, nullptr,
-                            s->d_parts, nullptr);
+  launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
+                           s->d_parts, nullptr);
   launch_sum_parts(s, pblocks, 1);
-  launch_cameras<T, cVerify>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, nullptr, s->d_delta,
-                             nullptr, lambda, nullptr, nullptr);
-  launch_sum_parts(s, nc, 2);
+  launch_cameras<T, cCheck>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, (const double*)s->x, nullptr,
+                            (const double*)s->rhs, lambda, (double*)s->q, s->d_delta);
+  launch_sum_parts(s, nc, 0);
+  launch_sum_parts(s, nc, 2, nc);
   FG_TRY(hipGetLastError());
   double step_norm = 0.0;
   const fg_status nst = norm_inf_impl<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0, &step_norm,
                                               3 * kParts);
   if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
+  cg_collect(s);
+  if (s->factor_failed) return FG_LINEAR_SOLVE_FAILED;  // NotPositiveDefinite, no iteration
   const double* rs = s->h_sums;                // ||Sx - rhs||^2, ||rhs||^2
   const double* pt = s->h_sums + kParts;       // n2, n2s, g.d, g2, g2s, ||J delta||^2 (points)
   const double* cm = s->h_sums + 2 * kParts;   // n2, n2s, g.d, g2, g2s (cameras)
@@ -627,7 +629,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc(&s->d_delta, 9 * nc + 3 * np));
   FG_TRY(dalloc(&s->d_rows, 2 * no));
   const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
-  const size_t rows = std::max({nc, pblocks, (size_t)1024});
+  const size_t rows = std::max({2 * nc, pblocks, (size_t)1024});  // cCheck writes 2 nc rows
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
   FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc, kCgBlock))));
   FG_TRY(dalloc((double**)&s->zb, 9 * nc));

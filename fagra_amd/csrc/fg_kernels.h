@@ -601,7 +601,11 @@ __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double
 //             U_p = sqrt(lambda) Dp ws_p - (alpha/beta) U_p, and ||U||^2 partials; with no
 //             forward table it only forms x_p and ||U||^2 of the current U (the start vector)
 // ---------------------------------------------------------------------------------------
-enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu = 5, kLsmr = 6 };
+//   kCheck:   the whole point side of a damped attempt's recovery and verification in two
+//             sweeps over the point's observations: sweep 1 as kSchur at z (t_p for the
+//             reduced residual ||S z - rhs||) and kRecover (delta_p from the same sum), sweep 2
+//             as kVerify at delta = (z / D, delta_p); both forward tables are the table of z
+enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu = 5, kLsmr = 6, kCheck = 7 };
 
 struct PointArgs {
   int nchunks, np;
@@ -683,6 +687,30 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
   }
 }
 
+// One lane's observations of a point-major pass (SELL chunk, two observations per step so
+// that both loads issue before either is used).
+template <typename T, int MODE>
+__device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int lane, const Vec4<T>* __restrict__ rec,
+                                            const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
+                                            const T* __restrict__ fw, const double dpx[3],
+                                            const double* __restrict__ rows_in, double* __restrict__ rows_out,
+                                            double w[3], double h[6], double& uu, double coef) {
+  const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
+  int j = 0;
+  for (; j + 1 < len; j += 2) {
+    const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
+    const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
+    const Vec4<T> r0 = rec[s0], r1 = rec[s1];
+    if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+    if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+  }
+  if (j < len) {
+    const int s0 = base + j * kWave + lane;
+    const int c0 = pa.slot_cam[s0];
+    if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+  }
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_points(
     PointArgs pa, const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
@@ -713,23 +741,45 @@ __global__ void __launch_bounds__(kBlock) k_points(
         dpx[1] = delta_p_in[3 * i + 1];
         dpx[2] = delta_p_in[3 * i + 2];
       }
-      if (MODE != kRhs) {
-        const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
-        int j = 0;
-        // two observations per step: both loads issue before either is used
-        for (; j + 1 < len; j += 2) {
-          const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
-          const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
-          const Vec4<T> r0 = rec[s0], r1 = rec[s1];
-          if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
-          if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+      if (MODE == kCheck) {
+        // sweep 1: w = sum Jp^T Jc z, Hpp re-formed; t_p as kSchur, delta_p as kRecover
+        point_sweep<T, kSchur>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+        double t[3] = {w[0], w[1], w[2]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const double d = double(dpv[3 * i + a]);
+          h[sym3(a, a)] += lambda * d * d;
         }
-        if (j < len) {
-          const int s0 = base + j * kWave + lane;
-          const int c0 = pa.slot_cam[s0];
-          if (c0 >= 0)
-            point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+        if (chol_packed<double, 3>(h)) {
+          solve_lower_packed<double, 3>(h, t);
+          solve_upper_packed<double, 3>(h, t);
+        } else {
+          t[0] = t[1] = t[2] = 0.0;
         }
+        reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
+        const double v[3] = {double(gp[3 * i]) + w[0], double(gp[3 * i + 1]) + w[1], double(gp[3 * i + 2]) + w[2]};
+        point_solve(mp, dpv, i, v, t);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          dpx[a] = -t[a];
+          delta_out[3 * i + a] = dpx[a];
+          w[a] = 0.0;
+        }
+        // sweep 2: kVerify at (z / D, delta_p)
+        point_sweep<T, kVerify>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const double d = double(dpv[3 * i + a]);
+          const double nr = w[a] + (lambda * (d * dpx[a])) * d;
+          n2 += nr * nr;
+          n2s += (nr / d) * (nr / d);
+          const double g = double(gp[3 * i + a]);
+          gd += g * dpx[a];
+          g2 += g * g;
+          g2s += (g / d) * (g / d);
+        }
+      } else if (MODE != kRhs) {
+        point_sweep<T, MODE>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
       }
       if (MODE == kSchur) {
         // Hpp + lambda Dp^2 re-formed from the same rows and factored in registers: the
@@ -796,7 +846,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
     const double b = block_sum(uu, smem);
     if (threadIdx.x == 0) parts[blockIdx.x] = b;
   }
-  if (MODE == kVerify) {
+  if (MODE == kVerify || MODE == kCheck) {
     const double v[6] = {n2, n2s, gd, g2, g2s, uu};
     for (int a = 0; a < 6; ++a) {
       const double b = block_sum(v[a], smem);
@@ -821,7 +871,11 @@ __global__ void __launch_bounds__(kBlock) k_points(
 //             U_c = sqrt(lambda) Dc ws_c - (alpha/beta) U_c and their ||.||^2 (no ws: x_c only)
 // T(acc) maps the rotation slots by (R Jr)^T.
 // ---------------------------------------------------------------------------------------
-enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5 };
+//   kCheck:   cResid and cVerify of one attempt in one pass: per row u = Jc z (forward table
+//             of z), v1 = u - Jp t_p for q = S z, v2 = u + Jp delta_p + r for the normal
+//             residual, delta_c = z / D written; residual partials in parts[c], verification
+//             partials in parts[nc + c]
+enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5, cCheck = 6 };
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_cameras(
@@ -841,12 +895,12 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
   if (ls != nullptr && ls->done) return;
   const int c = blockIdx.x;
   if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
-  if ((MODE == cVerify || MODE == cSchur || MODE == cResid) && threadIdx.x < kFwd)
+  if ((MODE == cVerify || MODE == cSchur || MODE == cResid || MODE == cCheck) && threadIdx.x < kFwd)
     cfw[threadIdx.x] = double(fw[kFwd * c + threadIdx.x]);
   __syncthreads();
-  double acc[9];
+  double acc[9], acc2[9];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) acc[a] = 0.0;
+  for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
   const int k0 = cam_off[c], k1 = cam_off[c + 1];
   auto one = [&](int k, const Vec4<T>& rec, const Vec4<double>& tv) {
     ObsGeom<double> geo;
@@ -858,6 +912,18 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     } else if (MODE == cLsmr) {
       v0 = tv.x;
       v1 = tv.y;
+    } else if (MODE == cCheck) {
+      const int ps = k_pslot[k];
+      const double xt[3] = {tv.x, tv.y, tv.z};
+      const double xd[3] = {delta_p[3 * ps], delta_p[3 * ps + 1], delta_p[3 * ps + 2]};
+      double u0, u1, a0, a1, b0, b1;
+      jc_apply(geo, cfw, u0, u1);
+      jp_apply(geo, ct, xt, a0, a1);
+      jp_apply(geo, ct, xd, b0, b1);
+      const Vec2<T> r = res_cm[k];
+      v0 = u0 - a0;
+      v1 = u1 - a1;
+      jct_factors(geo, u0 + b0 + double(r.x), u1 + b1 + double(r.y), acc2);
     } else if (MODE == cVerify) {
       const double x[3] = {tv.x, tv.y, tv.z};
       double u0, u1, w0, w1;
@@ -881,6 +947,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
   };
   auto gather = [&](int k) -> Vec4<double> {
     if (MODE == cJtu) return Vec4<double>{0.0, 0.0, 0.0, 0.0};
+    // cCheck gathers t_p here and delta_p in the row itself
     const int ps = k_pslot[k];
     if (MODE == cLsmr) {
       const Vec2<double> u = reinterpret_cast<const Vec2<double>*>(rows_in)[ps];
@@ -904,6 +971,14 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     const double v = wave_sum(acc[a]);
     if (lane == 0) red[a][wave] = v;
   }
+  __shared__ double red2[9][kBlock / kWave];
+  if (MODE == cCheck) {
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      const double v = wave_sum(acc2[a]);
+      if (lane == 0) red2[a][wave] = v;
+    }
+  }
   __syncthreads();
   double part0 = 0.0, part1 = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
   if (threadIdx.x == 0) {
@@ -922,7 +997,41 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
     double d[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] = double(dc[9 * c + i]);
-    if (MODE == cSchur || MODE == cResid) {
+    if (MODE == cCheck) {
+      // verification of delta = (z / D, delta_p) in original coordinates
+      double s2[9], y2[9];
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+        s2[a] = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s2[a] += red2[a][w];
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) y2[j] = ct[tRJ(0, j)] * s2[0] + ct[tRJ(1, j)] * s2[1] + ct[tRJ(2, j)] * s2[2];
+#pragma unroll
+      for (int j = 3; j < 9; ++j) y2[j] = s2[j];
+      double v0s = 0.0, v1s = 0.0, v2s = 0.0, v3s = 0.0, v4s = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const double dd = d[i];
+        const double dc_i = vin[9 * c + i] / dd;
+        out_d[9 * c + i] = dc_i;
+        const double nr = y2[i] + (lambda * (dd * dc_i)) * dd;
+        v0s += nr * nr;
+        v1s += (nr / dd) * (nr / dd);
+        const double g = double(gc[9 * c + i]);
+        v2s += g * dc_i;
+        v3s += g * g;
+        v4s += (g / dd) * (g / dd);
+      }
+      double* pv = parts + kParts * (nc + c);
+      pv[0] = v0s;
+      pv[1] = v1s;
+      pv[2] = v2s;
+      pv[3] = v3s;
+      pv[4] = v4s;
+      pv[5] = 0.0;
+    }
+    if (MODE == cSchur || MODE == cResid || MODE == cCheck) {
       const double* p = vin + 9 * c;
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
@@ -930,7 +1039,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
         out[9 * c + i] = q;
         if (MODE == cSchur) {
           part0 += p[i] * q;
-        } else {
+        } else {  // cResid, cCheck
           const double e = q - rhs[9 * c + i];
           part0 += e * e;
           part1 += rhs[9 * c + i] * rhs[9 * c + i];
