@@ -143,6 +143,11 @@ struct fg_schur {
   hipEvent_t ph_a = nullptr, ph_b = nullptr;
   double last_predicted = 0.0, last_step_norm = 0.0;
   bool step_valid = false;
+  // Device LM loop state (calls that share one synchronization)
+  bool lin_pending = false;       // linearization enqueued, its flag not read yet
+  bool damping_spec = false;      // scales at spec_floor enqueued and read back with the gradient
+  double spec_floor = kNaN;       // the loop's last column-norm floor
+  bool trial_cost_ready = false;  // the last solve staged its step and read back the trial cost
   // learned preconditioner
   fg_mlp* mlp = nullptr;
   void* mbj = nullptr;         // f64 [nc][45] explicit (B + lambda I)^-1
@@ -202,8 +207,10 @@ fg_status ensure_tables(fg_ba* b, int buf) {
   return FG_OK;
 }
 
+// The cost of buffer buf is enqueued by cost_launch and read by cost_finish once the stream
+// has been synchronized.
 template <typename T>
-fg_status cost_impl(fg_ba* b, int buf, double* out) {
+fg_status cost_launch(fg_ba* b, int buf) {
   FG_CHECK(ensure_tables<T>(b, buf));
   if (b->timing) (void)hipEventRecord(b->ev_a, b->stream);
   FG_TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int), b->stream));
@@ -217,7 +224,10 @@ fg_status cost_impl(fg_ba* b, int buf, double* out) {
   FG_TRY(hipMemcpyAsync(b->h_pinned, b->d_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   if (b->timing) (void)hipEventRecord(b->ev_b, b->stream);
-  FG_TRY(hipStreamSynchronize(b->stream));
+  return FG_OK;
+}
+
+fg_status cost_finish(fg_ba* b, double* out) {
   if (b->timing) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, b->ev_a, b->ev_b) == hipSuccess) b->cost_seconds += 1e-3 * ms;
@@ -227,6 +237,13 @@ fg_status cost_impl(fg_ba* b, int buf, double* out) {
   if (!std::isfinite(v) || v < 0.0) return FG_INVALID_EVALUATION;
   *out = v;
   return FG_OK;
+}
+
+template <typename T>
+fg_status cost_impl(fg_ba* b, int buf, double* out) {
+  FG_CHECK(cost_launch<T>(b, buf));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  return cost_finish(b, out);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -268,24 +285,33 @@ void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, c
                      (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, nullptr);
 }
 
+// Enqueues the linearization of the current state; its evaluation flag lands in d_flags[1]
+// (d_flags[0..3] are cleared first).
 template <typename T>
-fg_status linearize_impl(fg_schur* s) {
+fg_status linearize_launch(fg_schur* s) {
   fg_ba* b = s->ba;
   const int buf = b->cur;
   FG_CHECK(ensure_tables<T>(b, buf));
-  FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
+  FG_TRY(hipMemsetAsync(s->d_flags, 0, 4 * sizeof(int), b->stream));
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL(k_linearize_points<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np,
                      b->d_chunk_base, b->d_chunk_len, b->d_slot_cam, (const Vec2<double>*)b->d_slot_pix64,
                      (const T*)b->d_tab[buf], (const double*)b->d_tab64[buf], (const double*)b->d_pts[buf],
                      b->huber, (Vec4<T>*)s->rec,
-                     (Vec2<T>*)s->res, (double*)s->hpp, (T*)s->gp, s->d_flags);
+                     (Vec2<T>*)s->res, (double*)s->hpp, (T*)s->gp, s->d_flags + 1);
   FG_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
                      b->d_k_slot, (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[buf],
-                     (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags);
+                     (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags + 1);
   FG_TRY(hipGetLastError());
-  FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  return FG_OK;
+}
+
+template <typename T>
+fg_status linearize_impl(fg_schur* s) {
+  fg_ba* b = s->ba;
+  FG_CHECK(linearize_launch<T>(s));
+  FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags + 1, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
   if (s->h_int[0] != 0) return FG_INVALID_EVALUATION;
   return FG_OK;
@@ -316,17 +342,54 @@ fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m,
 }
 
 template <typename T>
+void launch_scales(fg_schur* s, double floor, int* flag) {
+  fg_ba* b = s->ba;
+  hipLaunchKernelGGL(k_scales<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     (int)b->np, (const T*)s->hcc, (const double*)s->hpp, (T)floor, (T*)s->dc, (T*)s->dp,
+                     (T*)s->bs, (double*)s->cs, flag);
+}
+
+template <typename T>
 fg_status prepare_damping_impl(fg_schur* s, double floor) {
   fg_ba* b = s->ba;
   Phase ph(s, &s->tim.preparation);
   FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
-  hipLaunchKernelGGL(k_scales<T>, dim3(grid_for(b->nc + b->np)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (int)b->np, (const T*)s->hcc, (const double*)s->hpp, (T)floor, (T*)s->dc, (T*)s->dp,
-                     (T*)s->bs, (double*)s->cs, s->d_flags);
+  launch_scales<T>(s, floor, s->d_flags);
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
   if (s->h_int[0] != 0) return FG_INVALID_EVALUATION;
+  return FG_OK;
+}
+
+// The gradient norm of a linearization enqueued by linearize_launch, read back together with
+// that linearization's evaluation flag; with a finite spec_floor the damping scales at that
+// floor are enqueued too and their flag comes back in h_int[6] (damping_spec).
+template <typename T>
+fg_status gradient_stage(fg_schur* s, double* out) {
+  fg_ba* b = s->ba;
+  const size_t n = 9 * b->nc, m = 3 * b->np;
+  const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
+  hipLaunchKernelGGL(k_norm_inf<T>, dim3(blocks), dim3(kBlock), 0, b->stream, (int)n, (const T*)s->gc, (int)m,
+                     (const T*)s->gp, s->d_cgparts, s->d_flags);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_cgparts, blocks, 1,
+                     s->d_scalar);
+  const bool spec = std::isfinite(s->spec_floor);
+  if (spec) launch_scales<T>(s, s->spec_floor, s->d_flags + 2);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipMemcpyAsync(s->h_norm, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipMemcpyAsync(s->h_int + 4, s->d_flags, 3 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  FG_TRY(hipStreamSynchronize(b->stream));
+  if (s->h_int[5] != 0) {
+    s->linearized = false;
+    return FG_INVALID_EVALUATION;
+  }
+  s->damping_spec = spec;
+  if (s->h_int[4] != 0) {
+    *out = kNaN;
+    return FG_INVALID_EVALUATION;
+  }
+  *out = s->h_norm[0];
   return FG_OK;
 }
 
@@ -451,10 +514,9 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   init.fallback_at = fallback_at;
   init.rel_tol = s->cfg.relative_tolerance;
   init.fallback_factor = s->cfg.fallback_factor;
-  FG_TRY(hipMemcpyAsync(s->d_sc, &init, sizeof(DevScalars), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_cg_init<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const double*)s->mbj, (const double*)s->gmat, (const double*)s->rhs, (double*)s->x,
-                     (double*)s->r, (double*)s->z, (double*)s->p, s->d_cgparts, (const CgState*)&s->d_sc->st[0]);
+                     (double*)s->r, (double*)s->z, (double*)s->p, s->d_cgparts, init, s->d_sc);
   FG_TRY(hipGetLastError());
   const bool timed = s->cfg.collect_timings != 0;
   hipGraphExec_t graph = nullptr;
@@ -513,9 +575,26 @@ void cg_collect(fg_schur* s) {
   s->tim.matvecs += (size_t)std::max(h.iters, 0);
 }
 
-// Schur::attempt (schur.rs:192-202): factor + reduced rhs, CG, recover, verify.
+// The trial state (cur + delta, both parameter blocks) into the other buffer; its camera
+// tables are rebuilt on first use.
 template <typename T>
-fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
+fg_status stage_launch(fg_ba* b, const double* d_delta) {
+  const int other = b->cur ^ 1;
+  const size_t n = 9 * b->nc + 3 * b->np;
+  hipLaunchKernelGGL(k_stage<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, (int)b->nc, (int)b->np,
+                     b->d_cams[b->cur], b->d_pts[b->cur], d_delta, d_delta + 9 * b->nc, b->d_cams[other],
+                     b->d_pts[other], (T*)b->d_pts_t[other]);
+  FG_TRY(hipGetLastError());
+  b->tab_valid[other] = false;
+  return FG_OK;
+}
+
+// Schur::attempt (schur.rs:192-202): factor + reduced rhs, CG, recover, verify. With
+// speculate_trial the trial state of the step and its cost are enqueued behind the
+// verification and come back with the same synchronization (the device LM loop stages and
+// costs every step it does not reject beforehand; trial_cost_ready marks the result).
+template <typename T>
+fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool speculate_trial = false) {
   fg_ba* b = s->ba;
   const double lambda = lambda_d;
   const int nc = (int)b->nc;
@@ -563,9 +642,14 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted) {
   launch_sum_parts(s, nc, 0);
   launch_sum_parts(s, nc, 2, nc);
   FG_TRY(hipGetLastError());
+  if (speculate_trial) {
+    FG_CHECK(stage_launch<T>(b, s->d_delta));
+    FG_CHECK(cost_launch<T>(b, b->cur ^ 1));
+  }
   double step_norm = 0.0;
   const fg_status nst = norm_inf_impl<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0, &step_norm,
                                               3 * kParts);
+  s->trial_cost_ready = speculate_trial;
   if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
   cg_collect(s);
   if (s->factor_failed) return FG_LINEAR_SOLVE_FAILED;  // NotPositiveDefinite, no iteration
@@ -694,13 +778,7 @@ fg_status upload_vector(fg_ba* b, const double* host, double* dev) {
 
 template <typename T>
 fg_status stage_impl(fg_ba* b, const double* d_delta) {
-  const int other = b->cur ^ 1;
-  const size_t n = 9 * b->nc + 3 * b->np;
-  hipLaunchKernelGGL(k_stage<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, (int)b->nc, (int)b->np,
-                     b->d_cams[b->cur], b->d_pts[b->cur], d_delta, d_delta + 9 * b->nc, b->d_cams[other],
-                     b->d_pts[other], (T*)b->d_pts_t[other]);
-  FG_TRY(hipGetLastError());
-  b->tab_valid[other] = false;
+  FG_CHECK(stage_launch<T>(b, d_delta));
   b->trial_active = true;
   return FG_OK;
 }
@@ -1255,15 +1333,16 @@ fg_status fg_schur_prepare_damping(fg_schur* s, double floor) {
   return st;
 }
 
-static fg_status solve_damped_device(fg_schur* s, double lambda, double* predicted) {
+static fg_status solve_damped_device(fg_schur* s, double lambda, double* predicted, bool speculate_trial = false) {
   reset_diag(s, lambda);
   s->step_valid = false;
+  s->trial_cost_ready = false;
   if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
   s->solves += 1;
   s->events_used = 0;
   double pred = 0.0;
-  fg_status st = s->cfg.precision == FG_FP32 ? attempt_impl<float>(s, lambda, &pred)
-                                             : attempt_impl<double>(s, lambda, &pred);
+  fg_status st = s->cfg.precision == FG_FP32 ? attempt_impl<float>(s, lambda, &pred, speculate_trial)
+                                             : attempt_impl<double>(s, lambda, &pred, speculate_trial);
   if (s->cfg.collect_timings && s->events_used > 0) {
     (void)hipEventSynchronize(s->events[s->events_used - 1]);
     // the first diag.iterations pairs bracket real products; later ones are the no-op tail
@@ -1677,7 +1756,7 @@ fg_status lsmr_run(fg_lsmr* l, int mode, bool damped, double lambda, bool* conve
 
 // solve (src/lsmr.rs:215-280) + recover_and_verify (:430-468) for a damped attempt.
 template <typename T>
-fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted) {
+fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted, bool speculate_trial) {
   fg_schur* s = l->s;
   fg_ba* b = s->ba;
   const int mode = l->prepared_mode;
@@ -1750,8 +1829,13 @@ fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted) {
     launch_sum_parts(s, sb, 3);
   }
   FG_TRY(hipGetLastError());
+  if (speculate_trial) {  // the trial state and its cost, as in the Schur attempt
+    FG_CHECK(stage_launch<T>(b, s->d_delta));
+    FG_CHECK(cost_launch<T>(b, b->cur ^ 1));
+  }
   double step_norm = 0.0;
   const fg_status nst = norm_inf_impl<double>(s, s->d_delta, l->n, nullptr, 0, &step_norm, 4 * kParts);
+  s->trial_cost_ready = speculate_trial;
   if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
   const bool finite_step = nst == FG_OK;
   const double* pt = s->h_sums + kParts;      // points: n2, n2s, g.d, g2, g2s, ||J delta||^2
@@ -1811,14 +1895,15 @@ fg_status lsmr_solve_impl(fg_lsmr* l) {
 }
 
 // solve_damped (src/lsmr.rs:560-595) with the step left on the device.
-fg_status lsmr_solve_damped_device(fg_lsmr* l, double lambda, double* pred) {
+fg_status lsmr_solve_damped_device(fg_lsmr* l, double lambda, double* pred, bool speculate_trial = false) {
   lsmr_reset_diag(l);
   l->s->step_valid = false;
+  l->s->trial_cost_ready = false;
   const int mode = l->prepared_mode;
   if (mode < 0 || mode != lsmr_requested_mode(l) || !std::isfinite(lambda) || lambda < 0.0)
     return FG_INVALID_OPTIONS;
-  const fg_status st = l->cfg.precision == FG_FP32 ? lsmr_solve_damped_impl<float>(l, lambda, pred)
-                                                   : lsmr_solve_damped_impl<double>(l, lambda, pred);
+  const fg_status st = l->cfg.precision == FG_FP32 ? lsmr_solve_damped_impl<float>(l, lambda, pred, speculate_trial)
+                                                   : lsmr_solve_damped_impl<double>(l, lambda, pred, speculate_trial);
   if (l->cfg.on_solve && l->diag.termination != FG_LSMR_NONE) l->cfg.on_solve(&l->diag, l->cfg.on_solve_user);
   if (st != FG_OK) return st;
   l->s->step_valid = true;
@@ -1951,18 +2036,74 @@ void fg_lsmr_timings_get(const fg_lsmr* l, fg_lsmr_timings* t) { *t = l->tim; }
 // ------------------------------------------------------------------------------------------
 // LM / GN on the device problem
 // ------------------------------------------------------------------------------------------
+// The loop calls linearize, gradient_norm, prepare_damping in that order, and stage, cost(trial)
+// right after a solve that it does not reject beforehand; each group shares one
+// synchronization. linearize only enqueues its kernels, and its evaluation flag is read with
+// the gradient norm; the damping scales at the loop's previous floor are enqueued with the
+// gradient norm and kept when prepare_damping asks for that floor; solve_damped enqueues the
+// staging and the cost of its step, which stage and cost then take. The same kernels run on
+// the same inputs as in the synchronous calls, so the results are identical. With timings
+// collected every call keeps its own synchronization.
+static bool deferred_calls(const fg_schur* s) { return !s->cfg.collect_timings && !s->ba->timing; }
+
+static void clear_deferred(fg_schur* s) {
+  s->lin_pending = false;
+  s->damping_spec = false;
+  s->trial_cost_ready = false;
+}
+
 static int cb_prepare(void* c) {
   fg_schur* s = (fg_schur*)c;
   s->ba->trial_active = false;
+  clear_deferred(s);
+  s->spec_floor = kNaN;
   return fg_schur_prepare(s, s->ba, 9 * s->ba->nc + 3 * s->ba->np);
 }
-static int cb_cost(void* c, int trial, double* out) { return fg_ba_cost(((fg_schur*)c)->ba, trial, out); }
-static int cb_linearize(void* c) { return fg_schur_linearize((fg_schur*)c); }
-static int cb_gradient_norm(void* c, double* out) { return fg_schur_gradient_norm((fg_schur*)c, out); }
-static int cb_prepare_damping(void* c, double f) { return fg_schur_prepare_damping((fg_schur*)c, f); }
+static int cb_cost(void* c, int trial, double* out) {
+  fg_schur* s = (fg_schur*)c;
+  if (trial && s->trial_cost_ready && s->ba->trial_active) {
+    s->trial_cost_ready = false;
+    return cost_finish(s->ba, out);
+  }
+  return fg_ba_cost(s->ba, trial, out);
+}
+static int cb_linearize(void* c) {
+  fg_schur* s = (fg_schur*)c;
+  clear_deferred(s);
+  if (!deferred_calls(s)) return fg_schur_linearize(s);
+  if (!s->prepared) return FG_INVALID_OPTIONS;
+  s->ready = false;
+  s->linearized = false;
+  FG_CHECK(s->cfg.precision == FG_FP32 ? linearize_launch<float>(s) : linearize_launch<double>(s));
+  s->linearized = true;
+  s->lin_pending = true;
+  return FG_OK;
+}
+static int cb_gradient_norm(void* c, double* out) {
+  fg_schur* s = (fg_schur*)c;
+  s->damping_spec = false;
+  if (!s->lin_pending) return fg_schur_gradient_norm(s, out);
+  s->lin_pending = false;
+  return s->cfg.precision == FG_FP32 ? gradient_stage<float>(s, out) : gradient_stage<double>(s, out);
+}
+static int cb_prepare_damping(void* c, double f) {
+  fg_schur* s = (fg_schur*)c;
+  const bool spec = s->damping_spec && f == s->spec_floor;
+  s->damping_spec = false;
+  if (!spec) {
+    const fg_status st = fg_schur_prepare_damping(s, f);
+    if (std::isfinite(f) && f > 0.0 && deferred_calls(s)) s->spec_floor = f;
+    return st;
+  }
+  s->ready = false;
+  if (!s->linearized) return FG_INVALID_OPTIONS;
+  if (s->h_int[6] != 0) return FG_INVALID_EVALUATION;
+  s->ready = true;
+  return FG_OK;
+}
 static int cb_solve_damped(void* c, double lambda, double* norm, double* pred) {
   fg_schur* s = (fg_schur*)c;
-  fg_status st = solve_damped_device(s, lambda, pred);
+  fg_status st = solve_damped_device(s, lambda, pred, deferred_calls(s));
   if (st == FG_OK) *norm = s->last_step_norm;
   return st;
 }
@@ -1977,6 +2118,10 @@ static int cb_solve(void* c, double* norm) {
 static int cb_stage(void* c) {
   fg_schur* s = (fg_schur*)c;
   if (!s->step_valid) return FG_INVALID_OPTIONS;
+  if (s->trial_cost_ready) {  // staged by the solve
+    s->ba->trial_active = true;
+    return FG_OK;
+  }
   return s->cfg.precision == FG_FP32 ? stage_impl<float>(s->ba, s->d_delta) : stage_impl<double>(s->ba, s->d_delta);
 }
 static void cb_accept(void* c) {
@@ -1984,8 +2129,13 @@ static void cb_accept(void* c) {
   fg_ba_accept(s->ba);
   s->linearized = false;
   s->ready = false;
+  clear_deferred(s);
 }
-static void cb_reject(void* c) { fg_ba_reject(((fg_schur*)c)->ba); }
+static void cb_reject(void* c) {
+  fg_schur* s = (fg_schur*)c;
+  fg_ba_reject(s->ba);
+  clear_deferred(s);
+}
 static void cb_work(void* c, size_t* solves, size_t* iterations) { fg_schur_statistics((fg_schur*)c, solves, iterations); }
 
 static fg_problem device_problem(fg_schur* s) {
@@ -2025,18 +2175,35 @@ fg_status fg_gn_solve_batch(fg_schur* s, fg_ba* b, const fg_optimize_options* op
 
 // The same optimizers over the LSMR backend: cost, staging and acceptance are those of the
 // bound problem; the linear model and damped steps are the LSMR backend's.
+// The Schur callbacks' shared synchronizations (cb_linearize ... cb_cost) over the owned
+// evaluator state.
 static int lcb_prepare(void* c) {
   fg_lsmr* l = (fg_lsmr*)c;
   l->s->ba->trial_active = false;
+  clear_deferred(l->s);
+  l->s->spec_floor = kNaN;
   return fg_lsmr_prepare(l, l->s->ba, 9 * l->s->ba->nc + 3 * l->s->ba->np);
 }
-static int lcb_cost(void* c, int trial, double* out) { return fg_ba_cost(((fg_lsmr*)c)->s->ba, trial, out); }
-static int lcb_linearize(void* c) { return fg_lsmr_linearize((fg_lsmr*)c); }
-static int lcb_gradient_norm(void* c, double* out) { return fg_lsmr_gradient_norm((fg_lsmr*)c, out); }
-static int lcb_prepare_damping(void* c, double f) { return fg_lsmr_prepare_damping((fg_lsmr*)c, f); }
+static int lcb_cost(void* c, int trial, double* out) { return cb_cost(((fg_lsmr*)c)->s, trial, out); }
+static int lcb_linearize(void* c) {
+  fg_lsmr* l = (fg_lsmr*)c;
+  l->prepared_mode = -1;
+  return cb_linearize(l->s);
+}
+static int lcb_gradient_norm(void* c, double* out) { return cb_gradient_norm(((fg_lsmr*)c)->s, out); }
+static int lcb_prepare_damping(void* c, double f) {
+  fg_lsmr* l = (fg_lsmr*)c;
+  l->prepared_mode = -1;
+  {
+    Phase ph(l->s, &l->tim.preparation);
+    FG_CHECK(cb_prepare_damping(l->s, f));
+  }
+  l->prepared_mode = lsmr_requested_mode(l);
+  return FG_OK;
+}
 static int lcb_solve_damped(void* c, double lambda, double* norm, double* pred) {
   fg_lsmr* l = (fg_lsmr*)c;
-  fg_status st = lsmr_solve_damped_device(l, lambda, pred);
+  fg_status st = lsmr_solve_damped_device(l, lambda, pred, deferred_calls(l->s));
   if (st == FG_OK) *norm = l->s->last_step_norm;
   return st;
 }

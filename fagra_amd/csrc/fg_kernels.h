@@ -1144,17 +1144,21 @@ __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_
   }
 }
 
-// x = 0, r = b, z = M r, p = z; per-workgroup partials r.r (col 0) and r.z (col 1).
+// x = 0, r = b, z = M r, p = z; per-workgroup partials r.r (col 0) and r.z (col 1). The PCG's
+// scalar state arrives as an argument and workgroup 0 publishes it (no host copy before the
+// solve).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_init(int nc, const int* __restrict__ cam_off,
                                                     const double* __restrict__ mbj,
                                                     const double* __restrict__ mlearn,
                                                     const double* __restrict__ rhs, double* __restrict__ x,
                                                     double* __restrict__ r, double* __restrict__ z,
-                                                    double* __restrict__ p, double* parts, const CgState* st) {
+                                                    double* __restrict__ p, double* parts, DevScalars init,
+                                                    DevScalars* sc) {
   __shared__ double smem[kBlock / kWave];
-  const bool learned = st->use_learned != 0;
-  const double lambda = st->lambda;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sc = init;
+  const bool learned = init.st[0].use_learned != 0;
+  const double lambda = init.st[0].lambda;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0;
   if (c < nc) {
