@@ -615,6 +615,18 @@ struct PointArgs {
   const int* slot_row;
 };
 
+// The eliminated-block solves t_p handed from the point half of a product to the camera half,
+// one 4-vector row per point in the storage type (16 bytes in f32, gathered once per row).
+template <typename T>
+__device__ __forceinline__ void store_tp(double* tp, int i, const double t[3]) {
+  reinterpret_cast<Vec4<T>*>(tp)[i] = Vec4<T>{T(t[0]), T(t[1]), T(t[2]), T(0)};
+}
+template <typename T>
+__device__ __forceinline__ Vec4<double> load_tp(const double* tp, int i) {
+  const Vec4<T> v = reinterpret_cast<const Vec4<T>*>(tp)[i];
+  return Vec4<double>{double(v.x), double(v.y), double(v.z), 0.0};
+}
+
 // One observation of a point-major pass: record, camera table and forward table in registers.
 template <typename T, int MODE>
 __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& rec,
@@ -756,7 +768,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
         } else {
           t[0] = t[1] = t[2] = 0.0;
         }
-        reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
+        store_tp<T>(t_out, i, t);
         const double v[3] = {double(gp[3 * i]) + w[0], double(gp[3 * i + 1]) + w[1], double(gp[3 * i + 2]) + w[2]};
         point_solve(mp, dpv, i, v, t);
 #pragma unroll
@@ -797,7 +809,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
         } else {
           t[0] = t[1] = t[2] = 0.0;  // unobserved point at lambda = 0: no elimination term
         }
-        reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
+        store_tp<T>(t_out, i, t);
       } else if (MODE == kRhs || MODE == kRecover) {
         double v[3];
         if (MODE == kRhs) {
@@ -811,7 +823,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
 #pragma unroll
           for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -t[a];
         } else {
-          reinterpret_cast<Vec4<double>*>(t_out)[i] = Vec4<double>{t[0], t[1], t[2], 0.0};
+          store_tp<T>(t_out, i, t);
         }
       } else if (MODE == kVerify) {
 #pragma unroll
@@ -954,7 +966,7 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
       return Vec4<double>{u.x, u.y, 0.0, 0.0};
     }
     if (MODE == cVerify) return Vec4<double>{delta_p[3 * ps], delta_p[3 * ps + 1], delta_p[3 * ps + 2], 0.0};
-    return reinterpret_cast<const Vec4<double>*>(tp)[ps];
+    return load_tp<T>(tp, ps);
   };
   int k = k0 + threadIdx.x;
   // two rows per step: records and gathers of both issue before either is used
