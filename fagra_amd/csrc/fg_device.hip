@@ -78,6 +78,7 @@ struct fg_ba {
   // typed ones in f64 precision).
   Vec2<double>* d_slot_pix64 = nullptr;
   double* d_tab64[2] = {nullptr, nullptr};
+  double* d_tabw[2] = {nullptr, nullptr};  // the T tables widened to f64 (aliases d_tab in f64)
   bool fp32 = false;
   // State: two buffers (accepted / trial) of f64 values plus compute-precision points and
   // camera tables.
@@ -114,6 +115,7 @@ struct fg_schur {
   void *dc = nullptr, *dp = nullptr, *bs = nullptr, *cs = nullptr, *lc = nullptr, *mp = nullptr;
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   void *rhs = nullptr, *zero = nullptr;
+  double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
   double *d_delta = nullptr;  // [9nc cameras | 3np points in slot order]
   double *d_rows = nullptr;   // 2no scratch for operator products
   double *d_parts = nullptr;  // max(nc, blocks) x kParts (per-camera / per-block partials)
@@ -197,11 +199,17 @@ struct Phase {
 template <typename T>
 fg_status ensure_tables(fg_ba* b, int buf) {
   if (b->tab_valid[buf]) return FG_OK;
-  hipLaunchKernelGGL(k_camera_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     b->d_cams[buf], (T*)b->d_tab[buf]);
-  if (b->fp32)
-    hipLaunchKernelGGL(k_camera_tables<double>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                       b->d_cams[buf], b->d_tab64[buf]);
+  // f64 tables first; in f32 the T table is their rounding, entry by entry (the point-major
+  // linearization relies on that to form it from the f64 table)
+  hipLaunchKernelGGL(k_camera_tables<double>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     b->d_cams[buf], b->d_tab64[buf]);
+  if (b->fp32) {
+    const int n = (int)(kCamTab * b->nc);
+    hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, n,
+                       (const double*)b->d_tab64[buf], (T*)b->d_tab[buf]);
+    hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, n, (const T*)b->d_tab[buf],
+                       b->d_tabw[buf]);
+  }
   FG_TRY(hipGetLastError());
   b->tab_valid[buf] = true;
   return FG_OK;
@@ -261,6 +269,9 @@ PointArgs point_args(const fg_ba* b) {
   return a;
 }
 
+// The forward table as f64 rows for the camera-major kernels.
+inline const double* fw_wide(const fg_schur* s) { return s->fw_w != nullptr ? s->fw_w : (const double*)s->fw; }
+
 template <typename T, int MODE>
 void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, double lambda,
                    double* delta_out, double* rows_out, double* parts, const CgState* st) {
@@ -277,11 +288,12 @@ void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, c
                     const double* vin, const double* delta_c, const double* rhs, double lambda, double* out,
                     double* out_d, const CgState* st = nullptr) {
   fg_ba* b = s->ba;
-  // the Schur, residual and verification modes read the forward table of their input
-  if (fw == nullptr) fw = (const T*)s->fw;
+  // the Schur, residual and verification modes read the forward table of their input, which
+  // is always s->fw (fw names it at the call), widened to f64
+  (void)fw;
   hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
-                     (const T*)b->d_tab[b->cur], (const double*)s->tp, delta_p, rows_in, fw, (const T*)s->dc,
+                     (const double*)b->d_tabw[b->cur], (const double*)s->tp, delta_p, rows_in, fw_wide(s), (const T*)s->dc,
                      (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, nullptr);
 }
 
@@ -296,7 +308,7 @@ fg_status linearize_launch(fg_schur* s) {
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   hipLaunchKernelGGL(k_linearize_points<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np,
                      b->d_chunk_base, b->d_chunk_len, b->d_slot_cam, (const Vec2<double>*)b->d_slot_pix64,
-                     (const T*)b->d_tab[buf], (const double*)b->d_tab64[buf], (const double*)b->d_pts[buf],
+                     (const double*)b->d_tab64[buf], (const double*)b->d_pts[buf],
                      b->huber, (Vec4<T>*)s->rec,
                      (Vec2<T>*)s->res, (double*)s->hpp, (T*)s->gp, s->d_flags + 1);
   FG_TRY(hipGetLastError());
@@ -418,7 +430,7 @@ template <typename T>
 fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out, const double* resid_rhs) {
   fg_ba* b = s->ba;
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw);
+                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w);
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
   launch_cameras<T, cResid>(s, nullptr, nullptr, (const T*)s->fw, x, nullptr,
                             resid_rhs ? resid_rhs : (const double*)s->zero, lambda, out, nullptr);
@@ -450,7 +462,7 @@ void launch_cg_iteration(fg_schur* s, int i, bool timed) {
   CgState* cur = &s->d_sc->st[(i + 1) & 1];
   hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
-                     (double*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
+                     (double*)s->p, (T*)s->fw, s->fw_w, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], (const DevScalars*)s->d_sc,
                      (const int*)s->d_fail);
   const bool ev = timed && s->events_used + 2 <= s->events.size();
@@ -545,7 +557,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   // the last update's convergence / exhaustion decision
   hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
-                     (double*)s->p, (T*)s->fw, (const double*)s->d_cgparts, cblocks,
+                     (double*)s->p, (T*)s->fw, s->fw_w, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
                      (const DevScalars*)s->d_sc, (const int*)s->d_fail);
   FG_TRY(hipGetLastError());
@@ -632,7 +644,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   // ||delta||_inf. With a failed block factorization the PCG did not iterate and these results
   // are discarded (NotPositiveDefinite).
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw);
+                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
                            s->d_parts, nullptr);
@@ -699,6 +711,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((double**)&s->mp, 6 * np));
   FG_TRY(dalloc((double**)&s->tp, 4 * np));
   FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
+  if (sizeof(T) == 4) FG_TRY(dalloc(&s->fw_w, kFwd * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((double**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
@@ -735,6 +748,7 @@ void free_model(fg_schur* s) {
     if (*v) (void)hipFree(*v);
     *v = nullptr;
   }
+  dfree(s->fw_w);
   dfree(s->d_delta);
   dfree(s->d_rows);
   dfree(s->d_parts);
@@ -802,7 +816,7 @@ fg_status jacobian_apply_impl(fg_schur* s, const double* x, double* y) {
   fg_ba* b = s->ba;
   FG_CHECK(upload_vector(b, x, s->d_delta));
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w);
   launch_points<T, kJv>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, 0.0, nullptr, s->d_rows, nullptr,
                         nullptr);
   FG_TRY(hipGetLastError());
@@ -899,7 +913,7 @@ fg_status time_products_impl(fg_schur* s, double lambda, const double* x, int re
   if (fails) return FG_LINEAR_SOLVE_FAILED;
   FG_TRY(hipMemcpyAsync(s->p, x, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->p, (const T*)s->dc, 1, (T*)s->fw);
+                     (const T*)b->d_tab[b->cur], (const double*)s->p, (const T*)s->dc, 1, (T*)s->fw, s->fw_w);
   // one untimed pass of each warms the caches the timed passes see
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
   launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, lambda,
@@ -1060,9 +1074,11 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
         hipMalloc(&b->d_tab[i], kCamTab * nc * ts) != hipSuccess)
       return fail(FG_DEVICE_ERROR);
     if (b->fp32) {
-      if (dalloc(&b->d_tab64[i], kCamTab * nc) != hipSuccess) return fail(FG_DEVICE_ERROR);
+      if (dalloc(&b->d_tab64[i], kCamTab * nc) != hipSuccess || dalloc(&b->d_tabw[i], kCamTab * nc) != hipSuccess)
+        return fail(FG_DEVICE_ERROR);
     } else {
       b->d_tab64[i] = (double*)b->d_tab[i];
+      b->d_tabw[i] = (double*)b->d_tab[i];
     }
   }
   if (!b->fp32) b->d_slot_pix64 = (Vec2<double>*)b->d_slot_pix;
@@ -1098,6 +1114,7 @@ void fg_ba_destroy(fg_ba* b) {
   if (b->d_slot_pix) (void)hipFree(b->d_slot_pix);
   for (int i = 0; i < 2; ++i) {
     if (b->fp32 && b->d_tab64[i]) (void)hipFree(b->d_tab64[i]);
+    if (b->fp32 && b->d_tabw[i]) (void)hipFree(b->d_tabw[i]);
     dfree(b->d_cams[i]);
     dfree(b->d_pts[i]);
     if (b->d_pts_t[i]) (void)hipFree(b->d_pts_t[i]);
@@ -1655,8 +1672,9 @@ void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
                      (const CgState*)nullptr, st);
   // the camera pass gathers U by observation slot (k_slot in the gather-index argument)
   hipLaunchKernelGGL((k_cameras<T, cLsmr>), dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                     b->d_k_slot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm, (const T*)b->d_tab[b->cur],
-                     (const double*)s->tp, (const double*)nullptr, (const double*)l->d_u, (const T*)s->fw,
+                     b->d_k_slot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
+                     (const double*)b->d_tabw[b->cur], (const double*)s->tp, (const double*)nullptr,
+                     (const double*)l->d_u, fw_wide(s),
                      (const T*)s->dc, (const T*)s->gc, damped && step ? ws : nullptr, (const double*)nullptr,
                      (const double*)nullptr, 0.0, damped && step ? l->d_ub : nullptr, l->d_zt,
                      l->d_up + l->pblocks, (const CgState*)nullptr, st);
@@ -1678,7 +1696,7 @@ void lsmr_iteration(fg_lsmr* l, int mode, bool damped) {
                      (const double*)s->lc, (const double*)s->mp, (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0,
                      scaled ? 1 : 0, 0, (const LsmrState*)l->d_st, l->d_ws);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw);
+                     (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw, s->fw_w);
   lsmr_passes<T>(l, l->d_ws, damped);
   hipLaunchKernelGGL(k_lsmr_v<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_zt,
                      (const double*)l->d_ub, (const double*)s->lc, (const double*)s->mp, (const T*)s->dc,
@@ -1813,7 +1831,7 @@ fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted, b
     hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(3 * np)), dim3(kBlock), 0, b->stream, 3 * np,
                        (const double*)(s->d_delta + 9 * nc), (const T*)s->dp, s->d_delta + 9 * nc);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw);
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * nc, nullptr, lambda, l->d_zt + 9 * nc, nullptr,
                             s->d_parts, nullptr);

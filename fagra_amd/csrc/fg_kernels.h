@@ -249,12 +249,54 @@ static __global__ void k_reduce_parts(const double* __restrict__ parts, int n, i
 // table in f64, the same way the Schur products form it, and Hpp and the gradient accumulate
 // in f64, so the point blocks are the exact Gram blocks of the stored Jacobian.
 // ---------------------------------------------------------------------------------------
+// One observation of the point-major linearization. cd is the f64 camera table; the table in
+// T is its rounding (k_camera_tables stores both from the same f64 values), formed here
+// instead of gathered a second time.
+template <typename T>
+__device__ __forceinline__ bool linearize_obs(const double cd[16], double x0, double x1, double x2,
+                                              const Vec2<double>& ob, double huber, Vec4<T>& r4, Vec2<T>& rw,
+                                              double h[6], double g[3]) {
+  double px, py, qx, qy, z;
+  bool ok = project(cd, x0, x1, x2, px, py, qx, qy, z);
+  const double rx = px - ob.x, ry = py - ob.y;
+  double cost, sw;
+  huber_weight(rx, ry, huber, cost, sw);
+  double ct[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) ct[i] = double(T(cd[i]));
+  r4 = Vec4<T>{T(qx), T(qy), T(z), T(sw)};
+  const double wrx = sw * rx, wry = sw * ry;
+  rw = Vec2<T>{T(wrx), T(wry)};
+  ObsGeom<double> geo, g64;
+  obs_geom(to_d(r4), ct, geo);
+  obs_geom(Vec4<double>{qx, qy, z, sw}, cd, g64);
+  // Jp columns: Jp[:, k] = sw * dp * R[:, k], from the stored record (Hpp) and from the f64
+  // geometry (gradient)
+  double jp0[3], jp1[3], e0[3], e1[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    jp0[k] = geo.sw * (geo.dp0[0] * ct[tR(0, k)] + geo.dp0[1] * ct[tR(1, k)] + geo.dp0[2] * ct[tR(2, k)]);
+    jp1[k] = geo.sw * (geo.dp1[0] * ct[tR(0, k)] + geo.dp1[1] * ct[tR(1, k)] + geo.dp1[2] * ct[tR(2, k)]);
+    e0[k] = g64.sw * (g64.dp0[0] * cd[tR(0, k)] + g64.dp0[1] * cd[tR(1, k)] + g64.dp0[2] * cd[tR(2, k)]);
+    e1[k] = g64.sw * (g64.dp1[0] * cd[tR(0, k)] + g64.dp1[1] * cd[tR(1, k)] + g64.dp1[2] * cd[tR(2, k)]);
+    ok = ok && isfinite(jp0[k]) && isfinite(jp1[k]);
+  }
+  ok = ok && isfinite(wrx) && isfinite(wry);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = a; b < 3; ++b) h[sym3(b, a)] += jp0[a] * jp0[b] + jp1[a] * jp1[b];
+    g[a] += e0[a] * wrx + e1[a] * wry;
+  }
+  return ok;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_linearize_points(
     int nchunks, int np, const int* __restrict__ chunk_base, const int* __restrict__ chunk_len,
-    const int* __restrict__ slot_cam, const Vec2<double>* __restrict__ slot_pix, const T* __restrict__ tab,
-    const double* __restrict__ tab64, const double* __restrict__ pts, double huber, Vec4<T>* __restrict__ rec,
-    Vec2<T>* __restrict__ res, double* __restrict__ hpp, T* __restrict__ gp, int* flags) {
+    const int* __restrict__ slot_cam, const Vec2<double>* __restrict__ slot_pix, const double* __restrict__ tab64,
+    const double* __restrict__ pts, double huber, Vec4<T>* __restrict__ rec, Vec2<T>* __restrict__ res,
+    double* __restrict__ hpp, T* __restrict__ gp, int* flags) {
   const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   if (chunk >= nchunks) return;
@@ -265,45 +307,39 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
   double h[6] = {0, 0, 0, 0, 0, 0};
   double g[3] = {0, 0, 0};
   bool bad = false;
-  for (int j = 0; j < len; ++j) {
-    const int s = base + j * kWave + lane;
-    const int c = slot_cam[s];
-    if (c < 0) continue;
-    double cd[16];
-    load_cam16(tab64, c, cd);
-    double px, py, qx, qy, z;
-    bool ok = project(cd, x0, x1, x2, px, py, qx, qy, z);
-    const Vec2<double> ob = slot_pix[s];
-    const double rx = px - ob.x, ry = py - ob.y;
-    double cost, sw;
-    huber_weight(rx, ry, huber, cost, sw);
-    double ct[16];
-    load_cam16_d(tab, c, ct);
-    const Vec4<T> r4 = {T(qx), T(qy), T(z), T(sw)};
-    rec[s] = r4;
-    const double wrx = sw * rx, wry = sw * ry;
-    res[s] = Vec2<T>{T(wrx), T(wry)};
-    ObsGeom<double> geo, g64;
-    obs_geom(to_d(r4), ct, geo);
-    obs_geom(Vec4<double>{qx, qy, z, sw}, cd, g64);
-    // Jp columns: Jp[:, k] = sw * dp * R[:, k], from the stored record (Hpp) and from the f64
-    // geometry (gradient)
-    double jp0[3], jp1[3], e0[3], e1[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      jp0[k] = geo.sw * (geo.dp0[0] * ct[tR(0, k)] + geo.dp0[1] * ct[tR(1, k)] + geo.dp0[2] * ct[tR(2, k)]);
-      jp1[k] = geo.sw * (geo.dp1[0] * ct[tR(0, k)] + geo.dp1[1] * ct[tR(1, k)] + geo.dp1[2] * ct[tR(2, k)]);
-      e0[k] = g64.sw * (g64.dp0[0] * cd[tR(0, k)] + g64.dp0[1] * cd[tR(1, k)] + g64.dp0[2] * cd[tR(2, k)]);
-      e1[k] = g64.sw * (g64.dp1[0] * cd[tR(0, k)] + g64.dp1[1] * cd[tR(1, k)] + g64.dp1[2] * cd[tR(2, k)]);
-      ok = ok && isfinite(jp0[k]) && isfinite(jp1[k]);
+  int j = 0;
+  // two observations per step: both tables and pixels load before either is used
+  for (; j + 1 < len; j += 2) {
+    const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
+    const int c0 = slot_cam[s0], c1 = slot_cam[s1];
+    double cd0[16], cd1[16];
+    if (c0 >= 0) load_cam16(tab64, c0, cd0);
+    if (c1 >= 0) load_cam16(tab64, c1, cd1);
+    const Vec2<double> ob0 = slot_pix[s0], ob1 = slot_pix[s1];
+    Vec4<T> r4;
+    Vec2<T> rw;
+    if (c0 >= 0) {
+      bad |= !linearize_obs<T>(cd0, x0, x1, x2, ob0, huber, r4, rw, h, g);
+      rec[s0] = r4;
+      res[s0] = rw;
     }
-    ok = ok && isfinite(wrx) && isfinite(wry);
-    bad |= !ok;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-      for (int b = a; b < 3; ++b) h[sym3(b, a)] += jp0[a] * jp0[b] + jp1[a] * jp1[b];
-      g[a] += e0[a] * wrx + e1[a] * wry;
+    if (c1 >= 0) {
+      bad |= !linearize_obs<T>(cd1, x0, x1, x2, ob1, huber, r4, rw, h, g);
+      rec[s1] = r4;
+      res[s1] = rw;
+    }
+  }
+  if (j < len) {
+    const int s0 = base + j * kWave + lane;
+    const int c0 = slot_cam[s0];
+    if (c0 >= 0) {
+      double cd0[16];
+      load_cam16(tab64, c0, cd0);
+      Vec4<T> r4;
+      Vec2<T> rw;
+      bad |= !linearize_obs<T>(cd0, x0, x1, x2, slot_pix[s0], huber, r4, rw, h, g);
+      rec[s0] = r4;
+      res[s0] = rw;
     }
   }
   if (bad) atomicOr(flags, 1);
@@ -337,10 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
   for (int i = 0; i < 9; ++i) gacc[i] = 0.0;
   bool bad = false;
   const int k0 = cam_off[c], k1 = cam_off[c + 1];
-  for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
-    const int s = k_slot[k];
-    const Vec4<T> r4 = rec[s];
-    const Vec2<T> rw = res[s];
+  auto one = [&](int k, const Vec4<T>& r4, const Vec2<T>& rw) {
     rec_cm[k] = r4;
     res_cm[k] = rw;
     ObsGeom<T> geo;
@@ -356,6 +389,19 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
+  };
+  int k = k0 + threadIdx.x;
+  // two rows per step: both gathers issue before either row is used
+  for (; k + kBlock < k1; k += 2 * kBlock) {
+    const int sa = k_slot[k], sb = k_slot[k + kBlock];
+    const Vec4<T> ra = rec[sa], rb = rec[sb];
+    const Vec2<T> wa = res[sa], wb = res[sb];
+    one(k, ra, wa);
+    one(k + kBlock, rb, wb);
+  }
+  if (k < k1) {
+    const int sa = k_slot[k];
+    one(k, rec[sa], res[sa]);
   }
   if (bad) atomicOr(flags, 1);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -564,26 +610,37 @@ __global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const 
 // Forward camera tables from a retained-coordinate vector. scaled != 0: z = x / Dc (the
 // reduced system's scaled coordinates); otherwise z = x. fw = (RJ z_rot, z_t, z_intr).
 // ---------------------------------------------------------------------------------------
+// fww (f32 storage only): the same table widened to f64, which the camera-major kernels read
+// with uniform loads.
 template <typename T>
 __device__ __forceinline__ void forward_table(const T* __restrict__ ct, const double* x, const T* __restrict__ d,
-                                              bool scaled, T* __restrict__ fw, double z[9]) {
+                                              bool scaled, T* __restrict__ fw, double* __restrict__ fww,
+                                              double z[9]) {
 #pragma unroll
   for (int j = 0; j < 9; ++j) z[j] = scaled ? x[j] / double(d[j]) : x[j];
+  T v[kFwd];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-    fw[i] = T(double(ct[tRJ(i, 0)]) * z[0] + double(ct[tRJ(i, 1)]) * z[1] + double(ct[tRJ(i, 2)]) * z[2]);
+    v[i] = T(double(ct[tRJ(i, 0)]) * z[0] + double(ct[tRJ(i, 1)]) * z[1] + double(ct[tRJ(i, 2)]) * z[2]);
 #pragma unroll
-  for (int j = 3; j < 9; ++j) fw[j] = T(z[j]);
-  fw[9] = fw[10] = fw[11] = T(0);
+  for (int j = 3; j < 9; ++j) v[j] = T(z[j]);
+  v[9] = v[10] = v[11] = T(0);
+#pragma unroll
+  for (int i = 0; i < kFwd; ++i) fw[i] = v[i];
+  if (fww != nullptr) {
+#pragma unroll
+    for (int i = 0; i < kFwd; ++i) fww[i] = double(v[i]);
+  }
 }
 
 template <typename T>
 __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double* __restrict__ x,
-                                 const T* __restrict__ dc, int scaled, T* __restrict__ fw) {
+                                 const T* __restrict__ dc, int scaled, T* __restrict__ fw, double* __restrict__ fww) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   double z[9];
-  forward_table(tab + kCamTab * c, x + 9 * c, dc + 9 * c, scaled != 0, fw + kFwd * c, z);
+  forward_table(tab + kCamTab * c, x + 9 * c, dc + 9 * c, scaled != 0, fw + kFwd * c,
+                fww != nullptr ? fww + kFwd * c : nullptr, z);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -889,16 +946,20 @@ __global__ void __launch_bounds__(kBlock) k_points(
 //             partials in parts[nc + c]
 enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5, cCheck = 6 };
 
+// Waves per SIMD the camera-major kernel is compiled for: the f32 product modes fit six (80
+// VGPRs), so all of a 1,500-camera launch is resident at once; the other modes keep the
+// compiler's default.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kBlock) k_cameras(
+constexpr int camera_waves() { return sizeof(T) == 4 && (MODE == 0 || MODE == 1) ? 6 : 1; }
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(camera_waves<T, MODE>()))) k_cameras(
     int nc, const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
-    const Vec2<T>* __restrict__ res_cm, const T* __restrict__ tab, const double* __restrict__ tp,
-    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const T* __restrict__ fw,
+    const Vec2<T>* __restrict__ res_cm, const double* __restrict__ tabw, const double* __restrict__ tp,
+    const double* __restrict__ delta_p, const double* __restrict__ rows_in, const double* __restrict__ fww,
     const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
     double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls) {
-  __shared__ double ct[kCamTab];
-  __shared__ double cfw[kFwd];
   __shared__ double red[9][kBlock / kWave];
   if (MODE == cSchur && st != nullptr) {
     if (st->done) return;
@@ -906,10 +967,10 @@ __global__ void __launch_bounds__(kBlock) k_cameras(
   }
   if (ls != nullptr && ls->done) return;
   const int c = blockIdx.x;
-  if (threadIdx.x < kCamTab) ct[threadIdx.x] = double(tab[kCamTab * c + threadIdx.x]);
-  if ((MODE == cVerify || MODE == cSchur || MODE == cResid || MODE == cCheck) && threadIdx.x < kFwd)
-    cfw[threadIdx.x] = double(fw[kFwd * c + threadIdx.x]);
-  __syncthreads();
+  // the camera's table and forward table as f64 rows (the T values widened): uniform over the
+  // workgroup, so they are read with scalar loads and stay out of the vector registers
+  const double* __restrict__ ct = tabw + kCamTab * c;
+  const double* __restrict__ cfw = fww + kFwd * c;
   double acc[9], acc2[9];
 #pragma unroll
   for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
@@ -1207,6 +1268,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
                                                          const T* __restrict__ tab, const T* __restrict__ dc,
                                                          const double* __restrict__ zb, const double* __restrict__ z,
                                                          double* __restrict__ p, T* __restrict__ fw,
+                                                         double* __restrict__ fww,
                                                          const double* __restrict__ parts, int nparts,
                                                          const CgState* __restrict__ in, CgState* out,
                                                          const DevScalars* __restrict__ sc,
@@ -1270,7 +1332,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
     p[9 * c + i] = pv[i];
   }
   double zz[9];
-  forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, zz);
+  forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, fww != nullptr ? fww + kFwd * c : nullptr, zz);
 }
 
 // x += alpha p; r -= alpha q; z = M r. Every workgroup forms p.q from the camera kernel's
@@ -1366,6 +1428,12 @@ __global__ void k_stage(int nc, int np, const double* __restrict__ cams, const d
     tpts[j] = v;
     tpts_t[j] = T(v);
   }
+}
+
+template <typename T>
+__global__ void k_widen(int n, const T* __restrict__ a, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = double(a[i]);
 }
 
 template <typename T>
