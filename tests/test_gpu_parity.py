@@ -209,3 +209,46 @@ def test_graph_batches_match_eager_launches(precision):
         assert s0 == s1 and t0 == t1 and i0 == i1
         if s0 == fa.OK:
             assert np.array_equal(d0, d1) and p0 == p1
+
+
+@pytest.mark.skip(reason="the LDS-resident f32 point pass (k_points_lds) left the GPU faulted on its first run "
+                         "and its cause is not found; the kernel and its selection hook are out of the library "
+                         "until it is")
+def test_lds_point_passes_match_global_gathers(monkeypatch):
+    """The f32 point passes with every camera held in LDS (k_points_lds: the Schur product's
+    elimination and the attempt's recovery/verification sweep) equal the same passes with
+    per-observation global gathers (FAGRA_LDS_POINTS=0): the same values go through the same
+    arithmetic, so the product and the step agree bit for bit; the verification partials are
+    summed over other workgroups and agree to rounding."""
+    p = small(seed=21, nc=48, npts=4000, huber=1.0)
+    x = np.random.default_rng(5).normal(size=9 * p.n_cameras)
+    runs = []
+    for flag in (None, "0"):
+        if flag is None:
+            monkeypatch.delenv("FAGRA_LDS_POINTS", raising=False)
+        else:
+            monkeypatch.setenv("FAGRA_LDS_POINTS", flag)
+        ba = fa.BalProblem(p, precision=fa.FP32)
+        s = fa.Schur(precision=fa.FP32)
+        assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+        lds = s.lds_workgroups()
+        st, y = s.reduced_apply(1e-3, x)
+        assert st == fa.OK
+        solves = []
+        for lam in (1e-3, 1e-1):
+            st, d, pr = s.solve_damped(lam)
+            assert st == fa.OK
+            dg = s.diagnostics()
+            solves.append((d, pr, dg.iterations, dg.reduced_relative_residual, dg.original_relative_residual))
+        runs.append((lds, y, solves))
+    (lds_a, y_a, sa), (lds_b, y_b, sb) = runs
+    assert lds_a > 0 and lds_b == 0
+    assert np.array_equal(y_a, y_b)
+    for (d0, p0, i0, r0, o0), (d1, p1, i1, r1, o1) in zip(sa, sb):
+        assert i0 == i1 and np.array_equal(d0, d1)
+        assert abs(p0 - p1) <= 1e-12 * abs(p1)
+        assert abs(r0 - r1) <= 1e-9 * abs(r1) and abs(o0 - o1) <= 1e-9 * abs(o1)
+    # f64 keeps the gathers
+    ba = fa.BalProblem(p, precision=fa.FP64)
+    s = fa.Schur(precision=fa.FP64)
+    assert s.prepare(ba) == fa.OK and s.lds_workgroups() == 0
