@@ -534,9 +534,8 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   hipGraphExec_t graph = nullptr;
   if (!timed && s->use_graphs) FG_CHECK(cg_batch_graph<T>(s, &graph));
   DevScalars* polls = (DevScalars*)s->h_poll;  // two pinned slots
-  int launched = 0, batch = 0;
-  bool stop = false;
-  while (!stop && launched < max_it) {
+  int launched = 0, issued = 0, read = 0;  // iterations launched; batches issued / read back
+  auto issue = [&]() -> fg_status {
     if (graph) {
       FG_TRY(hipGraphLaunch(graph, b->stream));
     } else {
@@ -544,15 +543,37 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
       FG_TRY(hipGetLastError());
     }
     launched += kBatch;
-    const int slot = batch & 1;
+    const int slot = issued & 1;
     FG_TRY(hipMemcpyAsync(&polls[slot], s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
     FG_TRY(hipEventRecord(s->poll_ev[slot], b->stream));
-    if (batch > 0) {
-      FG_TRY(hipEventSynchronize(s->poll_ev[slot ^ 1]));
-      const DevScalars& h = polls[slot ^ 1];
-      if (h.st[0].done || h.st[1].done) stop = true;
-    }
-    ++batch;
+    ++issued;
+    return FG_OK;
+  };
+  // The next batch normally goes out before the previous one is read, so the device never
+  // waits for the host; when the state read last predicts (from its residual's geometric rate)
+  // that the solve ends inside the batch already in flight, the host reads that batch first
+  // instead: a batch launched after the end is kBatch iterations of no-op kernels, a wrong
+  // prediction costs one host turnaround.
+  auto ends_in_flight = [&](const CgState& h) {
+    if (h.done || h.iters < 1 || !(h.bb > 0.0) || !(h.rr > h.tol2) || !(h.tol2 > 0.0)) return false;
+    const double rate = std::log(h.rr / h.bb) / h.iters;  // log of the mean decrease per iteration
+    if (!(rate < 0.0)) return false;
+    const double need = std::ceil(std::log(h.tol2 / h.rr) / rate);
+    return h.iters + need <= launched - 1;
+  };
+  FG_CHECK(issue());
+  CgState last{};
+  bool have = false;
+  while (launched < max_it) {  // past max_it the final decision is read with the verification
+    const bool ahead = !(have && ends_in_flight(last));
+    if (ahead) FG_CHECK(issue());
+    FG_TRY(hipEventSynchronize(s->poll_ev[read & 1]));
+    const DevScalars& h = polls[read & 1];
+    ++read;
+    last = h.st[0].iters >= h.st[1].iters ? h.st[0] : h.st[1];
+    have = true;
+    if (h.st[0].done || h.st[1].done) break;
+    if (issued == read) have = false;  // nothing in flight: the next batch goes out unconditionally
   }
   // the last update's convergence / exhaustion decision
   hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
