@@ -658,11 +658,13 @@ __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double
 //             U_p = sqrt(lambda) Dp ws_p - (alpha/beta) U_p, and ||U||^2 partials; with no
 //             forward table it only forms x_p and ||U||^2 of the current U (the start vector)
 // ---------------------------------------------------------------------------------------
-//   kCheck:   the whole point side of a damped attempt's recovery and verification in two
-//             sweeps over the point's observations: sweep 1 as kSchur at z (t_p for the
-//             reduced residual ||S z - rhs||) and kRecover (delta_p from the same sum), sweep 2
-//             as kVerify at delta = (z / D, delta_p); both forward tables are the table of z
+//   kCheck:   the whole point side of a damped attempt's recovery and verification in one
+//             sweep over the point's observations: kSchur at z (t_p for the reduced residual
+//             ||S z - rhs||), kRecover (delta_p from the same sum), and kVerify's terms at
+//             delta = (z / D, delta_p) expanded in sums of that sweep
 enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu = 5, kLsmr = 6, kCheck = 7 };
+// Sweep mode of kCheck only: kSchur's sums plus sum ||Jc z||^2 (uu) and sum Jp^T r (gr).
+constexpr int kSchurSums = 8;
 
 struct PointArgs {
   int nchunks, np;
@@ -690,12 +692,12 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
                                           const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
                                           const T* __restrict__ fw, const double dpx[3],
                                           const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                                          double w[3], double h[6], double& uu, double coef) {
+                                          double w[3], double h[6], double& uu, double coef, double gr[3]) {
   double ct[16];
   load_cam16_d(tab, c, ct);
   ObsGeom<double> geo;
   obs_geom(to_d(rec), ct, geo);
-  if (MODE == kSchur) {
+  if (MODE == kSchur || MODE == kSchurSums) {
     double f[kFwd];
     load_fwd_d(fw, c, f);
     double u0, u1;
@@ -711,6 +713,12 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
 #pragma unroll
       for (int b = a; b < 3; ++b) h[sym3(b, a)] += j0[a] * j0[b] + j1[a] * j1[b];
       w[a] += j0[a] * u0 + j1[a] * u1;
+    }
+    if (MODE == kSchurSums) {
+      uu += u0 * u0 + u1 * u1;
+      const Vec2<T> r = res[s];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) gr[a] += j0[a] * double(r.x) + j1[a] * double(r.y);
     }
   } else if (MODE == kRecover) {
     double f[kFwd];
@@ -763,20 +771,20 @@ __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int 
                                             const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
                                             const T* __restrict__ fw, const double dpx[3],
                                             const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                                            double w[3], double h[6], double& uu, double coef) {
+                                            double w[3], double h[6], double& uu, double coef, double gr[3]) {
   const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
   int j = 0;
   for (; j + 1 < len; j += 2) {
     const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
     const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
     const Vec4<T> r0 = rec[s0], r1 = rec[s1];
-    if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
-    if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+    if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
+    if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
   }
   if (j < len) {
     const int s0 = base + j * kWave + lane;
     const int c0 = pa.slot_cam[s0];
-    if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+    if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
   }
 }
 
@@ -811,8 +819,16 @@ __global__ void __launch_bounds__(kBlock) k_points(
         dpx[2] = delta_p_in[3 * i + 2];
       }
       if (MODE == kCheck) {
-        // sweep 1: w = sum Jp^T Jc z, Hpp re-formed; t_p as kSchur, delta_p as kRecover
-        point_sweep<T, kSchur>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+        // One sweep: w = sum Jp^T Jc z and Hpp re-formed as kSchur (t_p), delta_p as kRecover,
+        // plus sum ||Jc z||^2 and sum Jp^T r, from which the verification of
+        // delta = (z / D, delta_p) follows without a second sweep:
+        //   ||J delta||^2 over the point's rows = sum ||Jc z||^2 + 2 delta_p.w + delta_p^T Hpp delta_p
+        //   Jp^T (J delta + r)                  = w + Hpp delta_p + sum Jp^T r
+        double gr[3] = {0.0, 0.0, 0.0};
+        point_sweep<T, kSchurSums>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
+        double hu[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) hu[k] = h[k];  // undamped Hpp
         double t[3] = {w[0], w[1], w[2]};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -832,14 +848,17 @@ __global__ void __launch_bounds__(kBlock) k_points(
         for (int a = 0; a < 3; ++a) {
           dpx[a] = -t[a];
           delta_out[3 * i + a] = dpx[a];
-          w[a] = 0.0;
         }
-        // sweep 2: kVerify at (z / D, delta_p)
-        point_sweep<T, kVerify>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+        double hd[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          hd[a] = hu[sym3(a, 0)] * dpx[0] + hu[sym3(a, 1)] * dpx[1] + hu[sym3(a, 2)] * dpx[2];
+        uu += 2.0 * (dpx[0] * w[0] + dpx[1] * w[1] + dpx[2] * w[2]) +
+              (dpx[0] * hd[0] + dpx[1] * hd[1] + dpx[2] * hd[2]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           const double d = double(dpv[3 * i + a]);
-          const double nr = w[a] + (lambda * (d * dpx[a])) * d;
+          const double nr = (w[a] + hd[a] + gr[a]) + (lambda * (d * dpx[a])) * d;
           n2 += nr * nr;
           n2s += (nr / d) * (nr / d);
           const double g = double(gp[3 * i + a]);
@@ -848,7 +867,8 @@ __global__ void __launch_bounds__(kBlock) k_points(
           g2s += (g / d) * (g / d);
         }
       } else if (MODE != kRhs) {
-        point_sweep<T, MODE>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef);
+        double gr[3] = {0.0, 0.0, 0.0};  // kSchurSums only
+        point_sweep<T, MODE>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
       }
       if (MODE == kSchur) {
         // Hpp + lambda Dp^2 re-formed from the same rows and factored in registers: the
