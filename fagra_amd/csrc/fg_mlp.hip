@@ -55,10 +55,9 @@ fg_status mlp_reserve(fg_mlp* m, size_t n) {
 }
 
 void mlp_forward(fg_mlp* m, hipStream_t stream, size_t npad, const uint16_t* x, float* y) {
-  // one wave (16 rows) per workgroup: a 1,536-row batch spreads over 96 workgroups
-  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)(npad / 16)), dim3(64), 0, stream, x, m->w1, m->params + kB1,
-                     m->w2, m->params + kB2, m->w3, m->params + kB3, (float*)nullptr, (uint16_t*)nullptr,
-                     (float*)nullptr, (uint16_t*)nullptr, y);
+  // 16 rows per four-wave workgroup: a 1,536-row batch spreads over 96 workgroups
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)(npad / 16)), dim3(256), 0, stream, x, m->w1, m->params + kB1,
+                     m->w2, m->params + kB2, m->w3, m->params + kB3, y);
 }
 
 }  // namespace fg

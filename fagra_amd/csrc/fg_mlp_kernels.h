@@ -100,6 +100,48 @@ static __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* __re
   }
 }
 
+// Inference forward (no stored activations): one 16-row block per four-wave workgroup, the
+// waves splitting each layer's 16-column output tiles and exchanging the activations through
+// LDS. Every tile is the same MFMA chain over the same operands as in k_mlp_forward, so the
+// output rows are the same; the chain per wave is a quarter as long.
+static __global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w1,
+                                                   const float* __restrict__ b1, const uint16_t* __restrict__ w2,
+                                                   const float* __restrict__ b2, const uint16_t* __restrict__ w3,
+                                                   const float* __restrict__ b3, float* __restrict__ y) {
+  __shared__ uint16_t h1[16][kHidden];
+  __shared__ uint16_t h2[16][kHidden];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const int waves = blockDim.x >> 6;
+  const size_t m0 = (size_t)blockIdx.x * 16;
+  for (int nt = wave; nt < kHidden / 16; nt += waves) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(x + m0 * kInPad, kInPad, w1 + (size_t)nt * 16 * kInPad, kInPad, kInPad, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      h1[row][col] = f2bf(gelu(acc[i] + b1[col]));
+    }
+  }
+  __syncthreads();
+  for (int nt = wave; nt < kHidden / 16; nt += waves) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&h1[0][0], kHidden, w2 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      h2[row][col] = f2bf(gelu(acc[i] + b2[col]));
+    }
+  }
+  __syncthreads();
+  for (int nt = wave; nt < kOutPad / 16; nt += waves) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&h2[0][0], kHidden, w3 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      y[(m0 + row) * kOutPad + col] = acc[i] + b3[col];
+    }
+  }
+}
+
 // G = I + s tril(y): packed lower column-major 9x9 per camera.
 __device__ __forceinline__ void build_g(const float* yrow, float g[45]) {
   for (int k = 0; k < 45; ++k) g[k] = kOutScale * yrow[k];
