@@ -680,7 +680,6 @@ fg_status gs_alloc_dim(fg_gschur* g) {
 fg_status gs_finalize(fg_gschur* g) {
   if (!g->dirty) return FG_OK;
   const size_t ne = g->h_em.size();
-  size_t cap = 0;
   GS_TRY(grow(&g->d_em, &g->cap_em, ne));
   GS_TRY(grow(&g->d_res, &g->cap_rows, g->rows));
   GS_TRY(grow(&g->d_vals, &g->cap_vals, g->h_vals.size()));

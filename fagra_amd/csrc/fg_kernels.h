@@ -80,6 +80,53 @@ __device__ __forceinline__ T block_sum(T v, T* smem /* kBlock/kWave */) {
   return r;  // valid in thread 0
 }
 
+// Three block sums in one round (each the same tree as block_sum); valid in thread 0.
+__device__ __forceinline__ void block_sum3(double v[3]) {
+  __shared__ double red[3][kBlock / kWave];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = wave_sum(v[c]);
+  __syncthreads();
+  if (lane == 0)
+    for (int c = 0; c < 3; ++c) red[c][wave] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) {
+      double r = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) r += red[c][w];
+      v[c] = r;
+    }
+}
+
+// Deterministic sums of columns 0-2 of a (n x kParts) partial array by one workgroup (fixed
+// assignment, fixed tree, the same per column as block_sum_parts_col); results broadcast to
+// all threads, with two barriers for the three columns.
+__device__ __forceinline__ void block_sum_parts_cols3(const double* parts, int n, double out[3]) {
+  __shared__ double red[3][kBlock / kWave];
+  __shared__ double bc[3];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s[c] += parts[kParts * i + c];
+  }
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = wave_sum(s[c]);
+  if (lane == 0)
+    for (int c = 0; c < 3; ++c) red[c][wave] = s[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) {
+      double r = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) r += red[c][w];
+      bc[c] = r;
+    }
+  __syncthreads();
+  out[0] = bc[0];
+  out[1] = bc[1];
+  out[2] = bc[2];
+}
+
 // Deterministic sum of column col of a (n x kParts) partial array by one workgroup (fixed
 // assignment, fixed tree); result broadcast to all threads.
 __device__ __forceinline__ double block_sum_parts_col(const double* parts, int n, int col, double* smem) {
@@ -1174,7 +1221,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
       }
       parts[c] = part0;
     }
-    if (MODE != cJtu && MODE != cLsmr) {
+    if (MODE == cSchur) {
+      parts[c] = part0;  // p.q, one per camera (k_cg_update sums them contiguously)
+    } else if (MODE != cJtu && MODE != cLsmr) {
       double* pc = parts + kParts * c;
       pc[0] = part0;
       pc[1] = part1;
@@ -1293,7 +1342,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
                                                          const CgState* __restrict__ in, CgState* out,
                                                          const DevScalars* __restrict__ sc,
                                                          const int* __restrict__ factor_fails) {
-  __shared__ double smem[kBlock / kWave];
   const int max_iters = sc->max_iters, fallback_at = sc->fallback_at;
   const double rel_tol = sc->rel_tol, fallback_factor = sc->fallback_factor;
   CgState s = *in;
@@ -1301,9 +1349,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
     if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
     return;
   }
-  const double rr = block_sum_parts_col(parts, nparts, 0, smem);
-  const double rz = block_sum_parts_col(parts, nparts, 1, smem);
-  const double rzb = block_sum_parts_col(parts, nparts, 2, smem);
+  double sums[3];
+  block_sum_parts_cols3(parts, nparts, sums);
+  const double rr = sums[0], rz = sums[1], rzb = sums[2];
   double beta = 0.0;
   bool restart = false;
   s.rr = rr;
@@ -1374,7 +1422,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
   const double lambda = s.lambda;
   const int fallback_at = sc->fallback_at;
   double pq = 0.0;
-  for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += pq_parts[kParts * i];
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += pq_parts[i];  // one per camera, contiguous
   pq = block_sum(pq, smem);
   if (threadIdx.x == 0) bcast = pq;
   __syncthreads();
@@ -1415,13 +1463,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
       }
     }
   }
-  const double b0 = block_sum(rr, smem);
-  const double b1 = block_sum(rz, smem);
-  const double b2 = block_sum(rzb, smem);
+  double b[3] = {rr, rz, rzb};
+  block_sum3(b);
   if (threadIdx.x == 0) {
-    parts[kParts * blockIdx.x] = b0;
-    parts[kParts * blockIdx.x + 1] = b1;
-    parts[kParts * blockIdx.x + 2] = b2;
+    parts[kParts * blockIdx.x] = b[0];
+    parts[kParts * blockIdx.x + 1] = b[1];
+    parts[kParts * blockIdx.x + 2] = b[2];
   }
 }
 
