@@ -1018,16 +1018,30 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
     std::vector<int64_t> fill(b->pt_off.begin(), b->pt_off.end() - 1);
     for (size_t k = 0; k < no; ++k) b->pt_perm[fill[obs_point[b->cam_order[k]]]++] = (int64_t)k;
   }
-  // Point slots: stable counting sort by track length, longest first.
+  // Point slots: track length, longest first, then the camera of the point's first row (its
+  // lowest camera), then point index. Lanes of a chunk then run the same trip count, and at
+  // their first observation they mostly read the same camera's tables.
   int64_t maxlen = 0;
   for (size_t p = 0; p < np; ++p) maxlen = std::max(maxlen, b->pt_off[p + 1] - b->pt_off[p]);
   {
+    std::vector<int64_t> by_cam(np);  // points stably ordered by first camera (unobserved last)
+    {
+      std::vector<int64_t> first(np), cc(nc + 2, 0);
+      for (size_t p = 0; p < np; ++p) {
+        first[p] = b->pt_off[p + 1] > b->pt_off[p] ? obs_camera[b->cam_order[b->pt_perm[b->pt_off[p]]]]
+                                                   : (int64_t)nc;
+        cc[first[p] + 1]++;
+      }
+      for (size_t c = 0; c <= nc; ++c) cc[c + 1] += cc[c];
+      for (size_t p = 0; p < np; ++p) by_cam[cc[first[p]]++] = (int64_t)p;
+    }
     std::vector<int64_t> cnt(maxlen + 2, 0);
     for (size_t p = 0; p < np; ++p) cnt[maxlen - (b->pt_off[p + 1] - b->pt_off[p]) + 1]++;
     for (int64_t l = 0; l <= maxlen; ++l) cnt[l + 1] += cnt[l];
     b->point_of_slot.assign(np, 0);
     b->slot_of_point.assign(np, 0);
-    for (size_t p = 0; p < np; ++p) {
+    for (size_t i = 0; i < np; ++i) {
+      const size_t p = (size_t)by_cam[i];
       const int64_t key = maxlen - (b->pt_off[p + 1] - b->pt_off[p]);
       const int64_t sl = cnt[key]++;
       b->point_of_slot[sl] = (int32_t)p;
