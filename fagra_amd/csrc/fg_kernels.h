@@ -680,6 +680,27 @@ __device__ __forceinline__ void forward_table(const T* __restrict__ ct, const do
   }
 }
 
+// forward_table (scaled) from registers: rj = the table's R Jr rows (T values widened), d = Dc.
+template <typename T>
+__device__ __forceinline__ void forward_table_rows(const double rj[9], const double* x, const double d[9],
+                                                   T* __restrict__ fw, double* __restrict__ fww) {
+  double z[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) z[j] = x[j] / d[j];
+  T v[kFwd];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = T(rj[3 * i] * z[0] + rj[3 * i + 1] * z[1] + rj[3 * i + 2] * z[2]);
+#pragma unroll
+  for (int j = 3; j < 9; ++j) v[j] = T(z[j]);
+  v[9] = v[10] = v[11] = T(0);
+#pragma unroll
+  for (int i = 0; i < kFwd; ++i) fw[i] = v[i];
+  if (fww != nullptr) {
+#pragma unroll
+    for (int i = 0; i < kFwd; ++i) fww[i] = double(v[i]);
+  }
+}
+
 template <typename T>
 __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double* __restrict__ x,
                                  const T* __restrict__ dc, int scaled, T* __restrict__ fw, double* __restrict__ fww) {
@@ -1267,6 +1288,23 @@ static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, d
 // unobserved blocks divide by lambda, block_factors.rs:193-203), or with the learned
 // correction z = L^-T G G^T L^-1 r. Both inverses are formed explicitly per camera (packed
 // symmetric 45) once per solve, so one application is a symmetric 9x9 product.
+// precondition() with the camera's block already in registers.
+__device__ __forceinline__ void precondition_block(bool observed, const double m[45], double lambda, const double r[9],
+                                                   double z[9]) {
+  if (observed) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) s += m[sym9(i, j)] * r[j];
+      z[i] = s;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) z[i] = lambda != 0.0 ? r[i] / lambda : r[i];
+  }
+}
+
 __device__ __forceinline__ void precondition(int c, const int* __restrict__ cam_off, const double* __restrict__ minv,
                                              double lambda, const double r[9], double z[9]) {
   if (cam_off[c + 1] > cam_off[c]) {
@@ -1349,6 +1387,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
     if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
     return;
   }
+  // this camera's vectors and table row load before the partial sums, overlapping them
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double zc[9], pc[9], rj[9], dv[9];
+  if (c < nc) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      zc[i] = z[9 * c + i];
+      pc[i] = p[9 * c + i];
+      rj[i] = double(tab[kCamTab * c + tRJ(i / 3, i % 3)]);
+      dv[i] = double(dc[9 * c + i]);
+    }
+  }
   double sums[3];
   block_sum_parts_cols3(parts, nparts, sums);
   const double rr = sums[0], rz = sums[1], rzb = sums[2];
@@ -1388,19 +1438,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
   if (s.done) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   const bool first = in->iters < 0;
   double pv[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     if (restart) pv[i] = zb[9 * c + i];
-    else if (first) pv[i] = p[9 * c + i];
-    else pv[i] = z[9 * c + i] + beta * p[9 * c + i];
+    else if (first) pv[i] = pc[i];
+    else pv[i] = zc[i] + beta * pc[i];
     p[9 * c + i] = pv[i];
   }
-  double zz[9];
-  forward_table(tab + kCamTab * c, pv, dc + 9 * c, true, fw + kFwd * c, fww != nullptr ? fww + kFwd * c : nullptr, zz);
+  forward_table_rows<T>(rj, pv, dv, fw + kFwd * c, fww != nullptr ? fww + kFwd * c : nullptr);
 }
 
 // x += alpha p; r -= alpha q; z = M r. Every workgroup forms p.q from the camera kernel's
@@ -1421,6 +1469,24 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
   if (s.done) return;
   const double lambda = s.lambda;
   const int fallback_at = sc->fallback_at;
+  const bool learned = s.use_learned != 0;
+  // this camera's vectors and preconditioner block load before the p.q sum, overlapping it
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double pc[9], qc[9], xc[9], rc[9], m[45];
+  bool observed = false;
+  if (c < nc) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      pc[i] = p[9 * c + i];
+      qc[i] = q[9 * c + i];
+      xc[i] = x[9 * c + i];
+      rc[i] = r[9 * c + i];
+    }
+    observed = cam_off[c + 1] > cam_off[c];
+    const double* mc = (learned ? mlearn : mbj) + 45 * c;
+#pragma unroll
+    for (int i = 0; i < 45; ++i) m[i] = mc[i];
+  }
   double pq = 0.0;
   for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += pq_parts[i];  // one per camera, contiguous
   pq = block_sum(pq, smem);
@@ -1435,19 +1501,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
     return;
   }
   const double alpha = s.rz / pq;
-  const bool learned = s.use_learned != 0;
   const bool both = learned && s.iters + 1 == fallback_at;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, rz = 0.0, rzb = 0.0;
   if (c < nc) {
     double rv[9], zv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      x[9 * c + i] += alpha * p[9 * c + i];
-      rv[i] = r[9 * c + i] - alpha * q[9 * c + i];
+      x[9 * c + i] = xc[i] + alpha * pc[i];
+      rv[i] = rc[i] - alpha * qc[i];
       r[9 * c + i] = rv[i];
     }
-    precondition(c, cam_off, learned ? mlearn : mbj, lambda, rv, zv);
+    precondition_block(observed, m, lambda, rv, zv);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       z[9 * c + i] = zv[i];
