@@ -116,6 +116,7 @@ struct fg_schur {
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
+  void* prow = nullptr;    // T [nc][kPRow]: table row + forward table (point-major reads)
   double *d_delta = nullptr;  // [9nc cameras | 3np points in slot order]
   double *d_rows = nullptr;   // 2no scratch for operator products
   double *d_parts = nullptr;  // max(nc, blocks) x kParts (per-camera / per-block partials)
@@ -277,8 +278,10 @@ void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const dou
                    double* delta_out, double* rows_out, double* parts, const CgState* st) {
   fg_ba* b = s->ba;
   const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  // the point-major kernels read the product rows (s->prow) where a forward table is named
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
-                     (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur], fw,
+                     (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur],
+                     fw != nullptr ? (const T*)s->prow : nullptr,
                      (const double*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda,
                      (double*)s->tp, delta_out, rows_out, parts, st, nullptr);
 }
@@ -430,7 +433,7 @@ template <typename T>
 fg_status schur_product(fg_schur* s, const double* x, double lambda, double* out, const double* resid_rhs) {
   fg_ba* b = s->ba;
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w);
+                     (const T*)b->d_tab[b->cur], x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w, (T*)s->prow);
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
   launch_cameras<T, cResid>(s, nullptr, nullptr, (const T*)s->fw, x, nullptr,
                             resid_rhs ? resid_rhs : (const double*)s->zero, lambda, out, nullptr);
@@ -462,7 +465,7 @@ void launch_cg_iteration(fg_schur* s, int i, bool timed) {
   CgState* cur = &s->d_sc->st[(i + 1) & 1];
   hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
-                     (double*)s->p, (T*)s->fw, s->fw_w, (const double*)s->d_cgparts, cblocks,
+                     (double*)s->p, (T*)s->fw, s->fw_w, (T*)s->prow, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], (const DevScalars*)s->d_sc,
                      (const int*)s->d_fail);
   const bool ev = timed && s->events_used + 2 <= s->events.size();
@@ -578,7 +581,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   // the last update's convergence / exhaustion decision
   hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
-                     (double*)s->p, (T*)s->fw, s->fw_w, (const double*)s->d_cgparts, cblocks,
+                     (double*)s->p, (T*)s->fw, s->fw_w, (T*)s->prow, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
                      (const DevScalars*)s->d_sc, (const int*)s->d_fail);
   FG_TRY(hipGetLastError());
@@ -665,7 +668,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   // ||delta||_inf. With a failed block factorization the PCG did not iterate and these results
   // are discarded (NotPositiveDefinite).
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w);
+                     (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w, (T*)s->prow);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
                            s->d_parts, nullptr);
@@ -733,6 +736,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((double**)&s->tp, 4 * np));
   FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
   if (sizeof(T) == 4) FG_TRY(dalloc(&s->fw_w, kFwd * nc));
+  FG_TRY(dalloc((T**)&s->prow, kPRow * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((double**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
@@ -770,6 +774,8 @@ void free_model(fg_schur* s) {
     *v = nullptr;
   }
   dfree(s->fw_w);
+  if (s->prow) (void)hipFree(s->prow);
+  s->prow = nullptr;
   dfree(s->d_delta);
   dfree(s->d_rows);
   dfree(s->d_parts);
@@ -837,7 +843,7 @@ fg_status jacobian_apply_impl(fg_schur* s, const double* x, double* y) {
   fg_ba* b = s->ba;
   FG_CHECK(upload_vector(b, x, s->d_delta));
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w);
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w, (T*)s->prow);
   launch_points<T, kJv>(s, (const T*)s->fw, s->d_delta + 9 * b->nc, nullptr, 0.0, nullptr, s->d_rows, nullptr,
                         nullptr);
   FG_TRY(hipGetLastError());
@@ -934,7 +940,7 @@ fg_status time_products_impl(fg_schur* s, double lambda, const double* x, int re
   if (fails) return FG_LINEAR_SOLVE_FAILED;
   FG_TRY(hipMemcpyAsync(s->p, x, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->p, (const T*)s->dc, 1, (T*)s->fw, s->fw_w);
+                     (const T*)b->d_tab[b->cur], (const double*)s->p, (const T*)s->dc, 1, (T*)s->fw, s->fw_w, (T*)s->prow);
   // one untimed pass of each warms the caches the timed passes see
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
   launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, lambda,
@@ -1701,7 +1707,7 @@ void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
   const LsmrState* st = l->d_st;
   hipLaunchKernelGGL((k_points<T, kLsmr>), dim3(l->pblocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur],
-                     step ? (const T*)s->fw : nullptr, (const double*)s->mp, (const T*)s->gp, (const T*)s->dp,
+                     step ? (const T*)s->prow : nullptr, (const double*)s->mp, (const T*)s->gp, (const T*)s->dp,
                      step ? ws + 9 * nc : nullptr, (const double*)l->d_u, 0.0,
                      damped && step ? l->d_ub + 9 * nc : nullptr, l->d_zt + 9 * nc, l->d_u, l->d_up,
                      (const CgState*)nullptr, st);
@@ -1731,7 +1737,7 @@ void lsmr_iteration(fg_lsmr* l, int mode, bool damped) {
                      (const double*)s->lc, (const double*)s->mp, (const T*)s->dc, (const T*)s->dp, mode == 2 ? 1 : 0,
                      scaled ? 1 : 0, 0, (const LsmrState*)l->d_st, l->d_ws);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw, s->fw_w);
+                     (const T*)b->d_tab[b->cur], (const double*)l->d_ws, (const T*)s->dc, 0, (T*)s->fw, s->fw_w, (T*)s->prow);
   lsmr_passes<T>(l, l->d_ws, damped);
   hipLaunchKernelGGL(k_lsmr_v<T>, dim3(vblocks), dim3(kBlock), 0, b->stream, nc, np, (const double*)l->d_zt,
                      (const double*)l->d_ub, (const double*)s->lc, (const double*)s->mp, (const T*)s->dc,
@@ -1866,7 +1872,7 @@ fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted, b
     hipLaunchKernelGGL(k_recover_cameras<T>, dim3(grid_for(3 * np)), dim3(kBlock), 0, b->stream, 3 * np,
                        (const double*)(s->d_delta + 9 * nc), (const T*)s->dp, s->d_delta + 9 * nc);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
-                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w);
+                     (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w, (T*)s->prow);
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * nc, nullptr, lambda, l->d_zt + 9 * nc, nullptr,
                             s->d_parts, nullptr);

@@ -52,6 +52,38 @@ __device__ __forceinline__ void load_cam16_d(const T* __restrict__ tab, int c, d
 #pragma unroll
   for (int i = 0; i < 16; ++i) out[i] = double(t[i]);
 }
+// Product row of a camera (kPRow scalars): the table's (R | t) rows, (f, k1, k2, fw0), fw1-4,
+// fw5-8 -- everything an observation of a point-major product needs from its camera in six
+// 4-vector loads (a table row and a forward table take seven).
+constexpr int kPRow = 24;
+template <typename T>
+__device__ __forceinline__ void load_prow_d(const T* __restrict__ prow, int c, double ct[16], double f[kFwd]) {
+  const Vec4<T>* r4 = reinterpret_cast<const Vec4<T>*>(prow) + (kPRow / 4) * c;
+  Vec4<T> v[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v[i] = r4[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ct[4 * i] = double(v[i].x); ct[4 * i + 1] = double(v[i].y); ct[4 * i + 2] = double(v[i].z);
+    ct[4 * i + 3] = double(v[i].w);
+  }
+  ct[12] = double(v[3].x); ct[13] = double(v[3].y); ct[14] = double(v[3].z); ct[15] = 0.0;
+  f[0] = double(v[3].w);
+  f[1] = double(v[4].x); f[2] = double(v[4].y); f[3] = double(v[4].z); f[4] = double(v[4].w);
+  f[5] = double(v[5].x); f[6] = double(v[5].y); f[7] = double(v[5].z); f[8] = double(v[5].w);
+  f[9] = f[10] = f[11] = 0.0;
+}
+// The product row from a table row and the forward table values v (T).
+template <typename T>
+__device__ __forceinline__ void store_prow(const T* __restrict__ ct, const T v[kFwd], T* __restrict__ prow) {
+  Vec4<T>* r4 = reinterpret_cast<Vec4<T>*>(prow);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) r4[i] = Vec4<T>{ct[4 * i], ct[4 * i + 1], ct[4 * i + 2], ct[4 * i + 3]};
+  r4[3] = Vec4<T>{ct[tF], ct[tK1], ct[tK2], v[0]};
+  r4[4] = Vec4<T>{v[1], v[2], v[3], v[4]};
+  r4[5] = Vec4<T>{v[5], v[6], v[7], v[8]};
+}
+
 template <typename T>
 __device__ __forceinline__ void load_fwd_d(const T* __restrict__ fw, int c, double out[kFwd]) {
   T t[kFwd];
@@ -662,7 +694,7 @@ __global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const 
 template <typename T>
 __device__ __forceinline__ void forward_table(const T* __restrict__ ct, const double* x, const T* __restrict__ d,
                                               bool scaled, T* __restrict__ fw, double* __restrict__ fww,
-                                              double z[9]) {
+                                              T* __restrict__ prow, double z[9]) {
 #pragma unroll
   for (int j = 0; j < 9; ++j) z[j] = scaled ? x[j] / double(d[j]) : x[j];
   T v[kFwd];
@@ -678,12 +710,14 @@ __device__ __forceinline__ void forward_table(const T* __restrict__ ct, const do
 #pragma unroll
     for (int i = 0; i < kFwd; ++i) fww[i] = double(v[i]);
   }
+  store_prow(ct, v, prow);
 }
 
 // forward_table (scaled) from registers: rj = the table's R Jr rows (T values widened), d = Dc.
 template <typename T>
 __device__ __forceinline__ void forward_table_rows(const double rj[9], const double* x, const double d[9],
-                                                   T* __restrict__ fw, double* __restrict__ fww) {
+                                                   const T trow[16], T* __restrict__ fw, double* __restrict__ fww,
+                                                   T* __restrict__ prow) {
   double z[9];
 #pragma unroll
   for (int j = 0; j < 9; ++j) z[j] = x[j] / d[j];
@@ -699,16 +733,18 @@ __device__ __forceinline__ void forward_table_rows(const double rj[9], const dou
 #pragma unroll
     for (int i = 0; i < kFwd; ++i) fww[i] = double(v[i]);
   }
+  store_prow(trow, v, prow);
 }
 
 template <typename T>
 __global__ void k_forward_tables(int nc, const T* __restrict__ tab, const double* __restrict__ x,
-                                 const T* __restrict__ dc, int scaled, T* __restrict__ fw, double* __restrict__ fww) {
+                                 const T* __restrict__ dc, int scaled, T* __restrict__ fw, double* __restrict__ fww,
+                                 T* __restrict__ prow) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   double z[9];
   forward_table(tab + kCamTab * c, x + 9 * c, dc + 9 * c, scaled != 0, fw + kFwd * c,
-                fww != nullptr ? fww + kFwd * c : nullptr, z);
+                fww != nullptr ? fww + kFwd * c : nullptr, prow + kPRow * c, z);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -761,13 +797,15 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
                                           const T* __restrict__ fw, const double dpx[3],
                                           const double* __restrict__ rows_in, double* __restrict__ rows_out,
                                           double w[3], double h[6], double& uu, double coef, double gr[3]) {
-  double ct[16];
-  load_cam16_d(tab, c, ct);
+  // fw is the cameras' product rows in the modes that apply a camera vector
+  constexpr bool kFwdModes = MODE == kSchur || MODE == kSchurSums || MODE == kRecover || MODE == kVerify ||
+                             MODE == kJv || MODE == kLsmr;
+  double ct[16], f[kFwd];
+  if (kFwdModes && fw != nullptr) load_prow_d(fw, c, ct, f);
+  else load_cam16_d(tab, c, ct);
   ObsGeom<double> geo;
   obs_geom(to_d(rec), ct, geo);
   if (MODE == kSchur || MODE == kSchurSums) {
-    double f[kFwd];
-    load_fwd_d(fw, c, f);
     double u0, u1;
     jc_apply(geo, f, u0, u1);
     double j0[3], j1[3];
@@ -789,14 +827,10 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
       for (int a = 0; a < 3; ++a) gr[a] += j0[a] * double(r.x) + j1[a] * double(r.y);
     }
   } else if (MODE == kRecover) {
-    double f[kFwd];
-    load_fwd_d(fw, c, f);
     double u0, u1;
     jc_apply(geo, f, u0, u1);
     jpt_accumulate(geo, ct, u0, u1, w);
   } else if (MODE == kVerify || MODE == kJv) {
-    double f[kFwd];
-    load_fwd_d(fw, c, f);
     double u0, u1, v0, v1;
     jc_apply(geo, f, u0, u1);
     jp_apply(geo, ct, dpx, v0, v1);
@@ -818,8 +852,6 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
     const Vec2<double> old = reinterpret_cast<const Vec2<double>*>(rows_in)[s];
     double u0 = old.x, u1 = old.y;
     if (fw != nullptr) {
-      double f[kFwd];
-      load_fwd_d(fw, c, f);
       double v0, v1, x0, x1;
       jc_apply(geo, f, v0, v1);
       jp_apply(geo, ct, dpx, x0, x1);
@@ -1375,7 +1407,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
                                                          const T* __restrict__ tab, const T* __restrict__ dc,
                                                          const double* __restrict__ zb, const double* __restrict__ z,
                                                          double* __restrict__ p, T* __restrict__ fw,
-                                                         double* __restrict__ fww,
+                                                         double* __restrict__ fww, T* __restrict__ prow,
                                                          const double* __restrict__ parts, int nparts,
                                                          const CgState* __restrict__ in, CgState* out,
                                                          const DevScalars* __restrict__ sc,
@@ -1390,6 +1422,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
   // this camera's vectors and table row load before the partial sums, overlapping them
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double zc[9], pc[9], rj[9], dv[9];
+  T trow[16];
   if (c < nc) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -1398,6 +1431,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
       rj[i] = double(tab[kCamTab * c + tRJ(i / 3, i % 3)]);
       dv[i] = double(dc[9 * c + i]);
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) trow[i] = tab[kCamTab * c + i];
   }
   double sums[3];
   block_sum_parts_cols3(parts, nparts, sums);
@@ -1448,7 +1483,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
     else pv[i] = zc[i] + beta * pc[i];
     p[9 * c + i] = pv[i];
   }
-  forward_table_rows<T>(rj, pv, dv, fw + kFwd * c, fww != nullptr ? fww + kFwd * c : nullptr);
+  forward_table_rows<T>(rj, pv, dv, trow, fw + kFwd * c, fww != nullptr ? fww + kFwd * c : nullptr, prow + kPRow * c);
 }
 
 // x += alpha p; r -= alpha q; z = M r. Every workgroup forms p.q from the camera kernel's
