@@ -319,6 +319,10 @@ class Schur:
                                            ctypes.byref(b)), "fg_schur_time_products")
         return a.value, b.value
 
+    def record_mismatches(self):
+        """Observations whose two record copies differ (0 after a consistent linearization)."""
+        return lib().fg_schur_record_mismatches(self.h)
+
     def reduced_apply(self, lam, x):
         x = _f64(x)
         y = np.zeros_like(x)

@@ -156,6 +156,7 @@ SIGNATURES = {
     "fg_schur_camera_gram": (ctypes.c_int, [_VP, _dp]),
     "fg_schur_reduced_apply": (ctypes.c_int, [_VP, _d, _dp, _dp]),
     "fg_schur_time_products": (ctypes.c_int, [_VP, _d, _dp, ctypes.c_int, _dp, _dp]),
+    "fg_schur_record_mismatches": (ctypes.c_int, [_VP]),
     "fg_schur_jacobian_apply": (ctypes.c_int, [_VP, _dp, _dp]),
     "fg_schur_jacobian_transpose_apply": (ctypes.c_int, [_VP, _dp, _dp]),
     "fg_schur_damped_apply": (ctypes.c_int, [_VP, _d, ctypes.c_int, _dp, _dp]),

@@ -77,6 +77,7 @@ struct fg_ba {
   // f64 observations and camera tables for the linearization residual (the same buffers as the
   // typed ones in f64 precision).
   Vec2<double>* d_slot_pix64 = nullptr;
+  Vec2<double>* d_cam_pix64 = nullptr;  // f64 pixels in camera-major row order
   double* d_tab64[2] = {nullptr, nullptr};
   double* d_tabw[2] = {nullptr, nullptr};  // the T tables widened to f64 (aliases d_tab in f64)
   bool fp32 = false;
@@ -316,8 +317,9 @@ fg_status linearize_launch(fg_schur* s) {
                      (Vec2<T>*)s->res, (double*)s->hpp, (T*)s->gp, s->d_flags + 1);
   FG_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
-                     b->d_k_slot, (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[buf],
-                     (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags + 1);
+                     b->d_k_pslot, (const double*)b->d_pts[buf], (const Vec2<double>*)b->d_cam_pix64,
+                     (const double*)b->d_tab64[buf], (const T*)b->d_tab[buf], b->huber, (Vec4<T>*)s->rec_cm,
+                     (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags + 1);
   FG_TRY(hipGetLastError());
   return FG_OK;
 }
@@ -1109,6 +1111,17 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
              hipSuccess) {
     return fail(FG_DEVICE_ERROR);
   }
+  {
+    std::vector<double> cam_pix(2 * std::max<size_t>(1, no), 0.0);
+    for (size_t k = 0; k < no; ++k) {
+      cam_pix[2 * k] = obs_pixel[2 * b->cam_order[k]];
+      cam_pix[2 * k + 1] = obs_pixel[2 * b->cam_order[k] + 1];
+    }
+    if (dalloc((double**)&b->d_cam_pix64, cam_pix.size()) != hipSuccess ||
+        hipMemcpy(b->d_cam_pix64, cam_pix.data(), cam_pix.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return fail(FG_DEVICE_ERROR);
+  }
   for (int i = 0; i < 2; ++i) {
     if (dalloc(&b->d_cams[i], 9 * nc) != hipSuccess || dalloc(&b->d_pts[i], 3 * np) != hipSuccess ||
         hipMalloc(&b->d_pts_t[i], 3 * np * ts) != hipSuccess ||
@@ -1152,6 +1165,7 @@ void fg_ba_destroy(fg_ba* b) {
   dfree(b->d_pt_len);
   dfree(b->d_point_of_slot);
   if (b->fp32 && b->d_slot_pix64) (void)hipFree(b->d_slot_pix64);
+  if (b->d_cam_pix64) (void)hipFree(b->d_cam_pix64);
   if (b->d_slot_pix) (void)hipFree(b->d_slot_pix);
   for (int i = 0; i < 2; ++i) {
     if (b->fp32 && b->d_tab64[i]) (void)hipFree(b->d_tab64[i]);
@@ -1464,6 +1478,25 @@ fg_status fg_schur_capture(fg_schur* s, double lambda, float* features, float* t
   if (!s->ready || !std::isfinite(lambda) || lambda < 0.0) return FG_INVALID_OPTIONS;
   return s->cfg.precision == FG_FP32 ? capture_impl<float>(s, lambda, features, targets, counts)
                                      : capture_impl<double>(s, lambda, features, targets, counts);
+}
+
+int fg_schur_record_mismatches(fg_schur* s) {
+  if (!s->linearized) return -1;
+  fg_ba* b = s->ba;
+  if (hipMemsetAsync(s->d_flags + 3, 0, sizeof(int), b->stream) != hipSuccess) return -1;
+  if (s->cfg.precision == FG_FP32)
+    hipLaunchKernelGGL(k_record_mismatch<float>, dim3(grid_for(b->no)), dim3(kBlock), 0, b->stream, (int)b->no,
+                       b->d_k_slot, (const Vec4<float>*)s->rec, (const Vec2<float>*)s->res,
+                       (const Vec4<float>*)s->rec_cm, (const Vec2<float>*)s->res_cm, s->d_flags + 3);
+  else
+    hipLaunchKernelGGL(k_record_mismatch<double>, dim3(grid_for(b->no)), dim3(kBlock), 0, b->stream, (int)b->no,
+                       b->d_k_slot, (const Vec4<double>*)s->rec, (const Vec2<double>*)s->res,
+                       (const Vec4<double>*)s->rec_cm, (const Vec2<double>*)s->res_cm, s->d_flags + 3);
+  if (hipGetLastError() != hipSuccess) return -1;
+  if (hipMemcpyAsync(s->h_int + 7, s->d_flags + 3, sizeof(int), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return -1;
+  return s->h_int[7];
 }
 
 fg_status fg_schur_time_products(fg_schur* s, double lambda, const double* x, int repeats, double* sec_points,

@@ -328,6 +328,43 @@ static __global__ void k_reduce_parts(const double* __restrict__ parts, int n, i
 // table in f64, the same way the Schur products form it, and Hpp and the gradient accumulate
 // in f64, so the point blocks are the exact Gram blocks of the stored Jacobian.
 // ---------------------------------------------------------------------------------------
+// The compact record and weighted residual of one observation from the f64 camera table, the
+// f64 point and the f64 pixel: projection, residual, IRLS weight, rounded to T. Both
+// linearization passes form their copies with this function, with contraction off, so the
+// point-major and camera-major records are the same bits. Returns false on a zero depth or a
+// nonfinite pixel.
+template <typename T>
+__device__ __forceinline__ bool record_of(const double* __restrict__ cd, double x0, double x1, double x2,
+                                          const Vec2<double>& ob, double huber, Vec4<T>& r4, Vec2<T>& rw,
+                                          double& qx, double& qy, double& z, double& sw, double& wrx,
+                                          double& wry) {
+#pragma clang fp contract(off)
+  double px, py;
+  const double p0 = cd[tR(0, 0)] * x0 + cd[tR(0, 1)] * x1 + cd[tR(0, 2)] * x2 + cd[tT(0)];
+  const double p1 = cd[tR(1, 0)] * x0 + cd[tR(1, 1)] * x1 + cd[tR(1, 2)] * x2 + cd[tT(1)];
+  const double p2 = cd[tR(2, 0)] * x0 + cd[tR(2, 1)] * x1 + cd[tR(2, 2)] * x2 + cd[tT(2)];
+  qx = -p0 / p2;
+  qy = -p1 / p2;
+  z = p2;
+  const double r2 = qx * qx + qy * qy;
+  const double sc = cd[tF] * (1.0 + cd[tK1] * r2 + cd[tK2] * r2 * r2);
+  px = sc * qx;
+  py = sc * qy;
+  const bool ok = p2 != 0.0 && isfinite(px) && isfinite(py);
+  const double rx = px - ob.x, ry = py - ob.y;
+  const double s2 = rx * rx + ry * ry;
+  if (!(huber > 0.0) || s2 <= huber * huber) {
+    sw = 1.0;
+  } else {
+    sw = sqrt(huber / sqrt(s2));
+  }
+  r4 = Vec4<T>{T(qx), T(qy), T(z), T(sw)};
+  wrx = sw * rx;
+  wry = sw * ry;
+  rw = Vec2<T>{T(wrx), T(wry)};
+  return ok;
+}
+
 // One observation of the point-major linearization. cd is the f64 camera table; the table in
 // T is its rounding (k_camera_tables stores both from the same f64 values), formed here
 // instead of gathered a second time.
@@ -335,17 +372,11 @@ template <typename T>
 __device__ __forceinline__ bool linearize_obs(const double cd[16], double x0, double x1, double x2,
                                               const Vec2<double>& ob, double huber, Vec4<T>& r4, Vec2<T>& rw,
                                               double h[6], double g[3]) {
-  double px, py, qx, qy, z;
-  bool ok = project(cd, x0, x1, x2, px, py, qx, qy, z);
-  const double rx = px - ob.x, ry = py - ob.y;
-  double cost, sw;
-  huber_weight(rx, ry, huber, cost, sw);
+  double qx, qy, z, sw, wrx, wry;
+  bool ok = record_of<T>(cd, x0, x1, x2, ob, huber, r4, rw, qx, qy, z, sw, wrx, wry);
   double ct[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) ct[i] = double(T(cd[i]));
-  r4 = Vec4<T>{T(qx), T(qy), T(z), T(sw)};
-  const double wrx = sw * rx, wry = sw * ry;
-  rw = Vec2<T>{T(wrx), T(wry)};
   ObsGeom<double> geo, g64;
   obs_geom(to_d(r4), ct, geo);
   obs_geom(Vec4<double>{qx, qy, z, sw}, cd, g64);
@@ -429,14 +460,18 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
 }
 
 // ---------------------------------------------------------------------------------------
-// Linearization, camera-major pass (one workgroup per camera): camera-major copies of the
-// records, the camera Gram block Hcc = Jc^T Jc (45 packed) and gradient gc = Jc^T r.
+// Linearization, camera-major pass (one workgroup per camera): the camera-major copies of the
+// records, formed again from the f64 points (gathered by point slot from a 12 MB array in the
+// 1,500-camera case, instead of gathering the 78 MB of slot-order records and residuals) with
+// the same record_of as the point-major pass, then the camera Gram block Hcc = Jc^T Jc (45
+// packed) and gradient gc = Jc^T r.
 // ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
-    const int* __restrict__ cam_off, const int* __restrict__ k_slot, const Vec4<T>* __restrict__ rec,
-    const Vec2<T>* __restrict__ res, const T* __restrict__ tab, Vec4<T>* __restrict__ rec_cm,
-    Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc, T* __restrict__ gc, int* flags) {
+    const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const double* __restrict__ pts,
+    const Vec2<double>* __restrict__ cam_pix, const double* __restrict__ tab64, const T* __restrict__ tab,
+    double huber, Vec4<T>* __restrict__ rec_cm, Vec2<T>* __restrict__ res_cm, T* __restrict__ hcc,
+    T* __restrict__ gc, int* flags) {
   // The weighted residual was formed in f64 and rounded once to T (relative error eps of the
   // residual itself), so the gradient needs only f64 accumulation here.
   __shared__ T ct[kCamTab];
@@ -452,7 +487,12 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
   for (int i = 0; i < 9; ++i) gacc[i] = 0.0;
   bool bad = false;
   const int k0 = cam_off[c], k1 = cam_off[c + 1];
-  auto one = [&](int k, const Vec4<T>& r4, const Vec2<T>& rw) {
+  const double* __restrict__ cd = tab64 + kCamTab * c;  // uniform: scalar loads
+  auto one = [&](int k, const double x[3], const Vec2<double>& ob) {
+    Vec4<T> r4;
+    Vec2<T> rw;
+    double qx, qy, z, sw, wrx, wry;
+    bad |= !record_of<T>(cd, x[0], x[1], x[2], ob, huber, r4, rw, qx, qy, z, sw, wrx, wry);
     rec_cm[k] = r4;
     res_cm[k] = rw;
     ObsGeom<T> geo;
@@ -470,17 +510,19 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
     for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
   };
   int k = k0 + threadIdx.x;
-  // two rows per step: both gathers issue before either row is used
+  // two rows per step: both points and pixels load before either row is used
   for (; k + kBlock < k1; k += 2 * kBlock) {
-    const int sa = k_slot[k], sb = k_slot[k + kBlock];
-    const Vec4<T> ra = rec[sa], rb = rec[sb];
-    const Vec2<T> wa = res[sa], wb = res[sb];
-    one(k, ra, wa);
-    one(k + kBlock, rb, wb);
+    const int pa = k_pslot[k], pb = k_pslot[k + kBlock];
+    const double xa[3] = {pts[3 * pa], pts[3 * pa + 1], pts[3 * pa + 2]};
+    const double xb[3] = {pts[3 * pb], pts[3 * pb + 1], pts[3 * pb + 2]};
+    const Vec2<double> oa = cam_pix[k], ob = cam_pix[k + kBlock];
+    one(k, xa, oa);
+    one(k + kBlock, xb, ob);
   }
   if (k < k1) {
-    const int sa = k_slot[k];
-    one(k, rec[sa], res[sa]);
+    const int pa = k_pslot[k];
+    const double xa[3] = {pts[3 * pa], pts[3 * pa + 1], pts[3 * pa + 2]};
+    one(k, xa, cam_pix[k]);
   }
   if (bad) atomicOr(flags, 1);
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -501,6 +543,27 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
     if (threadIdx.x < 45) hcc[45 * c + threadIdx.x] = T(s);
     else gc[9 * c + threadIdx.x - 45] = T(s);
   }
+}
+
+__device__ __forceinline__ bool memcmp_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool memcmp_bits(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// Consistency check of the two record copies: counts camera-major rows whose record or
+// weighted residual differs in any bit from the point-major copy of the same observation.
+template <typename T>
+__global__ void k_record_mismatch(int no, const int* __restrict__ k_slot, const Vec4<T>* __restrict__ rec,
+                                  const Vec2<T>* __restrict__ res, const Vec4<T>* __restrict__ rec_cm,
+                                  const Vec2<T>* __restrict__ res_cm, int* count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= no) return;
+  const int s = k_slot[k];
+  const Vec4<T> a = rec[s], b = rec_cm[k];
+  const Vec2<T> u = res[s], v = res_cm[k];
+  const bool same = memcmp_bits(a.x, b.x) && memcmp_bits(a.y, b.y) && memcmp_bits(a.z, b.z) &&
+                    memcmp_bits(a.w, b.w) && memcmp_bits(u.x, v.x) && memcmp_bits(u.y, v.y);
+  if (!same) atomicAdd(count, 1);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -158,6 +158,11 @@ fg_status fg_schur_reduced_apply(fg_schur*, double lambda, const double* x, doub
  * launch of each. */
 fg_status fg_schur_time_products(fg_schur*, double lambda, const double* x, int repeats, double* sec_points,
                                  double* sec_cameras);
+/* Consistency hook, no reference counterpart: after a linearization, the number of
+ * observations whose camera-major record or weighted residual (formed by the camera-major
+ * pass) differs in any bit from the point-major copy; 0 when the two passes agree, -1 when not
+ * linearized or on a device error. */
+int fg_schur_record_mismatches(fg_schur*);
 /* Jacobian::apply / transpose_apply (src/linear_model.rs:39-87). */
 fg_status fg_schur_jacobian_apply(fg_schur*, const double* x, double* y);
 fg_status fg_schur_jacobian_transpose_apply(fg_schur*, const double* y, double* x);

@@ -252,3 +252,22 @@ def test_lds_point_passes_match_global_gathers(monkeypatch):
     ba = fa.BalProblem(p, precision=fa.FP64)
     s = fa.Schur(precision=fa.FP64)
     assert s.prepare(ba) == fa.OK and s.lds_workgroups() == 0
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_camera_major_records_equal_point_major_records(precision):
+    """The camera-major linearization pass forms its records from the points again; every
+    record and weighted residual must equal the point-major copy bit for bit, so that both
+    halves of the Schur products use one Jacobian (also at a second, perturbed state and with
+    Huber weights active)."""
+    for seed, nc, npts, huber in [(31, 12, 900, 0.0), (32, 40, 5000, 1.0)]:
+        p = small(seed=seed, nc=nc, npts=npts, huber=huber)
+        ba = fa.BalProblem(p, precision=precision)
+        s = fa.Schur(precision=precision)
+        assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK
+        assert s.record_mismatches() == 0
+        cams, pts = ba.state()
+        rng = np.random.default_rng(seed)
+        ba.set_state(cams + 1e-3 * rng.normal(size=cams.shape), pts + 1e-2 * rng.normal(size=pts.shape))
+        assert s.linearize() == fa.OK
+        assert s.record_mismatches() == 0
