@@ -130,6 +130,8 @@ struct fg_schur {
   double* d_scalar = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirrors
   void* h_poll = nullptr;      // two DevScalars, pinned
+  DevScalars* h_mirror = nullptr;  // coherent host memory the PCG directions publish their state to
+  DevScalars* d_mirror = nullptr;  // its device address
   double* h_sums = nullptr;   // 4 x kParts
   double* h_norm = nullptr;
   int* h_int = nullptr;
@@ -469,7 +471,7 @@ void launch_cg_iteration(fg_schur* s, int i, bool timed) {
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
                      (double*)s->p, (T*)s->fw, s->fw_w, (T*)s->prow, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[i & 1], &s->d_sc->st[(i + 1) & 1], (const DevScalars*)s->d_sc,
-                     (const int*)s->d_fail);
+                     (const int*)s->d_fail, &s->d_mirror->st[(i + 1) & 1]);
   const bool ev = timed && s->events_used + 2 <= s->events.size();
   if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, 0.0, nullptr, nullptr, nullptr, cur);
@@ -538,7 +540,10 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   const bool timed = s->cfg.collect_timings != 0;
   hipGraphExec_t graph = nullptr;
   if (!timed && s->use_graphs) FG_CHECK(cg_batch_graph<T>(s, &graph));
-  DevScalars* polls = (DevScalars*)s->h_poll;  // two pinned slots
+  // The directions publish their state to coherent host memory (h_mirror); after a batch's
+  // event the host reads it there, possibly already overtaken by the next batch (a later
+  // state; a torn read only misguides the prediction below, and a set done flag was set).
+  s->h_mirror->st[0].done = s->h_mirror->st[1].done = 0;
   int launched = 0, issued = 0, read = 0;  // iterations launched; batches issued / read back
   auto issue = [&]() -> fg_status {
     if (graph) {
@@ -548,9 +553,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
       FG_TRY(hipGetLastError());
     }
     launched += kBatch;
-    const int slot = issued & 1;
-    FG_TRY(hipMemcpyAsync(&polls[slot], s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
-    FG_TRY(hipEventRecord(s->poll_ev[slot], b->stream));
+    FG_TRY(hipEventRecord(s->poll_ev[issued & 1], b->stream));
     ++issued;
     return FG_OK;
   };
@@ -573,7 +576,8 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
     const bool ahead = !(have && ends_in_flight(last));
     if (ahead) FG_CHECK(issue());
     FG_TRY(hipEventSynchronize(s->poll_ev[read & 1]));
-    const DevScalars& h = polls[read & 1];
+    DevScalars h;
+    std::memcpy(&h, (const void*)s->h_mirror, sizeof(DevScalars));
     ++read;
     last = h.st[0].iters >= h.st[1].iters ? h.st[0] : h.st[1];
     have = true;
@@ -585,7 +589,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
                      (double*)s->p, (T*)s->fw, s->fw_w, (T*)s->prow, (const double*)s->d_cgparts, cblocks,
                      (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
-                     (const DevScalars*)s->d_sc, (const int*)s->d_fail);
+                     (const DevScalars*)s->d_sc, (const int*)s->d_fail, &s->d_mirror->st[(launched + 1) & 1]);
   FG_TRY(hipGetLastError());
   // read back with the verification's synchronization (cg_collect)
   FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
@@ -1288,6 +1292,8 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
   if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail();
   if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc(&s->h_poll, 2 * sizeof(DevScalars), 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_mirror, sizeof(DevScalars), hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&s->d_mirror, s->h_mirror, 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_sums, 4 * kParts * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_norm, 2 * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_int, 8 * sizeof(int), 0) != hipSuccess)
@@ -1311,6 +1317,7 @@ void fg_schur_destroy(fg_schur* s) {
   dfree(s->d_sums);
   if (s->h_sc) (void)hipHostFree(s->h_sc);
   if (s->h_poll) (void)hipHostFree(s->h_poll);
+  if (s->h_mirror) (void)hipHostFree(s->h_mirror);
   if (s->h_sums) (void)hipHostFree(s->h_sums);
   if (s->h_norm) (void)hipHostFree(s->h_norm);
   if (s->h_int) (void)hipHostFree(s->h_int);

@@ -1474,12 +1474,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
                                                          const double* __restrict__ parts, int nparts,
                                                          const CgState* __restrict__ in, CgState* out,
                                                          const DevScalars* __restrict__ sc,
-                                                         const int* __restrict__ factor_fails) {
+                                                         const int* __restrict__ factor_fails, CgState* mirror) {
   const int max_iters = sc->max_iters, fallback_at = sc->fallback_at;
   const double rel_tol = sc->rel_tol, fallback_factor = sc->fallback_factor;
   CgState s = *in;
   if (s.done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *out = s;
+      if (mirror) *mirror = s;
+    }
     return;
   }
   // this camera's vectors and table row load before the partial sums, overlapping them
@@ -1534,7 +1537,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
       s.rz = rz;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *out = s;
+  // workgroup 0 publishes the state, and a copy to host memory the host polls (mirror)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *out = s;
+    if (mirror) *mirror = s;
+  }
   if (s.done) return;
   if (c >= nc) return;
   const bool first = in->iters < 0;
