@@ -130,6 +130,8 @@ struct fg_schur {
   double* d_scalar = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirrors
   void* h_poll = nullptr;      // two DevScalars, pinned
+  AttemptReadback* h_att = nullptr;  // coherent host memory: a damped attempt's results
+  AttemptReadback* d_att = nullptr;
   DevScalars* h_mirror = nullptr;  // coherent host memory the PCG directions publish their state to
   DevScalars* d_mirror = nullptr;  // its device address
   double* h_sums = nullptr;   // 4 x kParts
@@ -203,17 +205,11 @@ struct Phase {
 template <typename T>
 fg_status ensure_tables(fg_ba* b, int buf) {
   if (b->tab_valid[buf]) return FG_OK;
-  // f64 tables first; in f32 the T table is their rounding, entry by entry (the point-major
-  // linearization relies on that to form it from the f64 table)
-  hipLaunchKernelGGL(k_camera_tables<double>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                     b->d_cams[buf], b->d_tab64[buf]);
-  if (b->fp32) {
-    const int n = (int)(kCamTab * b->nc);
-    hipLaunchKernelGGL(k_convert<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, n,
-                       (const double*)b->d_tab64[buf], (T*)b->d_tab[buf]);
-    hipLaunchKernelGGL(k_widen<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, n, (const T*)b->d_tab[buf],
-                       b->d_tabw[buf]);
-  }
+  // f64 tables; in f32 the T table is their rounding, entry by entry (the point-major
+  // linearization relies on that to form it from the f64 table), and its f64 widening
+  hipLaunchKernelGGL(k_camera_tables<T>, dim3(grid_for(b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     b->d_cams[buf], b->d_tab64[buf], b->fp32 ? (T*)b->d_tab[buf] : (T*)nullptr,
+                     b->fp32 ? b->d_tabw[buf] : (double*)nullptr);
   FG_TRY(hipGetLastError());
   b->tab_valid[buf] = true;
   return FG_OK;
@@ -222,7 +218,7 @@ fg_status ensure_tables(fg_ba* b, int buf) {
 // The cost of buffer buf is enqueued by cost_launch and read by cost_finish once the stream
 // has been synchronized.
 template <typename T>
-fg_status cost_launch(fg_ba* b, int buf) {
+fg_status cost_launch(fg_ba* b, int buf, bool copy_back = true) {
   FG_CHECK(ensure_tables<T>(b, buf));
   if (b->timing) (void)hipEventRecord(b->ev_a, b->stream);
   FG_TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int), b->stream));
@@ -233,8 +229,10 @@ fg_status cost_launch(fg_ba* b, int buf) {
   hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)b->d_parts, blocks, 0,
                      b->d_out);
   FG_TRY(hipGetLastError());
-  FG_TRY(hipMemcpyAsync(b->h_pinned, b->d_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  if (copy_back) {
+    FG_TRY(hipMemcpyAsync(b->h_pinned, b->d_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    FG_TRY(hipMemcpyAsync(b->h_flags, b->d_flags, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  }
   if (b->timing) (void)hipEventRecord(b->ev_b, b->stream);
   return FG_OK;
 }
@@ -336,8 +334,9 @@ fg_status linearize_impl(fg_schur* s) {
   return FG_OK;
 }
 
+// ||a, c||_inf into d_scalar with the nonfinite flag in d_flags[0] (enqueued only).
 template <typename T>
-fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m, double* out, int sums = 0) {
+fg_status norm_inf_launch(fg_schur* s, const T* a, size_t n, const T* c, size_t m) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_flags, 0, sizeof(int), b->stream));
   const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
@@ -346,6 +345,13 @@ fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m,
   hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_cgparts, blocks, 1,
                      s->d_scalar);
   FG_TRY(hipGetLastError());
+  return FG_OK;
+}
+
+template <typename T>
+fg_status norm_inf_impl(fg_schur* s, const T* a, size_t n, const T* c, size_t m, double* out, int sums = 0) {
+  fg_ba* b = s->ba;
+  FG_CHECK(norm_inf_launch<T>(s, a, n, c, m));
   // sums > 0: the first `sums` values of d_sums come back with the same synchronization
   if (sums > 0)
     FG_TRY(hipMemcpyAsync(s->h_sums, s->d_sums, sums * sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -591,9 +597,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
                      (const CgState*)&s->d_sc->st[launched & 1], &s->d_sc->st[(launched + 1) & 1],
                      (const DevScalars*)s->d_sc, (const int*)s->d_fail, &s->d_mirror->st[(launched + 1) & 1]);
   FG_TRY(hipGetLastError());
-  // read back with the verification's synchronization (cg_collect)
-  FG_TRY(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipMemcpyAsync(s->h_int + 1, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  // the state and the factor failures are read with the verification's results (attempt_impl)
   s->cg_launched = launched;
   return FG_OK;
 }
@@ -675,24 +679,42 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   // are discarded (NotPositiveDefinite).
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
                      (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w, (T*)s->prow);
+  // partials: camera residual rows [0, nc), camera verification rows [nc, 2 nc), point rows
+  // from 2 nc; their column sums in one launch
   const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
   launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
-                           s->d_parts, nullptr);
-  launch_sum_parts(s, pblocks, 1);
+                           s->d_parts + (size_t)kParts * 2 * nc, nullptr);
   launch_cameras<T, cCheck>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, (const double*)s->x, nullptr,
                             (const double*)s->rhs, lambda, (double*)s->q, s->d_delta);
-  launch_sum_parts(s, nc, 0);
-  launch_sum_parts(s, nc, 2, nc);
+  const SumJobs jobs{{nc, pblocks, nc, 0}, {0, 2 * nc, nc, 0}, {0, 1, 2, 0}};
+  hipLaunchKernelGGL(k_sum_parts_jobs, dim3(3), dim3(kBlock), 0, b->stream, (const double*)s->d_parts, jobs,
+                     s->d_sums);
   FG_TRY(hipGetLastError());
   if (speculate_trial) {
     FG_CHECK(stage_launch<T>(b, s->d_delta));
-    FG_CHECK(cost_launch<T>(b, b->cur ^ 1));
+    FG_CHECK(cost_launch<T>(b, b->cur ^ 1, false));
+  }
+  FG_CHECK(norm_inf_launch<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0));
+  // one kernel gathers the PCG state, the verification sums, the step norm and the trial
+  // cost into host memory; one synchronization
+  hipLaunchKernelGGL(k_collect_attempt, dim3(1), dim3(kBlock), 0, b->stream, (const DevScalars*)s->d_sc,
+                     (const int*)s->d_fail, speculate_trial ? (const double*)b->d_out : (const double*)nullptr,
+                     speculate_trial ? (const int*)b->d_flags : (const int*)nullptr, (const double*)s->d_scalar,
+                     (const int*)s->d_flags, (const double*)s->d_sums, s->d_att);
+  FG_TRY(hipGetLastError());
+  FG_TRY(hipStreamSynchronize(b->stream));
+  const AttemptReadback& att = *s->h_att;
+  *s->h_sc = att.sc;
+  s->h_int[1] = att.fail;
+  std::memcpy(s->h_sums, att.sums, 3 * kParts * sizeof(double));
+  if (speculate_trial) {
+    b->h_pinned[0] = att.cost;
+    b->h_flags[0] = att.cost_flag;
   }
   double step_norm = 0.0;
-  const fg_status nst = norm_inf_impl<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0, &step_norm,
-                                              3 * kParts);
+  const fg_status nst = att.norm_flag != 0 ? FG_INVALID_EVALUATION : FG_OK;
+  if (nst == FG_OK) step_norm = att.norm;
   s->trial_cost_ready = speculate_trial;
-  if (nst != FG_OK && nst != FG_INVALID_EVALUATION) return nst;
   cg_collect(s);
   if (s->factor_failed) return FG_LINEAR_SOLVE_FAILED;  // NotPositiveDefinite, no iteration
   const double* rs = s->h_sums;                // ||Sx - rhs||^2, ||rhs||^2
@@ -757,7 +779,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc(&s->d_delta, 9 * nc + 3 * np));
   FG_TRY(dalloc(&s->d_rows, 2 * no));
   const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
-  const size_t rows = std::max({2 * nc, pblocks, (size_t)1024});  // cCheck writes 2 nc rows
+  const size_t rows = std::max(2 * nc + pblocks, (size_t)1024);  // the attempt's check: 2 nc camera rows + point rows
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
   FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc, kCgBlock))));
   FG_TRY(dalloc((double**)&s->zb, 9 * nc));
@@ -1293,6 +1315,8 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
   if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc(&s->h_poll, 2 * sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_mirror, sizeof(DevScalars), hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_att, sizeof(AttemptReadback), hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&s->d_att, s->h_att, 0) != hipSuccess ||
       hipHostGetDevicePointer((void**)&s->d_mirror, s->h_mirror, 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_sums, 4 * kParts * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_norm, 2 * sizeof(double), 0) != hipSuccess ||
@@ -1318,6 +1342,7 @@ void fg_schur_destroy(fg_schur* s) {
   if (s->h_sc) (void)hipHostFree(s->h_sc);
   if (s->h_poll) (void)hipHostFree(s->h_poll);
   if (s->h_mirror) (void)hipHostFree(s->h_mirror);
+  if (s->h_att) (void)hipHostFree(s->h_att);
   if (s->h_sums) (void)hipHostFree(s->h_sums);
   if (s->h_norm) (void)hipHostFree(s->h_norm);
   if (s->h_int) (void)hipHostFree(s->h_int);

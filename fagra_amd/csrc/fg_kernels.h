@@ -175,7 +175,10 @@ __device__ __forceinline__ double block_sum_parts_col(const double* parts, int n
 // Camera tables: Projection::new (examples/bal/model.rs:126-146) in f64, stored in T.
 // ---------------------------------------------------------------------------------------
 template <typename T>
-__global__ void k_camera_tables(int nc, const double* __restrict__ cams, T* __restrict__ tab) {
+// f64 table into tab64; with f32 storage (lo != nullptr) also its entry-by-entry rounding into
+// lo and that rounding widened back to f64 into lo_wide, in the same pass.
+__global__ void k_camera_tables(int nc, const double* __restrict__ cams, double* __restrict__ tab64,
+                                T* __restrict__ lo, double* __restrict__ lo_wide) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   const double* cam = cams + 9 * c;
@@ -214,18 +217,28 @@ __global__ void k_camera_tables(int nc, const double* __restrict__ cams, T* __re
                wi = qw * qi * 2;
   const double R[9] = {ww + ii - jj - kk2, ij - wk, wj + ik, wk + ij, ww - ii + jj - kk2,
                        jk - wi, ik - wj, wi + jk, ww - ii - jj + kk2};
-  T* o = tab + kCamTab * c;
-  for (int i = 0; i < kCamTab; ++i) o[i] = T(0);
+  double o[kCamTab];
+  for (int i = 0; i < kCamTab; ++i) o[i] = 0.0;
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) {
-      o[tR(i, j)] = T(R[3 * i + j]);
+      o[tR(i, j)] = R[3 * i + j];
       double s = 0;
       for (int k = 0; k < 3; ++k) s += R[3 * i + k] * Jr[3 * k + j];
-      o[tRJ(i, j)] = T(s);
+      o[tRJ(i, j)] = s;
     }
-    o[tT(i)] = T(cam[3 + i]);
+    o[tT(i)] = cam[3 + i];
   }
-  o[tF] = T(cam[6]); o[tK1] = T(cam[7]); o[tK2] = T(cam[8]);
+  o[tF] = cam[6]; o[tK1] = cam[7]; o[tK2] = cam[8];
+#pragma unroll
+  for (int i = 0; i < kCamTab; ++i) tab64[kCamTab * c + i] = o[i];
+  if (lo != nullptr) {
+#pragma unroll
+    for (int i = 0; i < kCamTab; ++i) {
+      const T r = T(o[i]);
+      lo[kCamTab * c + i] = r;
+      lo_wide[kCamTab * c + i] = double(r);
+    }
+  }
 }
 
 // Blockwise Huber (src/factors.rs:42-52): cost and sqrt(weight).
@@ -1353,6 +1366,38 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
 // ---------------------------------------------------------------------------------------
 // Column sums of a (rows x kParts) partial array in fixed order (deterministic).
 // ---------------------------------------------------------------------------------------
+// Everything the host reads after a damped attempt, gathered by one kernel into host memory:
+// the PCG state and block-factor failures, the trial cost and its flag, ||delta||_inf and its
+// nonfinite flag, and the verification sums.
+struct AttemptReadback {
+  DevScalars sc;
+  int fail, cost_flag, norm_flag, pad;
+  double cost, norm;
+  double sums[3 * kParts];
+};
+static __global__ void k_collect_attempt(const DevScalars* __restrict__ sc, const int* __restrict__ fail,
+                                         const double* __restrict__ cost, const int* __restrict__ cost_flag,
+                                         const double* __restrict__ norm, const int* __restrict__ norm_flag,
+                                         const double* __restrict__ sums, AttemptReadback* out) {
+  const int t = threadIdx.x;
+  if (t < 3 * kParts) out->sums[t] = sums[t];
+  if (t == 0) {
+    out->sc = *sc;
+    out->fail = *fail;
+    out->cost_flag = cost_flag ? *cost_flag : 0;
+    out->cost = cost ? *cost : 0.0;
+    out->norm_flag = *norm_flag;
+    out->norm = *norm;
+  }
+}
+
+// Up to four column-sum jobs in one launch, one workgroup each: job j sums rows[j] rows of the
+// partial array starting at row first[j] into out[slot[j] * kParts ...].
+struct SumJobs {
+  int rows[4], first[4], slot[4];
+};
+static __global__ void k_sum_parts_jobs(const double* __restrict__ parts, SumJobs jobs, double* __restrict__ sums);
+
 static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* __restrict__ out) {
   __shared__ double smem[kParts][kBlock / kWave];
   double s[kParts];
@@ -1372,6 +1417,31 @@ static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, d
     double t = 0.0;
     for (int w = 0; w < kBlock / kWave; ++w) t += smem[threadIdx.x][w];
     out[threadIdx.x] = t;
+  }
+}
+
+static __global__ void k_sum_parts_jobs(const double* __restrict__ parts, SumJobs jobs, double* __restrict__ sums) {
+  __shared__ double smem[kParts][kBlock / kWave];
+  const int j = blockIdx.x;
+  const double* p = parts + (size_t)kParts * jobs.first[j];
+  const int rows = jobs.rows[j];
+  double s[kParts];
+#pragma unroll
+  for (int col = 0; col < kParts; ++col) s[col] = 0.0;
+  for (int i = threadIdx.x; i < rows; i += blockDim.x)
+#pragma unroll
+    for (int col = 0; col < kParts; ++col) s[col] += p[kParts * i + col];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int col = 0; col < kParts; ++col) {
+    const double v = wave_sum(s[col]);
+    if (lane == 0) smem[col][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kParts) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += smem[threadIdx.x][w];
+    sums[kParts * jobs.slot[j] + threadIdx.x] = t;
   }
 }
 
