@@ -419,16 +419,18 @@ fg_status gradient_stage(fg_schur* s, double* out) {
 }
 
 // Factor the blocks at lambda; returns the number of failed blocks through *fails.
-template <typename T>
 // fails == nullptr: the failure count stays on the device (d_fail), where the PCG's first
-// direction reads it.
-fg_status factor_impl(fg_schur* s, double lambda, int* fails) {
+// direction reads it. rhs_points: the point pass also forms the right-hand side's point solves
+// into s->tp.
+template <typename T>
+fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = false) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
   hipLaunchKernelGGL(k_factor_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
   hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
-                     b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail);
+                     b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail, (const T*)s->gp,
+                     (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr);
   FG_TRY(hipGetLastError());
   if (fails == nullptr) return FG_OK;
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -647,9 +649,9 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   {
     Phase ph(s, &s->tim.factorization);
     // a failed block is reported by the PCG (NotPositiveDefinite before its first iteration)
-    FG_CHECK(factor_impl<T>(s, lambda, nullptr));
+    // the point factorization also forms M gp (the point half of the reduced right-hand side)
+    FG_CHECK(factor_impl<T>(s, lambda, nullptr, true));
     // rhs = Dc^-1 (-gc + sum Jc^T Jp M gp)
-    launch_points<T, kRhs>(s, nullptr, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
     launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (double*)s->rhs,
                             nullptr);
     FG_TRY(hipGetLastError());

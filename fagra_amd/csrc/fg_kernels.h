@@ -675,14 +675,27 @@ __global__ void k_scales(int nc, int np, const T* __restrict__ hcc, const double
 // are skipped (no Gram block exists for them in the reference): a zero factor, zero solve.
 // ---------------------------------------------------------------------------------------
 template <typename T>
+__device__ __forceinline__ void point_solve(const double* __restrict__ lp, const T* __restrict__ dpv, int i,
+                                            const double v[3], double t[3]);
+template <typename T>
+__device__ __forceinline__ void store_tp(double* tp, int i, const double t[3]);
+
+// With t_out: also the right-hand side's point solves t_p = (Hpp + lambda Dp^2)^-1 gp of
+// factor_and_reduce_rhs (what k_points<kRhs> computes), from the factor just stored.
+template <typename T>
 __global__ void k_factor_points(int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
-                                double lambda, double* __restrict__ lp, int* fail_count) {
+                                double lambda, double* __restrict__ lp, int* fail_count,
+                                const T* __restrict__ gp, const T* __restrict__ dpv, double* __restrict__ t_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   double* l = lp + 6 * p;
   if (pt_len[p] == 0) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) l[i] = 0.0;
+    if (t_out != nullptr) {
+      const double z[3] = {0.0, 0.0, 0.0};
+      store_tp<T>(t_out, p, z);
+    }
     return;
   }
   double a[6];
@@ -697,6 +710,12 @@ __global__ void k_factor_points(int np, const int* __restrict__ pt_len, const do
 #pragma unroll
   for (int i = 0; i < 6; ++i) l[i] = ok ? a[i] : 0.0;  // zero factor: failed (see cameras)
   if (!ok) atomicAdd(fail_count, 1);
+  if (t_out != nullptr) {
+    const double v[3] = {double(gp[3 * p]), double(gp[3 * p + 1]), double(gp[3 * p + 2])};
+    double t[3];
+    point_solve(lp, dpv, p, v, t);
+    store_tp<T>(t_out, p, t);
+  }
 }
 
 // t = (Hpp + lambda D^2)^-1 v = D^-1 (C + lambda I)^-1 D^-1 v with the stored factor of point i.
