@@ -103,6 +103,16 @@ struct fg_ba {
   bool timing = false;
 };
 
+// Host side of a PCG solve in progress (launched iterations, issued and read batches, the
+// last state read).
+struct CgRun {
+  hipGraphExec_t graph = nullptr;
+  bool timed = false;
+  int max_it = 0, launched = 0, issued = 0, read = 0;
+  CgState last{};
+  bool have = false;
+};
+
 struct fg_schur {
   fg_schur_config cfg;
   fg_ba* ba = nullptr;
@@ -145,7 +155,7 @@ struct fg_schur {
   hipGraphExec_t cg_graph[2] = {nullptr, nullptr};  // PCG batch per accepted-state buffer
   int cg_graph_size[2] = {0, 0};                      // sizeof(T) it was captured for
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
-  int cg_launched = 0;  // PCG iterations launched by the last cg_impl (state slot parity)
+  int cg_launched = 0;  // PCG iterations launched by the last cg_finish (state slot parity)
   std::vector<hipEvent_t> events;  // matvec kernel timing pool (pairs)
   size_t events_used = 0;
   hipEvent_t ph_a = nullptr, ph_b = nullptr;
@@ -156,6 +166,11 @@ struct fg_schur {
   bool damping_spec = false;      // scales at spec_floor enqueued and read back with the gradient
   double spec_floor = kNaN;       // the loop's last column-norm floor
   bool trial_cost_ready = false;  // the last solve staged its step and read back the trial cost
+  double hint_lambda = kNaN;      // the damping the loop's next solve will use (hinted)
+  bool pf_active = false;         // attempt_begin ran for pf_lambda with the gradient norm
+  double pf_lambda = 0.0;
+  CgRun pf_run;
+  hipEvent_t pf_ev = nullptr;     // end of the gradient stage's readback
   // learned preconditioner
   fg_mlp* mlp = nullptr;
   void* mbj = nullptr;         // f64 [nc][45] explicit (B + lambda I)^-1
@@ -391,7 +406,7 @@ fg_status prepare_damping_impl(fg_schur* s, double floor) {
 // that linearization's evaluation flag; with a finite spec_floor the damping scales at that
 // floor are enqueued too and their flag comes back in h_int[6] (damping_spec).
 template <typename T>
-fg_status gradient_stage(fg_schur* s, double* out) {
+fg_status gradient_launch(fg_schur* s, bool* spec_out) {
   fg_ba* b = s->ba;
   const size_t n = 9 * b->nc, m = 3 * b->np;
   const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
@@ -404,7 +419,12 @@ fg_status gradient_stage(fg_schur* s, double* out) {
   FG_TRY(hipGetLastError());
   FG_TRY(hipMemcpyAsync(s->h_norm, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipMemcpyAsync(s->h_int + 4, s->d_flags, 3 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipStreamSynchronize(b->stream));
+  *spec_out = spec;
+  return FG_OK;
+}
+
+// The gradient stage's results once gradient_launch's copies have landed.
+fg_status gradient_finish(fg_schur* s, bool spec, double* out) {
   if (s->h_int[5] != 0) {
     s->linearized = false;
     return FG_INVALID_EVALUATION;
@@ -416,6 +436,14 @@ fg_status gradient_stage(fg_schur* s, double* out) {
   }
   *out = s->h_norm[0];
   return FG_OK;
+}
+
+template <typename T>
+fg_status gradient_stage(fg_schur* s, double* out) {
+  bool spec = false;
+  FG_CHECK(gradient_launch<T>(s, &spec));
+  FG_TRY(hipStreamSynchronize(s->ba->stream));
+  return gradient_finish(s, spec, out);
 }
 
 // Factor the blocks at lambda; returns the number of failed blocks through *fails.
@@ -523,8 +551,25 @@ fg_status cg_batch_graph(fg_schur* s, hipGraphExec_t* out) {
   return FG_OK;
 }
 
+// Host side of one PCG solve in progress: cg_begin initializes and launches the first batch,
+// cg_finish polls and launches the rest.
 template <typename T>
-fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
+fg_status cg_issue(fg_schur* s, CgRun& run) {
+  fg_ba* b = s->ba;
+  if (run.graph) {
+    FG_TRY(hipGraphLaunch(run.graph, b->stream));
+  } else {
+    for (int it = 0; it < kBatch; ++it) launch_cg_iteration<T>(s, run.launched + it, run.timed);
+    FG_TRY(hipGetLastError());
+  }
+  run.launched += kBatch;
+  FG_TRY(hipEventRecord(s->poll_ev[run.issued & 1], b->stream));
+  ++run.issued;
+  return FG_OK;
+}
+
+template <typename T>
+fg_status cg_begin(fg_schur* s, double lambda, bool learned, CgRun& run) {
   fg_ba* b = s->ba;
   const int nc = (int)b->nc;
   const int cblocks = grid_for(nc, kCgBlock);
@@ -545,26 +590,22 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
                      (const double*)s->mbj, (const double*)s->gmat, (const double*)s->rhs, (double*)s->x,
                      (double*)s->r, (double*)s->z, (double*)s->p, s->d_cgparts, init, s->d_sc);
   FG_TRY(hipGetLastError());
-  const bool timed = s->cfg.collect_timings != 0;
-  hipGraphExec_t graph = nullptr;
-  if (!timed && s->use_graphs) FG_CHECK(cg_batch_graph<T>(s, &graph));
+  run = CgRun{};
+  run.max_it = max_it;
+  run.timed = s->cfg.collect_timings != 0;
+  if (!run.timed && s->use_graphs) FG_CHECK(cg_batch_graph<T>(s, &run.graph));
   // The directions publish their state to coherent host memory (h_mirror); after a batch's
   // event the host reads it there, possibly already overtaken by the next batch (a later
-  // state; a torn read only misguides the prediction below, and a set done flag was set).
+  // state; a torn read only misguides the prediction in cg_finish, and a set done flag was set).
   s->h_mirror->st[0].done = s->h_mirror->st[1].done = 0;
-  int launched = 0, issued = 0, read = 0;  // iterations launched; batches issued / read back
-  auto issue = [&]() -> fg_status {
-    if (graph) {
-      FG_TRY(hipGraphLaunch(graph, b->stream));
-    } else {
-      for (int it = 0; it < kBatch; ++it) launch_cg_iteration<T>(s, launched + it, timed);
-      FG_TRY(hipGetLastError());
-    }
-    launched += kBatch;
-    FG_TRY(hipEventRecord(s->poll_ev[issued & 1], b->stream));
-    ++issued;
-    return FG_OK;
-  };
+  return cg_issue<T>(s, run);
+}
+
+template <typename T>
+fg_status cg_finish(fg_schur* s, CgRun& run) {
+  fg_ba* b = s->ba;
+  const int nc = (int)b->nc;
+  const int cblocks = grid_for(nc, kCgBlock);
   // The next batch normally goes out before the previous one is read, so the device never
   // waits for the host; when the state read last predicts (from its residual's geometric rate)
   // that the solve ends inside the batch already in flight, the host reads that batch first
@@ -575,24 +616,22 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
     const double rate = std::log(h.rr / h.bb) / h.iters;  // log of the mean decrease per iteration
     if (!(rate < 0.0)) return false;
     const double need = std::ceil(std::log(h.tol2 / h.rr) / rate);
-    return h.iters + need <= launched - 1;
+    return h.iters + need <= run.launched - 1;
   };
-  FG_CHECK(issue());
-  CgState last{};
-  bool have = false;
-  while (launched < max_it) {  // past max_it the final decision is read with the verification
-    const bool ahead = !(have && ends_in_flight(last));
-    if (ahead) FG_CHECK(issue());
-    FG_TRY(hipEventSynchronize(s->poll_ev[read & 1]));
+  while (run.launched < run.max_it) {  // past max_it the final decision is read with the verification
+    const bool ahead = !(run.have && ends_in_flight(run.last));
+    if (ahead) FG_CHECK(cg_issue<T>(s, run));
+    FG_TRY(hipEventSynchronize(s->poll_ev[run.read & 1]));
     DevScalars h;
     std::memcpy(&h, (const void*)s->h_mirror, sizeof(DevScalars));
-    ++read;
-    last = h.st[0].iters >= h.st[1].iters ? h.st[0] : h.st[1];
-    have = true;
+    ++run.read;
+    run.last = h.st[0].iters >= h.st[1].iters ? h.st[0] : h.st[1];
+    run.have = true;
     if (h.st[0].done || h.st[1].done) break;
-    if (issued == read) have = false;  // nothing in flight: the next batch goes out unconditionally
+    if (run.issued == run.read) run.have = false;  // nothing in flight: the next batch goes out unconditionally
   }
   // the last update's convergence / exhaustion decision
+  const int launched = run.launched;
   hipLaunchKernelGGL(k_cg_direction<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
                      (const T*)b->d_tab[b->cur], (const T*)s->dc, (const double*)s->zb, (const double*)s->z,
                      (double*)s->p, (T*)s->fw, s->fw_w, (T*)s->prow, (const double*)s->d_cgparts, cblocks,
@@ -604,7 +643,7 @@ fg_status cg_impl(fg_schur* s, double lambda, bool learned) {
   return FG_OK;
 }
 
-// PCG statistics and diagnostics from the state read back by cg_impl (after a synchronization).
+// PCG statistics and diagnostics from the state read back after cg_finish (after a synchronization).
 void cg_collect(fg_schur* s) {
   const int launched = s->cg_launched;
   s->factor_failed = s->h_int[1] != 0;
@@ -637,12 +676,10 @@ fg_status stage_launch(fg_ba* b, const double* d_delta) {
   return FG_OK;
 }
 
-// Schur::attempt (schur.rs:192-202): factor + reduced rhs, CG, recover, verify. With
-// speculate_trial the trial state of the step and its cost are enqueued behind the
-// verification and come back with the same synchronization (the device LM loop stages and
-// costs every step it does not reject beforehand; trial_cost_ready marks the result).
+// Schur::attempt (schur.rs:192-202), the first part: factor + reduced rhs, the learned
+// preconditioner, the PCG's start (initialization and first batch), all enqueued only.
 template <typename T>
-fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool speculate_trial = false) {
+fg_status attempt_begin(fg_schur* s, double lambda_d, CgRun& run) {
   fg_ba* b = s->ba;
   const double lambda = lambda_d;
   const int nc = (int)b->nc;
@@ -668,9 +705,27 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
                        (const float*)s->my, (const double*)s->lc, (double*)s->gmat);
     FG_TRY(hipGetLastError());
   }
+  Phase ph(s, &s->tim.linear_solve);
+  return cg_begin<T>(s, lambda, learned, run);
+}
+
+// Schur::attempt (schur.rs:192-202), the rest: the PCG to its end, recovery and verification.
+// prefetched: attempt_begin ran earlier for this lambda (enqueued with the gradient norm).
+// With speculate_trial the trial state of the step and its cost are enqueued behind the
+// verification and come back with the same synchronization (the device LM loop stages and
+// costs every step it does not reject beforehand; trial_cost_ready marks the result).
+template <typename T>
+fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool speculate_trial = false,
+                       CgRun* prefetched = nullptr) {
+  fg_ba* b = s->ba;
+  const double lambda = lambda_d;
+  const int nc = (int)b->nc;
+  CgRun local;
+  CgRun& run = prefetched ? *prefetched : local;
+  if (!prefetched) FG_CHECK(attempt_begin<T>(s, lambda_d, run));
   {
     Phase ph(s, &s->tim.linear_solve);
-    FG_CHECK(cg_impl<T>(s, lambda, learned));
+    FG_CHECK(cg_finish<T>(s, run));
   }
   Phase ph(s, &s->tim.verification);
   // recover_step (schur.rs:312-344) and verify_step (schur.rs:346-397) in one point pass and
@@ -1326,7 +1381,8 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
     return fail();
   if (hipEventCreate(&s->ph_a) != hipSuccess || hipEventCreate(&s->ph_b) != hipSuccess ||
       hipEventCreateWithFlags(&s->poll_ev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&s->poll_ev[1], hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&s->poll_ev[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->pf_ev, hipEventDisableTiming) != hipSuccess)
     return fail();
   *out = s;
   return FG_OK;
@@ -1353,6 +1409,7 @@ void fg_schur_destroy(fg_schur* s) {
     if (e) (void)hipEventDestroy(e);
   if (s->ph_a) (void)hipEventDestroy(s->ph_a);
   if (s->ph_b) (void)hipEventDestroy(s->ph_b);
+  if (s->pf_ev) (void)hipEventDestroy(s->pf_ev);
   delete s;
 }
 
@@ -1439,7 +1496,8 @@ fg_status fg_schur_prepare_damping(fg_schur* s, double floor) {
   return st;
 }
 
-static fg_status solve_damped_device(fg_schur* s, double lambda, double* predicted, bool speculate_trial = false) {
+static fg_status solve_damped_device(fg_schur* s, double lambda, double* predicted, bool speculate_trial = false,
+                                     CgRun* prefetched = nullptr) {
   reset_diag(s, lambda);
   s->step_valid = false;
   s->trial_cost_ready = false;
@@ -1447,8 +1505,8 @@ static fg_status solve_damped_device(fg_schur* s, double lambda, double* predict
   s->solves += 1;
   s->events_used = 0;
   double pred = 0.0;
-  fg_status st = s->cfg.precision == FG_FP32 ? attempt_impl<float>(s, lambda, &pred, speculate_trial)
-                                             : attempt_impl<double>(s, lambda, &pred, speculate_trial);
+  fg_status st = s->cfg.precision == FG_FP32 ? attempt_impl<float>(s, lambda, &pred, speculate_trial, prefetched)
+                                             : attempt_impl<double>(s, lambda, &pred, speculate_trial, prefetched);
   if (s->cfg.collect_timings && s->events_used > 0) {
     (void)hipEventSynchronize(s->events[s->events_used - 1]);
     // the first diag.iterations pairs bracket real products; later ones are the no-op tail
@@ -2172,10 +2230,46 @@ void fg_lsmr_timings_get(const fg_lsmr* l, fg_lsmr_timings* t) { *t = l->tim; }
 // collected every call keeps its own synchronization.
 static bool deferred_calls(const fg_schur* s) { return !s->cfg.collect_timings && !s->ba->timing; }
 
+// A started attempt that will not be used: wait for its kernels, so that no PCG batch of it
+// publishes a state while the next solve polls.
+static void drop_prefetch(fg_schur* s) {
+  if (!s->pf_active) return;
+  s->pf_active = false;
+  (void)hipStreamSynchronize(s->ba->stream);
+}
+
 static void clear_deferred(fg_schur* s) {
   s->lin_pending = false;
   s->damping_spec = false;
   s->trial_cost_ready = false;
+  drop_prefetch(s);
+}
+
+// The damping of the loop's next solve, known before its linearization (fg::lm_optimize_hinted).
+static void cb_hint(void* c, double lambda) { ((fg_schur*)c)->hint_lambda = lambda; }
+
+// gradient_norm with the next attempt's start behind it: once the norm and the damping scales
+// are enqueued, the factorization, right-hand side, preconditioner and first PCG batch of the
+// hinted damping follow on the stream, and the host waits only for the norm. The started
+// attempt is used when prepare_damping keeps the scales and solve_damped asks for that damping;
+// otherwise its results are overwritten by the attempt that runs (stream order).
+extern "C++" template <typename T>
+fg_status gradient_and_prefetch(fg_schur* s, double* out) {
+  bool spec = false;
+  FG_CHECK(gradient_launch<T>(s, &spec));
+  const double lambda = s->hint_lambda;
+  s->hint_lambda = kNaN;
+  const bool learned = s->cfg.preconditioner == FG_PRECOND_LEARNED;
+  if (spec && std::isfinite(lambda) && lambda >= 0.0 && (!learned || s->mlp)) {
+    FG_TRY(hipEventRecord(s->pf_ev, s->ba->stream));
+    FG_CHECK(attempt_begin<T>(s, lambda, s->pf_run));
+    s->pf_active = true;
+    s->pf_lambda = lambda;
+    FG_TRY(hipEventSynchronize(s->pf_ev));
+  } else {
+    FG_TRY(hipStreamSynchronize(s->ba->stream));
+  }
+  return gradient_finish(s, spec, out);
 }
 
 static int cb_prepare(void* c) {
@@ -2208,15 +2302,17 @@ static int cb_linearize(void* c) {
 static int cb_gradient_norm(void* c, double* out) {
   fg_schur* s = (fg_schur*)c;
   s->damping_spec = false;
+  drop_prefetch(s);
   if (!s->lin_pending) return fg_schur_gradient_norm(s, out);
   s->lin_pending = false;
-  return s->cfg.precision == FG_FP32 ? gradient_stage<float>(s, out) : gradient_stage<double>(s, out);
+  return s->cfg.precision == FG_FP32 ? gradient_and_prefetch<float>(s, out) : gradient_and_prefetch<double>(s, out);
 }
 static int cb_prepare_damping(void* c, double f) {
   fg_schur* s = (fg_schur*)c;
   const bool spec = s->damping_spec && f == s->spec_floor;
   s->damping_spec = false;
   if (!spec) {
+    drop_prefetch(s);  // started on other scales
     const fg_status st = fg_schur_prepare_damping(s, f);
     if (std::isfinite(f) && f > 0.0 && deferred_calls(s)) s->spec_floor = f;
     return st;
@@ -2229,7 +2325,14 @@ static int cb_prepare_damping(void* c, double f) {
 }
 static int cb_solve_damped(void* c, double lambda, double* norm, double* pred) {
   fg_schur* s = (fg_schur*)c;
-  fg_status st = solve_damped_device(s, lambda, pred, deferred_calls(s));
+  CgRun* pre = nullptr;
+  if (s->pf_active && s->pf_lambda == lambda) {
+    pre = &s->pf_run;
+    s->pf_active = false;
+  } else {
+    drop_prefetch(s);
+  }
+  fg_status st = solve_damped_device(s, lambda, pred, deferred_calls(s), pre);
   if (st == FG_OK) *norm = s->last_step_norm;
   return st;
 }
@@ -2287,7 +2390,10 @@ fg_status fg_lm_solve_batch(fg_schur* s, fg_ba* b, const fg_optimize_options* op
   if (!b || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
   FG_CHECK(bind(s, b));
   fg_problem pb = device_problem(s);
-  fg_lm_optimize(&pb, options, lm, on_accept, user, out);
+  s->hint_lambda = kNaN;
+  fg::lm_optimize_hinted(&pb, options, lm, on_accept, user, out, deferred_calls(s) ? cb_hint : nullptr);
+  s->hint_lambda = kNaN;
+  drop_prefetch(s);
   return (fg_status)out->status;
 }
 

@@ -39,4 +39,8 @@ inline size_t pad64(size_t n) { return (n + 63) / 64 * 64; }
 // host array out, on stream s. Status codes of SelectedCovariance::compute.
 fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* columns, size_t k, double tolerance,
                            double* out);
+// fg_lm_optimize with a damping hint before each linearization (fg_lm.cpp).
+void lm_optimize_hinted(const fg_problem* pb, const fg_optimize_options* options, const fg_lm_options* lm,
+                        fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out,
+                        void (*hint)(void* ctx, double damping));
 }  // namespace fg

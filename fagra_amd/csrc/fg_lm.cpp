@@ -11,6 +11,7 @@
 #include <limits>
 
 #include "../../include/fagra_gpu.h"
+#include "fg_internal.h"
 
 namespace {
 
@@ -83,6 +84,17 @@ fg_status fg_lm_options_default(fg_lm_options* o) {
 
 void fg_lm_optimize(const fg_problem* pb, const fg_optimize_options* options, const fg_lm_options* lm,
                     fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out) {
+  fg::lm_optimize_hinted(pb, options, lm, on_accept, user, out, nullptr);
+}
+
+}  // extern "C"
+
+// The LM loop with an optional hint: right before each linearization, `hint` receives the
+// damping the first trial of that iteration will use (0 for an undamped trial), so a backend
+// can start that solve while the host still checks the gradient.
+void fg::lm_optimize_hinted(const fg_problem* pb, const fg_optimize_options* options, const fg_lm_options* lm,
+                            fg_on_accept_fn on_accept, void* user, fg_lm_outcome* out,
+                            void (*hint)(void* ctx, double damping)) {
   reset(out);
   auto finish = [&](int status) {
     out->status = status;
@@ -112,6 +124,7 @@ void fg_lm_optimize(const fg_problem* pb, const fg_optimize_options* options, co
       return finish(FG_OK);
     }
     out->linearizations += 1;
+    if (hint) hint(pb->ctx, stalled ? 0.0 : lambda);
     st = pb->linearize(pb->ctx);
     if (st != FG_OK) return finish(st);
     double gradient = 0.0;
@@ -196,6 +209,8 @@ void fg_lm_optimize(const fg_problem* pb, const fg_optimize_options* options, co
     if (!accepted) return finish(FG_NO_PROGRESS);
   }
 }
+
+extern "C" {
 
 void fg_gn_optimize(const fg_problem* pb, const fg_optimize_options* options, fg_lm_outcome* out) {
   reset(out);
