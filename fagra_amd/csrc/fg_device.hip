@@ -402,9 +402,9 @@ fg_status prepare_damping_impl(fg_schur* s, double floor) {
   return FG_OK;
 }
 
-// The gradient norm of a linearization enqueued by linearize_launch, read back together with
-// that linearization's evaluation flag; with a finite spec_floor the damping scales at that
-// floor are enqueued too and their flag comes back in h_int[6] (damping_spec).
+// Enqueues the gradient norm of a linearization enqueued by linearize_launch and its readback
+// together with that linearization's evaluation flag; with a finite spec_floor the damping
+// scales at that floor are enqueued too and their flag comes back in h_int[6] (damping_spec).
 template <typename T>
 fg_status gradient_launch(fg_schur* s, bool* spec_out) {
   fg_ba* b = s->ba;
@@ -436,14 +436,6 @@ fg_status gradient_finish(fg_schur* s, bool spec, double* out) {
   }
   *out = s->h_norm[0];
   return FG_OK;
-}
-
-template <typename T>
-fg_status gradient_stage(fg_schur* s, double* out) {
-  bool spec = false;
-  FG_CHECK(gradient_launch<T>(s, &spec));
-  FG_TRY(hipStreamSynchronize(s->ba->stream));
-  return gradient_finish(s, spec, out);
 }
 
 // Factor the blocks at lambda; returns the number of failed blocks through *fails.

@@ -1756,12 +1756,6 @@ __global__ void k_stage(int nc, int np, const double* __restrict__ cams, const d
 }
 
 template <typename T>
-__global__ void k_widen(int n, const T* __restrict__ a, double* __restrict__ b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = double(a[i]);
-}
-
-template <typename T>
 __global__ void k_convert(int n, const double* __restrict__ a, T* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = T(a[i]);
