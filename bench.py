@@ -74,9 +74,14 @@ def run_mode(fa, problem_dev, precision, precond, cg_tol, target, max_iter, coll
     t0 = [0.0]
 
     trace = []
+    last_log = [0.0]
 
     def on_accept(st):
-        trace.append((time.perf_counter() - t0[0], st.cost))
+        now = time.perf_counter() - t0[0]
+        trace.append((now, st.cost))
+        if now - last_log[0] > 20.0:  # long solves (the converged C* on large workloads) show progress
+            last_log[0] = now
+            log(f"bench: ... {st.accepted_steps} steps, cost {st.cost!r} at {now:.0f}s")
         if "t" not in hit and st.cost <= target:
             hit["t"] = time.perf_counter() - t0[0]
             hit["steps"] = st.accepted_steps

@@ -69,6 +69,7 @@ struct fg_ba {
   std::vector<int32_t> point_of_slot, slot_of_point;
   int* d_point_of_slot = nullptr;  // uploaded on the first covariance query
   int nchunks = 0;
+  int nsplit = 0;  // leading chunks of kSplitLen or more steps (one workgroup each in k_points)
   size_t nslots = 0;
   // Device static data.
   int *d_chunk_base = nullptr, *d_chunk_len = nullptr, *d_slot_cam = nullptr, *d_slot_row = nullptr;
@@ -274,10 +275,16 @@ fg_status cost_impl(fg_ba* b, int buf, double* out) {
 // ------------------------------------------------------------------------------------------
 // Backend launches
 // ------------------------------------------------------------------------------------------
+// Workgroups of a k_points launch: one per long chunk, then one per kChunksPerBlock chunks.
+inline int point_blocks(const fg_ba* b) {
+  return std::max(1, b->nsplit + grid_for((size_t)(b->nchunks - b->nsplit), kChunksPerBlock));
+}
+
 template <typename T>
 PointArgs point_args(const fg_ba* b) {
   PointArgs a;
   a.nchunks = b->nchunks;
+  a.nsplit = b->nsplit;
   a.np = (int)b->np;
   a.chunk_base = b->d_chunk_base;
   a.chunk_len = b->d_chunk_len;
@@ -293,7 +300,7 @@ template <typename T, int MODE>
 void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const double* rows_in, double lambda,
                    double* delta_out, double* rows_out, double* parts, const CgState* st) {
   fg_ba* b = s->ba;
-  const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  const int blocks = point_blocks(b);
   // the point-major kernels read the product rows (s->prow) where a forward table is named
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur],
@@ -730,7 +737,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
                      (const T*)b->d_tab[b->cur], (const double*)s->x, (const T*)s->dc, 1, (T*)s->fw, s->fw_w, (T*)s->prow);
   // partials: camera residual rows [0, nc), camera verification rows [nc, 2 nc), point rows
   // from 2 nc; their column sums in one launch
-  const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  const int pblocks = point_blocks(b);
   launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
                            s->d_parts + (size_t)kParts * 2 * nc, nullptr);
   launch_cameras<T, cCheck>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, (const double*)s->x, nullptr,
@@ -827,7 +834,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc(&s->mtarget, 45 * nc));
   FG_TRY(dalloc(&s->d_delta, 9 * nc + 3 * np));
   FG_TRY(dalloc(&s->d_rows, 2 * no));
-  const size_t pblocks = (size_t)std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  const size_t pblocks = (size_t)point_blocks(b);
   const size_t rows = std::max(2 * nc + pblocks, (size_t)1024);  // the attempt's check: 2 nc camera rows + point rows
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
   FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc, kCgBlock))));
@@ -1139,6 +1146,12 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
     chunk_base[q + 1] = chunk_base[q] + chunk_len[q] * kWave;
   }
   b->nslots = (size_t)chunk_base[b->nchunks];
+  // chunk lengths do not increase (longest tracks first); FAGRA_SPLIT_LEN overrides the
+  // threshold (0: no chunk is split), for comparing both mappings on one problem
+  int split_len = kSplitLen;
+  if (const char* e = std::getenv("FAGRA_SPLIT_LEN")) split_len = std::atoi(e) > 0 ? std::atoi(e) : std::numeric_limits<int>::max();
+  b->nsplit = 0;
+  while (b->nsplit < b->nchunks && chunk_len[b->nsplit] >= split_len) ++b->nsplit;
   std::vector<int> slot_cam(b->nslots, -1), slot_row(b->nslots, -1), k_slot(no), k_pslot(no), pt_len(np);
   std::vector<double> slot_pix(2 * b->nslots, 0.0);
   for (size_t sl = 0; sl < np; ++sl) {
@@ -1793,7 +1806,7 @@ fg_status lsmr_alloc_vectors(fg_lsmr* l) {
   const size_t n = 9 * b->nc + 3 * b->np;
   if (l->n == n && l->nslots == b->nslots && l->d_u) return FG_OK;
   lsmr_free_vectors(l);
-  l->pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  l->pblocks = point_blocks(b);
   const size_t ublocks = (size_t)l->pblocks + b->nc;
   const size_t vblocks = (size_t)grid_for(b->nc + b->np);
   FG_TRY(dalloc(&l->d_u, 2 * std::max<size_t>(1, b->nslots)));
@@ -1990,7 +2003,7 @@ fg_status lsmr_solve_damped_impl(fg_lsmr* l, double lambda, double* predicted, b
                        (const double*)(s->d_delta + 9 * nc), (const T*)s->dp, s->d_delta + 9 * nc);
   hipLaunchKernelGGL(k_forward_tables<T>, dim3(grid_for(nc)), dim3(kBlock), 0, b->stream, nc,
                      (const T*)b->d_tab[b->cur], (const double*)s->d_delta, (const T*)s->dc, 0, (T*)s->fw, s->fw_w, (T*)s->prow);
-  const int pblocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  const int pblocks = point_blocks(b);
   launch_points<T, kVerify>(s, (const T*)s->fw, s->d_delta + 9 * nc, nullptr, lambda, l->d_zt + 9 * nc, nullptr,
                             s->d_parts, nullptr);
   launch_sum_parts(s, pblocks, 1);

@@ -865,8 +865,14 @@ enum PointMode { kSchur = 0, kRecover = 1, kRhs = 2, kVerify = 3, kJv = 4, kJtu 
 // Sweep mode of kCheck only: kSchur's sums plus sum ||Jc z||^2 (uu) and sum Jp^T r (gr).
 constexpr int kSchurSums = 8;
 
+// Chunks at least this long (the first ones: points are ordered longest track first) are
+// swept by all four waves of one workgroup, each taking a quarter of the observation steps,
+// so that a long-tailed track distribution does not leave a few waves looping for the rest.
+constexpr int kSplitLen = 32;
+
 struct PointArgs {
   int nchunks, np;
+  int nsplit;  // leading chunks that each take a whole workgroup
   const int* chunk_base;
   const int* chunk_len;
   const int* slot_cam;
@@ -962,21 +968,21 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
 // One lane's observations of a point-major pass (SELL chunk, two observations per step so
 // that both loads issue before either is used).
 template <typename T, int MODE>
-__device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int lane, const Vec4<T>* __restrict__ rec,
-                                            const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
-                                            const T* __restrict__ fw, const double dpx[3],
+__device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int lane, int j0, int j1,
+                                            const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res,
+                                            const T* __restrict__ tab, const T* __restrict__ fw, const double dpx[3],
                                             const double* __restrict__ rows_in, double* __restrict__ rows_out,
                                             double w[3], double h[6], double& uu, double coef, double gr[3]) {
-  const int base = pa.chunk_base[chunk], len = pa.chunk_len[chunk];
-  int j = 0;
-  for (; j + 1 < len; j += 2) {
+  const int base = pa.chunk_base[chunk];
+  int j = j0;
+  for (; j + 1 < j1; j += 2) {
     const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
     const int c0 = pa.slot_cam[s0], c1 = pa.slot_cam[s1];
     const Vec4<T> r0 = rec[s0], r1 = rec[s1];
     if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, r0, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
     if (c1 >= 0) point_obs<T, MODE>(pa, s1, c1, r1, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
   }
-  if (j < len) {
+  if (j < j1) {
     const int s0 = base + j * kWave + lane;
     const int c0 = pa.slot_cam[s0];
     if (c0 >= 0) point_obs<T, MODE>(pa, s0, c0, rec[s0], res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
@@ -999,129 +1005,169 @@ __global__ void __launch_bounds__(kBlock) k_points(
   // kLsmr: U update coefficient alpha/beta and sqrt(lambda) of the damping rows
   const double coef = MODE == kLsmr ? ls->alpha / ls->beta : 0.0;
   const double root = MODE == kLsmr ? ls->root : 0.0;
-  const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
+  const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
+  // The first pa.nsplit chunks (the longest tracks) take a whole workgroup each: its four waves
+  // sweep consecutive quarters of the chunk's observation steps, add their lanes' sums in LDS
+  // in wave order, and wave 0 finishes the points.
+  const bool split = (int)blockIdx.x < pa.nsplit;
+  const int chunk = split ? (int)blockIdx.x : pa.nsplit + ((int)blockIdx.x - pa.nsplit) * kChunksPerBlock + wave;
+  const int i = chunk * kWave + lane;
+  const bool live = chunk < pa.nchunks && i < pa.np;
   double uu = 0.0, n2 = 0.0, n2s = 0.0, gd = 0.0, g2 = 0.0, g2s = 0.0;  // verify partials
-  if (chunk < pa.nchunks) {
-    const int i = chunk * kWave + lane;
-    if (i < pa.np) {
-      double w[3] = {0.0, 0.0, 0.0};
-      double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      double dpx[3] = {0.0, 0.0, 0.0};
-      if (MODE == kVerify || MODE == kJv || (MODE == kLsmr && fw != nullptr)) {
-        dpx[0] = delta_p_in[3 * i];
-        dpx[1] = delta_p_in[3 * i + 1];
-        dpx[2] = delta_p_in[3 * i + 2];
+  double w[3] = {0.0, 0.0, 0.0};
+  double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double gr[3] = {0.0, 0.0, 0.0};  // kCheck: sum Jp^T r
+  double dpx[3] = {0.0, 0.0, 0.0};
+  if (live) {
+    if (MODE == kVerify || MODE == kJv || (MODE == kLsmr && fw != nullptr)) {
+      dpx[0] = delta_p_in[3 * i];
+      dpx[1] = delta_p_in[3 * i + 1];
+      dpx[2] = delta_p_in[3 * i + 2];
+    }
+    if (MODE != kRhs) {
+      const int len = pa.chunk_len[chunk];
+      const int j0 = split ? len * wave / kChunksPerBlock : 0;
+      const int j1 = split ? len * (wave + 1) / kChunksPerBlock : len;
+      // kCheck's one sweep: w = sum Jp^T Jc z and Hpp re-formed as kSchur, plus sum ||Jc z||^2
+      // (into uu) and sum Jp^T r
+      point_sweep<T, MODE == kCheck ? kSchurSums : MODE>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in,
+                                                          rows_out, w, h, uu, coef, gr);
+    }
+  }
+  // sums a split chunk's waves must combine before the points are finished (uu is additive
+  // over the block and goes to the block sum as it is)
+  constexpr int kMeet = MODE == kCheck ? 12 : MODE == kSchur ? 9 : (MODE == kRhs || MODE == kJv) ? 0 : 3;
+  if (kMeet > 0 && split) {
+    __shared__ double meet[kMeet > 0 ? kMeet * kWave : 1];
+    double v[12];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = w[a];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) v[3 + a] = h[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[9 + a] = gr[a];
+    for (int src = 1; src < kChunksPerBlock; ++src) {
+      __syncthreads();
+      if (wave == src) {
+#pragma unroll
+        for (int a = 0; a < kMeet; ++a) meet[a * kWave + lane] = v[a];
       }
-      if (MODE == kCheck) {
-        // One sweep: w = sum Jp^T Jc z and Hpp re-formed as kSchur (t_p), delta_p as kRecover,
-        // plus sum ||Jc z||^2 and sum Jp^T r, from which the verification of
-        // delta = (z / D, delta_p) follows without a second sweep:
-        //   ||J delta||^2 over the point's rows = sum ||Jc z||^2 + 2 delta_p.w + delta_p^T Hpp delta_p
-        //   Jp^T (J delta + r)                  = w + Hpp delta_p + sum Jp^T r
-        double gr[3] = {0.0, 0.0, 0.0};
-        point_sweep<T, kSchurSums>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
-        double hu[6];
+      __syncthreads();
+      if (wave == 0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) hu[k] = h[k];  // undamped Hpp
-        double t[3] = {w[0], w[1], w[2]};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const double d = double(dpv[3 * i + a]);
-          h[sym3(a, a)] += lambda * d * d;
-        }
-        if (chol_packed<double, 3>(h)) {
-          solve_lower_packed<double, 3>(h, t);
-          solve_upper_packed<double, 3>(h, t);
-        } else {
-          t[0] = t[1] = t[2] = 0.0;
-        }
-        store_tp<T>(t_out, i, t);
-        const double v[3] = {double(gp[3 * i]) + w[0], double(gp[3 * i + 1]) + w[1], double(gp[3 * i + 2]) + w[2]};
-        point_solve(mp, dpv, i, v, t);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          dpx[a] = -t[a];
-          delta_out[3 * i + a] = dpx[a];
-        }
-        double hd[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-          hd[a] = hu[sym3(a, 0)] * dpx[0] + hu[sym3(a, 1)] * dpx[1] + hu[sym3(a, 2)] * dpx[2];
-        uu += 2.0 * (dpx[0] * w[0] + dpx[1] * w[1] + dpx[2] * w[2]) +
-              (dpx[0] * hd[0] + dpx[1] * hd[1] + dpx[2] * hd[2]);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const double d = double(dpv[3 * i + a]);
-          const double nr = (w[a] + hd[a] + gr[a]) + (lambda * (d * dpx[a])) * d;
-          n2 += nr * nr;
-          n2s += (nr / d) * (nr / d);
-          const double g = double(gp[3 * i + a]);
-          gd += g * dpx[a];
-          g2 += g * g;
-          g2s += (g / d) * (g / d);
-        }
-      } else if (MODE != kRhs) {
-        double gr[3] = {0.0, 0.0, 0.0};  // kSchurSums only
-        point_sweep<T, MODE>(pa, chunk, lane, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef, gr);
+        for (int a = 0; a < kMeet; ++a) v[a] += meet[a * kWave + lane];
       }
-      if (MODE == kSchur) {
-        // Hpp + lambda Dp^2 re-formed from the same rows and factored in registers: the
-        // eliminated projection is then exact for the Jp of this pass (the stored factor of
-        // the same block, scaled and rounded differently, slowed tight PCG solves 2x)
-        double t[3] = {w[0], w[1], w[2]};
+    }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const double d = double(dpv[3 * i + a]);
-          h[sym3(a, a)] += lambda * d * d;
-        }
-        if (chol_packed<double, 3>(h)) {
-          solve_lower_packed<double, 3>(h, t);
-          solve_upper_packed<double, 3>(h, t);
-        } else {
-          t[0] = t[1] = t[2] = 0.0;  // unobserved point at lambda = 0: no elimination term
-        }
+    for (int a = 0; a < 3; ++a) w[a] = v[a];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) h[a] = v[3 + a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gr[a] = v[9 + a];
+  }
+  if (live && (!split || wave == 0)) {
+    if (MODE == kCheck) {
+      // The verification of delta = (z / D, delta_p) follows from the sweep's sums without
+      // a second sweep:
+      //   ||J delta||^2 over the point's rows = sum ||Jc z||^2 + 2 delta_p.w + delta_p^T Hpp delta_p
+      //   Jp^T (J delta + r)                  = w + Hpp delta_p + sum Jp^T r
+      double hu[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) hu[k] = h[k];  // undamped Hpp
+      double t[3] = {w[0], w[1], w[2]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double d = double(dpv[3 * i + a]);
+        h[sym3(a, a)] += lambda * d * d;
+      }
+      if (chol_packed<double, 3>(h)) {
+        solve_lower_packed<double, 3>(h, t);
+        solve_upper_packed<double, 3>(h, t);
+      } else {
+        t[0] = t[1] = t[2] = 0.0;
+      }
+      store_tp<T>(t_out, i, t);
+      const double v[3] = {double(gp[3 * i]) + w[0], double(gp[3 * i + 1]) + w[1], double(gp[3 * i + 2]) + w[2]};
+      point_solve(mp, dpv, i, v, t);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        dpx[a] = -t[a];
+        delta_out[3 * i + a] = dpx[a];
+      }
+      double hd[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        hd[a] = hu[sym3(a, 0)] * dpx[0] + hu[sym3(a, 1)] * dpx[1] + hu[sym3(a, 2)] * dpx[2];
+      uu += 2.0 * (dpx[0] * w[0] + dpx[1] * w[1] + dpx[2] * w[2]) +
+            (dpx[0] * hd[0] + dpx[1] * hd[1] + dpx[2] * hd[2]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double d = double(dpv[3 * i + a]);
+        const double nr = (w[a] + hd[a] + gr[a]) + (lambda * (d * dpx[a])) * d;
+        n2 += nr * nr;
+        n2s += (nr / d) * (nr / d);
+        const double g = double(gp[3 * i + a]);
+        gd += g * dpx[a];
+        g2 += g * g;
+        g2s += (g / d) * (g / d);
+      }
+    } else if (MODE == kSchur) {
+      // Hpp + lambda Dp^2 re-formed from the same rows and factored in registers: the
+      // eliminated projection is then exact for the Jp of this pass (the stored factor of
+      // the same block, scaled and rounded differently, slowed tight PCG solves 2x)
+      double t[3] = {w[0], w[1], w[2]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double d = double(dpv[3 * i + a]);
+        h[sym3(a, a)] += lambda * d * d;
+      }
+      if (chol_packed<double, 3>(h)) {
+        solve_lower_packed<double, 3>(h, t);
+        solve_upper_packed<double, 3>(h, t);
+      } else {
+        t[0] = t[1] = t[2] = 0.0;  // unobserved point at lambda = 0: no elimination term
+      }
+      store_tp<T>(t_out, i, t);
+    } else if (MODE == kRhs || MODE == kRecover) {
+      double v[3];
+      if (MODE == kRhs) {
+        v[0] = double(gp[3 * i]); v[1] = double(gp[3 * i + 1]); v[2] = double(gp[3 * i + 2]);
+      } else {
+        v[0] = double(gp[3 * i]) + w[0]; v[1] = double(gp[3 * i + 1]) + w[1]; v[2] = double(gp[3 * i + 2]) + w[2];
+      }
+      double t[3];
+      point_solve(mp, dpv, i, v, t);
+      if (MODE == kRecover) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -t[a];
+      } else {
         store_tp<T>(t_out, i, t);
-      } else if (MODE == kRhs || MODE == kRecover) {
-        double v[3];
-        if (MODE == kRhs) {
-          v[0] = double(gp[3 * i]); v[1] = double(gp[3 * i + 1]); v[2] = double(gp[3 * i + 2]);
-        } else {
-          v[0] = double(gp[3 * i]) + w[0]; v[1] = double(gp[3 * i + 1]) + w[1]; v[2] = double(gp[3 * i + 2]) + w[2];
-        }
-        double t[3];
-        point_solve(mp, dpv, i, v, t);
-        if (MODE == kRecover) {
+      }
+    } else if (MODE == kVerify) {
 #pragma unroll
-          for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = -t[a];
-        } else {
-          store_tp<T>(t_out, i, t);
-        }
-      } else if (MODE == kVerify) {
+      for (int a = 0; a < 3; ++a) {
+        const double d = double(dpv[3 * i + a]);
+        const double nr = w[a] + (lambda * (d * dpx[a])) * d;
+        if (delta_out) delta_out[3 * i + a] = nr;  // the normal residual itself, on request
+        n2 += nr * nr;
+        n2s += (nr / d) * (nr / d);
+        const double g = double(gp[3 * i + a]);
+        gd += g * delta_p_in[3 * i + a];
+        g2 += g * g;
+        g2s += (g / d) * (g / d);
+      }
+    } else if (MODE == kJtu) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const double d = double(dpv[3 * i + a]);
-          const double nr = w[a] + (lambda * (d * dpx[a])) * d;
-          if (delta_out) delta_out[3 * i + a] = nr;  // the normal residual itself, on request
-          n2 += nr * nr;
-          n2s += (nr / d) * (nr / d);
-          const double g = double(gp[3 * i + a]);
-          gd += g * delta_p_in[3 * i + a];
-          g2 += g * g;
-          g2s += (g / d) * (g / d);
-        }
-      } else if (MODE == kJtu) {
+      for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = w[a];
+    } else if (MODE == kLsmr) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) delta_out[3 * i + a] = w[a];
-      } else if (MODE == kLsmr) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          delta_out[3 * i + a] = w[a];
-          if (t_out != nullptr && fw != nullptr) {  // damping rows of this point
-            const double ub = root * double(dpv[3 * i + a]) * dpx[a] - coef * t_out[3 * i + a];
-            t_out[3 * i + a] = ub;
-            uu += ub * ub;
-          }
+      for (int a = 0; a < 3; ++a) {
+        delta_out[3 * i + a] = w[a];
+        if (t_out != nullptr && fw != nullptr) {  // damping rows of this point
+          const double ub = root * double(dpv[3 * i + a]) * dpx[a] - coef * t_out[3 * i + a];
+          t_out[3 * i + a] = ub;
+          uu += ub * ub;
         }
       }
     }

@@ -271,3 +271,46 @@ def test_camera_major_records_equal_point_major_records(precision):
         ba.set_state(cams + 1e-3 * rng.normal(size=cams.shape), pts + 1e-2 * rng.normal(size=pts.shape))
         assert s.linearize() == fa.OK
         assert s.record_mismatches() == 0
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_split_chunks_match_oracle(precision, monkeypatch):
+    """Every chunk swept by the four waves of one workgroup (FAGRA_SPLIT_LEN=2, read when the
+    problem is uploaded): the reduced product, the Jacobian products, a damped Schur solve and
+    an LSMR solve still match the oracle, and the reduced product matches the unsplit mapping
+    of the same problem."""
+    p = small(seed=33, nc=10, npts=700, huber=1.0)
+    monkeypatch.setenv("FAGRA_SPLIT_LEN", "0")
+    _, _, _, s_plain = linearized_pair(p, precision)
+    monkeypatch.setenv("FAGRA_SPLIT_LEN", "2")
+    ba_o, s_o, ba_g, s_g = linearized_pair(p, precision)
+    rng = np.random.default_rng(5)
+    for lam in [1e-3, 1.0]:
+        x = rng.normal(size=9 * p.n_cameras)
+        st_o, y_o = s_o.reduced_apply(lam, x)
+        st_g, y_g = s_g.reduced_apply(lam, x)
+        st_p, y_p = s_plain.reduced_apply(lam, x)
+        assert st_o == st_g == st_p == 0
+        assert rel(y_g, y_o) < 5 * PROD_TOL[precision], lam
+        assert rel(y_g, y_p) < PROD_TOL[precision], lam
+    n, rows = 9 * p.n_cameras + 3 * p.n_points, 2 * p.n_observations
+    x, y = rng.normal(size=n), rng.normal(size=rows)
+    assert rel(s_g.jacobian_apply(x), s_o.jacobian_apply(x)) < PROD_TOL[precision]
+    assert rel(s_g.jacobian_transpose_apply(y), s_o.jacobian_transpose_apply(y)) < PROD_TOL[precision]
+    tol = 1e-10 if precision == fa.FP64 else 1e-6
+    s_o.relative_tolerance = tol
+    s_g.relative_tolerance = tol
+    bound = 1e-8 if precision == fa.FP64 else 3e-3
+    for lam in [1e-3, 1.0]:
+        st_o, d_o, pr_o = s_o.solve_damped(lam)
+        st_g, d_g, pr_g = s_g.solve_damped(lam)
+        assert st_o == st_g == 0, (lam, s_g.diagnostics().termination)
+        assert rel(d_g, d_o) < bound, lam
+        assert abs(pr_g - pr_o) <= bound * abs(pr_o), lam
+    l_g = fa.Lsmr(precision=precision, diagonal_preconditioning=True, block_preconditioning=True,
+                  relative_tolerance=1e-12 if precision == fa.FP64 else 1e-8, max_iterations=2000)
+    assert l_g.prepare(ba_g) == fa.OK and l_g.linearize() == fa.OK and l_g.prepare_damping(1e-6) == fa.OK
+    st, d_l, pr_l = l_g.solve_damped(1.0)
+    st_o, d_o, pr_o = s_o.solve_damped(1.0)
+    assert st == fa.OK and st_o == 0
+    assert rel(d_l, d_o) < (1e-6 if precision == fa.FP64 else 3e-3)
