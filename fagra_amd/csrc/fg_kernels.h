@@ -891,19 +891,21 @@ __device__ __forceinline__ Vec4<double> load_tp(const double* tp, int i) {
   return Vec4<double>{double(v.x), double(v.y), double(v.z), 0.0};
 }
 
-// One observation of a point-major pass: record, camera table and forward table in registers.
+// The point-major modes that apply a camera vector read the cameras' product rows.
+template <int MODE>
+constexpr bool point_fwd_mode() {
+  return MODE == kSchur || MODE == kSchurSums || MODE == kRecover || MODE == kVerify || MODE == kJv ||
+         MODE == kLsmr;
+}
+
+// One observation of a point-major pass with its camera's table (and, with fwd, product row)
+// already in registers.
 template <typename T, int MODE>
-__device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& rec,
-                                          const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
-                                          const T* __restrict__ fw, const double dpx[3],
-                                          const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                                          double w[3], double h[6], double& uu, double coef, double gr[3]) {
-  // fw is the cameras' product rows in the modes that apply a camera vector
-  constexpr bool kFwdModes = MODE == kSchur || MODE == kSchurSums || MODE == kRecover || MODE == kVerify ||
-                             MODE == kJv || MODE == kLsmr;
-  double ct[16], f[kFwd];
-  if (kFwdModes && fw != nullptr) load_prow_d(fw, c, ct, f);
-  else load_cam16_d(tab, c, ct);
+__device__ __forceinline__ void point_obs_row(const PointArgs& pa, int s, const Vec4<T>& rec,
+                                              const Vec2<T>* __restrict__ res, const double ct[16],
+                                              const double f[kFwd], bool fwd, const double dpx[3],
+                                              const double* __restrict__ rows_in, double* __restrict__ rows_out,
+                                              double w[3], double h[6], double& uu, double coef, double gr[3]) {
   ObsGeom<double> geo;
   obs_geom(to_d(rec), ct, geo);
   if (MODE == kSchur || MODE == kSchurSums) {
@@ -952,7 +954,7 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
   } else if (MODE == kLsmr) {
     const Vec2<double> old = reinterpret_cast<const Vec2<double>*>(rows_in)[s];
     double u0 = old.x, u1 = old.y;
-    if (fw != nullptr) {
+    if (fwd) {
       double v0, v1, x0, x1;
       jc_apply(geo, f, v0, v1);
       jp_apply(geo, ct, dpx, x0, x1);
@@ -965,15 +967,91 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
   }
 }
 
-// One lane's observations of a point-major pass (SELL chunk, two observations per step so
-// that both loads issue before either is used).
+// One observation of a point-major pass: record, camera table and forward table in registers.
 template <typename T, int MODE>
+__device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, const Vec4<T>& rec,
+                                          const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
+                                          const T* __restrict__ fw, const double dpx[3],
+                                          const double* __restrict__ rows_in, double* __restrict__ rows_out,
+                                          double w[3], double h[6], double& uu, double coef, double gr[3]) {
+  double ct[16], f[kFwd];
+  if (point_fwd_mode<MODE>() && fw != nullptr) load_prow_d(fw, c, ct, f);
+  else load_cam16_d(tab, c, ct);
+  point_obs_row<T, MODE>(pa, s, rec, res, ct, f, fw != nullptr, dpx, rows_in, rows_out, w, h, uu, coef, gr);
+}
+
+// Cooperative product-row gather for FP32 rows (24 floats = 6 Vec4 per camera). A lane-private
+// gather makes every 16-byte load instruction touch up to 64 cache lines, one per lane, and
+// the per-line lookups bound the point pass. Here the wave's 64 rows are fetched as 384
+// consecutive 16-byte pieces: in round k (0..5) lane m fetches piece P = 64k + m, i.e. piece
+// P % 6 of the row of lane P / 6, so one instruction covers the lines of about 11 rows. The
+// rows meet in the wave's LDS slice (row stride 7 Vec4 = 112 bytes, so the 16-byte reads of
+// 64 consecutive rows fall on distinct banks in every group of 8 lanes) and each lane reads
+// its own. Every lane of the wave must call both (lane shuffles); c < 0 rows are not fetched.
+constexpr int kCoopStride = 7;
+__device__ __forceinline__ void coop_fetch(const float* __restrict__ prow, int c, int lane, Vec4<float> v[6]) {
+  const Vec4<float>* r4 = reinterpret_cast<const Vec4<float>*>(prow);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / 6;
+    const int cr = __shfl(c, r, kWave);
+    v[k] = Vec4<float>{0.f, 0.f, 0.f, 0.f};
+    if (cr >= 0) v[k] = r4[(kPRow / 4) * cr + (piece - 6 * r)];
+  }
+}
+__device__ __forceinline__ void coop_exchange(const Vec4<float> v[6], int lane, Vec4<float>* __restrict__ lds,
+                                              double ct[16], double f[kFwd]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / 6;
+    lds[r * kCoopStride + (piece - 6 * r)] = v[k];
+  }
+  __builtin_amdgcn_wave_barrier();
+  Vec4<float> u[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) u[k] = lds[lane * kCoopStride + k];
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ct[4 * k] = double(u[k].x); ct[4 * k + 1] = double(u[k].y); ct[4 * k + 2] = double(u[k].z);
+    ct[4 * k + 3] = double(u[k].w);
+  }
+  ct[12] = double(u[3].x); ct[13] = double(u[3].y); ct[14] = double(u[3].z); ct[15] = 0.0;
+  f[0] = double(u[3].w);
+  f[1] = double(u[4].x); f[2] = double(u[4].y); f[3] = double(u[4].z); f[4] = double(u[4].w);
+  f[5] = double(u[5].x); f[6] = double(u[5].y); f[7] = double(u[5].z); f[8] = double(u[5].w);
+  f[9] = f[10] = f[11] = 0.0;
+}
+
+// One lane's observations of a point-major pass (SELL chunk, two observations per step so
+// that both loads issue before either is used). With COOP (FP32 product rows) the rows come
+// through coop_fetch / coop_exchange, and every lane of the wave must take part.
+template <typename T, int MODE, bool COOP = false>
 __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int lane, int j0, int j1,
                                             const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res,
                                             const T* __restrict__ tab, const T* __restrict__ fw, const double dpx[3],
                                             const double* __restrict__ rows_in, double* __restrict__ rows_out,
-                                            double w[3], double h[6], double& uu, double coef, double gr[3]) {
+                                            double w[3], double h[6], double& uu, double coef, double gr[3],
+                                            Vec4<float>* __restrict__ coop = nullptr) {
   const int base = pa.chunk_base[chunk];
+  if constexpr (COOP) {
+    const float* pr = reinterpret_cast<const float*>(fw);
+    // one observation per step: two in flight cost 40 more VGPRs and a wave per SIMD, and
+    // measured slower
+    for (int j = j0; j < j1; ++j) {
+      const int s0 = base + j * kWave + lane;
+      const int c0 = pa.slot_cam[s0];
+      const Vec4<T> r0 = rec[s0];
+      Vec4<float> v0[6];
+      coop_fetch(pr, c0, lane, v0);
+      double ct[16], f[kFwd];
+      coop_exchange(v0, lane, coop, ct, f);
+      if (c0 >= 0) point_obs_row<T, MODE>(pa, s0, r0, res, ct, f, true, dpx, rows_in, rows_out, w, h, uu, coef, gr);
+    }
+    return;
+  }
   int j = j0;
   for (; j + 1 < j1; j += 2) {
     const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
@@ -1019,20 +1097,28 @@ __global__ void __launch_bounds__(kBlock) k_points(
   double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double gr[3] = {0.0, 0.0, 0.0};  // kCheck: sum Jp^T r
   double dpx[3] = {0.0, 0.0, 0.0};
-  if (live) {
-    if (MODE == kVerify || MODE == kJv || (MODE == kLsmr && fw != nullptr)) {
-      dpx[0] = delta_p_in[3 * i];
-      dpx[1] = delta_p_in[3 * i + 1];
-      dpx[2] = delta_p_in[3 * i + 2];
-    }
-    if (MODE != kRhs) {
-      const int len = pa.chunk_len[chunk];
-      const int j0 = split ? len * wave / kChunksPerBlock : 0;
-      const int j1 = split ? len * (wave + 1) / kChunksPerBlock : len;
-      // kCheck's one sweep: w = sum Jp^T Jc z and Hpp re-formed as kSchur, plus sum ||Jc z||^2
-      // (into uu) and sum Jp^T r
-      point_sweep<T, MODE == kCheck ? kSchurSums : MODE>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in,
-                                                          rows_out, w, h, uu, coef, gr);
+  if (live && (MODE == kVerify || MODE == kJv || (MODE == kLsmr && fw != nullptr))) {
+    dpx[0] = delta_p_in[3 * i];
+    dpx[1] = delta_p_in[3 * i + 1];
+    dpx[2] = delta_p_in[3 * i + 2];
+  }
+  // kCheck's one sweep: w = sum Jp^T Jc z and Hpp re-formed as kSchur, plus sum ||Jc z||^2
+  // (into uu) and sum Jp^T r
+  constexpr int kSweep = MODE == kCheck ? kSchurSums : MODE;
+  // FP32 product rows are gathered cooperatively by the whole wave (lanes past np sweep their
+  // chunk's empty slots, whose camera is -1)
+  constexpr bool kCoop = sizeof(T) == 4 && point_fwd_mode<kSweep>();
+  __shared__ Vec4<float> coop[kCoop ? kChunksPerBlock * kWave * kCoopStride : 1];
+  if (MODE != kRhs && chunk < pa.nchunks) {
+    const int len = pa.chunk_len[chunk];
+    const int j0 = split ? len * wave / kChunksPerBlock : 0;
+    const int j1 = split ? len * (wave + 1) / kChunksPerBlock : len;
+    if (kCoop && fw != nullptr) {
+      point_sweep<T, kSweep, kCoop>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu,
+                                    coef, gr, coop + wave * kWave * kCoopStride);
+    } else if (live) {
+      point_sweep<T, kSweep>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef,
+                             gr);
     }
   }
   // sums a split chunk's waves must combine before the points are finished (uu is additive
