@@ -522,17 +522,8 @@ __global__ void __launch_bounds__(kBlock) k_linearize_cameras(
 #pragma unroll
     for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
   };
-  int k = k0 + threadIdx.x;
-  // two rows per step: both points and pixels load before either row is used
-  for (; k + kBlock < k1; k += 2 * kBlock) {
-    const int pa = k_pslot[k], pb = k_pslot[k + kBlock];
-    const double xa[3] = {pts[3 * pa], pts[3 * pa + 1], pts[3 * pa + 2]};
-    const double xb[3] = {pts[3 * pb], pts[3 * pb + 1], pts[3 * pb + 2]};
-    const Vec2<double> oa = cam_pix[k], ob = cam_pix[k + kBlock];
-    one(k, xa, oa);
-    one(k + kBlock, xb, ob);
-  }
-  if (k < k1) {
+  // one row per step: two in flight held 171 (FP32) / 256 (FP64) VGPRs, 2 / 1 waves per SIMD
+  for (int k = k0 + threadIdx.x; k < k1; k += kBlock) {
     const int pa = k_pslot[k];
     const double xa[3] = {pts[3 * pa], pts[3 * pa + 1], pts[3 * pa + 2]};
     one(k, xa, cam_pix[k]);
@@ -1106,8 +1097,9 @@ __global__ void __launch_bounds__(kBlock) k_points(
   // (into uu) and sum Jp^T r
   constexpr int kSweep = MODE == kCheck ? kSchurSums : MODE;
   // FP32 product rows are gathered cooperatively by the whole wave (lanes past np sweep their
-  // chunk's empty slots, whose camera is -1)
-  constexpr bool kCoop = sizeof(T) == 4 && point_fwd_mode<kSweep>();
+  // chunk's empty slots, whose camera is -1); not in kCheck, whose larger sweep went from 96 to
+  // 132 VGPRs with it and ran slower (63.8 against 56.2 us per launch)
+  constexpr bool kCoop = sizeof(T) == 4 && point_fwd_mode<kSweep>() && MODE != kCheck;
   __shared__ Vec4<float> coop[kCoop ? kChunksPerBlock * kWave * kCoopStride : 1];
   if (MODE != kRhs && chunk < pa.nchunks) {
     const int len = pa.chunk_len[chunk];
