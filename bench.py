@@ -31,6 +31,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "s to LM converge @1e-5 rel cost, 1.5k-cam BA: learned+FP32 vs plain f64 PCG"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the Schur-product kernels on rings1500 from the hardware-counter
+# runs (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over bench.py --profile-products), scaled
+# by what the same counters read for a 256 MiB device copy: bytes = 2.0 x FETCH_SIZE KiB x 1024
+# + 1.0 x WRITE_SIZE KiB x 1024. Counters cannot be read in the timed run itself, so the
+# roofline's "traffic" carries these figures for the workload they were measured on.
+MEASURED_TRAFFIC = {
+    ("rings1500", "k_points<float, kSchur>"): (2 * 35547.9 + 7812.5) * 1024,
+    ("rings1500", "k_cameras<float, cSchur>"): (2 * 71510.9 + 187.5) * 1024,
+    ("rings1500", "k_points<double, kSchur>"): (2 * 64440.1 + 15625.0) * 1024,
+    ("rings1500", "k_cameras<double, cSchur>"): (2 * 192539.2 + 194.5) * 1024,
+}
 
 
 def log(*a):
@@ -346,7 +357,7 @@ def main():
         kname, kbytes, ksec = "k_cameras<%s, cSchur>" % ("float" if prec_name == "fp32" else "double"), b_cams, t_cams
     achieved = kbytes / ksec / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((args.workload, kname)), "kernel": kname,
                 "bytes_per_launch": kbytes, "avg_launch_us": ksec * 1e6, "launches_timed": args.kernel_repeats,
                 "schur_product": {"k_points_us": t_pts * 1e6, "k_cameras_us": t_cams * 1e6,
                                   "bytes": b_pts + b_cams,
