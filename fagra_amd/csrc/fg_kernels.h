@@ -414,50 +414,74 @@ __device__ __forceinline__ bool linearize_obs(const double cd[16], double x0, do
   return ok;
 }
 
+// Cooperative row gather. A lane-private gather of a per-camera row makes every 16-byte load
+// instruction touch up to 64 cache lines, one per lane, and those per-line lookups bound the
+// point-major passes. Here a wave fetches the 64 rows it needs (R 16-byte pieces each, row c
+// starting at piece stride * c) as 64 R consecutive pieces: in round k (0 .. R-1) lane m fetches
+// piece P = 64k + m, i.e. piece P % R of the row of lane P / R (whose camera a lane shuffle
+// provides), so one instruction covers the lines of about 64 / R rows. The rows meet in the
+// wave's LDS slice (row stride R + 1 pieces, so the 16-byte reads of consecutive rows fall on
+// distinct banks in every group of 8 lanes) and each lane reads its own. Every lane of the
+// wave must call both halves (lane shuffles); c < 0 rows are not fetched. Pure data movement:
+// the values are bit-identical to a lane-private load.
+template <int R>
+__device__ __forceinline__ void coop_fetch(const Vec4<float>* __restrict__ rows, int stride, int c, int lane,
+                                           Vec4<float> v[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / R;
+    const int cr = __shfl(c, r, kWave);
+    v[k] = Vec4<float>{0.f, 0.f, 0.f, 0.f};
+    if (cr >= 0) v[k] = rows[stride * cr + (piece - R * r)];
+  }
+}
+template <int R>
+__device__ __forceinline__ void coop_exchange(const Vec4<float> v[R], int lane, Vec4<float>* __restrict__ lds,
+                                              Vec4<float> u[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / R;
+    lds[r * (R + 1) + (piece - R * r)] = v[k];
+  }
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < R; ++k) u[k] = lds[lane * (R + 1) + k];
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_linearize_points(
     int nchunks, int np, const int* __restrict__ chunk_base, const int* __restrict__ chunk_len,
     const int* __restrict__ slot_cam, const Vec2<double>* __restrict__ slot_pix, const double* __restrict__ tab64,
     const double* __restrict__ pts, double huber, Vec4<T>* __restrict__ rec, Vec2<T>* __restrict__ res,
     double* __restrict__ hpp, T* __restrict__ gp, int* flags) {
-  const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
+  // the f64 camera tables (first 16 of 32 scalars: 8 pieces of a 16-piece row) are gathered
+  // cooperatively by the wave, so every lane of a live chunk sweeps it (lanes past np read
+  // their empty slots, whose camera is -1)
+  __shared__ Vec4<float> rows[kChunksPerBlock * kWave * 9];
+  const int wave = threadIdx.x / kWave;
+  const int chunk = blockIdx.x * kChunksPerBlock + wave;
   const int lane = threadIdx.x % kWave;
-  if (chunk >= nchunks) return;
+  if (chunk >= nchunks) return;  // uniform over the wave
   const int i = chunk * kWave + lane;
-  if (i >= np) return;
-  const double x0 = pts[3 * i], x1 = pts[3 * i + 1], x2 = pts[3 * i + 2];
+  const bool live = i < np;
+  const double x0 = live ? pts[3 * i] : 0.0, x1 = live ? pts[3 * i + 1] : 0.0, x2 = live ? pts[3 * i + 2] : 0.0;
   const int base = chunk_base[chunk], len = chunk_len[chunk];
+  Vec4<float>* lds = rows + wave * kWave * 9;
   double h[6] = {0, 0, 0, 0, 0, 0};
   double g[3] = {0, 0, 0};
   bool bad = false;
-  int j = 0;
-  // two observations per step: both tables and pixels load before either is used
-  for (; j + 1 < len; j += 2) {
-    const int s0 = base + j * kWave + lane, s1 = s0 + kWave;
-    const int c0 = slot_cam[s0], c1 = slot_cam[s1];
-    double cd0[16], cd1[16];
-    if (c0 >= 0) load_cam16(tab64, c0, cd0);
-    if (c1 >= 0) load_cam16(tab64, c1, cd1);
-    const Vec2<double> ob0 = slot_pix[s0], ob1 = slot_pix[s1];
-    Vec4<T> r4;
-    Vec2<T> rw;
-    if (c0 >= 0) {
-      bad |= !linearize_obs<T>(cd0, x0, x1, x2, ob0, huber, r4, rw, h, g);
-      rec[s0] = r4;
-      res[s0] = rw;
-    }
-    if (c1 >= 0) {
-      bad |= !linearize_obs<T>(cd1, x0, x1, x2, ob1, huber, r4, rw, h, g);
-      rec[s1] = r4;
-      res[s1] = rw;
-    }
-  }
-  if (j < len) {
+  for (int j = 0; j < len; ++j) {
     const int s0 = base + j * kWave + lane;
     const int c0 = slot_cam[s0];
+    Vec4<float> v[8], u[8];
+    coop_fetch<8>(reinterpret_cast<const Vec4<float>*>(tab64), kCamTab / 2, c0, lane, v);
+    coop_exchange<8>(v, lane, lds, u);
     if (c0 >= 0) {
       double cd0[16];
-      load_cam16(tab64, c0, cd0);
+      __builtin_memcpy(cd0, u, sizeof(cd0));
       Vec4<T> r4;
       Vec2<T> rw;
       bad |= !linearize_obs<T>(cd0, x0, x1, x2, slot_pix[s0], huber, r4, rw, h, g);
@@ -465,6 +489,7 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
       res[s0] = rw;
     }
   }
+  if (!live) return;
   if (bad) atomicOr(flags, 1);
 #pragma unroll
   for (int k = 0; k < 6; ++k) hpp[6 * i + k] = h[k];
@@ -971,39 +996,14 @@ __device__ __forceinline__ void point_obs(const PointArgs& pa, int s, int c, con
   point_obs_row<T, MODE>(pa, s, rec, res, ct, f, fw != nullptr, dpx, rows_in, rows_out, w, h, uu, coef, gr);
 }
 
-// Cooperative product-row gather for FP32 rows (24 floats = 6 Vec4 per camera). A lane-private
-// gather makes every 16-byte load instruction touch up to 64 cache lines, one per lane, and
-// the per-line lookups bound the point pass. Here the wave's 64 rows are fetched as 384
-// consecutive 16-byte pieces: in round k (0..5) lane m fetches piece P = 64k + m, i.e. piece
-// P % 6 of the row of lane P / 6, so one instruction covers the lines of about 11 rows. The
-// rows meet in the wave's LDS slice (row stride 7 Vec4 = 112 bytes, so the 16-byte reads of
-// 64 consecutive rows fall on distinct banks in every group of 8 lanes) and each lane reads
-// its own. Every lane of the wave must call both (lane shuffles); c < 0 rows are not fetched.
-constexpr int kCoopStride = 7;
-__device__ __forceinline__ void coop_fetch(const float* __restrict__ prow, int c, int lane, Vec4<float> v[6]) {
-  const Vec4<float>* r4 = reinterpret_cast<const Vec4<float>*>(prow);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int piece = kWave * k + lane;
-    const int r = piece / 6;
-    const int cr = __shfl(c, r, kWave);
-    v[k] = Vec4<float>{0.f, 0.f, 0.f, 0.f};
-    if (cr >= 0) v[k] = r4[(kPRow / 4) * cr + (piece - 6 * r)];
-  }
-}
-__device__ __forceinline__ void coop_exchange(const Vec4<float> v[6], int lane, Vec4<float>* __restrict__ lds,
-                                              double ct[16], double f[kFwd]) {
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int piece = kWave * k + lane;
-    const int r = piece / 6;
-    lds[r * kCoopStride + (piece - 6 * r)] = v[k];
-  }
-  __builtin_amdgcn_wave_barrier();
-  Vec4<float> u[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) u[k] = lds[lane * kCoopStride + k];
-  __builtin_amdgcn_wave_barrier();
+// The FP32 product row (24 floats = 6 pieces) of lane's camera c, through coop_fetch /
+// coop_exchange, widened to f64.
+constexpr int kCoopStride = 7;  // LDS Vec4 per row for R = 6
+__device__ __forceinline__ void coop_prow(const float* __restrict__ prow, int c, int lane,
+                                          Vec4<float>* __restrict__ lds, double ct[16], double f[kFwd]) {
+  Vec4<float> v[6], u[6];
+  coop_fetch<6>(reinterpret_cast<const Vec4<float>*>(prow), kPRow / 4, c, lane, v);
+  coop_exchange<6>(v, lane, lds, u);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     ct[4 * k] = double(u[k].x); ct[4 * k + 1] = double(u[k].y); ct[4 * k + 2] = double(u[k].z);
@@ -1018,7 +1018,7 @@ __device__ __forceinline__ void coop_exchange(const Vec4<float> v[6], int lane, 
 
 // One lane's observations of a point-major pass (SELL chunk, two observations per step so
 // that both loads issue before either is used). With COOP (FP32 product rows) the rows come
-// through coop_fetch / coop_exchange, and every lane of the wave must take part.
+// through coop_prow, and every lane of the wave must take part.
 template <typename T, int MODE, bool COOP = false>
 __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int lane, int j0, int j1,
                                             const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res,
@@ -1035,10 +1035,8 @@ __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int 
       const int s0 = base + j * kWave + lane;
       const int c0 = pa.slot_cam[s0];
       const Vec4<T> r0 = rec[s0];
-      Vec4<float> v0[6];
-      coop_fetch(pr, c0, lane, v0);
       double ct[16], f[kFwd];
-      coop_exchange(v0, lane, coop, ct, f);
+      coop_prow(pr, c0, lane, coop, ct, f);
       if (c0 >= 0) point_obs_row<T, MODE>(pa, s0, r0, res, ct, f, true, dpx, rows_in, rows_out, w, h, uu, coef, gr);
     }
     return;
