@@ -272,6 +272,43 @@ __device__ __forceinline__ bool project(const T* __restrict__ tab, T x0, T x1, T
   return p2 != T(0) && isfinite(px) && isfinite(py);
 }
 
+// Cooperative row gather. A lane-private gather of a per-camera row makes every 16-byte load
+// instruction touch up to 64 cache lines, one per lane, and those per-line lookups bound the
+// point-major passes. Here a wave fetches the 64 rows it needs (R 16-byte pieces each, row c
+// starting at piece stride * c) as 64 R consecutive pieces: in round k (0 .. R-1) lane m fetches
+// piece P = 64k + m, i.e. piece P % R of the row of lane P / R (whose camera a lane shuffle
+// provides), so one instruction covers the lines of about 64 / R rows. The rows meet in the
+// wave's LDS slice (row stride R + 1 pieces, so the 16-byte reads of consecutive rows fall on
+// distinct banks in every group of 8 lanes) and each lane reads its own. Every lane of the
+// wave must call both halves (lane shuffles); c < 0 rows are not fetched. Pure data movement:
+// the values are bit-identical to a lane-private load.
+template <int R>
+__device__ __forceinline__ void coop_fetch(const Vec4<float>* __restrict__ rows, int stride, int c, int lane,
+                                           Vec4<float> v[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / R;
+    const int cr = __shfl(c, r, kWave);
+    v[k] = Vec4<float>{0.f, 0.f, 0.f, 0.f};
+    if (cr >= 0) v[k] = rows[stride * cr + (piece - R * r)];
+  }
+}
+template <int R>
+__device__ __forceinline__ void coop_exchange(const Vec4<float> v[R], int lane, Vec4<float>* __restrict__ lds,
+                                              Vec4<float> u[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / R;
+    lds[r * (R + 1) + (piece - R * r)] = v[k];
+  }
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < R; ++k) u[k] = lds[lane * (R + 1) + k];
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ---------------------------------------------------------------------------------------
 // Cost: sum over observations of 0.5||r||^2 or Huber, accumulated in f64 (Frame::cost,
 // model.rs:201-213). Deterministic: per-lane sequential, fixed block tree; per-workgroup
@@ -283,21 +320,31 @@ __global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int*
                                                  const Vec2<T>* __restrict__ slot_pix, const T* __restrict__ tab,
                                                  const T* __restrict__ pts, T huber, double* parts, int* flags) {
   __shared__ double smem[kBlock / kWave];
-  const int chunk = blockIdx.x * kChunksPerBlock + threadIdx.x / kWave;
+  // the first 16 table scalars of each observation's camera (R pieces of 16 bytes) are gathered
+  // cooperatively by the wave, so every lane of a live chunk sweeps it (lanes past np read their
+  // empty slots, whose camera is -1)
+  constexpr int R = 16 * sizeof(T) / 16;
+  __shared__ Vec4<float> rows[kChunksPerBlock * kWave * (R + 1)];
+  const int wave = threadIdx.x / kWave;
+  const int chunk = blockIdx.x * kChunksPerBlock + wave;
   const int lane = threadIdx.x % kWave;
   double acc = 0.0;
   bool bad = false;
   if (chunk < nchunks) {
     const int i = chunk * kWave + lane;
-    if (i < np) {
-      const T x0 = pts[3 * i], x1 = pts[3 * i + 1], x2 = pts[3 * i + 2];
-      const int base = chunk_base[chunk], len = chunk_len[chunk];
-      for (int j = 0; j < len; ++j) {
-        const int s = base + j * kWave + lane;
-        const int c = slot_cam[s];
-        if (c < 0) continue;
+    const bool live = i < np;
+    const T x0 = live ? pts[3 * i] : T(0), x1 = live ? pts[3 * i + 1] : T(0), x2 = live ? pts[3 * i + 2] : T(0);
+    const int base = chunk_base[chunk], len = chunk_len[chunk];
+    Vec4<float>* lds = rows + wave * kWave * (R + 1);
+    for (int j = 0; j < len; ++j) {
+      const int s = base + j * kWave + lane;
+      const int c = slot_cam[s];
+      Vec4<float> v[R], u[R];
+      coop_fetch<R>(reinterpret_cast<const Vec4<float>*>(tab), kCamTab * (int)sizeof(T) / 16, c, lane, v);
+      coop_exchange<R>(v, lane, lds, u);
+      if (c >= 0) {
         T ct[16];
-        load_cam16(tab, c, ct);
+        __builtin_memcpy(ct, u, sizeof(ct));
         T px, py, qx, qy, z;
         bool ok = project(ct, x0, x1, x2, px, py, qx, qy, z);
         const Vec2<T> ob = slot_pix[s];
@@ -412,43 +459,6 @@ __device__ __forceinline__ bool linearize_obs(const double cd[16], double x0, do
     g[a] += e0[a] * wrx + e1[a] * wry;
   }
   return ok;
-}
-
-// Cooperative row gather. A lane-private gather of a per-camera row makes every 16-byte load
-// instruction touch up to 64 cache lines, one per lane, and those per-line lookups bound the
-// point-major passes. Here a wave fetches the 64 rows it needs (R 16-byte pieces each, row c
-// starting at piece stride * c) as 64 R consecutive pieces: in round k (0 .. R-1) lane m fetches
-// piece P = 64k + m, i.e. piece P % R of the row of lane P / R (whose camera a lane shuffle
-// provides), so one instruction covers the lines of about 64 / R rows. The rows meet in the
-// wave's LDS slice (row stride R + 1 pieces, so the 16-byte reads of consecutive rows fall on
-// distinct banks in every group of 8 lanes) and each lane reads its own. Every lane of the
-// wave must call both halves (lane shuffles); c < 0 rows are not fetched. Pure data movement:
-// the values are bit-identical to a lane-private load.
-template <int R>
-__device__ __forceinline__ void coop_fetch(const Vec4<float>* __restrict__ rows, int stride, int c, int lane,
-                                           Vec4<float> v[R]) {
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int piece = kWave * k + lane;
-    const int r = piece / R;
-    const int cr = __shfl(c, r, kWave);
-    v[k] = Vec4<float>{0.f, 0.f, 0.f, 0.f};
-    if (cr >= 0) v[k] = rows[stride * cr + (piece - R * r)];
-  }
-}
-template <int R>
-__device__ __forceinline__ void coop_exchange(const Vec4<float> v[R], int lane, Vec4<float>* __restrict__ lds,
-                                              Vec4<float> u[R]) {
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int piece = kWave * k + lane;
-    const int r = piece / R;
-    lds[r * (R + 1) + (piece - R * r)] = v[k];
-  }
-  __builtin_amdgcn_wave_barrier();
-#pragma unroll
-  for (int k = 0; k < R; ++k) u[k] = lds[lane * (R + 1) + k];
-  __builtin_amdgcn_wave_barrier();
 }
 
 template <typename T>
