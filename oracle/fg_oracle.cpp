@@ -1,8 +1,13 @@
 // fg_oracle.cpp -- TEST INFRASTRUCTURE ONLY (see fg_oracle.h).
 //
-// A plain, sequential f64 restatement of the reference's algorithm, written from the
-// reference's source read as text. Each routine cites the file:line it follows. The
-// product library never links or calls this file.
+// A plain f64 restatement of the reference's algorithm, written from the reference's
+// source read as text. Each routine cites the file:line it follows. The product library
+// never links or calls this file.
+//
+// Loops over independent blocks (variable blocks, edges, cameras, points) run on OpenMP
+// threads. Every sum is still taken term by term in the order of the sequential loop it
+// replaces (a block's terms in emission order), so results are bit-identical for any
+// thread count; orc_set_threads(1) is the single-threaded restatement.
 //
 // Third-party arithmetic that the reference delegates (faer 0.24.4 CG/LLT/triangular
 // solves, nalgebra 0.34.2 quaternion rotation) is restated from the published
@@ -16,6 +21,8 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+
+#include <omp.h>
 
 namespace {
 
@@ -184,27 +191,115 @@ struct Projection {
 struct Block {
   size_t row, col, rows, cols, start;
 };
+// Block index for the threaded loops: runs of blocks that share their first row (one
+// emission each), and the blocks of every column start in block order. `clean` holds when
+// the column groups are disjoint and each has one width, i.e. every column is covered by the
+// blocks of exactly one group; only then do per-group loops visit a column's terms in the
+// sequential order.
+struct BlockIndex {
+  bool built = false, clean = false;
+  std::vector<size_t> row_runs;             // first block of each run, then blocks.size()
+  std::vector<size_t> group_col, group_width;
+  std::vector<size_t> group_off, group_blocks;  // CSR over groups, block order
+  std::vector<size_t> group_of_col;         // column start -> group (kNone elsewhere)
+};
+
+// Stable counting sort of items by key: CSR offsets (keys + 1) and items in input order.
+void bucket(size_t keys, size_t n, const size_t* key, std::vector<size_t>& off, std::vector<size_t>& items) {
+  off.assign(keys + 1, 0);
+  for (size_t i = 0; i < n; ++i)
+    if (key[i] != kNone) off[key[i] + 1]++;
+  for (size_t k = 0; k < keys; ++k) off[k + 1] += off[k];
+  items.assign(off[keys], 0);
+  std::vector<size_t> fill(off.begin(), off.end() - 1);
+  for (size_t i = 0; i < n; ++i)
+    if (key[i] != kNone) items[fill[key[i]]++] = i;
+}
+
 struct Jacobian {
   size_t rows = 0, cols = 0;
   std::vector<Block> blocks;
   std::vector<double> values;
+  mutable BlockIndex index;
+
+  void invalidate() { index.built = false; }
+  const BlockIndex& indexed() const {
+    BlockIndex& ix = index;
+    if (ix.built) return ix;
+    size_t nb = blocks.size();
+    ix.row_runs.clear();
+    for (size_t b = 0; b < nb; ++b)
+      if (b == 0 || blocks[b].row != blocks[b - 1].row) ix.row_runs.push_back(b);
+    ix.row_runs.push_back(nb);
+    std::vector<size_t> start_of(cols, kNone);
+    std::vector<size_t> key(nb, kNone);
+    ix.group_col.clear();
+    ix.group_width.clear();
+    ix.clean = true;
+    for (size_t b = 0; b < nb; ++b) {
+      if (blocks[b].cols == 0) continue;
+      size_t c = blocks[b].col;
+      if (c >= cols) { ix.clean = false; continue; }
+      if (start_of[c] == kNone) {
+        start_of[c] = ix.group_col.size();
+        ix.group_col.push_back(c);
+        ix.group_width.push_back(blocks[b].cols);
+      } else if (ix.group_width[start_of[c]] != blocks[b].cols) {
+        ix.clean = false;
+      }
+      key[b] = start_of[c];
+    }
+    bucket(ix.group_col.size(), nb, key.data(), ix.group_off, ix.group_blocks);
+    size_t end = 0;
+    for (size_t c = 0; c < cols && ix.clean; ++c) {
+      if (start_of[c] == kNone) continue;
+      if (c < end || c + ix.group_width[start_of[c]] > cols) ix.clean = false;
+      end = c + ix.group_width[start_of[c]];
+    }
+    ix.group_of_col.swap(start_of);
+    ix.built = true;
+    return ix;
+  }
+
   // src/linear_model.rs:39-55
   void apply(const double* x, double* y) const {
     std::fill(y, y + rows, 0.0);
-    for (const Block& b : blocks)
-      for (size_t j = 0; j < b.cols; ++j) {
-        double v = x[b.col + j];
-        for (size_t i = 0; i < b.rows; ++i) y[b.row + i] += values[b.start + j * b.rows + i] * v;
+    const BlockIndex& ix = indexed();
+    size_t runs = ix.row_runs.size() - 1;
+    // rows of different runs are disjoint (each run is one emission)
+#pragma omp parallel for schedule(static, 1024)
+    for (size_t r = 0; r < runs; ++r)
+      for (size_t bi = ix.row_runs[r]; bi < ix.row_runs[r + 1]; ++bi) {
+        const Block& b = blocks[bi];
+        for (size_t j = 0; j < b.cols; ++j) {
+          double v = x[b.col + j];
+          for (size_t i = 0; i < b.rows; ++i) y[b.row + i] += values[b.start + j * b.rows + i] * v;
+        }
       }
   }
   // src/linear_model.rs:64-87
   void transpose_apply(const double* y, double* x) const {
     std::fill(x, x + cols, 0.0);
-    for (const Block& b : blocks)
-      for (size_t j = 0; j < b.cols; ++j) {
-        double s = 0;
-        for (size_t i = 0; i < b.rows; ++i) s += values[b.start + j * b.rows + i] * y[b.row + i];
-        x[b.col + j] += s;
+    const BlockIndex& ix = indexed();
+    if (!ix.clean) {
+      for (const Block& b : blocks)
+        for (size_t j = 0; j < b.cols; ++j) {
+          double s = 0;
+          for (size_t i = 0; i < b.rows; ++i) s += values[b.start + j * b.rows + i] * y[b.row + i];
+          x[b.col + j] += s;
+        }
+      return;
+    }
+    size_t groups = ix.group_col.size();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t g = 0; g < groups; ++g)
+      for (size_t k = ix.group_off[g]; k < ix.group_off[g + 1]; ++k) {
+        const Block& b = blocks[ix.group_blocks[k]];
+        for (size_t j = 0; j < b.cols; ++j) {
+          double s = 0;
+          for (size_t i = 0; i < b.rows; ++i) s += values[b.start + j * b.rows + i] * y[b.row + i];
+          x[b.col + j] += s;
+        }
       }
   }
 };
@@ -240,6 +335,7 @@ struct LinearizedModel {
     jacobian.rows = 0;
     jacobian.blocks.clear();
     jacobian.values.clear();
+    jacobian.invalidate();
     negative_residual.clear();
     std::fill(gradient.begin(), gradient.end(), 0.0);
   }
@@ -248,6 +344,7 @@ struct LinearizedModel {
                   const size_t* widths, const double* values) {
     size_t row = jacobian.rows;
     jacobian.rows = row + rows;
+    jacobian.invalidate();
     for (size_t i = 0; i < rows; ++i) negative_residual.push_back(-residual[i]);
     size_t offset = 0;
     for (size_t bi = 0; bi < nblocks; ++bi) {
@@ -267,11 +364,26 @@ struct LinearizedModel {
   int prepare_scales(double floor) {  // :211-230
     if (!is_finite(floor) || floor <= 0.0) return ORC_INVALID_OPTIONS;
     scales.assign(jacobian.cols, 0.0);
-    for (const Block& b : jacobian.blocks)
-      for (size_t c = 0; c < b.cols; ++c) {
-        double& n = scales[b.col + c];
-        for (size_t r = 0; r < b.rows; ++r) n = std::hypot(n, jacobian.values[b.start + c * b.rows + r]);
-      }
+    const BlockIndex& ix = jacobian.indexed();
+    if (ix.clean) {
+      // a column's chain runs over its group's blocks in block order, as below
+      size_t groups = ix.group_col.size();
+#pragma omp parallel for schedule(dynamic, 64)
+      for (size_t g = 0; g < groups; ++g)
+        for (size_t k = ix.group_off[g]; k < ix.group_off[g + 1]; ++k) {
+          const Block& b = jacobian.blocks[ix.group_blocks[k]];
+          for (size_t c = 0; c < b.cols; ++c) {
+            double& n = scales[b.col + c];
+            for (size_t r = 0; r < b.rows; ++r) n = std::hypot(n, jacobian.values[b.start + c * b.rows + r]);
+          }
+        }
+    } else {
+      for (const Block& b : jacobian.blocks)
+        for (size_t c = 0; c < b.cols; ++c) {
+          double& n = scales[b.col + c];
+          for (size_t r = 0; r < b.rows; ++r) n = std::hypot(n, jacobian.values[b.start + c * b.rows + r]);
+        }
+    }
     if (norm_inf(scales.data(), scales.size(), nullptr) != ORC_OK) return ORC_INVALID_EVALUATION;
     for (double& s : scales) s = std::max(s, floor);
     return ORC_OK;
@@ -352,7 +464,9 @@ struct BlockFactors {
     blocks.clear();
     gram.clear();
     owners.assign(jac.cols, kNone);
-    for (const Block& b : jac.blocks) {
+    std::vector<size_t> ids(jac.blocks.size(), kNone);
+    for (size_t bi = 0; bi < jac.blocks.size(); ++bi) {
+      const Block& b = jac.blocks[bi];
       if (b.cols == 0) continue;
       size_t end = b.col + b.cols;
       if (end > jac.cols) return ORC_DIMENSION_MISMATCH;
@@ -370,7 +484,9 @@ struct BlockFactors {
         if (blocks[owner].col != b.col || blocks[owner].width != b.cols) return ORC_DIMENSION_MISMATCH;
         id = owner;
       }
-      size_t start = blocks[id].start;
+      ids[bi] = id;
+    }
+    auto add = [&](const Block& b, size_t start) {
       for (size_t col = 0; col < b.cols; ++col)
         for (size_t row = col; row < b.cols; ++row) {
           double s = 0;
@@ -379,6 +495,20 @@ struct BlockFactors {
                  (jac.values[b.start + row * b.rows + k] / scales[b.col + row]);
           gram[start + col * b.cols + row] += s;
         }
+    };
+    const BlockIndex& ix = jac.indexed();
+    if (ix.clean) {
+      // the blocks of a variable block are exactly its column group, visited in block order
+      size_t nv = blocks.size();
+#pragma omp parallel for schedule(dynamic, 64)
+      for (size_t v = 0; v < nv; ++v) {
+        size_t g = ix.group_of_col[blocks[v].col];
+        for (size_t k = ix.group_off[g]; k < ix.group_off[g + 1]; ++k)
+          add(jac.blocks[ix.group_blocks[k]], blocks[v].start);
+      }
+    } else {
+      for (size_t bi = 0; bi < jac.blocks.size(); ++bi)
+        if (ids[bi] != kNone) add(jac.blocks[bi], blocks[ids[bi]].start);
     }
     factors.assign(gram.size(), 0.0);
     return ORC_OK;
@@ -386,7 +516,10 @@ struct BlockFactors {
   size_t factor(double lambda) {  // :113-138
     factors = gram;
     size_t fallbacks = 0;
-    for (VariableBlock& b : blocks) {
+    size_t nv = blocks.size();
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : fallbacks)
+    for (size_t bi = 0; bi < nv; ++bi) {
+      VariableBlock& b = blocks[bi];
       double* v = factors.data() + b.start;
       for (size_t j = 0; j < b.width; ++j) v[j * b.width + j] += lambda;
       bool ok = norm_inf(v, b.width * b.width, nullptr) == ORC_OK && cholesky_lower(v, b.width);
@@ -399,7 +532,10 @@ struct BlockFactors {
     return fallbacks;
   }
   void apply_factor_inverse(double* rhs, bool transpose) const {  // :142-160
-    for (const VariableBlock& b : blocks) {
+    size_t nv = blocks.size();
+#pragma omp parallel for schedule(dynamic, 256)
+    for (size_t bi = 0; bi < nv; ++bi) {
+      const VariableBlock& b = blocks[bi];
       if (b.fallback) continue;
       const double* l = factors.data() + b.start;
       if (transpose) solve_lower(l, b.width, rhs + b.col);
@@ -408,15 +544,21 @@ struct BlockFactors {
   }
   // :178-204 over coordinates [offset, offset+n)
   void solve_normal(double* rhs, size_t offset, size_t n, double lambda) const {
-    for (const VariableBlock& b : blocks) {
+    size_t nv = blocks.size();
+#pragma omp parallel for schedule(dynamic, 256)
+    for (size_t bi = 0; bi < nv; ++bi) {
+      const VariableBlock& b = blocks[bi];
       if (b.col < offset || b.col >= offset + n) continue;
       const double* l = factors.data() + b.start;
       double* part = rhs + (b.col - offset);
       solve_lower(l, b.width, part);
       solve_upper_transpose(l, b.width, part);
     }
-    for (size_t i = 0; i < n; ++i)
-      if (owners[offset + i] == kNone && lambda != 0.0) rhs[i] /= lambda;
+    if (lambda != 0.0) {
+#pragma omp parallel for schedule(static)
+      for (size_t i = 0; i < n; ++i)
+        if (owners[offset + i] == kNone) rhs[i] /= lambda;
+    }
   }
 };
 
@@ -437,6 +579,8 @@ struct orc_schur {
   BlockFactors blocks;
   StepWorkspace work;
   std::vector<Edge> edges;
+  // edges of each retained / eliminated block in edge order (built by prepare_damping)
+  std::vector<size_t> retained_off, retained_edges, eliminated_off, eliminated_edges;
   std::vector<double> cross, reduced_rhs, reduced_step, scaled_work, reduced_residual;
   bool ready = false;
   size_t solves = 0, iterations = 0;
@@ -452,21 +596,40 @@ struct orc_schur {
     diag.original_relative_residual = kNaN;
   }
 
-  // src/schur.rs:558-582
+  void index_edges() {
+    size_t ne = edges.size(), dim = model.jacobian.cols;
+    std::vector<size_t> key(ne);
+    for (size_t e = 0; e < ne; ++e) key[e] = edges[e].retained / rw;
+    bucket(retained / rw, ne, key.data(), retained_off, retained_edges);
+    for (size_t e = 0; e < ne; ++e) key[e] = edges[e].eliminated / ew;
+    bucket((dim - retained) / ew, ne, key.data(), eliminated_off, eliminated_edges);
+  }
+  // src/schur.rs:558-582. The sequential loop visits edges in order; an output block's terms
+  // come from its own edges only, so each block sums its edges in edge order on one thread.
   void cross_apply(double* out, const double* rhs, bool transpose, double alpha) const {
-    for (size_t e = 0; e < edges.size(); ++e)
-      for (size_t p = 0; p < ew; ++p)
-        for (size_t c = 0; c < rw; ++c) {
-          double value = alpha * cross[(e * ew + p) * rw + c];
-          if (transpose) out[edges[e].eliminated + p] += value * rhs[edges[e].retained + c];
-          else out[edges[e].retained + c] += value * rhs[edges[e].eliminated + p];
-        }
+    const std::vector<size_t>& off = transpose ? eliminated_off : retained_off;
+    const std::vector<size_t>& list = transpose ? eliminated_edges : retained_edges;
+    size_t nblocks = off.empty() ? 0 : off.size() - 1;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (size_t blk = 0; blk < nblocks; ++blk)
+      for (size_t k = off[blk]; k < off[blk + 1]; ++k) {
+        size_t e = list[k];
+        for (size_t p = 0; p < ew; ++p)
+          for (size_t c = 0; c < rw; ++c) {
+            double value = alpha * cross[(e * ew + p) * rw + c];
+            if (transpose) out[edges[e].eliminated + p] += value * rhs[edges[e].retained + c];
+            else out[edges[e].retained + c] += value * rhs[edges[e].eliminated + p];
+          }
+      }
   }
   // Reduced::apply, src/schur.rs:606-652
   void reduced_apply(double lambda, const double* x, double* out) const {
     size_t n = model.jacobian.cols, elim = n - retained;
     for (size_t i = 0; i < retained; ++i) out[i] = lambda * x[i];
-    for (const VariableBlock& b : blocks.blocks) {
+    size_t nv = blocks.blocks.size();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t bi = 0; bi < nv; ++bi) {
+      const VariableBlock& b = blocks.blocks[bi];
       if (b.col >= retained) continue;
       const double* g = blocks.gram.data() + b.start;
       for (size_t j = 0; j < b.width; ++j)
@@ -671,8 +834,11 @@ int orc_schur_prepare_damping(orc_schur* s, double floor) {  // :490-526
   st = s->blocks.prepare(s->model.jacobian, s->model.scales);
   if (st != ORC_OK) return st;
   s->cross.assign(s->rw * s->ew * s->edges.size(), 0.0);
+  s->index_edges();
   const LinearizedModel& m = s->model;
-  for (size_t e = 0; e < s->edges.size(); ++e) {
+  size_t ne = s->edges.size();
+#pragma omp parallel for schedule(static, 1024)
+  for (size_t e = 0; e < ne; ++e) {
     const Block& left = m.jacobian.blocks[s->edges[e].retained_jacobian];
     const Block& right = m.jacobian.blocks[s->edges[e].eliminated_jacobian];
     for (size_t p = 0; p < s->ew; ++p)
@@ -1122,6 +1288,7 @@ struct orc_ba {
   std::vector<double> obs_px;
   double huber = 0.0;  // 0 = plain least squares
   std::vector<int64_t> order, cam_off;  // rows in camera-stable order (Frame batches)
+  std::vector<size_t> pt_off, pt_rows;   // each point's rows, ascending (its emission order)
   bool trial_active = false;
   orc_schur* backend = nullptr;  // used by the LM callbacks
 
@@ -1140,8 +1307,11 @@ struct orc_ba {
 
   // src/optimization.rs:391-430 + Frame::cost (model.rs:201-213): per batch sum in payload
   // order, checked, then added to the total in batch order.
+  // Batch sums are independent; the total adds them in batch order.
   int cost(const std::vector<double>& c, const std::vector<double>& p, double* out) const {
-    double total = 0.0;
+    std::vector<double> batches(nc, 0.0);
+    std::vector<int> status(nc, ORC_OK);
+#pragma omp parallel for schedule(dynamic, 4)
     for (size_t cam = 0; cam < nc; ++cam) {
       if (cam_off[cam] == cam_off[cam + 1]) continue;
       Projection proj(&c[9 * cam]);
@@ -1150,22 +1320,111 @@ struct orc_ba {
         int64_t o = order[k];
         double px[2];
         int st = proj.value(&p[3 * obs_pt[o]], px);
-        if (st != ORC_OK) return st;
+        if (st != ORC_OK) {
+          status[cam] = st;
+          break;
+        }
         double r[2] = {px[0] - obs_px[2 * o], px[1] - obs_px[2 * o + 1]};
         double cst, sc;
         robust(r, &cst, &sc);
         batch += cst;
       }
-      if (!is_finite(batch) || batch < 0.0) return ORC_INVALID_EVALUATION;
-      total += batch;
+      if (status[cam] == ORC_OK && (!is_finite(batch) || batch < 0.0)) status[cam] = ORC_INVALID_EVALUATION;
+      batches[cam] = batch;
+    }
+    double total = 0.0;
+    for (size_t cam = 0; cam < nc; ++cam) {
+      if (cam_off[cam] == cam_off[cam + 1]) continue;
+      if (status[cam] != ORC_OK) return status[cam];
+      total += batches[cam];
     }
     if (!is_finite(total) || total < 0.0) return ORC_INVALID_EVALUATION;
     *out = total;
     return ORC_OK;
   }
 
-  // Frame::linearize (model.rs:214-236) through the sink checks (optimization.rs:541-613).
+  // One observation's emission: weighted residual rw and the 24 Jacobian values (camera
+  // block 2x9 then point block 2x3, column-major), as Frame::linearize emits it.
+  int emission(const Projection& proj, int64_t o, double rw[2], double vals[24]) const {
+    const double* x = &pts[3 * obs_pt[o]];
+    double px[2];
+    int st = proj.value(x, px);
+    if (st != ORC_OK) return st;
+    double r[2] = {px[0] - obs_px[2 * o], px[1] - obs_px[2 * o + 1]};
+    double jc[18], jp[6];
+    proj.jacobians(x, jc, jp);
+    double cst, sc;
+    robust(r, &cst, &sc);
+    for (int i = 0; i < 18; ++i) vals[i] = sc * jc[i];
+    for (int i = 0; i < 6; ++i) vals[18 + i] = sc * jp[i];
+    rw[0] = sc * r[0];
+    rw[1] = sc * r[1];
+    for (int i = 0; i < 24; ++i)
+      if (!is_finite(vals[i])) return ORC_INVALID_EVALUATION;
+    if (!is_finite(rw[0]) || !is_finite(rw[1])) return ORC_INVALID_EVALUATION;
+    return ORC_OK;
+  }
+
+  // Frame::linearize for every batch in bulk: the same values, blocks, edges and gradient as
+  // the per-emission path below (emission k of the camera-stable order owns rows 2k, 2k+1,
+  // blocks 2k, 2k+1 and values 24k..24k+24), with the gradient of a camera summed over its
+  // rows and that of a point over its rows in ascending order, which is the order in which
+  // the per-emission path adds them.
   int linearize(orc_schur* s) const {
+    if (s->retained != 9 * nc || s->rw != 9 || s->ew != 3 || s->model.jacobian.cols != 9 * nc + 3 * np)
+      return linearize_sequential(s);
+    orc_schur_clear(s);
+    LinearizedModel& m = s->model;
+    m.jacobian.rows = 2 * no;
+    m.jacobian.blocks.resize(2 * no);
+    m.jacobian.values.resize(24 * no);
+    m.negative_residual.resize(2 * no);
+    s->edges.resize(no);
+    std::vector<int> status(nc, ORC_OK);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (size_t cam = 0; cam < nc; ++cam) {
+      if (cam_off[cam] == cam_off[cam + 1]) continue;
+      Projection proj(&cams[9 * cam]);
+      for (int64_t k = cam_off[cam]; k < cam_off[cam + 1]; ++k) {
+        int64_t o = order[k];
+        double rw[2];
+        int st = emission(proj, o, rw, &m.jacobian.values[24 * k]);
+        if (st != ORC_OK) {
+          status[cam] = st;
+          break;
+        }
+        size_t row = 2 * (size_t)k, pcol = 3 * (size_t)obs_pt[o];
+        m.negative_residual[row] = -rw[0];
+        m.negative_residual[row + 1] = -rw[1];
+        m.jacobian.blocks[row] = Block{row, 9 * cam, 2, 9, 24 * (size_t)k};
+        m.jacobian.blocks[row + 1] = Block{row, 9 * nc + pcol, 2, 3, 24 * (size_t)k + 18};
+        s->edges[k] = Edge{9 * cam, pcol, row, row + 1};
+      }
+    }
+    for (size_t cam = 0; cam < nc; ++cam)
+      if (status[cam] != ORC_OK) return status[cam];
+    // gradient[col + j] += sum_i J_ij r_i per block (src/linear_model.rs:199-207)
+    auto add = [&](size_t k, size_t first, size_t width, double* g) {
+      double r[2] = {-m.negative_residual[2 * k], -m.negative_residual[2 * k + 1]};
+      const double* v = &m.jacobian.values[24 * k + first];
+      for (size_t j = 0; j < width; ++j) {
+        double sum = 0;
+        for (size_t i = 0; i < 2; ++i) sum += v[j * 2 + i] * r[i];
+        g[j] += sum;
+      }
+    };
+#pragma omp parallel for schedule(dynamic, 4)
+    for (size_t cam = 0; cam < nc; ++cam)
+      for (int64_t k = cam_off[cam]; k < cam_off[cam + 1]; ++k) add(k, 0, 9, &m.gradient[9 * cam]);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (size_t p = 0; p < np; ++p)
+      for (size_t i = pt_off[p]; i < pt_off[p + 1]; ++i) add(pt_rows[i], 18, 3, &m.gradient[9 * nc + 3 * p]);
+    return ORC_OK;
+  }
+
+  // Frame::linearize (model.rs:214-236) through the sink checks (optimization.rs:541-613),
+  // one emission at a time.
+  int linearize_sequential(orc_schur* s) const {
     orc_schur_clear(s);
     for (size_t cam = 0; cam < nc; ++cam) {
       if (cam_off[cam] == cam_off[cam + 1]) continue;
@@ -1220,6 +1479,9 @@ orc_ba* orc_ba_new(size_t nc, size_t np, size_t no, const double* cameras, const
   b->order.assign(no, 0);
   std::vector<int64_t> fill(b->cam_off.begin(), b->cam_off.end() - 1);
   for (size_t o = 0; o < no; ++o) b->order[fill[obs_camera[o]]++] = (int64_t)o;
+  std::vector<size_t> key(no);
+  for (size_t k = 0; k < no; ++k) key[k] = (size_t)obs_point[b->order[k]];
+  bucket(np, no, key.data(), b->pt_off, b->pt_rows);
   return b;
 }
 void orc_ba_free(orc_ba* b) { delete b; }
@@ -1246,6 +1508,9 @@ void orc_ba_set_state(orc_ba* b, const double* cameras, const double* points) {
   b->pts.assign(points, points + 3 * b->np);
 }
 int orc_ba_linearize(orc_ba* b, orc_schur* s) { return b->linearize(s); }
+int orc_ba_linearize_sequential(orc_ba* b, orc_schur* s) { return b->linearize_sequential(s); }
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 void orc_ba_retract(orc_ba* b, const double* delta) {
   for (size_t i = 0; i < 9 * b->nc; ++i) b->cams[i] += delta[i];
   for (size_t i = 0; i < 3 * b->np; ++i) b->pts[i] += delta[9 * b->nc + i];
@@ -1519,8 +1784,10 @@ int ba_stage(void* c, const double* delta) {
   orc_ba* b = ((BaCtx*)c)->ba;
   b->trial_cams = b->cams;
   b->trial_pts = b->pts;
-  for (size_t i = 0; i < 9 * b->nc; ++i) b->trial_cams[i] += delta[i];
-  for (size_t i = 0; i < 3 * b->np; ++i) b->trial_pts[i] += delta[9 * b->nc + i];
+  size_t nc9 = 9 * b->nc, np3 = 3 * b->np;
+  for (size_t i = 0; i < nc9; ++i) b->trial_cams[i] += delta[i];
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < np3; ++i) b->trial_pts[i] += delta[nc9 + i];
   b->trial_active = true;
   return ORC_OK;
 }

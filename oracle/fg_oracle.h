@@ -136,6 +136,11 @@ void orc_ba_get_state(const orc_ba*, double* cameras, double* points);
 void orc_ba_set_state(orc_ba*, const double* cameras, const double* points);
 /* Linearize the current state into a Schur backend (clears it first). */
 int orc_ba_linearize(orc_ba*, orc_schur*);
+/* the same linearization one emission at a time through orc_schur_accumulate */
+int orc_ba_linearize_sequential(orc_ba*, orc_schur*);
+/* OpenMP threads of the oracle's block loops (results do not depend on it) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 /* Retraction of a full-length step into the state (additive). */
 void orc_ba_retract(orc_ba*, const double* delta);
 

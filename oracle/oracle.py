@@ -1,8 +1,10 @@
 """ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
-The oracle (``fg_oracle.cpp``) is a sequential f64 restatement of the reference's
-bundle-adjustment path.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` import this module, always as the checker.
+The oracle (``fg_oracle.cpp``) is an f64 restatement of the reference's bundle-adjustment
+path. Its loops over independent blocks run on OpenMP threads with every sum taken in the
+sequential order, so results are bit-identical for any thread count (``set_threads``).
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+import this module, always as the checker.
 """
 import ctypes
 import os
@@ -163,6 +165,9 @@ def lib():
         "orc_ba_get_state": (None, [_VP, _dp, _dp]),
         "orc_ba_set_state": (None, [_VP, _dp, _dp]),
         "orc_ba_linearize": (ctypes.c_int, [_VP, _VP]),
+        "orc_ba_linearize_sequential": (ctypes.c_int, [_VP, _VP]),
+        "orc_set_threads": (None, [ctypes.c_int]),
+        "orc_get_threads": (ctypes.c_int, []),
         "orc_ba_retract": (None, [_VP, _dp]),
         "orc_lm_optimize": (None, [ctypes.POINTER(Problem), ctypes.POINTER(OptimizeOptions),
                                    ctypes.POINTER(LmOptions), ctypes.POINTER(LmOutcome)]),
@@ -179,6 +184,15 @@ def lib():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's block loops (1 = the single-threaded restatement)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def _d(a):
@@ -452,6 +466,10 @@ class BA:
 
     def linearize(self, schur):
         return lib().orc_ba_linearize(self.h, schur.h)
+
+    def linearize_sequential(self, schur):
+        """The same linearization one emission at a time through Schur.accumulate."""
+        return lib().orc_ba_linearize_sequential(self.h, schur.h)
 
     def retract(self, delta):
         d, pd = _d(delta)
