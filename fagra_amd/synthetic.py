@@ -13,8 +13,8 @@ Workloads (names carry their sizes):
 * ``rings1500``     1,500 cameras on rings of radius 8/10/12, 500,000 points,
                     tracks U{3..10}, 2% outliers uniform over the image (only on points
                     that stay well triangulated by their inliers), Huber 1 px
-* ``zipf4000``      4,000 cameras, 1,000,000 points, Zipf(1.5) tracks clipped to
-                    2..200, focal U(400, 800), Huber 1 px
+* ``zipf4000``      4,000 cameras on one circle of radius 10, 1,000,000 points, Zipf(1.5)
+                    tracks clipped to 2..200, focal U(400, 800), no outliers, Huber 1 px
 
 Initial estimates perturb every rotation by N(0, 0.01 rad) per axis and every
 translation and point coordinate by N(0, 0.05).
@@ -161,8 +161,8 @@ CONFIGS = {
                     outliers=0.0, focal=("const", 500.0)),
     "rings1500": dict(n_cameras=1500, n_points=500000, radii=(8.0, 10.0, 12.0), track=("uniform", 3, 10),
                       huber=1.0, outliers=0.02, focal=("const", 500.0)),
-    "zipf4000": dict(n_cameras=4000, n_points=1000000, radii=(8.0, 10.0, 12.0), track=("zipf", 1.5, 2, 200),
-                     huber=1.0, outliers=0.02, focal=("uniform", 400.0, 800.0)),
+    "zipf4000": dict(n_cameras=4000, n_points=1000000, radii=(10.0,), track=("zipf", 1.5, 2, 200),
+                     huber=1.0, outliers=0.0, focal=("uniform", 400.0, 800.0)),
 }
 
 

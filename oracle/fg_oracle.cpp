@@ -1803,9 +1803,15 @@ void ba_reject(void* c) { ((BaCtx*)c)->ba->trial_active = false; }
 void ba_work(void* c, size_t* solves, size_t* iterations) {
   orc_schur_statistics(((BaCtx*)c)->s, solves, iterations);
 }
+// One record per accepted step: cost, attempts so far, damping of the accepted trial.
 void ba_on_accept(void* c, const orc_lm_outcome* st) {
   BaCtx* x = (BaCtx*)c;
-  if (x->trace && x->trace_len < x->trace_cap) x->trace[x->trace_len++] = st->cost;
+  if (x->trace && x->trace_len < x->trace_cap) {
+    double* r = x->trace + 3 * x->trace_len++;
+    r[0] = st->cost;
+    r[1] = (double)st->attempts;
+    r[2] = st->damping;
+  }
 }
 }  // namespace
 

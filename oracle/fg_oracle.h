@@ -203,10 +203,12 @@ int orc_dense_covariance(const double* information, size_t n, const size_t* colu
 int orc_schur_covariance(const orc_schur* s, const size_t* offsets, const size_t* widths, size_t nblocks,
                          double tolerance, double* out);
 
-/* BA convenience: LM (or GN) with the Schur backend on a BA problem. */
+/* BA convenience: LM (or GN) with the Schur backend on a BA problem. The trace receives one
+ * record of three doubles per accepted step (cost, attempts so far, damping of the accepted
+ * trial), up to trace_capacity records. */
 void orc_ba_lm(orc_ba*, size_t max_iterations_cg, double relative_tolerance,
                const orc_optimize_options*, const orc_lm_options*, orc_lm_outcome*,
-               double* cost_trace, size_t cost_trace_capacity, size_t* cost_trace_len);
+               double* trace, size_t trace_capacity, size_t* trace_len);
 void orc_ba_gn(orc_ba*, size_t max_iterations_cg, double relative_tolerance,
                const orc_optimize_options*, orc_lm_outcome*);
 

@@ -479,11 +479,12 @@ class BA:
         options = options or OptimizeOptions.default()
         lm_options = lm_options or LmOptions.default()
         out = LmOutcome()
-        tr, ptr = _out(max(trace_capacity, 1))
+        tr, ptr = _out(3 * max(trace_capacity, 1))
         n = ctypes.c_size_t()
         lib().orc_ba_lm(self.h, max_cg, tol, ctypes.byref(options), ctypes.byref(lm_options),
                         ctypes.byref(out), ptr, trace_capacity, ctypes.byref(n))
-        return out, tr[:n.value].copy()
+        # per accepted step: cost, attempts so far, damping of the accepted trial
+        return out, tr[:3 * n.value].reshape(n.value, 3).copy()
 
     def gn(self, options=None, max_cg=1000, tol=1e-10):
         options = options or OptimizeOptions.default()

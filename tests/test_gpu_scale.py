@@ -1,0 +1,214 @@
+"""HIP path vs the CPU oracle at full workload size (GPU only).
+
+Workloads: rings1500 (1,500 cameras, 500,000 points, 3,249,262 observations, 2% outliers;
+block-Jacobi and learned preconditioner) and zipf4000 (4,000 cameras, 1,000,000 points, Zipf(1.5)
+tracks clipped to 2..200, 21.4M observations). The oracle runs on the host's cores with its
+threaded block loops, which are bit-identical to its sequential restatement.
+
+Checked against the oracle on the same inputs: the row order, camera offsets and point
+permutation (bit-exact), the initial cost, the linearized model, the reduced-camera product
+(src/schur.rs:606-652) at three dampings, and one damped solve (src/schur.rs:527-545) with
+each preconditioner. Bars as in the small-problem parity tests: FP64 products 1e-10, steps
+1e-8; FP32 products 2e-5 (x5), steps 3e-3 (the reference's own f32 Schur bound,
+src/schur.rs:788).
+
+Trajectories: committed oracle fixtures (tests/golden/trajectory_*.json, written by
+tests/golden/make_trajectory.py) hold the oracle's LM run at CG tolerance 1e-10 -- one record
+per accepted step (cost, attempts) and a digest of the final state. The FP64 GPU run of the
+same length must agree step by step (cost 1e-9 relative, same attempts) and in the sampled
+final state; the FP32 runs (block-Jacobi and learned) at the benchmark's CG tolerance 1e-3 must
+end within 1e-5 of the oracle's final cost.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fagra_amd as fa
+from fagra_amd import learned, synthetic
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+PROD_TOL = {fa.FP64: 1e-10, fa.FP32: 2e-5}
+STEP_TOL = {fa.FP64: 1e-8, fa.FP32: 3e-3}
+CG_TOL = {fa.FP64: 1e-10, fa.FP32: 1e-6}
+TRAJECTORIES = {"rings1500": "trajectory_rings1500_0_cg1e-10_400.json",
+                "zipf4000": "trajectory_zipf4000_0_cg1e-10_60.json"}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module", params=["rings1500", "zipf4000"])
+def scale(request):
+    """The workload, the oracle linearized and damping-prepared at the initial state, and
+    lazily built GPU problems and oracle results shared by the tests of one workload."""
+    name = request.param
+    print(f"\nscale: generating {name}", flush=True)
+    p = synthetic.generate(name, seed=0)
+    ba_o = oracle.BA(p, huber=p.huber)
+    st, s_o = oracle.ba_schur(ba_o)
+    assert st == oracle.OK
+    assert s_o.prepare_damping(1e-6) == oracle.OK
+    ctx = {"name": name, "p": p, "ba_o": ba_o, "s_o": s_o, "dev": {}, "solve": {}}
+    yield ctx
+    for d in ctx["dev"].values():
+        d.close()
+    ctx.clear()
+
+
+def device(ctx, precision):
+    if precision not in ctx["dev"]:
+        ctx["dev"][precision] = fa.BalProblem(ctx["p"], precision=precision)
+    return ctx["dev"][precision]
+
+
+def linearized(ctx, precision, **kw):
+    ba = device(ctx, precision)
+    ba.set_state(ctx["p"].cameras, ctx["p"].points)
+    s = fa.Schur(precision=precision, **kw)
+    assert s.prepare(ba) == fa.OK
+    assert s.linearize() == fa.OK
+    assert s.prepare_damping(1e-6) == fa.OK
+    return s
+
+
+def oracle_solve(ctx, lam, tol):
+    key = (lam, tol)
+    if key not in ctx["solve"]:
+        s_o = ctx["s_o"]
+        s_o.relative_tolerance = tol
+        st, d, pred = s_o.solve_damped(lam)
+        assert st == oracle.OK
+        ctx["solve"][key] = (d, pred, s_o.diagnostics().iterations)
+    return ctx["solve"][key]
+
+
+def test_layout_is_bit_exact_at_scale(scale):
+    g = device(scale, fa.FP64).layout()
+    o = scale["ba_o"].layout()
+    for a, b in zip(g, o):
+        assert a.dtype == b.dtype and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32])
+def test_initial_cost_and_model_at_scale(scale, precision):
+    st_o, c_o = scale["ba_o"].cost()
+    st_g, c_g = device(scale, precision).cost()
+    assert st_o == st_g == fa.OK
+    print(f"{scale['name']} precision {precision}: cost {c_g!r} oracle {c_o!r} rel {abs(c_g - c_o) / c_o:.2e}")
+    assert abs(c_g - c_o) <= (1e-12 if precision == fa.FP64 else 1e-5) * c_o
+    s = linearized(scale, precision)
+    g, d, nr = s.model()
+    s_o = scale["s_o"]
+    tol = PROD_TOL[precision]
+    assert rel(g, s_o.gradient()) < 10 * tol
+    assert rel(d, s_o.scales()) < tol
+    assert rel(nr, s_o.negative_residual()) < tol
+    s.close()
+
+
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32])
+def test_reduced_apply_at_scale(scale, precision):
+    s = linearized(scale, precision)
+    rng = np.random.default_rng(7)
+    for lam in (1e-4, 1e-2, 1.0):
+        x = rng.normal(size=9 * scale["p"].n_cameras)
+        st_o, y_o = scale["s_o"].reduced_apply(lam, x)
+        st_g, y_g = s.reduced_apply(lam, x)
+        assert st_o == st_g == fa.OK
+        e = rel(y_g, y_o)
+        print(f"{scale['name']} precision {precision} lambda {lam:g}: reduced product rel {e:.2e}")
+        assert e < 5 * PROD_TOL[precision], lam
+    s.close()
+
+
+@pytest.mark.parametrize("precision,precond", [(fa.FP64, fa.PRECOND_BLOCK_JACOBI),
+                                               (fa.FP32, fa.PRECOND_BLOCK_JACOBI),
+                                               (fa.FP32, fa.PRECOND_LEARNED)])
+def test_damped_solve_at_scale(scale, precision, precond):
+    if precond == fa.PRECOND_LEARNED and not learned.available():
+        pytest.skip("no trained preconditioner weights")
+    lam, tol = 1e-3, CG_TOL[precision]
+    d_o, pred_o, it_o = oracle_solve(scale, lam, tol)
+    s = linearized(scale, precision, relative_tolerance=tol, preconditioner=precond)
+    st, d, pred = s.solve_damped(lam)
+    diag = s.diagnostics()
+    assert st == fa.OK and diag.termination == fa.CG_CONVERGED
+    e = rel(d, d_o)
+    print(f"{scale['name']} precision {precision} preconditioner {precond}: step rel {e:.2e}, predicted "
+          f"{pred!r} oracle {pred_o!r}, PCG iterations {diag.iterations} oracle {it_o}")
+    assert e < STEP_TOL[precision]
+    assert abs(pred - pred_o) <= STEP_TOL[precision] * abs(pred_o)
+    s.close()
+
+
+def load_trajectory(name, p):
+    path = os.path.join(GOLDEN, TRAJECTORIES[name])
+    if not os.path.exists(path):
+        pytest.skip(f"no oracle trajectory fixture for {name}")
+    with open(path) as f:
+        fx = json.load(f)
+    pd = fx["problem"]
+    assert (pd["n_cameras"], pd["n_points"], pd["n_observations"]) == (p.n_cameras, p.n_points, p.n_observations)
+    assert pd["pixel_sum"] == float(np.sum(p.obs_pixel)) and pd["obs_point_sum"] == int(np.sum(p.obs_point))
+    return fx
+
+
+def sampled_state_error(ba, fx):
+    cams, pts = ba.state()
+    x = np.concatenate([np.ravel(cams), np.ravel(pts)])
+    st = fx["state"]
+    ref = np.asarray(st["sample_values"])
+    return rel(x[np.asarray(st["sample_index"])], ref), (np.linalg.norm(cams), np.linalg.norm(pts))
+
+
+def run_lm(ba, p, precision, steps, cg_tol, precond=fa.PRECOND_BLOCK_JACOBI):
+    ba.set_state(p.cameras, p.points)
+    backend = fa.Schur(precision=precision, relative_tolerance=cg_tol, preconditioner=precond)
+    lm = fa.LevenbergMarquardt(backend)
+    trace = []
+    lm.on_accept = lambda st: trace.append((st.cost, st.attempts))
+    out = lm.solve_batch(ba, fa.OptimizeOptions(steps, 1e-8, 0.0, 0.0))
+    backend.close()
+    return out, trace
+
+
+def test_fp64_trajectory_matches_oracle_fixture(scale):
+    p = scale["p"]
+    fx = load_trajectory(scale["name"], p)
+    ba = device(scale, fa.FP64)
+    out, trace = run_lm(ba, p, fa.FP64, fx["max_iterations"], 1e-10)
+    ref = fx["trace"]
+    assert abs(out.initial_cost - fx["initial_cost"]) <= 1e-12 * fx["initial_cost"]
+    worst = max(abs(c - r[0]) / r[0] for (c, _), r in zip(trace, ref))
+    e_state, norms = sampled_state_error(ba, fx)
+    print(f"{scale['name']} fp64 {len(trace)} steps: final {out.cost!r} oracle {fx['final_cost']!r} "
+          f"worst step rel {worst:.2e}, sampled state rel {e_state:.2e}, attempts {out.attempts}/{fx['attempts']}")
+    assert len(trace) == len(ref) and out.status == fx["status"]
+    assert [a for _, a in trace] == [int(r[1]) for r in ref]
+    assert worst <= 1e-9
+    assert abs(out.cost - fx["final_cost"]) <= 1e-9 * fx["final_cost"]
+    assert e_state <= 1e-6
+    assert abs(norms[0] - fx["state"]["camera_norm"]) <= 1e-6 * fx["state"]["camera_norm"]
+    assert abs(norms[1] - fx["state"]["point_norm"]) <= 1e-6 * fx["state"]["point_norm"]
+
+
+@pytest.mark.parametrize("precond", [fa.PRECOND_BLOCK_JACOBI, fa.PRECOND_LEARNED])
+def test_fp32_final_cost_matches_oracle_fixture(scale, precond):
+    if precond == fa.PRECOND_LEARNED and not learned.available():
+        pytest.skip("no trained preconditioner weights")
+    p = scale["p"]
+    fx = load_trajectory(scale["name"], p)
+    ba = device(scale, fa.FP32)
+    out, trace = run_lm(ba, p, fa.FP32, 150, 1e-3, precond)
+    dc = (out.cost - fx["final_cost"]) / fx["final_cost"]
+    e_state, _ = sampled_state_error(ba, fx)
+    print(f"{scale['name']} fp32 preconditioner {precond}: {out.accepted_steps} steps, final {out.cost!r} "
+          f"oracle {fx['final_cost']!r} ({fx['accepted_steps']} steps at CG 1e-10) rel {dc:.2e}, "
+          f"sampled state rel {e_state:.2e}")
+    assert abs(dc) <= 1e-5

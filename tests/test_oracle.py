@@ -259,7 +259,9 @@ def test_lm_converges_and_matches_independent_objective():
     assert 0.5 * 0.25 * dof * 0.8 < out.cost < 0.5 * 0.25 * 2 * p.n_observations
     assert out.linearizations == out.accepted_steps + 1
     assert out.attempts == out.accepted_steps + out.rejected_steps
-    assert np.all(np.diff(trace) < 0)
+    assert np.all(np.diff(trace[:, 0]) < 0)
+    assert np.all(np.diff(trace[:, 1]) >= 1) and trace[-1, 1] <= out.attempts
+    assert len(trace) == out.accepted_steps
 
 
 def test_ba_schur_solve_matches_dense_augmented_lstsq():
