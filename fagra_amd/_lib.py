@@ -196,6 +196,7 @@ SIGNATURES = {
     "fg_gschur_last_step_norm": (_d, [_VP]),
     "fg_covariance_options_default": (ctypes.c_int, [ctypes.POINTER(CovarianceOptions)]),
     "fg_dense_covariance": (ctypes.c_int, [_dp, _sz, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
+    "fg_dense_covariance_noalloc": (ctypes.c_int, [_dp, _sz, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
     "fg_schur_covariance": (ctypes.c_int, [_VP, _szp, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
     "fg_gschur_covariance": (ctypes.c_int, [_VP, _szp, _szp, _sz, ctypes.POINTER(CovarianceOptions), _dp]),
     "fg_bal_scan": (ctypes.c_int, [ctypes.c_char_p, _sz, ctypes.POINTER(BalSizes), ctypes.c_char_p, _sz]),

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "fg_internal.h"
@@ -252,43 +253,57 @@ int grid_of(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kB
 
 }  // namespace
 
-namespace fg {
+namespace {
 
-fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* columns, size_t k, double tolerance,
-                           double* out) {
-  if (!std::isfinite(tolerance) || tolerance < 0.0 || tolerance >= 1.0) return FG_INVALID_RANK_TOLERANCE;
-  for (size_t a = 0; a < k; ++a)
-    if (columns[a] >= n) return FG_UNKNOWN_KEY;
-  if (n > (size_t)INT_MAX / 2 || k > (size_t)INT_MAX / 2 || (k && n > SIZE_MAX / 8 / k)) return FG_INVALID_INFORMATION;
-  if (n == 0) return FG_OK;  // k == 0 here: the empty selection of an empty model
-  double *diag = nullptr, *y = nullptr, *c = nullptr;
-  int *flags = nullptr, *cols = nullptr;
-  auto cleanup = [&]() {
-    for (void* p : {(void*)diag, (void*)y, (void*)c, (void*)flags, (void*)cols})
-      if (p) (void)hipFree(p);
-  };
+// Device workspace of one covariance: the diagonal of H, the flags, Y (n x k), the k x k
+// covariance, the selected columns and the inverse of every panel's diagonal factor tile. Each
+// part starts on a 256-byte boundary.
+struct CovWork {
+  double* diag;
+  int* flags;
+  double* y;
+  double* c;
+  int* cols;
+  double* linv;
+};
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t cov_work_bytes(size_t n, size_t k) {
+  return align256(n * sizeof(double)) + align256(4 * sizeof(int)) + align256((n * k > 0 ? n * k : 1) * sizeof(double)) +
+         align256((k * k > 0 ? k * k : 1) * sizeof(double)) + align256((k > 0 ? k : 1) * sizeof(int)) +
+         align256(((n + kNB - 1) / kNB) * kNB * kNB * sizeof(double));
+}
+CovWork cov_work_carve(unsigned char* base, size_t n, size_t k) {
+  CovWork w;
+  w.diag = (double*)base;
+  base += align256(n * sizeof(double));
+  w.flags = (int*)base;
+  base += align256(4 * sizeof(int));
+  w.y = (double*)base;
+  base += align256((n * k > 0 ? n * k : 1) * sizeof(double));
+  w.c = (double*)base;
+  base += align256((k * k > 0 ? k * k : 1) * sizeof(double));
+  w.cols = (int*)base;
+  base += align256((k > 0 ? k : 1) * sizeof(int));
+  w.linv = (double*)base;
+  return w;
+}
+
+// The factorization, rank screen and selected solves of H (device, n x n, overwritten by its
+// factor) in the workspace w, on stream s; host_cols are the k selected coordinates (checked by
+// the caller). Allocates nothing.
+fg_status dense_covariance_ws(hipStream_t s, double* h, size_t n, const int* host_cols, size_t k, double tolerance,
+                              double* out, const CovWork& w) {
+  double *diag = w.diag, *y = w.y, *c = w.c, *linv = w.linv;
+  int *flags = w.flags, *cols = w.cols;
   auto fail = [&](fg_status st) {
     (void)hipStreamSynchronize(s);
-    cleanup();
     return st;
   };
-  if (hipMalloc(&diag, n * sizeof(double)) != hipSuccess || hipMalloc(&flags, 4 * sizeof(int)) != hipSuccess ||
-      hipMalloc(&y, std::max<size_t>(1, n * k) * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c, std::max<size_t>(1, k * k) * sizeof(double)) != hipSuccess ||
-      hipMalloc(&cols, std::max<size_t>(1, k) * sizeof(int)) != hipSuccess)
-    return fail(FG_DEVICE_ERROR);
   const int ni = (int)n, ki = (int)k;
-  const int panels = (ni + kNB - 1) / kNB;
-  double* linv = nullptr;  // per panel, the inverse of its diagonal factor tile
-  if (hipMalloc(&linv, (size_t)panels * kNB * kNB * sizeof(double)) != hipSuccess) return fail(FG_DEVICE_ERROR);
-  struct Free {
-    double* p;
-    ~Free() { (void)hipFree(p); }
-  } free_linv{linv};
-  std::vector<int> hc(columns, columns + k);
   int init[4] = {0, 0, INT_MAX, 0};
   if (hipMemcpyAsync(flags, init, sizeof(init), hipMemcpyHostToDevice, s) != hipSuccess ||
-      (k && hipMemcpyAsync(cols, hc.data(), k * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess))
+      (k && hipMemcpyAsync(cols, host_cols, k * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess))
     return fail(FG_DEVICE_ERROR);
   hipLaunchKernelGGL(k_cov_check, dim3(grid_of(n * n)), dim3(kBlock), 0, s, (const double*)h, n, diag, flags);
   int hf[4];
@@ -314,12 +329,9 @@ fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* col
   if (hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail(FG_DEVICE_ERROR);
-  if (hf[1]) return fail(FG_SINGULAR_INFORMATION);  // the factorization failed
-  if (hf[2] != INT_MAX) return fail((hf[2] & 3) == 2 ? FG_INVALID_INFORMATION : FG_SINGULAR_INFORMATION);
-  if (k == 0) {
-    cleanup();
-    return FG_OK;
-  }
+  if (hf[1]) return FG_SINGULAR_INFORMATION;  // the factorization failed
+  if (hf[2] != INT_MAX) return (hf[2] & 3) == 2 ? FG_INVALID_INFORMATION : FG_SINGULAR_INFORMATION;
+  if (k == 0) return FG_OK;
   hipLaunchKernelGGL(k_cov_unit, dim3(grid_of(n * k)), dim3(kBlock), 0, s, y, ni, ki, (const int*)cols);
   for (int k0 = 0; k0 < ni; k0 += kNB) {
     const int kb = std::min(kNB, ni - k0), rest = ni - k0 - kb;
@@ -336,8 +348,67 @@ fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* col
       hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail(FG_DEVICE_ERROR);
-  cleanup();
   return (hf[0] & 4) ? FG_INVALID_INFORMATION : FG_OK;
+}
+
+// Checks shared by the covariance entry points (src/covariance.rs:136-160, 214-230).
+fg_status check_selection(size_t n, const size_t* columns, size_t k, double tolerance) {
+  if (!std::isfinite(tolerance) || tolerance < 0.0 || tolerance >= 1.0) return FG_INVALID_RANK_TOLERANCE;
+  for (size_t a = 0; a < k; ++a)
+    if (columns[a] >= n) return FG_UNKNOWN_KEY;
+  if (n > (size_t)INT_MAX / 2 || k > (size_t)INT_MAX / 2 || (k && n > SIZE_MAX / 8 / k)) return FG_INVALID_INFORMATION;
+  return FG_OK;
+}
+
+// Largest problem of fg_dense_covariance_noalloc, and its scratch: the n x n staging copy of the
+// information followed by the workspace of an (n, k) covariance.
+constexpr size_t kNoAllocMaxN = 2048, kNoAllocMaxK = 256;
+constexpr size_t kScratchBytes = align256(kNoAllocMaxN * kNoAllocMaxN * sizeof(double)) +
+                                 cov_work_bytes(kNoAllocMaxN, kNoAllocMaxK);
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// Device memory of fg_dense_covariance_noalloc, one copy per GPU for the life of the process.
+// Host code never uses this name as an address: it reads each device's address once with
+// hipGetSymbolAddress and passes that pointer.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of scratch, looked up on the first call on that device.
+fg_status scratch_of_current_device(unsigned char** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return FG_DEVICE_ERROR;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FG_DEVICE_ERROR;
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FG_OK;
+}
+
+}  // namespace
+
+namespace fg {
+
+fg_status dense_covariance(hipStream_t s, double* h, size_t n, const size_t* columns, size_t k, double tolerance,
+                           double* out) {
+  fg_status st = check_selection(n, columns, k, tolerance);
+  if (st != FG_OK) return st;
+  if (n == 0) return FG_OK;  // k == 0 here: the empty selection of an empty model
+  void* base = nullptr;
+  if (hipMalloc(&base, cov_work_bytes(n, k)) != hipSuccess) return FG_DEVICE_ERROR;
+  std::vector<int> hc(columns, columns + k);
+  st = dense_covariance_ws(s, h, n, hc.data(), k, tolerance, out, cov_work_carve((unsigned char*)base, n, k));
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(base);
+  return st;
 }
 
 }  // namespace fg
@@ -347,6 +418,36 @@ extern "C" {
 fg_status fg_covariance_options_default(fg_covariance_options* o) {
   o->relative_pivot_tolerance = 64.0 * 2.220446049250313e-16;  // 64 epsilon (covariance.rs:145-151)
   return FG_OK;
+}
+
+fg_status fg_dense_covariance_noalloc(const double* information, size_t n, const size_t* columns, size_t k,
+                                      const fg_covariance_options* options, double* out) {
+  // the checks of fg_dense_covariance, in its order
+  if (!std::isfinite(options->relative_pivot_tolerance) || options->relative_pivot_tolerance < 0.0 ||
+      options->relative_pivot_tolerance >= 1.0)
+    return FG_INVALID_RANK_TOLERANCE;
+  for (size_t a = 0; a < k; ++a) {
+    if (columns[a] >= n) return FG_UNKNOWN_KEY;
+    for (size_t b = 0; b < a; ++b)
+      if (columns[b] == columns[a]) return FG_DUPLICATE_COVARIANCE_BLOCK;
+  }
+  fg_status st = check_selection(n, columns, k, options->relative_pivot_tolerance);
+  if (st != FG_OK) return st;
+  if (n > kNoAllocMaxN || k > kNoAllocMaxK) return FG_INVALID_OPTIONS;  // beyond the scratch
+  if (n == 0) return FG_OK;
+  unsigned char* base = nullptr;
+  st = scratch_of_current_device(&base);
+  if (st != FG_OK) return st;
+  hipStream_t s = nullptr;  // the null stream: no stream is created either
+  double* h = (double*)base;
+  const CovWork w = cov_work_carve(base + align256(kNoAllocMaxN * kNoAllocMaxN * sizeof(double)), n, k);
+  int hc[kNoAllocMaxK];
+  for (size_t a = 0; a < k; ++a) hc[a] = (int)columns[a];
+  if (hipMemcpyAsync(h, information, n * n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+    return FG_DEVICE_ERROR;
+  st = dense_covariance_ws(s, h, n, hc, k, options->relative_pivot_tolerance, out, w);
+  if (hipStreamSynchronize(s) != hipSuccess) return FG_DEVICE_ERROR;
+  return st;
 }
 
 fg_status fg_dense_covariance(const double* information, size_t n, const size_t* columns, size_t k,

@@ -359,6 +359,16 @@ fg_status fg_covariance_options_default(fg_covariance_options*);
  * (src/normal.rs:372-392). */
 fg_status fg_dense_covariance(const double* information, size_t n, const size_t* columns, size_t k,
                               const fg_covariance_options*, double* out);
+/* The same as fg_dense_covariance without allocating: no reference counterpart. For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The staging copy of the information and the workspace live
+ * in a file-scope device array that each GPU holds from its first call until the process
+ * ends (about 39 MB); the call makes no hipMalloc / hipFree and creates no stream (it runs on
+ * the null stream). Limits: n <= 2048 and k <= 256, else FG_INVALID_OPTIONS (before any
+ * device work). Every call on a GPU reuses that GPU's array: calls that overlap on one GPU
+ * (from several host threads) race. Results are those of fg_dense_covariance. */
+fg_status fg_dense_covariance_noalloc(const double* information, size_t n, const size_t* columns, size_t k,
+                                      const fg_covariance_options*, double* out);
 /* CovarianceBackend for Schur (src/schur.rs:548-556): undamped information J^T J of the current
  * linearization, selected variable blocks given by (offset, width) in reference coordinates
  * (cameras 9c/9, points 9 nc + 3p/3; generic: the block grid of the partition), in selection
