@@ -216,6 +216,7 @@ SIGNATURES = {
     "fg_mlp_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_VP)]),
     "fg_mlp_destroy": (None, [_VP]),
     "fg_mlp_set_params": (ctypes.c_int, [_VP, _fp]),
+    "fg_mlp_set_input_range": (ctypes.c_int, [_VP, _fp, _fp]),
     "fg_mlp_get_params": (ctypes.c_int, [_VP, _fp]),
     "fg_mlp_forward": (ctypes.c_int, [_VP, _sz, _fp, _fp]),
     "fg_mlp_loss_grad": (ctypes.c_int, [_VP, _sz, _fp, _fp, _dp, _fp]),

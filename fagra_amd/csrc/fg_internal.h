@@ -26,6 +26,9 @@ struct fg_mlp {
   double* h_parts = nullptr;
   hipStream_t stream = nullptr;
   int step = 0;
+  // Input range of the trained network (device [2][kIn]: lower bounds, then upper bounds);
+  // cameras whose features leave it get G = I. Null: the network is applied everywhere.
+  float* range = nullptr;
 };
 
 namespace fg {

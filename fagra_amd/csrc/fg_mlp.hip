@@ -162,6 +162,7 @@ void fg_mlp_destroy(fg_mlp* m) {
                   m->x, m->z1, m->z2, m->h1, m->h2, m->y, m->dy, m->dz2, m->dz1, m->target, m->parts};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (m->range) (void)hipFree(m->range);
   if (m->h_parts) (void)hipHostFree(m->h_parts);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -181,6 +182,27 @@ fg_status fg_mlp_set_params(fg_mlp* m, const float* params) {
   FG_TRY(hipMemsetAsync(m->adam_v, 0, kParams * 4, m->stream));
   m->step = 0;
   if (convert(m) != FG_OK) return FG_DEVICE_ERROR;
+  FG_TRY(hipStreamSynchronize(m->stream));
+  return FG_OK;
+}
+
+fg_status fg_mlp_set_input_range(fg_mlp* m, const float* lo, const float* hi) {
+  if (!lo || !hi) {  // clear: the network applies to every camera
+    if (m->range) {
+      (void)hipStreamSynchronize(m->stream);
+      (void)hipFree(m->range);
+      m->range = nullptr;
+    }
+    return FG_OK;
+  }
+  float r[2 * kIn];
+  for (int k = 0; k < kIn; ++k) {
+    if (!(lo[k] <= hi[k])) return FG_INVALID_OPTIONS;  // also rejects NaN bounds
+    r[k] = lo[k];
+    r[kIn + k] = hi[k];
+  }
+  if (!m->range) FG_TRY(hipMalloc((void**)&m->range, sizeof(r)));
+  FG_TRY(hipMemcpyAsync(m->range, r, sizeof(r), hipMemcpyHostToDevice, m->stream));
   FG_TRY(hipStreamSynchronize(m->stream));
   return FG_OK;
 }

@@ -152,9 +152,13 @@ __device__ __forceinline__ void build_g(const float* yrow, float g[45]) {
 // the network output y and the block-Jacobi factor L of B + lambda I. One 64-lane group per
 // camera: lanes 0-8 form column j of W = L^-T G (a triangular solve each), lanes 0-44 then
 // form one packed entry of M = W W^T.
+// With range (device [2][kIn], lower then upper bounds) a camera whose features xf (f32
+// [nc][kIn]) leave the range the network was trained on keeps G = I, i.e. block-Jacobi.
 static __global__ void __launch_bounds__(256) k_mlp_build_g(int nc, const int* __restrict__ cam_off,
                                                             const float* __restrict__ y,
                                                             const double* __restrict__ lc,
+                                                            const float* __restrict__ xf,
+                                                            const float* __restrict__ range,
                                                             double* __restrict__ mout) {
   __shared__ double w[4][9][9];
   const int grp = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -163,10 +167,16 @@ static __global__ void __launch_bounds__(256) k_mlp_build_g(int nc, const int* _
   if (active && lane < 9) {
     const int j = lane;
     const float* yr = y + (size_t)kOutPad * c;
+    bool inside = true;
+    if (range)
+      for (int k = 0; k < kIn; ++k) {
+        const float f = xf[(size_t)kIn * c + k];
+        inside = inside && f >= range[k] && f <= range[kIn + k];
+      }
     double l[45], col[9];
     for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
     for (int i = 0; i < 9; ++i)  // column j of G = I + s tril(y)
-      col[i] = i < j ? 0.0 : double(kOutScale * yr[sym_index(9, i, j)]) + (i == j ? 1.0 : 0.0);
+      col[i] = i < j ? 0.0 : (inside ? double(kOutScale * yr[sym_index(9, i, j)]) : 0.0) + (i == j ? 1.0 : 0.0);
     solve_upper_packed<double, 9>(l, col);
     for (int i = 0; i < 9; ++i) w[grp][i][j] = col[i];
   }

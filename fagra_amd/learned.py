@@ -16,6 +16,7 @@ from . import _lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "learned_precond.f32")
+RANGE = os.path.join(HERE, "learned_precond.range.f32")  # 47 lower then 47 upper feature bounds
 N_IN, N_OUT = 47, 45
 
 
@@ -41,11 +42,28 @@ class Network:
             self.set_params(params)
 
     @classmethod
-    def load(cls, path=WEIGHTS):
-        return cls(params=np.fromfile(path, dtype=np.float32))
+    def load(cls, path=WEIGHTS, range_path=RANGE):
+        net = cls(params=np.fromfile(path, dtype=np.float32))
+        if range_path and os.path.exists(range_path):
+            r = np.fromfile(range_path, dtype=np.float32)
+            net.set_input_range(r[:N_IN], r[N_IN:])
+        return net
 
-    def save(self, path=WEIGHTS):
+    def save(self, path=WEIGHTS, range_path=RANGE):
         self.params().astype(np.float32).tofile(path)
+        if range_path and getattr(self, "input_range", None) is not None:
+            np.concatenate(self.input_range).astype(np.float32).tofile(range_path)
+
+    def set_input_range(self, lo, hi):
+        """Cameras whose features leave [lo, hi] keep block-Jacobi (None clears the range)."""
+        if lo is None:
+            _lib.check(_lib.load().fg_mlp_set_input_range(self.h, None, None), "fg_mlp_set_input_range")
+            self.input_range = None
+            return
+        a, pa = _f(lo)
+        b, pb = _f(hi)
+        _lib.check(_lib.load().fg_mlp_set_input_range(self.h, pa, pb), "fg_mlp_set_input_range")
+        self.input_range = (a.copy(), b.copy())
 
     def close(self):
         if getattr(self, "h", None):
@@ -104,15 +122,25 @@ def capture(schur, lam):
     return f, t, c
 
 
-def collect(seeds, workload="ring256", precision=None, cg_tol=1e-3, max_iter=15, log=None):
+def feature_range(features, margin=0.02):
+    """Per-feature bounds of a training set, widened by margin times their span; the damping
+    feature (the last) keeps the exact span of the dampings trained on."""
+    lo, hi = features.min(axis=0), features.max(axis=0)
+    pad = margin * (hi - lo)
+    lo, hi = lo - pad, hi + pad
+    lo[-1], hi[-1] = features[:, -1].min(), features[:, -1].max()
+    return lo.astype(np.float32), hi.astype(np.float32)
+
+
+def collect(seeds, workload="ring256", precision=None, cg_tol=1e-3, max_iter=15, log=None, **overrides):
     """Run LM on seeded problems and capture every attempted Schur system (one sample per
-    observed camera and attempt)."""
+    observed camera and attempt); overrides go to the workload generator."""
     import fagra_amd as fa
     from . import synthetic
     precision = fa.FP32 if precision is None else precision
     feats, targs = [], []
     for seed in seeds:
-        p = synthetic.generate(workload, seed=seed)
+        p = synthetic.generate(workload, seed=seed, **overrides)
         ba = fa.BalProblem(p, precision=precision)
         holder = {}
 

@@ -181,6 +181,10 @@ fg_status fg_mlp_create(uint64_t seed, fg_mlp** out);
 void fg_mlp_destroy(fg_mlp*);
 fg_status fg_mlp_set_params(fg_mlp*, const float* params);
 fg_status fg_mlp_get_params(fg_mlp*, float* params);
+/* Input range of the trained network (lo, hi: 47 feature bounds each): a camera whose features
+ * leave it -- a damping or a Gram block unlike any the network was trained on -- keeps plain
+ * block-Jacobi (G = I). Null lo and hi clear the range. */
+fg_status fg_mlp_set_input_range(fg_mlp*, const float* lo, const float* hi);
 /* out (n x 45) = network(features n x 47) */
 fg_status fg_mlp_forward(fg_mlp*, size_t n, const float* features, float* out);
 /* Mean loss ||G^T A G - I||_F^2 over n rows (targets n x 45 packed lower) and its gradient. */

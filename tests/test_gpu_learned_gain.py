@@ -1,0 +1,92 @@
+"""What the shipped preconditioner network gains over block-Jacobi, measured (GPU only).
+
+The network (fagra_amd/learned_precond.f32, trained by fagra_amd/train_precond.py on Schur
+systems that LM emits on seeded 256- and 400-camera problems, damping 1e-12 .. 1e12) corrects
+the block-Jacobi inverse (B_i + lambda I)^-1 (src/schur.rs:658-682) towards the inverse of the
+camera's diagonal block of the reduced system, S_cc. Two questions, each on held-out inputs:
+
+* Does it learn that map? Its loss ||G^T A G - I||^2 on held-out systems against G = I
+  (block-Jacobi): measured 0.434 -> 0.0034.
+* Does it cut PCG iterations to the same tolerance (1e-3, the benchmark's)? On these problems
+  block-Jacobi PCG already converges in 2-19 iterations per solve and the exact S_cc blocks
+  are no better a preconditioner than B_i, so the network's iterations stay within a few per
+  cent of block-Jacobi's: measured 360 against 356 over the held-out grid below (and 917
+  against 942 over the benchmark's LM solve to its target). The test asserts the 5% bound, and
+  that cameras outside the network's input range fall back to block-Jacobi.
+"""
+import numpy as np
+import pytest
+
+import fagra_amd as fa
+from fagra_amd import learned, synthetic
+
+pytestmark = pytest.mark.gpu
+
+LAMBDAS = [1e-12, 1e-10, 1e-8, 1e-6, 1e-4, 1e-3, 1e-2, 1e-1]
+
+
+def iterations(ba, precond, net, tol=1e-3):
+    s = fa.Schur(precision=fa.FP32, relative_tolerance=tol, preconditioner=precond, network=net)
+    assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+    out = []
+    for lam in LAMBDAS:
+        st, _, _ = s.solve_damped(lam)
+        d = s.diagnostics()
+        assert st == fa.OK and d.termination == fa.CG_CONVERGED, (lam, st, d.termination)
+        out.append(d.iterations)
+    s.close()
+    return out
+
+
+def at_lm_state(p, steps):
+    ba = fa.BalProblem(p, precision=fa.FP32)
+    if steps:
+        lm = fa.LevenbergMarquardt(fa.Schur(precision=fa.FP32, relative_tolerance=1e-3))
+        lm.solve_batch(ba, fa.OptimizeOptions(steps, 0.0, 0.0, 0.0))
+    return ba
+
+
+@pytest.fixture(scope="module")
+def net():
+    if not learned.available():
+        pytest.skip("no trained preconditioner weights")
+    return learned.Network.load()
+
+
+def test_network_fits_the_schur_diagonal_on_held_out_systems(net):
+    feats, targs = learned.collect([1000], max_iter=40)
+    identity = learned.Network(seed=0)  # zero output layer: G = I, plain block-Jacobi
+    base = identity.loss_grad(feats, targs)[0]
+    trained = net.loss_grad(feats, targs)[0]
+    print(f"held-out loss: block-Jacobi {base:.4f}, network {trained:.5f} over {len(feats)} camera systems")
+    assert trained < 0.05 * base
+
+
+def test_pcg_iterations_against_block_jacobi(net):
+    cases = [("ring256", 1000, {}, (0, 20, 60)), ("rings1500", 0, {}, (0, 106)),
+             ("zipf4000", 1000, {"n_cameras": 400, "n_points": 60000}, (0, 20))]
+    tot_bj = tot_ln = 0
+    for name, seed, overrides, states in cases:
+        p = synthetic.generate(name, seed=seed, **overrides)
+        for steps in states:
+            ba = at_lm_state(p, steps)
+            bj = iterations(ba, fa.PRECOND_BLOCK_JACOBI, None)
+            ln = iterations(ba, fa.PRECOND_LEARNED, net)
+            print(f"{name} seed {seed} after {steps} LM steps: block-Jacobi {bj} learned {ln}", flush=True)
+            tot_bj += sum(bj)
+            tot_ln += sum(ln)
+            ba.close()
+    print(f"PCG iterations to 1e-3 over the grid: block-Jacobi {tot_bj}, learned {tot_ln}")
+    assert tot_ln <= 1.05 * tot_bj
+
+
+def test_cameras_outside_the_input_range_keep_block_jacobi(net):
+    p = synthetic.generate("ring256", seed=1001)
+    ba = at_lm_state(p, 0)
+    gated = learned.Network(params=net.params())
+    big = np.full(learned.N_IN, 1e30, dtype=np.float32)
+    gated.set_input_range(big, big)  # no camera's features lie in [1e30, 1e30]
+    bj = iterations(ba, fa.PRECOND_BLOCK_JACOBI, None)
+    ln = iterations(ba, fa.PRECOND_LEARNED, gated)
+    print(f"gated out: block-Jacobi {bj} learned {ln}")
+    assert all(abs(a - b) <= 1 for a, b in zip(bj, ln))
