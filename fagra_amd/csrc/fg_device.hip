@@ -125,6 +125,7 @@ struct fg_schur {
   void *rec = nullptr, *res = nullptr, *rec_cm = nullptr, *res_cm = nullptr;
   void *hpp = nullptr, *gp = nullptr, *hcc = nullptr, *gc = nullptr;
   void *dc = nullptr, *dp = nullptr, *bs = nullptr, *cs = nullptr, *lc = nullptr, *mp = nullptr;
+  void* lpp = nullptr;  // f64 [np][6]: factor of Hpp + lambda Dp^2 of the current attempt
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
@@ -306,7 +307,7 @@ void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const dou
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur],
                      fw != nullptr ? (const T*)s->prow : nullptr,
                      (const double*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda,
-                     (double*)s->tp, delta_out, rows_out, parts, st, nullptr);
+                     (double*)s->tp, delta_out, rows_out, parts, st, nullptr, (const double*)s->lpp);
 }
 
 template <typename T, int MODE>
@@ -457,7 +458,8 @@ fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = 
                      b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
   hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
                      b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail, (const T*)s->gp,
-                     (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr);
+                     (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp,
+                     (double*)s->lpp);
   FG_TRY(hipGetLastError());
   if (fails == nullptr) return FG_OK;
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -818,6 +820,7 @@ fg_status alloc_model(fg_schur* s) {
   // of the PCG direction is stored in T
   FG_TRY(dalloc((double**)&s->lc, 45 * nc));
   FG_TRY(dalloc((double**)&s->mp, 6 * np));
+  FG_TRY(dalloc((double**)&s->lpp, 6 * np));
   FG_TRY(dalloc((double**)&s->tp, 4 * np));
   FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
   if (sizeof(T) == 4) FG_TRY(dalloc(&s->fw_w, kFwd * nc));
@@ -853,7 +856,7 @@ void free_model(fg_schur* s) {
     s->cg_graph_size[i] = 0;
   }
   for (void** v : {&s->rec, &s->res, &s->rec_cm, &s->res_cm, &s->hpp, &s->gp, &s->hcc, &s->gc, &s->dc, &s->dp,
-                   &s->bs, &s->cs, &s->lc, &s->mp, &s->tp, &s->fw, &s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs,
+                   &s->bs, &s->cs, &s->lc, &s->mp, &s->lpp, &s->tp, &s->fw, &s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs,
                    &s->zero, &s->gmat, &s->mbj}) {
     if (*v) (void)hipFree(*v);
     *v = nullptr;
@@ -1841,7 +1844,7 @@ void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
                      step ? (const T*)s->prow : nullptr, (const double*)s->mp, (const T*)s->gp, (const T*)s->dp,
                      step ? ws + 9 * nc : nullptr, (const double*)l->d_u, 0.0,
                      damped && step ? l->d_ub + 9 * nc : nullptr, l->d_zt + 9 * nc, l->d_u, l->d_up,
-                     (const CgState*)nullptr, st);
+                     (const CgState*)nullptr, st, (const double*)nullptr);
   // the camera pass gathers U by observation slot (k_slot in the gather-index argument)
   hipLaunchKernelGGL((k_cameras<T, cLsmr>), dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                      b->d_k_slot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,

@@ -708,13 +708,30 @@ __device__ __forceinline__ void store_tp(double* tp, int i, const double t[3]);
 
 // With t_out: also the right-hand side's point solves t_p = (Hpp + lambda Dp^2)^-1 gp of
 // factor_and_reduce_rhs (what k_points<kRhs> computes), from the factor just stored.
+// With lpp: also the unscaled factor of Hpp + lambda Dp^2 from the linearization's f64 Hpp
+// (the Gram of the Jp the products re-form), which k_points<kSchur> solves with; a zero
+// factor where it fails (an unobserved point at lambda = 0), where kSchur's t is zero.
 template <typename T>
 __global__ void k_factor_points(int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
                                 double lambda, double* __restrict__ lp, int* fail_count,
-                                const T* __restrict__ gp, const T* __restrict__ dpv, double* __restrict__ t_out) {
+                                const T* __restrict__ gp, const T* __restrict__ dpv, double* __restrict__ t_out,
+                                const double* __restrict__ hpp, double* __restrict__ lpp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   double* l = lp + 6 * p;
+  if (lpp != nullptr) {
+    double h[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h[k] = pt_len[p] == 0 ? 0.0 : hpp[6 * p + k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double d = double(dpv[3 * p + a]);
+      h[sym3(a, a)] += lambda * d * d;
+    }
+    const bool ok = pt_len[p] != 0 && chol_packed<double, 3>(h);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) lpp[6 * p + k] = ok ? h[k] : 0.0;
+  }
   if (pt_len[p] == 0) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) l[i] = 0.0;
@@ -934,7 +951,12 @@ __device__ __forceinline__ void point_obs_row(const PointArgs& pa, int s, const 
                                               double w[3], double h[6], double& uu, double coef, double gr[3]) {
   ObsGeom<double> geo;
   obs_geom(to_d(rec), ct, geo);
-  if (MODE == kSchur || MODE == kSchurSums) {
+  if (MODE == kSchur) {
+    // w += Jp^T Jc z; the point block comes factored (k_factor_points)
+    double u0, u1;
+    jc_apply(geo, f, u0, u1);
+    jpt_accumulate(geo, ct, u0, u1, w);
+  } else if (MODE == kSchurSums) {
     double u0, u1;
     jc_apply(geo, f, u0, u1);
     double j0[3], j1[3];
@@ -1026,6 +1048,76 @@ __device__ __forceinline__ void coop_prow(const float* __restrict__ prow, int c,
   f[9] = f[10] = f[11] = 0.0;
 }
 
+// The wave's 64 FP32 product rows (6 pieces of 16 bytes each) straight from L2 into its 6 KB
+// LDS slice with global_load_lds: round k moves pieces 64k .. 64k + 63, lane m fetching piece
+// P = 64k + m, i.e. piece P % 6 of the row of lane P / 6 (whose camera a lane shuffle
+// provides), and the hardware stores it at slice + 16 P, so row r lands at slice + 96 r. No
+// registers stage the data and no LDS store instructions are issued. Rows of empty slots
+// (camera -1) fetch camera 0's row, which no lane reads. Every lane of the wave must call this.
+// The waits: the previous row's LDS reads complete before the new data may land (lgkmcnt),
+// and the new data has landed before it is read (vmcnt).
+constexpr int kRowPieces = kPRow / 4;  // 6
+// The camera of the row each of lane's pieces belongs to (lane shuffles), camera -1 as 0.
+__device__ __forceinline__ void piece_cameras(int c, int lane, int cr[kRowPieces]) {
+#pragma unroll
+  for (int k = 0; k < kRowPieces; ++k) cr[k] = max(__shfl(c, (kWave * k + lane) / kRowPieces, kWave), 0);
+}
+__device__ __forceinline__ void rows_to_lds_dma(const Vec4<float>* __restrict__ prow, const int cr[kRowPieces],
+                                                int lane, Vec4<float>* __restrict__ lds) {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+#pragma unroll
+  for (int k = 0; k < kRowPieces; ++k) {
+    const int piece = kWave * k + lane;
+    const int r = piece / kRowPieces;
+    __builtin_amdgcn_global_load_lds(
+        reinterpret_cast<const void*>(prow + __umul24(cr[k], kRowPieces) + (piece - kRowPieces * r)),
+        (__attribute__((address_space(3))) void*)(lds + kWave * k), 16, 0, 0);
+  }
+}
+// The wait for rows_to_lds_dma (and every other vector-memory load issued so far).
+__device__ __forceinline__ void rows_landed() {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+}
+
+// w += Jp^T Jc z for one observation of the f32 Schur product, its camera's product row read
+// from LDS piece by piece as the computation needs it (pieces 0-2: rows of R with t_i; 3: f,
+// k1, k2, fw0; 4, 5: fw1-8), which keeps fewer values live than the whole row in f64 registers.
+// The same arithmetic as obs_geom, jc_apply and jpt_accumulate.
+__device__ __forceinline__ void schur_obs_lds(const Vec4<float>* __restrict__ row, const Vec4<float>& rec,
+                                              double w[3]) {
+  const double qx = rec.x, qy = rec.y, z = rec.z, sw = rec.w;
+  const Vec4<float> p3 = row[3];
+  const double f = p3.x, k1 = p3.y, k2 = p3.z;
+  const double r2 = qx * qx + qy * qy;
+  const double radial = 1.0 + k1 * r2 + k2 * r2 * r2;
+  const double c2 = 2.0 * k1 + 4.0 * k2 * r2;
+  const double a = f * (radial + c2 * qx * qx);
+  const double b = f * (c2 * qx * qy);
+  const double d = f * (radial + c2 * qy * qy);
+  const double iz = -1.0 / z;
+  const double dp00 = iz * a, dp01 = iz * b, dp02 = iz * (a * qx + b * qy);
+  const double dp10 = iz * b, dp11 = iz * d, dp12 = iz * (b * qx + d * qy);
+  const Vec4<float> p0 = row[0], p1 = row[1], p2 = row[2];
+  const double pt0 = -qx * z - double(p0.w), pt1 = -qy * z - double(p1.w), pt2 = z - double(p2.w);
+  const double fr2 = f * r2, fr4 = fr2 * r2;
+  const Vec4<float> p4 = row[4], p5 = row[5];
+  const double f0 = p3.w, f1 = p4.x, f2 = p4.y, f3 = p4.z, f4 = p4.w, f5 = p5.x, f6 = p5.y, f7 = p5.z,
+               f8 = p5.w;
+  const double v0 = f1 * pt2 - f2 * pt1 + f3;
+  const double v1 = f2 * pt0 - f0 * pt2 + f4;
+  const double v2 = f0 * pt1 - f1 * pt0 + f5;
+  const double sv = radial * f6 + fr2 * f7 + fr4 * f8;
+  const double u0 = sw * (dp00 * v0 + dp01 * v1 + dp02 * v2 + qx * sv);
+  const double u1 = sw * (dp10 * v0 + dp11 * v1 + dp12 * v2 + qy * sv);
+  const double e0 = sw * (u0 * dp00 + u1 * dp10);
+  const double e1 = sw * (u0 * dp01 + u1 * dp11);
+  const double e2 = sw * (u0 * dp02 + u1 * dp12);
+  w[0] += double(p0.x) * e0 + double(p1.x) * e1 + double(p2.x) * e2;
+  w[1] += double(p0.y) * e0 + double(p1.y) * e1 + double(p2.y) * e2;
+  w[2] += double(p0.z) * e0 + double(p1.z) * e1 + double(p2.z) * e2;
+}
+
 // One lane's observations of a point-major pass (SELL chunk, two observations per step so
 // that both loads issue before either is used). With COOP (FP32 product rows) the rows come
 // through coop_prow, and every lane of the wave must take part.
@@ -1037,7 +1129,39 @@ __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int 
                                             double w[3], double h[6], double& uu, double coef, double gr[3],
                                             Vec4<float>* __restrict__ coop = nullptr) {
   const int base = pa.chunk_base[chunk];
-  if constexpr (COOP) {
+  if constexpr (COOP && MODE == kSchur && sizeof(T) == 4) {
+    // Step j's rows travel while step j + 1's camera index and record are fetched, so one
+    // memory round trip is exposed per step instead of three (index, rows, record).
+    const Vec4<float>* pr = reinterpret_cast<const Vec4<float>*>(fw);
+    const Vec4<float>* rc = reinterpret_cast<const Vec4<float>*>(rec);
+    // Step j's rows travel while step j + 1's camera index and record are fetched, so one
+    // memory round trip is exposed per step instead of three (index, rows, record); the
+    // shuffles of step j + 1's cameras are issued before step j's arithmetic.
+    int c_cur = -1;
+    Vec4<float> r_cur{0.f, 0.f, 0.f, 0.f};
+    if (j0 < j1) {
+      c_cur = pa.slot_cam[base + j0 * kWave + lane];
+      r_cur = rc[base + j0 * kWave + lane];
+    }
+    int cr[kRowPieces];
+    piece_cameras(c_cur, lane, cr);
+    for (int j = j0; j < j1; ++j) {
+      rows_to_lds_dma(pr, cr, lane, coop);
+      int c_nxt = -1;
+      Vec4<float> r_nxt{0.f, 0.f, 0.f, 0.f};
+      if (j + 1 < j1) {  // uniform over the wave
+        const int s1 = base + (j + 1) * kWave + lane;
+        c_nxt = pa.slot_cam[s1];
+        r_nxt = rc[s1];
+      }
+      rows_landed();
+      piece_cameras(c_nxt, lane, cr);
+      if (c_cur >= 0) schur_obs_lds(coop + lane * kRowPieces, r_cur, w);
+      c_cur = c_nxt;
+      r_cur = r_nxt;
+    }
+    return;
+  } else if constexpr (COOP) {
     const float* pr = reinterpret_cast<const float*>(fw);
     // one observation per step: two in flight cost 40 more VGPRs and a wave per SIMD, and
     // measured slower
@@ -1066,13 +1190,19 @@ __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int 
   }
 }
 
+// Waves per SIMD the point-major kernel is compiled for: the f32 Schur product is held to 96
+// VGPRs for five (its LDS row exchange, 30 KB per workgroup, also admits five); the other
+// modes keep the compiler's default.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kBlock) k_points(
+constexpr int point_waves() { return 1; }
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(point_waves<T, MODE>()))) k_points(
     PointArgs pa, const Vec4<T>* __restrict__ rec, const Vec2<T>* __restrict__ res, const T* __restrict__ tab,
     const T* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
     const T* __restrict__ dpv, const double* __restrict__ delta_p_in, const double* __restrict__ rows_in,
     double lambda, double* __restrict__ t_out, double* __restrict__ delta_out, double* __restrict__ rows_out,
-    double* parts, const CgState* st, const LsmrState* ls) {
+    double* parts, const CgState* st, const LsmrState* ls, const double* __restrict__ lpp) {
   __shared__ double smem[kBlock / kWave];
   if (MODE == kSchur && st != nullptr) {
     if (st->done) return;
@@ -1108,14 +1238,21 @@ __global__ void __launch_bounds__(kBlock) k_points(
   // chunk's empty slots, whose camera is -1); not in kCheck, whose larger sweep went from 96 to
   // 132 VGPRs with it and ran slower (63.8 against 56.2 us per launch)
   constexpr bool kCoop = sizeof(T) == 4 && point_fwd_mode<kSweep>() && MODE != kCheck;
-  __shared__ Vec4<float> coop[kCoop ? kChunksPerBlock * kWave * kCoopStride : 1];
+  // kSchur's DMA rows need 6 pieces per row, the register exchange of the other modes 7
+  constexpr int kCoopRow = MODE == kSchur ? kRowPieces : kCoopStride;
+  __shared__ Vec4<float> coop[kCoop ? kChunksPerBlock * kWave * kCoopRow : 1];
   if (MODE != kRhs && chunk < pa.nchunks) {
     const int len = pa.chunk_len[chunk];
     const int j0 = split ? len * wave / kChunksPerBlock : 0;
     const int j1 = split ? len * (wave + 1) / kChunksPerBlock : len;
-    if (kCoop && fw != nullptr) {
+    if constexpr (kCoop && MODE == kSchur) {
+      // every Schur product names its forward table; only this sweep is compiled, so the
+      // register budget is the sweep's own
       point_sweep<T, kSweep, kCoop>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu,
-                                    coef, gr, coop + wave * kWave * kCoopStride);
+                                    coef, gr, coop + wave * kWave * kCoopRow);
+    } else if (kCoop && fw != nullptr) {
+      point_sweep<T, kSweep, kCoop>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu,
+                                    coef, gr, coop + wave * kWave * kCoopRow);
     } else if (live) {
       point_sweep<T, kSweep>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu, coef,
                              gr);
@@ -1123,7 +1260,7 @@ __global__ void __launch_bounds__(kBlock) k_points(
   }
   // sums a split chunk's waves must combine before the points are finished (uu is additive
   // over the block and goes to the block sum as it is)
-  constexpr int kMeet = MODE == kCheck ? 12 : MODE == kSchur ? 9 : (MODE == kRhs || MODE == kJv) ? 0 : 3;
+  constexpr int kMeet = MODE == kCheck ? 12 : (MODE == kRhs || MODE == kJv) ? 0 : 3;
   if (kMeet > 0 && split) {
     __shared__ double meet[kMeet > 0 ? kMeet * kWave : 1];
     double v[12];
@@ -1199,18 +1336,17 @@ __global__ void __launch_bounds__(kBlock) k_points(
         g2s += (g / d) * (g / d);
       }
     } else if (MODE == kSchur) {
-      // Hpp + lambda Dp^2 re-formed from the same rows and factored in registers: the
-      // eliminated projection is then exact for the Jp of this pass (the stored factor of
-      // the same block, scaled and rounded differently, slowed tight PCG solves 2x)
+      // t = (Hpp + lambda Dp^2)^-1 w with the unscaled factor of this attempt: Hpp is the f64
+      // Gram of the Jp this pass re-forms (accumulated at linearization in the same order), so
+      // the eliminated projection stays exact for it (a factor of the scaled block C + lambda
+      // I, rounded differently, slowed tight PCG solves 2x)
       double t[3] = {w[0], w[1], w[2]};
+      double l[6];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double d = double(dpv[3 * i + a]);
-        h[sym3(a, a)] += lambda * d * d;
-      }
-      if (chol_packed<double, 3>(h)) {
-        solve_lower_packed<double, 3>(h, t);
-        solve_upper_packed<double, 3>(h, t);
+      for (int k = 0; k < 6; ++k) l[k] = lpp[6 * i + k];
+      if (l[0] != 0.0) {
+        solve_lower_packed<double, 3>(l, t);
+        solve_upper_packed<double, 3>(l, t);
       } else {
         t[0] = t[1] = t[2] = 0.0;  // unobserved point at lambda = 0: no elimination term
       }
