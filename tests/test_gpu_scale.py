@@ -13,11 +13,13 @@ each preconditioner. Bars as in the small-problem parity tests: FP64 products 1e
 src/schur.rs:788).
 
 Trajectories: committed oracle fixtures (tests/golden/trajectory_*.json, written by
-tests/golden/make_trajectory.py) hold the oracle's LM run at CG tolerance 1e-10 -- one record
-per accepted step (cost, attempts) and a digest of the final state. The FP64 GPU run of the
-same length must agree step by step (cost 1e-9 relative, same attempts) and in the sampled
-final state; the FP32 runs (block-Jacobi and learned) at the benchmark's CG tolerance 1e-3 must
-end within 1e-5 of the oracle's final cost.
+tests/golden/make_trajectory.py) hold the oracle's LM run at CG tolerance 1e-10 for its first
+steps -- one record per accepted step (cost, attempts) and a digest of the final state. The
+FP64 GPU run of the same length must agree step by step (cost 1e-9 relative, same attempts) and
+in the sampled final state. Converged: tests/golden/converged_<workload>_0.json (written by
+tests/golden/make_converged.py) holds the oracle's converged cost and state; the FP32 runs
+(block-Jacobi and learned) at the benchmark's CG tolerance 1e-3 must end within 1e-5 of that
+cost, and the converged FP64 GPU run within 1e-9 of it.
 """
 import json
 import os
@@ -35,8 +37,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 PROD_TOL = {fa.FP64: 1e-10, fa.FP32: 2e-5}
 STEP_TOL = {fa.FP64: 1e-8, fa.FP32: 3e-3}
 CG_TOL = {fa.FP64: 1e-10, fa.FP32: 1e-6}
-TRAJECTORIES = {"rings1500": "trajectory_rings1500_0_cg1e-10_400.json",
-                "zipf4000": "trajectory_zipf4000_0_cg1e-10_60.json"}
+TRAJECTORIES = {"rings1500": "trajectory_rings1500_0_cg1e-10_30.json",
+                "zipf4000": "trajectory_zipf4000_0_cg1e-10_8.json"}
+CONVERGED = {"rings1500": "converged_rings1500_0.json", "zipf4000": "converged_zipf4000_0.json"}
 
 
 def rel(a, b):
@@ -147,10 +150,10 @@ def test_damped_solve_at_scale(scale, precision, precond):
     s.close()
 
 
-def load_trajectory(name, p):
-    path = os.path.join(GOLDEN, TRAJECTORIES[name])
+def load_trajectory(name, p, table=TRAJECTORIES):
+    path = os.path.join(GOLDEN, table[name])
     if not os.path.exists(path):
-        pytest.skip(f"no oracle trajectory fixture for {name}")
+        pytest.skip(f"no oracle fixture {table[name]} for {name}")
     with open(path) as f:
         fx = json.load(f)
     pd = fx["problem"]
@@ -203,12 +206,37 @@ def test_fp32_final_cost_matches_oracle_fixture(scale, precond):
     if precond == fa.PRECOND_LEARNED and not learned.available():
         pytest.skip("no trained preconditioner weights")
     p = scale["p"]
-    fx = load_trajectory(scale["name"], p)
+    fx = load_trajectory(scale["name"], p, CONVERGED)
     ba = device(scale, fa.FP32)
     out, trace = run_lm(ba, p, fa.FP32, 150, 1e-3, precond)
     dc = (out.cost - fx["final_cost"]) / fx["final_cost"]
     e_state, _ = sampled_state_error(ba, fx)
     print(f"{scale['name']} fp32 preconditioner {precond}: {out.accepted_steps} steps, final {out.cost!r} "
-          f"oracle {fx['final_cost']!r} ({fx['accepted_steps']} steps at CG 1e-10) rel {dc:.2e}, "
+          f"oracle converged {fx['final_cost']!r} ({fx['accepted_steps']} steps at CG 1e-10) rel {dc:.2e}, "
           f"sampled state rel {e_state:.2e}")
     assert abs(dc) <= 1e-5
+
+
+def test_converged_fp64_matches_oracle_fixture(scale):
+    # the oracle's procedure: LM calls of segment_steps accepted steps each, every call
+    # starting from the last accepted state
+    p = scale["p"]
+    fx = load_trajectory(scale["name"], p, CONVERGED)
+    ba = device(scale, fa.FP64)
+    ba.set_state(p.cameras, p.points)
+    costs, done = [], 0
+    while done < fx["accepted_steps"]:
+        lm = fa.LevenbergMarquardt(fa.Schur(precision=fa.FP64, relative_tolerance=1e-10))
+        n = min(fx["segment_steps"], fx["accepted_steps"] - done)
+        out = lm.solve_batch(ba, fa.OptimizeOptions(n, 1e-8, 0.0, 0.0))
+        done += out.accepted_steps
+        costs.append(out.cost)
+        if out.status != fa.NO_CONVERGENCE or out.accepted_steps == 0:
+            break
+    worst = max(abs(a - b) / b for a, b in zip(costs, fx["segment_costs"]))
+    dc = (costs[-1] - fx["final_cost"]) / fx["final_cost"]
+    e_state, _ = sampled_state_error(ba, fx)
+    print(f"{scale['name']} fp64 {done} steps in {len(costs)} segments: final {costs[-1]!r} oracle "
+          f"{fx['final_cost']!r} rel {dc:.2e}, worst segment rel {worst:.2e}, sampled state rel {e_state:.2e}")
+    assert len(costs) == len(fx["segment_costs"])
+    assert worst <= 1e-9 and abs(dc) <= 1e-9
