@@ -59,14 +59,15 @@ def kernel_bytes(nc, npts, no, precision):
     cache traffic, not algorithmic bytes).
 
     k_points<kSchur> (one lane per point): per observation the compact record (4 x T) and its
-    camera index (4 B); per point the column scales (3 x T) in and the solve t (one 4 x T
-    row) out; per camera the table row it reads (16 x T) and the forward table of p (12 x T).
+    camera index (4 B); per point the attempt's factor of Hpp + lambda D^2 (6 f64, written once
+    per damped solve by the point factorization) in and the solve t (one 4 x T row) out; per
+    camera the table row it reads (16 x T) and the forward table of p (12 x T).
     k_cameras<cSchur> (one workgroup per camera): per observation the record copy (4 x T) and
     the point-slot index (4 B); per point t back in (4 x T); per camera the table (32 x T),
     the forward table (12 x T), the column scales (9 x T) and p in, q out (9 x 8 B each).
     """
     t = 4 if precision == "fp32" else 8
-    points = no * (4 * t + 4) + npts * (3 * t + 4 * t) + nc * (16 + 12) * t
+    points = no * (4 * t + 4) + npts * (6 * 8 + 4 * t) + nc * (16 + 12) * t
     cameras = no * (4 * t + 4) + npts * 4 * t + nc * ((32 + 12 + 9) * t + 2 * 9 * 8)
     return points, cameras
 

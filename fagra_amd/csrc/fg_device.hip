@@ -67,6 +67,9 @@ struct fg_ba {
   std::vector<int64_t> cam_order, cam_off, pt_perm, pt_off;
   // Point slots: points by track length, longest first (stable).
   std::vector<int32_t> point_of_slot, slot_of_point;
+  // Camera-major device rows: each camera's rows grouped by the slot range of their point
+  // (point_ranges<T>() ranges); cm_ref[k] is the reference row of device row k.
+  std::vector<int> cm_ref;
   int* d_point_of_slot = nullptr;  // uploaded on the first covariance query
   int nchunks = 0;
   int nsplit = 0;  // leading chunks of kSplitLen or more steps (one workgroup each in k_points)
@@ -74,11 +77,15 @@ struct fg_ba {
   // Device static data.
   int *d_chunk_base = nullptr, *d_chunk_len = nullptr, *d_slot_cam = nullptr, *d_slot_row = nullptr;
   int *d_cam_off = nullptr, *d_k_slot = nullptr, *d_k_pslot = nullptr, *d_pt_len = nullptr;
+  int *d_cam_off_r = nullptr, *d_cm_ref = nullptr;  // (camera, range) row offsets; device row -> reference row
+  // Backends bound to this problem: destroying the problem unbinds them (their model arrays
+  // are freed and they must be prepared again), so either may be destroyed first.
+  std::vector<fg_schur*> users;
   void* d_slot_pix = nullptr;  // Vec2<T>[nslots]
   // f64 observations and camera tables for the linearization residual (the same buffers as the
   // typed ones in f64 precision).
   Vec2<double>* d_slot_pix64 = nullptr;
-  Vec2<double>* d_cam_pix64 = nullptr;  // f64 pixels in camera-major row order
+  Vec2<double>* d_cam_pix64 = nullptr;  // f64 pixels in camera-major device row order
   double* d_tab64[2] = {nullptr, nullptr};
   double* d_tabw[2] = {nullptr, nullptr};  // the T tables widened to f64 (aliases d_tab in f64)
   bool fp32 = false;
@@ -127,6 +134,7 @@ struct fg_schur {
   void *dc = nullptr, *dp = nullptr, *bs = nullptr, *cs = nullptr, *lc = nullptr, *mp = nullptr;
   void* lpp = nullptr;  // f64 [np][6]: factor of Hpp + lambda Dp^2 of the current attempt
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double* qy = nullptr;  // the PCG product's shares of q, 9 per (camera, point range)
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
   void* prow = nullptr;    // T [nc][kPRow]: table row + forward table (point-major reads)
@@ -318,10 +326,14 @@ void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, c
   // the Schur, residual and verification modes read the forward table of their input, which
   // is always s->fw (fw names it at the call), widened to f64
   (void)fw;
-  hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
+  // the Schur mode runs one workgroup per (camera, point range) and writes its shares of q
+  // into out (9 point_ranges<T>() nc)
+  const unsigned grid = (unsigned)b->nc * (MODE == cSchur ? point_ranges<T>() : 1);
+  hipLaunchKernelGGL((k_cameras<T, MODE>), dim3(grid), dim3(camera_block<T, MODE>()), 0, b->stream, (int)b->nc,
                      b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
                      (const double*)b->d_tabw[b->cur], (const double*)s->tp, delta_p, rows_in, fw_wide(s), (const T*)s->dc,
-                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, nullptr);
+                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, nullptr,
+                     (const int*)b->d_cam_off_r, (const int*)b->d_cm_ref);
 }
 
 // Enqueues the linearization of the current state; its evaluation flag lands in d_flags[1]
@@ -513,15 +525,15 @@ void launch_cg_iteration(fg_schur* s, int i, bool timed) {
   if (ev) (void)hipEventRecord(s->events[s->events_used], b->stream);
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, 0.0, nullptr, nullptr, nullptr, cur);
   launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, 0.0,
-                            (double*)s->q, nullptr, cur);
+                            (double*)s->qy, nullptr, cur);
   if (ev) {
     (void)hipEventRecord(s->events[s->events_used + 1], b->stream);
     s->events_used += 2;
   }
   hipLaunchKernelGGL(k_cg_update<T>, dim3(cblocks), dim3(kCgBlock), 0, b->stream, nc, b->d_cam_off,
-                     (const double*)s->mbj, (const double*)s->gmat, (const double*)s->p, (const double*)s->q,
+                     (const double*)s->mbj, (const double*)s->gmat, (const double*)s->p, (const double*)s->qy,
                      (double*)s->x, (double*)s->r, (double*)s->z, (double*)s->zb, (const double*)s->d_parts,
-                     s->d_cgparts, cur, (const DevScalars*)s->d_sc);
+                     point_ranges<T>() * nc, s->d_cgparts, cur, (const DevScalars*)s->d_sc);
 }
 
 template <typename T>
@@ -825,6 +837,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
   if (sizeof(T) == 4) FG_TRY(dalloc(&s->fw_w, kFwd * nc));
   FG_TRY(dalloc((T**)&s->prow, kPRow * nc));
+  FG_TRY(dalloc(&s->qy, 9 * (size_t)point_ranges<T>() * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((double**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
@@ -839,7 +852,9 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc(&s->d_delta, 9 * nc + 3 * np));
   FG_TRY(dalloc(&s->d_rows, 2 * no));
   const size_t pblocks = (size_t)point_blocks(b);
-  const size_t rows = std::max(2 * nc + pblocks, (size_t)1024);  // the attempt's check: 2 nc camera rows + point rows
+  // the attempt's check: 2 nc camera rows + point rows; the Schur product's camera pass: one
+  // p.q partial per (camera, range)
+  const size_t rows = std::max({2 * nc + pblocks, (size_t)1024, (point_ranges<T>() * nc + kParts - 1) / kParts});
   FG_TRY(dalloc(&s->d_parts, kParts * rows));
   FG_TRY(dalloc(&s->d_cgparts, kParts * std::max<size_t>(1024, (size_t)grid_for(nc, kCgBlock))));
   FG_TRY(dalloc((double**)&s->zb, 9 * nc));
@@ -862,6 +877,7 @@ void free_model(fg_schur* s) {
     *v = nullptr;
   }
   dfree(s->fw_w);
+  dfree(s->qy);
   if (s->prow) (void)hipFree(s->prow);
   s->prow = nullptr;
   dfree(s->d_delta);
@@ -975,8 +991,8 @@ fg_status model_impl(fg_schur* s, double* gradient, double* scales, double* negr
   }
   if (negres)
     for (size_t k = 0; k < no; ++k) {
-      negres[2 * k] = -(double)res[k].x;
-      negres[2 * k + 1] = -(double)res[k].y;
+      negres[2 * (size_t)b->cm_ref[k]] = -(double)res[k].x;
+      negres[2 * (size_t)b->cm_ref[k] + 1] = -(double)res[k].y;
     }
   return FG_OK;
 }
@@ -1032,7 +1048,7 @@ fg_status time_products_impl(fg_schur* s, double lambda, const double* x, int re
   // one untimed pass of each warms the caches the timed passes see
   launch_points<T, kSchur>(s, (const T*)s->fw, nullptr, nullptr, lambda, nullptr, nullptr, nullptr, nullptr);
   launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, lambda,
-                            (double*)s->q, nullptr);
+                            (double*)s->qy, nullptr);
   hipEvent_t ev[3];
   for (auto& e : ev) FG_TRY(hipEventCreate(&e));
   FG_TRY(hipEventRecord(ev[0], b->stream));
@@ -1041,7 +1057,7 @@ fg_status time_products_impl(fg_schur* s, double lambda, const double* x, int re
   FG_TRY(hipEventRecord(ev[1], b->stream));
   for (int i = 0; i < repeats; ++i)
     launch_cameras<T, cSchur>(s, nullptr, nullptr, (const T*)s->fw, (const double*)s->p, nullptr, nullptr, lambda,
-                              (double*)s->q, nullptr);
+                              (double*)s->qy, nullptr);
   FG_TRY(hipEventRecord(ev[2], b->stream));
   FG_TRY(hipGetLastError());
   FG_TRY(hipEventSynchronize(ev[2]));
@@ -1174,6 +1190,29 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
       k_pslot[k] = (int)sl;
     }
   }
+  // Camera-major device rows: within each camera, the rows of points in slot range g (slots
+  // [g np / R, (g + 1) np / R)) before those of range g + 1, reference order within a range.
+  // Index arrays of the camera-major kernels (k_slot, k_pslot, pixels) follow the device rows;
+  // slot_row keeps the reference row, which is the order of J v and J^T u row vectors.
+  const int ranges = precision == FG_FP32 ? point_ranges<float>() : point_ranges<double>();
+  std::vector<int> cam_off_r(nc * ranges + 1, 0);
+  {
+    auto range_of = [&](int64_t k) { return (int)((int64_t)k_pslot[k] * ranges / (int64_t)np); };
+    for (size_t c = 0; c < nc; ++c)
+      for (int64_t k = b->cam_off[c]; k < b->cam_off[c + 1]; ++k) cam_off_r[c * ranges + range_of(k) + 1]++;
+    for (size_t g = 0; g < nc * ranges; ++g) cam_off_r[g + 1] += cam_off_r[g];
+    std::vector<int> fill(cam_off_r.begin(), cam_off_r.end() - 1);
+    b->cm_ref.assign(no, 0);
+    for (size_t c = 0; c < nc; ++c)
+      for (int64_t k = b->cam_off[c]; k < b->cam_off[c + 1]; ++k) b->cm_ref[fill[c * ranges + range_of(k)]++] = (int)k;
+    std::vector<int> ks(no), kp(no);
+    for (size_t k = 0; k < no; ++k) {
+      ks[k] = k_slot[b->cm_ref[k]];
+      kp[k] = k_pslot[b->cm_ref[k]];
+    }
+    k_slot.swap(ks);
+    k_pslot.swap(kp);
+  }
   std::vector<int> cam_off32(nc + 1);
   for (size_t c = 0; c <= nc; ++c) cam_off32[c] = (int)b->cam_off[c];
   auto fail = [&](fg_status st) {
@@ -1187,7 +1226,8 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
   };
   if (!up(&b->d_chunk_base, chunk_base) || !up(&b->d_chunk_len, chunk_len) || !up(&b->d_slot_cam, slot_cam) ||
       !up(&b->d_slot_row, slot_row) || !up(&b->d_cam_off, cam_off32) || !up(&b->d_k_slot, k_slot) ||
-      !up(&b->d_k_pslot, k_pslot) || !up(&b->d_pt_len, pt_len))
+      !up(&b->d_k_pslot, k_pslot) || !up(&b->d_pt_len, pt_len) || !up(&b->d_cam_off_r, cam_off_r) ||
+      !up(&b->d_cm_ref, b->cm_ref))
     return fail(FG_DEVICE_ERROR);
   const size_t ts = precision == FG_FP32 ? sizeof(float) : sizeof(double);
   if (hipMalloc(&b->d_slot_pix, std::max<size_t>(1, 2 * b->nslots) * ts) != hipSuccess) return fail(FG_DEVICE_ERROR);
@@ -1206,8 +1246,8 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
   {
     std::vector<double> cam_pix(2 * std::max<size_t>(1, no), 0.0);
     for (size_t k = 0; k < no; ++k) {
-      cam_pix[2 * k] = obs_pixel[2 * b->cam_order[k]];
-      cam_pix[2 * k + 1] = obs_pixel[2 * b->cam_order[k] + 1];
+      cam_pix[2 * k] = obs_pixel[2 * b->cam_order[b->cm_ref[k]]];
+      cam_pix[2 * k + 1] = obs_pixel[2 * b->cam_order[b->cm_ref[k]] + 1];
     }
     if (dalloc((double**)&b->d_cam_pix64, cam_pix.size()) != hipSuccess ||
         hipMemcpy(b->d_cam_pix64, cam_pix.data(), cam_pix.size() * sizeof(double), hipMemcpyHostToDevice) !=
@@ -1244,9 +1284,12 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
   return FG_OK;
 }
 
+static void unbind(fg_schur* s);
+
 void fg_ba_destroy(fg_ba* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  while (!b->users.empty()) unbind(b->users.back());
   dfree(b->d_chunk_base);
   dfree(b->d_chunk_len);
   dfree(b->d_slot_cam);
@@ -1254,6 +1297,8 @@ void fg_ba_destroy(fg_ba* b) {
   dfree(b->d_cam_off);
   dfree(b->d_k_slot);
   dfree(b->d_k_pslot);
+  dfree(b->d_cam_off_r);
+  dfree(b->d_cm_ref);
   dfree(b->d_pt_len);
   dfree(b->d_point_of_slot);
   if (b->fp32 && b->d_slot_pix64) (void)hipFree(b->d_slot_pix64);
@@ -1355,8 +1400,14 @@ fg_status fg_schur_config_default(fg_schur_config* cfg, int precision) {
   cfg->collect_timings = 0;
   cfg->precision = precision;
   cfg->preconditioner = FG_PRECOND_BLOCK_JACOBI;
-  cfg->fallback_factor = 0.5;
-  cfg->fallback_iterations = 10;
+  // learned preconditioner: back to block-Jacobi unless the residual has fallen by 0.2 within
+  // 8 iterations (a mean rate of 0.82 per iteration). Over held-out LM states of three problem
+  // families this catches the late long-track states where the network's blocks converge
+  // slower than block-Jacobi (10 iterations / 0.5 let one run 58 iterations against 27) and
+  // leaves the benchmark's LM run alone (6 iterations / 0.2 restarted some of its solves and
+  // took 1,031 PCG iterations against 917)
+  cfg->fallback_factor = 0.2;
+  cfg->fallback_iterations = 8;
   return FG_OK;
 }
 
@@ -1400,6 +1451,7 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
 void fg_schur_destroy(fg_schur* s) {
   if (!s) return;
   if (s->ba && s->ba->stream) (void)hipStreamSynchronize(s->ba->stream);
+  unbind(s);
   free_model(s);
   dfree(s->d_sc);
   dfree(s->d_fail);
@@ -1430,16 +1482,23 @@ fg_status fg_schur_set_config(fg_schur* s, const fg_schur_config* cfg) {
 
 // Bind a problem: (re)allocate the model arrays for its sizes. No allocation when the same
 // problem is bound again.
+static void unbind(fg_schur* s) {
+  if (s->ba == nullptr) return;
+  auto& u = s->ba->users;
+  u.erase(std::remove(u.begin(), u.end(), s), u.end());
+  free_model(s);
+  s->ba = nullptr;
+  s->prepared = s->linearized = s->ready = false;
+}
+
 static fg_status bind(fg_schur* s, fg_ba* b) {
   if (b == nullptr || b->precision != s->cfg.precision) return FG_INVALID_OPTIONS;
   if (s->ba == b) return FG_OK;
-  if (s->ba) free_model(s);
+  unbind(s);
   s->ba = b;
+  b->users.push_back(s);
   fg_status st = s->cfg.precision == FG_FP32 ? alloc_model<float>(s) : alloc_model<double>(s);
-  if (st != FG_OK) {
-    free_model(s);
-    s->ba = nullptr;
-  }
+  if (st != FG_OK) unbind(s);
   return st;
 }
 
@@ -1852,7 +1911,7 @@ void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
                      (const double*)l->d_u, fw_wide(s),
                      (const T*)s->dc, (const T*)s->gc, damped && step ? ws : nullptr, (const double*)nullptr,
                      (const double*)nullptr, 0.0, damped && step ? l->d_ub : nullptr, l->d_zt,
-                     l->d_up + l->pblocks, (const CgState*)nullptr, st);
+                     l->d_up + l->pblocks, (const CgState*)nullptr, st, (const int*)nullptr, (const int*)nullptr);
   hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, l->pblocks + nc, 0,
                      l->d_st);
 }
@@ -2093,7 +2152,7 @@ fg_status lsmr_solve_damped_device(fg_lsmr* l, double lambda, double* pred, bool
   l->s->step_valid = false;
   l->s->trial_cost_ready = false;
   const int mode = l->prepared_mode;
-  if (mode < 0 || mode != lsmr_requested_mode(l) || !std::isfinite(lambda) || lambda < 0.0)
+  if (!l->s->ready || mode < 0 || mode != lsmr_requested_mode(l) || !std::isfinite(lambda) || lambda < 0.0)
     return FG_INVALID_OPTIONS;
   const fg_status st = l->cfg.precision == FG_FP32 ? lsmr_solve_damped_impl<float>(l, lambda, pred, speculate_trial)
                                                    : lsmr_solve_damped_impl<double>(l, lambda, pred, speculate_trial);

@@ -6,8 +6,11 @@
 //     observations are stored SELL-style: entry (j, lane) at chunk_base + 64 j + lane, so
 //     every per-observation load of a point-major kernel is a coalesced 64-lane access.
 //     Padding entries have camera -1.
-//   * Camera-major kernels walk the reference's camera-stable rows k in [off[c], off[c+1])
-//     with one workgroup per camera and keep a camera-major copy of the records.
+//   * Camera-major kernels walk camera c's rows k in [off[c], off[c+1]) and keep a
+//     camera-major copy of the records. Within a camera the rows are grouped by the point-slot
+//     range of their point (point_ranges<T>() ranges of equal slot count; the reference's row
+//     order within a group), so the Schur product's camera pass can give each range to its own
+//     workgroup (see cSchur below); cm_ref maps a camera-major row to the reference row.
 //   * Camera tables: kCamTab scalars per camera (see fg_math.h).
 #pragma once
 #include "fg_math.h"
@@ -17,6 +20,15 @@ namespace fg {
 constexpr int kBlock = 256;          // threads per workgroup
 constexpr int kParts = 6;            // partial sums per workgroup / camera
 constexpr int kChunksPerBlock = kBlock / kWave;
+// Point-slot ranges of the Schur product's camera pass. With 8 ranges, workgroup b takes
+// camera b / 8 and the rows of points in slot range b % 8; workgroup index mod 8 is the XCD
+// that runs it, and each of the 8 XCDs has its own L2, so each XCD gathers the per-point solves
+// of one eighth of the points (2 MB of 32-byte f64 rows at 500,000 points) instead of all of
+// them. On the 1,500-camera workload this took the f64 camera pass from 60.7 to 50.3 us; the
+// f32 pass (16-byte rows) was slower split in every form tried (34.4 us unsplit against
+// 41.6-56.2 us with 2, 4 or 8 ranges and 64-256 threads per workgroup), so it keeps one range.
+template <typename T>
+constexpr int point_ranges() { return sizeof(T) == 8 ? 8 : 1; }
 
 // PCG state. Kernels of launch i read slot i % 2 (the direction kernel) or (i + 1) % 2 (the
 // product and update kernels); the direction kernel of launch i writes slot (i + 1) % 2 from
@@ -924,6 +936,10 @@ struct PointArgs {
 
 // The eliminated-block solves t_p handed from the point half of a product to the camera half,
 // one 4-vector row per point in the storage type (16 bytes in f32, gathered once per row).
+// A rounded t makes the f32 product S~p = lambda p + Dc^-1 Jc^T (u - Jp t~), whose curvature
+// p.S~p is off by (Jp^T u).(t~ - t), first order in the rounding: along the similarity gauge at
+// lambda ~ 1e-12 it can come out nonpositive (NotPositiveDefinite, a rejected LM trial). f64
+// rows remove that but slowed the f32 camera pass from 34.4 to 47-56 us in every layout tried.
 template <typename T>
 __device__ __forceinline__ void store_tp(double* tp, int i, const double t[3]) {
   reinterpret_cast<Vec4<T>*>(tp)[i] = Vec4<T>{T(t[0]), T(t[1]), T(t[2]), T(0)};
@@ -1413,11 +1429,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
 //   kSchur:   q = lambda p + Dc^-1 T(acc),  v = Jc Dc^-1 p - Jp t, with p.q partials. This is
 //             S p = (B + lambda I) p - E (C + lambda I)^-1 E^T p (Reduced::apply,
 //             schur.rs:606-652) in its factored form Dc^-1 Jc^T (u - Jp t) with u = Jc Dc^-1 p,
-//             which needs no assembled B and cancels per row on 2-vectors.
+//             which needs no assembled B and cancels per row on 2-vectors. One workgroup per
+//             (camera, point range) (point_ranges<T>() per camera, rows off_r[b]..off_r[b + 1]):
+//             it writes its share y_b = Dc^-1 T(acc_b) to out[9 b ..] and p_c.y_b (plus
+//             lambda ||p_c||^2 in range 0) to parts[b]; k_cg_update forms
+//             q_c = lambda p_c + sum_b y_b in range order. With one range the partial is
+//             p_c.(lambda p_c + y_c), the same rounding as the q the update forms.
 //   kResid:   q as kSchur, minus rhs, with ||q - rhs||^2 and ||rhs||^2 partials
 //   kRhs:     rhs = Dc^-1 (-gc + T(acc)),  v = Jp t
 //   kVerify:  nr = T(acc) + lambda Dc^2 delta_c, v = J delta + r; norms and ||J delta||^2
-//   kJtu:     x_c = T(acc), v = u_k
+//   kJtu:     x_c = T(acc), v = u of the row's reference index cm_ref[k]
 //   kLsmr:    x_c = T(acc) with v = U gathered from slot order (k_pslot holds the
 //             observation slots), then the camera damping rows
 //             U_c = sqrt(lambda) Dc ws_c - (alpha/beta) U_c and their ||.||^2 (no ws: x_c only)
@@ -1435,21 +1456,31 @@ enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr
 template <typename T, int MODE>
 constexpr int camera_waves() { return sizeof(T) == 4 && (MODE == 0 || MODE == 1) ? 6 : 1; }
 
+// Threads per workgroup of the camera-major kernel: one wave for the Schur product's
+// (camera, point range) workgroups when a camera's rows are split over ranges (a few hundred
+// rows, about four per lane, and no LDS round for the sums; 128 threads measured 51.9 us and
+// 256 threads 74 us against 50.3 us); a whole camera's rows (kBlock) otherwise.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(camera_waves<T, MODE>()))) k_cameras(
+constexpr int camera_block() { return MODE == cSchur && point_ranges<T>() > 1 ? kWave : kBlock; }
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__((camera_block<T, MODE>())) __attribute__((amdgpu_waves_per_eu(camera_waves<T, MODE>()))) k_cameras(
     int nc, const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
     const Vec2<T>* __restrict__ res_cm, const double* __restrict__ tabw, const double* __restrict__ tp,
     const double* __restrict__ delta_p, const double* __restrict__ rows_in, const double* __restrict__ fww,
     const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
-    double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls) {
-  __shared__ double red[9][kBlock / kWave];
+    double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls,
+    const int* __restrict__ cam_off_r, const int* __restrict__ cm_ref) {
+  constexpr int kB = camera_block<T, MODE>();
+  constexpr int kR = point_ranges<T>();
+  __shared__ double red[9][kB / kWave];
   if (MODE == cSchur && st != nullptr) {
     if (st->done) return;
     lambda = st->lambda;
   }
   if (ls != nullptr && ls->done) return;
-  const int c = blockIdx.x;
+  const int c = MODE == cSchur ? (int)blockIdx.x / kR : (int)blockIdx.x;
   // the camera's table and forward table as f64 rows (the T values widened): uniform over the
   // workgroup, so they are read with scalar loads and stay out of the vector registers
   const double* __restrict__ ct = tabw + kCamTab * c;
@@ -1457,14 +1488,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
   double acc[9], acc2[9];
 #pragma unroll
   for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
-  const int k0 = cam_off[c], k1 = cam_off[c + 1];
+  const int k0 = MODE == cSchur ? cam_off_r[blockIdx.x] : cam_off[c];
+  const int k1 = MODE == cSchur ? cam_off_r[blockIdx.x + 1] : cam_off[c + 1];
   auto one = [&](int k, const Vec4<T>& rec, const Vec4<double>& tv) {
     ObsGeom<double> geo;
     obs_geom(to_d(rec), ct, geo);
     double v0, v1;
     if (MODE == cJtu) {
-      v0 = rows_in[2 * k];
-      v1 = rows_in[2 * k + 1];
+      const int row = cm_ref[k];
+      v0 = rows_in[2 * row];
+      v1 = rows_in[2 * row + 1];
     } else if (MODE == cLsmr) {
       v0 = tv.x;
       v1 = tv.y;
@@ -1514,11 +1547,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
   };
   int k = k0 + threadIdx.x;
   // two rows per step: records and gathers of both issue before either is used
-  for (; k + kBlock < k1; k += 2 * kBlock) {
-    const Vec4<T> ra = rec_cm[k], rb = rec_cm[k + kBlock];
-    const Vec4<double> ta = gather(k), tb = gather(k + kBlock);
+  for (; k + kB < k1; k += 2 * kB) {
+    const Vec4<T> ra = rec_cm[k], rb = rec_cm[k + kB];
+    const Vec4<double> ta = gather(k), tb = gather(k + kB);
     one(k, ra, ta);
-    one(k + kBlock, rb, tb);
+    one(k + kB, rb, tb);
   }
   if (k < k1) one(k, rec_cm[k], gather(k));
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -1527,7 +1560,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
     const double v = wave_sum(acc[a]);
     if (lane == 0) red[a][wave] = v;
   }
-  __shared__ double red2[9][kBlock / kWave];
+  __shared__ double red2[9][kB / kWave];
   if (MODE == cCheck) {
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
@@ -1542,7 +1575,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       s[a] = 0.0;
-      for (int w = 0; w < kBlock / kWave; ++w) s[a] += red[a][w];
+      for (int w = 0; w < kB / kWave; ++w) s[a] += red[a][w];
     }
     // T(acc): rotation slots through (R Jr)^T
     double y[9];
@@ -1559,7 +1592,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
 #pragma unroll
       for (int a = 0; a < 9; ++a) {
         s2[a] = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) s2[a] += red2[a][w];
+        for (int w = 0; w < kB / kWave; ++w) s2[a] += red2[a][w];
       }
 #pragma unroll
       for (int j = 0; j < 3; ++j) y2[j] = ct[tRJ(0, j)] * s2[0] + ct[tRJ(1, j)] * s2[1] + ct[tRJ(2, j)] * s2[2];
@@ -1587,19 +1620,29 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
       pv[4] = v4s;
       pv[5] = 0.0;
     }
-    if (MODE == cSchur || MODE == cResid || MODE == cCheck) {
+    if (MODE == cSchur) {
+      const double* p = vin + 9 * c;
+      const bool first = blockIdx.x % kR == 0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const double yi = y[i] / d[i];
+        out[9 * blockIdx.x + i] = yi;
+        if (kR == 1) {
+          part0 += p[i] * (lambda * p[i] + yi);
+        } else {
+          part0 += p[i] * yi;
+          if (first) part0 += lambda * (p[i] * p[i]);
+        }
+      }
+    } else if (MODE == cResid || MODE == cCheck) {
       const double* p = vin + 9 * c;
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         const double q = lambda * p[i] + y[i] / d[i];
         out[9 * c + i] = q;
-        if (MODE == cSchur) {
-          part0 += p[i] * q;
-        } else {  // cResid, cCheck
-          const double e = q - rhs[9 * c + i];
-          part0 += e * e;
-          part1 += rhs[9 * c + i] * rhs[9 * c + i];
-        }
+        const double e = q - rhs[9 * c + i];
+        part0 += e * e;
+        part1 += rhs[9 * c + i] * rhs[9 * c + i];
       }
     } else if (MODE == cRhs) {
 #pragma unroll
@@ -1638,7 +1681,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
       parts[c] = part0;
     }
     if (MODE == cSchur) {
-      parts[c] = part0;  // p.q, one per camera (k_cg_update sums them contiguously)
+      parts[blockIdx.x] = part0;  // p.q, one per (camera, range) (k_cg_update sums them contiguously)
     } else if (MODE != cJtu && MODE != cLsmr) {
       double* pc = parts + kParts * c;
       pc[0] = part0;
@@ -1914,17 +1957,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(int nc, const int* __re
 }
 
 // x += alpha p; r -= alpha q; z = M r. Every workgroup forms p.q from the camera kernel's
-// per-camera partials (alpha = r.z / p.q; p.q <= 0 ends the solve NotPositiveDefinite).
+// npq per-(camera, range) partials (alpha = r.z / p.q; p.q <= 0 ends the solve
+// NotPositiveDefinite), and q_c = lambda p_c + the camera's point_ranges<T>() shares y in order.
 // Partials: r.r (col 0), r.z (col 1) and, before a possible fallback, r.z_blockJacobi (col 2).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restrict__ cam_off,
                                                       const double* __restrict__ mbj,
                                                       const double* __restrict__ mlearn,
                                                       const double* __restrict__ p,
-                                                      const double* __restrict__ q, double* __restrict__ x,
+                                                      const double* __restrict__ qy, double* __restrict__ x,
                                                       double* __restrict__ r, double* __restrict__ z,
                                                       double* __restrict__ zb, const double* __restrict__ pq_parts,
-                                                      double* parts, CgState* st, const DevScalars* __restrict__ sc) {
+                                                      int npq, double* parts, CgState* st,
+                                                      const DevScalars* __restrict__ sc) {
   __shared__ double smem[kBlock / kWave];
   __shared__ double bcast;
   const CgState s = *st;
@@ -1940,17 +1985,43 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int nc, const int* __restr
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       pc[i] = p[9 * c + i];
-      qc[i] = q[9 * c + i];
       xc[i] = x[9 * c + i];
       rc[i] = r[9 * c + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      double y = 0.0;
+      for (int g = 0; g < point_ranges<T>(); ++g) y += qy[9 * (point_ranges<T>() * c + g) + i];
+      qc[i] = lambda * pc[i] + y;
     }
     observed = cam_off[c + 1] > cam_off[c];
     const double* mc = (learned ? mlearn : mbj) + 45 * c;
 #pragma unroll
     for (int i = 0; i < 45; ++i) m[i] = mc[i];
   }
+  // p.q from its partials: each thread sums a contiguous run (an even count, so its start is
+  // 16-byte aligned) two values a load with four independent sums, so the loads of its run
+  // are in flight together (one dependent sum over 188 partials per thread took 44 us with
+  // 12,000 partials); every workgroup uses the same order
   double pq = 0.0;
-  for (int i = threadIdx.x; i < nc; i += blockDim.x) pq += pq_parts[i];  // one per camera, contiguous
+  {
+    const int per = ((npq + (int)blockDim.x - 1) / (int)blockDim.x + 1) & ~1;
+    const int i0 = min(npq, (int)threadIdx.x * per), i1 = min(npq, i0 + per);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = i0;
+    for (; i + 8 <= i1; i += 8) {
+      const Vec2<double> v0 = *reinterpret_cast<const Vec2<double>*>(pq_parts + i);
+      const Vec2<double> v1 = *reinterpret_cast<const Vec2<double>*>(pq_parts + i + 2);
+      const Vec2<double> v2 = *reinterpret_cast<const Vec2<double>*>(pq_parts + i + 4);
+      const Vec2<double> v3 = *reinterpret_cast<const Vec2<double>*>(pq_parts + i + 6);
+      a0 += v0.x + v0.y;
+      a1 += v1.x + v1.y;
+      a2 += v2.x + v2.y;
+      a3 += v3.x + v3.y;
+    }
+    for (; i < i1; ++i) a0 += pq_parts[i];
+    pq = (a0 + a1) + (a2 + a3);
+  }
   pq = block_sum(pq, smem);
   if (threadIdx.x == 0) bcast = pq;
   __syncthreads();

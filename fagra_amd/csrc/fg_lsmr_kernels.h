@@ -105,7 +105,8 @@ __global__ void __launch_bounds__(kBlock) k_lsmr_v(int nc, int np, const double*
     }
     if (block) lsmr_block_precond(blk, nc, lc, lp, z, width, true);
     for (int i = 0; i < width; ++i) {
-      const double vn = z[i] - keep * v[off + i];
+      // the first v reads no old V: the buffer is uninitialized then, and 0 * NaN is NaN
+      const double vn = init ? z[i] : z[i] - keep * v[off + i];
       v[off + i] = vn;
       ss += vn * vn;
     }

@@ -92,8 +92,8 @@ typedef struct {
   int collect_timings;          /* default 0 */
   int precision;                /* fg_precision */
   int preconditioner;           /* fg_preconditioner */
-  double fallback_factor;       /* learned: required residual reduction ... */
-  size_t fallback_iterations;   /* ... within this many iterations */
+  double fallback_factor;       /* learned: required residual reduction (default 0.2) ... */
+  size_t fallback_iterations;   /* ... within this many iterations (default 8) */
   fg_on_solve_fn on_solve;      /* observe every numerically attempted solve */
   void* on_solve_user;
 } fg_schur_config;

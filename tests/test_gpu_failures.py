@@ -140,3 +140,27 @@ def test_trials_across_a_camera_plane_are_rejected_as_the_oracle_rejects_them():
     # the point's projection in camera c is near-singular along the way, which amplifies the
     # rounding differences of summation order: measured 1.3e-9 at worst over the 20 steps
     assert worst <= 1e-8
+
+
+@pytest.mark.parametrize("backend", ["schur", "lsmr"])
+def test_problem_destroyed_before_its_backend(backend):
+    """A backend outlives the problem it was prepared on: the problem's destruction unbinds it,
+    later calls report INVALID_OPTIONS instead of reaching freed memory, and the backend
+    prepares again on a new problem and solves as a fresh one does."""
+    make = (lambda: fa.Schur(precision=fa.FP64)) if backend == "schur" else (lambda: fa.Lsmr(precision=fa.FP64))
+    p = small()
+    ba = fa.BalProblem(p, precision=fa.FP64)
+    s = make()
+    assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+    st, first, _ = s.solve_damped(1e-3)
+    assert st == fa.OK
+    ba.close()
+    assert s.linearize() == fa.INVALID_OPTIONS
+    assert s.prepare_damping(1e-6) == fa.INVALID_OPTIONS
+    assert s.solve_damped(1e-3)[0] == fa.INVALID_OPTIONS
+    ba2 = fa.BalProblem(p, precision=fa.FP64)
+    assert s.prepare(ba2) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+    st, again, _ = s.solve_damped(1e-3)
+    assert st == fa.OK and np.array_equal(again, first)
+    ba2.close()
+    s.close()  # destroying an unbound backend

@@ -226,9 +226,10 @@ def test_converged_fp64_matches_oracle_fixture(scale):
     ba.set_state(p.cameras, p.points)
     costs, done = [], 0
     while done < fx["accepted_steps"]:
-        lm = fa.LevenbergMarquardt(fa.Schur(precision=fa.FP64, relative_tolerance=1e-10))
+        backend = fa.Schur(precision=fa.FP64, relative_tolerance=1e-10)
         n = min(fx["segment_steps"], fx["accepted_steps"] - done)
-        out = lm.solve_batch(ba, fa.OptimizeOptions(n, 1e-8, 0.0, 0.0))
+        out = fa.LevenbergMarquardt(backend).solve_batch(ba, fa.OptimizeOptions(n, 1e-8, 0.0, 0.0))
+        backend.close()
         done += out.accepted_steps
         costs.append(out.cost)
         if out.status != fa.NO_CONVERGENCE or out.accepted_steps == 0:
