@@ -218,6 +218,72 @@ def time_linearizations(fa, dev, precision, repeats):
     return el
 
 
+def widened_rows(no_cpu):
+    """Measurements of the rows beyond the BAL Schur path (python bench.py --widened), each beside
+    the oracle timed on the same bounded sample as its CPU baseline: the generic-emission Schur backend (a planar pose / landmark model, one damped solve)
+    and the selected covariance (a dense 3,072-coordinate information matrix)."""
+    import fagra_amd as fa
+    from fagra_amd import generic
+    from fagra_amd.planar import Planar
+    out = {}
+    m = Planar(n_poses=2000, n_landmarks=50000, n_obs=400000, seed=7, noise=0.01)
+    g = generic.GenericSchur(3 * m.np_, 3, 2, relative_tolerance=1e-6)
+    t = time.perf_counter()
+    assert g.prepare(m.dimension) == fa.OK and m.linearize(g) == fa.OK
+    st, _ = g.gradient_norm()
+    t_acc = time.perf_counter() - t
+    t = time.perf_counter()
+    assert g.prepare_damping(1e-6) == fa.OK
+    t_prep = time.perf_counter() - t
+    g.solve_damped(1e-3)  # warm
+    reps, t = 5, time.perf_counter()
+    for _ in range(reps):
+        st, d, pred = g.solve_damped(1e-3)
+    t_solve = (time.perf_counter() - t) / reps
+    gen = {"model": "planar: 2000 poses (3), 50000 landmarks (2), 400000 observations (2 rows) + prior",
+           "dimension": m.dimension, "host_accumulate_s": t_acc, "prepare_damping_s": t_prep,
+           "solve_damped_s": t_solve, "pcg_iterations": int(g.diagnostics().iterations), "status": st}
+    if not no_cpu:
+        from oracle import oracle as O
+        O.set_threads(1)  # the single-core baseline
+        o = O.Schur(3 * m.np_, 3, 2)
+        o.relative_tolerance = 1e-6
+        assert o.prepare(m.dimension) == O.OK
+        r, pr = m.residuals(m.x)
+        jp, jl = m.jacobians(m.x)
+        for e in range(m.pi.size):
+            o.accumulate(r[e], [(3 * int(m.pi[e]), jp[e]), (3 * m.np_ + 2 * int(m.li[e]), jl[e])])
+        o.accumulate(pr, [(0, np.eye(3))])
+        t = time.perf_counter()
+        o.prepare_damping(1e-6)
+        t_oprep = time.perf_counter() - t
+        t = time.perf_counter()
+        st_o, d_o, _ = o.solve_damped(1e-3)
+        t_osolve = time.perf_counter() - t
+        gen["cpu_baseline"] = {"prepare_damping_s": t_oprep, "solve_damped_s": t_osolve, "cores": 1,
+                             "step_rel_diff": float(np.linalg.norm(d - d_o) / np.linalg.norm(d_o))}
+    out["generic_schur"] = gen
+    n, k = 3072, 64
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(n + 64, n))
+    H = np.asfortranarray(A.T @ A)  # column-major, as the ABI takes it
+    cols = rng.permutation(n)[:k]
+    fa.dense_covariance(H[:128, :128], cols[:4] % 128)  # warm
+    t = time.perf_counter()
+    st, C = fa.dense_covariance(H, cols)
+    t_cov = time.perf_counter() - t
+    cov = {"n": n, "selected": k, "seconds": t_cov, "status": st,
+           "tflops_cholesky": (n ** 3 / 3) / t_cov / 1e12}
+    if not no_cpu:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        st_o, C_o = O.dense_covariance(H, cols)
+        cov["cpu_baseline"] = {"seconds": time.perf_counter() - t, "cores": 1,
+                             "max_abs_diff": float(np.max(np.abs(C - C_o)))}
+    out["covariance"] = cov
+    return out
+
+
 def profile_products(args):
     """Counter-run workload: repeats of each Schur-product kernel at the initial state of the
     workload in both precisions, then a plain 256 MiB device copy whose FETCH_SIZE and
@@ -266,6 +332,9 @@ def main():
     ap.add_argument("--lin-repeats", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32,learned")
+    ap.add_argument("--widened", action="store_true",
+                    help="only measure the generic-emission Schur backend and the dense covariance, each "
+                         "beside the oracle (one JSON line)")
     ap.add_argument("--profile-products", action="store_true",
                     help="only time the two Schur-product kernels (FP32, FP64) and a 256 MiB device copy, "
                          "for hardware-counter runs")
@@ -276,6 +345,9 @@ def main():
         sys.exit(2)
     if args.profile_products:
         return profile_products(args)
+    if args.widened:
+        print(json.dumps({"widened": widened_rows(args.no_cpu)}))
+        return
 
     import fagra_amd as fa
     from fagra_amd import learned, synthetic

@@ -10,7 +10,7 @@ camera's diagonal block of the reduced system, S_cc. Two questions, each on held
 * Does it cut PCG iterations to the same tolerance (1e-3, the benchmark's)? On these problems
   block-Jacobi PCG already converges in 4-27 iterations per solve and the exact S_cc blocks
   are no better a preconditioner than B_i, so the network's iterations stay within a few per
-  cent of block-Jacobi's: measured 415 against 417 over the held-out grid below (and 917
+  cent of block-Jacobi's: measured 393 against 390 over the held-out grid below (and 917
   against 942 over the benchmark's LM solve to its target). On late long-track states the
   network's blocks converge slower than block-Jacobi's; the fallback (residual not down by 0.2
   within 8 iterations) restarts those solves with block-Jacobi. The test asserts the 5% bound
