@@ -42,51 +42,72 @@ __global__ void k_cov_check(const double* __restrict__ h, size_t n, double* __re
   }
 }
 
-// Cholesky of the diagonal tile [k0, k0 + kb) by one wavefront (lane i owns row i of the tile
-// in LDS), then the inverse of its factor into linv (64 x 64, row-major, identity-padded beyond
-// kb) for the panel and selected-column solves. A nonpositive or nonfinite pivot sets flags[1]
-// (faer's LLT error, SingularInformation).
+// Cholesky of the diagonal tile [k0, k0 + kb) by one wavefront, then the inverse of its factor
+// into linv (64 x 64, row-major, identity-padded beyond kb) for the panel and selected-column
+// solves. A nonpositive or nonfinite pivot sets flags[1] (faer's LLT error,
+// SingularInformation). Lane i keeps row i of the tile in registers (both loops are unrolled,
+// so every index is a compile-time one); the column being eliminated is broadcast through a
+// 64-entry LDS vector. The inverse is formed column by column in the same way: lane c keeps
+// column c of L^-1 and reads the rows of L from LDS, which nothing writes meanwhile, so those
+// loads are in flight together. (Rows kept in LDS and updated in place there took 161 us per
+// tile: each read-modify-write waited for the previous one.) Entries above the diagonal of a
+// row are never read back: a lane at or above the column being eliminated only changes them.
 __global__ void __launch_bounds__(kWave) k_cov_potrf(double* __restrict__ a, int n, int k0, int kb,
                                                      double* __restrict__ linv, int* flags) {
   __shared__ double t[kNB][kNB + 1];
-  __shared__ double v[kNB][kNB + 1];
+  __shared__ double col[kNB];
   __shared__ int fail;
   if (flags[1]) return;
   const int i = threadIdx.x;
-  for (int c = 0; c < kNB; ++c) t[i][c] = (i < kb && c <= i) ? a[(size_t)(k0 + c) * n + k0 + i] : (i == c ? 1.0 : 0.0);
+  double row[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) row[c] = (i < kb && c <= i) ? a[(size_t)(k0 + c) * n + k0 + i] : (i == c ? 1.0 : 0.0);
   if (i == 0) fail = 0;
   __syncthreads();
-  for (int j = 0; j < kb; ++j) {
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
     if (i == j) {
-      const double d = t[j][j];
-      if (!(d > 0.0) || !isfinite(d)) fail = 1;
-      else t[j][j] = sqrt(d);
+      const double d = row[j];
+      if (!(d > 0.0) || !isfinite(d)) {
+        fail = 1;
+      } else {
+        row[j] = sqrt(d);
+        col[j] = row[j];
+      }
     }
     __syncthreads();
     if (fail) break;
-    if (i > j && i < kb) t[i][j] /= t[j][j];
-    __syncthreads();
-    if (i > j && i < kb) {
-      const double f = t[i][j];
-      for (int l = j + 1; l <= i; ++l) t[i][l] -= f * t[l][j];
+    if (i > j) {
+      row[j] /= col[j];
+      col[i] = row[j];
     }
+    __syncthreads();
+    const double f = row[j];  // L[i][j] below the diagonal; zero above it
+#pragma unroll
+    for (int l = j + 1; l < kNB; ++l) row[l] -= f * col[l];
     __syncthreads();
   }
   if (fail) {
     if (i == 0) flags[1] = 1;
     return;
   }
-  for (int c = 0; c <= i && i < kb; ++c) a[(size_t)(k0 + c) * n + k0 + i] = t[i][c];
-  // column i of L^-1 (forward substitution on e_i); rows above i are zero
-  for (int r = 0; r < kNB; ++r) v[r][i] = 0.0;
-  v[i][i] = 1.0 / t[i][i];
-  for (int r = i + 1; r < kNB; ++r) {
-    double s = 0.0;
-    for (int q = i; q < r; ++q) s += t[r][q] * v[q][i];
-    v[r][i] = -s / t[r][r];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    if (i < kb && c <= i) a[(size_t)(k0 + c) * n + k0 + i] = row[c];
+    t[i][c] = c <= i ? row[c] : 0.0;
   }
   __syncthreads();
-  for (int c = 0; c < kNB; ++c) linv[i * kNB + c] = v[i][c];
+  // column i of L^-1 by forward substitution on e_i; rows above i come out zero
+  double v[kNB];
+#pragma unroll
+  for (int r = 0; r < kNB; ++r) {
+    double s[4] = {r == i ? 1.0 : 0.0, 0.0, 0.0, 0.0};  // four chains, added in a fixed order
+#pragma unroll
+    for (int q = 0; q < r; ++q) s[q & 3] -= t[r][q] * v[q];
+    v[r] = ((s[0] + s[1]) + (s[2] + s[3])) / t[r][r];
+  }
+#pragma unroll
+  for (int r = 0; r < kNB; ++r) linv[r * kNB + i] = v[r];
 }
 
 // Panel rows below the tile, X = A_panel L^-T as a product with the tile inverse: 64 rows per
