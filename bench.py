@@ -62,9 +62,10 @@ def kernel_bytes(nc, npts, no, precision):
     camera index (4 B); per point the attempt's factor of Hpp + lambda D^2 (6 f64, written once
     per damped solve by the point factorization) in and the solve t (one 4 x T row) out; per
     camera the table row it reads (16 x T) and the forward table of p (12 x T).
-    k_cameras<cSchur> (one workgroup per camera): per observation the record copy (4 x T) and
-    the point-slot index (4 B); per point t back in (4 x T); per camera the table (32 x T),
-    the forward table (12 x T), the column scales (9 x T) and p in, q out (9 x 8 B each).
+    k_schur_cameras (one wave per camera and point-slot range): per observation the record copy
+    (4 x T) and the point-slot index (4 B); per point t back in (4 x T); per camera the table
+    (32 x T), the forward table (12 x T), the column scales (9 x T) and p in, q out (9 x 8 B
+    each).
     """
     t = 4 if precision == "fp32" else 8
     points = no * (4 * t + 4) + npts * (6 * 8 + 4 * t) + nc * (16 + 12) * t
@@ -434,7 +435,7 @@ def main():
     if t_pts >= t_cams:
         kname, kbytes, ksec = f"k_points<{tname}, kSchur>", b_pts, t_pts
     else:
-        kname, kbytes, ksec = f"k_cameras<{tname}, cSchur>", b_cams, t_cams
+        kname, kbytes, ksec = f"k_schur_cameras<{tname}>", b_cams, t_cams
     achieved = kbytes / ksec / 1e9
     traffic = MEASURED_TRAFFIC.get((args.workload, kname))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -326,14 +326,22 @@ void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, c
   // the Schur, residual and verification modes read the forward table of their input, which
   // is always s->fw (fw names it at the call), widened to f64
   (void)fw;
-  // the Schur mode runs one workgroup per (camera, point range) and writes its shares of q
-  // into out (9 point_ranges<T>() nc)
-  const unsigned grid = (unsigned)b->nc * (MODE == cSchur ? point_ranges<T>() : 1);
-  hipLaunchKernelGGL((k_cameras<T, MODE>), dim3(grid), dim3(camera_block<T, MODE>()), 0, b->stream, (int)b->nc,
-                     b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
-                     (const double*)b->d_tabw[b->cur], (const double*)s->tp, delta_p, rows_in, fw_wide(s), (const T*)s->dc,
-                     (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st, nullptr,
-                     (const int*)b->d_cam_off_r, (const int*)b->d_cm_ref);
+  if constexpr (MODE == cSchur) {
+    // one wave per (camera, point range), four cameras of one range per workgroup; the shares
+    // of q go to out (9 point_ranges<T>() nc)
+    constexpr int kR = point_ranges<T>(), kWavesPerBlock = kBlock / kWave;
+    const unsigned grid = (unsigned)((b->nc + kWavesPerBlock - 1) / kWavesPerBlock) * kR;
+    hipLaunchKernelGGL((k_schur_cameras<T>), dim3(grid), dim3(kBlock), 0, b->stream, (int)b->nc,
+                       (const int*)b->d_cam_off_r, b->d_k_pslot, (const Vec4<T>*)s->rec_cm,
+                       (const double*)b->d_tabw[b->cur], (const double*)s->tp, fw_wide(s), (const T*)s->dc, vin,
+                       lambda, out, s->d_parts, st);
+  } else {
+    hipLaunchKernelGGL((k_cameras<T, MODE>), dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, (int)b->nc,
+                       b->d_cam_off, b->d_k_pslot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,
+                       (const double*)b->d_tabw[b->cur], (const double*)s->tp, delta_p, rows_in, fw_wide(s),
+                       (const T*)s->dc, (const T*)s->gc, vin, delta_c, rhs, lambda, out, out_d, s->d_parts, st,
+                       nullptr, (const int*)b->d_cm_ref);
+  }
 }
 
 // Enqueues the linearization of the current state; its evaluation flag lands in d_flags[1]
@@ -1911,7 +1919,7 @@ void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
                      (const double*)l->d_u, fw_wide(s),
                      (const T*)s->dc, (const T*)s->gc, damped && step ? ws : nullptr, (const double*)nullptr,
                      (const double*)nullptr, 0.0, damped && step ? l->d_ub : nullptr, l->d_zt,
-                     l->d_up + l->pblocks, (const CgState*)nullptr, st, (const int*)nullptr, (const int*)nullptr);
+                     l->d_up + l->pblocks, (const CgState*)nullptr, st, (const int*)nullptr);
   hipLaunchKernelGGL(k_lsmr_norm, dim3(1), dim3(kBlock), 0, b->stream, (const double*)l->d_up, l->pblocks + nc, 0,
                      l->d_st);
 }

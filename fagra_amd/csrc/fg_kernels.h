@@ -10,7 +10,7 @@
 //     camera-major copy of the records. Within a camera the rows are grouped by the point-slot
 //     range of their point (point_ranges<T>() ranges of equal slot count; the reference's row
 //     order within a group), so the Schur product's camera pass can give each range to its own
-//     workgroup (see cSchur below); cm_ref maps a camera-major row to the reference row.
+//     wave (k_schur_cameras); cm_ref maps a camera-major row to the reference row.
 //   * Camera tables: kCamTab scalars per camera (see fg_math.h).
 #pragma once
 #include "fg_math.h"
@@ -20,15 +20,16 @@ namespace fg {
 constexpr int kBlock = 256;          // threads per workgroup
 constexpr int kParts = 6;            // partial sums per workgroup / camera
 constexpr int kChunksPerBlock = kBlock / kWave;
-// Point-slot ranges of the Schur product's camera pass. With 8 ranges, workgroup b takes
-// camera b / 8 and the rows of points in slot range b % 8; workgroup index mod 8 is the XCD
-// that runs it, and each of the 8 XCDs has its own L2, so each XCD gathers the per-point solves
-// of one eighth of the points (2 MB of 32-byte f64 rows at 500,000 points) instead of all of
-// them. On the 1,500-camera workload this took the f64 camera pass from 60.7 to 50.3 us; the
-// f32 pass (16-byte rows) was slower split in every form tried (34.4 us unsplit against
-// 41.6-56.2 us with 2, 4 or 8 ranges and 64-256 threads per workgroup), so it keeps one range.
+// Point-slot ranges of the Schur product's camera pass (k_schur_cameras). Workgroup b takes
+// range b % R, and workgroup index mod 8 is the XCD that runs it; each of the 8 XCDs has its own
+// L2, so with R ranges an XCD gathers the per-point solves of 1/R of the points instead of all
+// of them. FP64 (32-byte solves, 16 MB at 500,000 points): 8 ranges, 2 MB per XCD (camera pass
+// 50.6 -> 48.2 us on the 1,500-camera workload; unsplit, every XCD touches every line of the
+// solves). FP32 (16-byte solves): 2 ranges, measured fastest of 1, 2, 4 and 8 (33.5, 30.2, 32.4
+// and 35.0 us): the finer splits launch more, shorter waves whose fixed costs (scalar loads,
+// pipeline fill, the 9-sum wave reduction) outweigh the L2 hits they gain.
 template <typename T>
-constexpr int point_ranges() { return sizeof(T) == 8 ? 8 : 1; }
+constexpr int point_ranges() { return sizeof(T) == 4 ? 2 : 8; }
 
 // PCG state. Kernels of launch i read slot i % 2 (the direction kernel) or (i + 1) % 2 (the
 // product and update kernels); the direction kernel of launch i writes slot (i + 1) % 2 from
@@ -1096,42 +1097,55 @@ __device__ __forceinline__ void rows_landed() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// -1/z from the hardware reciprocal refined by two Newton steps (within an ulp of the division,
+// which the compiler expands to eleven instructions). z > 0 for every linearized row.
+__device__ __forceinline__ double neg_inv_depth(double z) {
+  double r = __builtin_amdgcn_rcp(z);
+  double e = __builtin_fma(-z, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-z, r, 1.0);
+  return -__builtin_fma(r, e, r);
+}
+
 // w += Jp^T Jc z for one observation of the f32 Schur product, its camera's product row read
 // from LDS piece by piece as the computation needs it (pieces 0-2: rows of R with t_i; 3: f,
 // k1, k2, fw0; 4, 5: fw1-8), which keeps fewer values live than the whole row in f64 registers.
-// The same arithmetic as obs_geom, jc_apply and jpt_accumulate.
+// With the projection derivative dp = iz A P of schur_camera_row (iz = -1/z, A = f (radial I +
+// c2 q q^T), P = [I | q]) and v = w' x (P - t) + z_t: u = Jc z = sw v', v' = iz A P v + q s, and
+// Jp^T u = R^T e with e = sw^2 iz P^T A v'; f64 arithmetic.
 __device__ __forceinline__ void schur_obs_lds(const Vec4<float>* __restrict__ row, const Vec4<float>& rec,
                                               double w[3]) {
   const double qx = rec.x, qy = rec.y, z = rec.z, sw = rec.w;
   const Vec4<float> p3 = row[3];
   const double f = p3.x, k1 = p3.y, k2 = p3.z;
-  const double r2 = qx * qx + qy * qy;
-  const double radial = 1.0 + k1 * r2 + k2 * r2 * r2;
-  const double c2 = 2.0 * k1 + 4.0 * k2 * r2;
-  const double a = f * (radial + c2 * qx * qx);
-  const double b = f * (c2 * qx * qy);
-  const double d = f * (radial + c2 * qy * qy);
-  const double iz = -1.0 / z;
-  const double dp00 = iz * a, dp01 = iz * b, dp02 = iz * (a * qx + b * qy);
-  const double dp10 = iz * b, dp11 = iz * d, dp12 = iz * (b * qx + d * qy);
+  const double r2 = __builtin_fma(qx, qx, qy * qy);
+  const double radial = __builtin_fma(r2, __builtin_fma(k2, r2, k1), 1.0);
+  const double fc2 = f * __builtin_fma(4.0 * k2, r2, 2.0 * k1);
+  const double frad = f * radial;
+  const double cx = fc2 * qx, cy = fc2 * qy;
+  const double a = __builtin_fma(cx, qx, frad), b = cx * qy, d = __builtin_fma(cy, qy, frad);
+  const double iz = neg_inv_depth(z);
   const Vec4<float> p0 = row[0], p1 = row[1], p2 = row[2];
-  const double pt0 = -qx * z - double(p0.w), pt1 = -qy * z - double(p1.w), pt2 = z - double(p2.w);
-  const double fr2 = f * r2, fr4 = fr2 * r2;
+  const double pt0 = __builtin_fma(-qx, z, -double(p0.w));
+  const double pt1 = __builtin_fma(-qy, z, -double(p1.w));
+  const double pt2 = z - double(p2.w);
   const Vec4<float> p4 = row[4], p5 = row[5];
   const double f0 = p3.w, f1 = p4.x, f2 = p4.y, f3 = p4.z, f4 = p4.w, f5 = p5.x, f6 = p5.y, f7 = p5.z,
                f8 = p5.w;
-  const double v0 = f1 * pt2 - f2 * pt1 + f3;
-  const double v1 = f2 * pt0 - f0 * pt2 + f4;
-  const double v2 = f0 * pt1 - f1 * pt0 + f5;
-  const double sv = radial * f6 + fr2 * f7 + fr4 * f8;
-  const double u0 = sw * (dp00 * v0 + dp01 * v1 + dp02 * v2 + qx * sv);
-  const double u1 = sw * (dp10 * v0 + dp11 * v1 + dp12 * v2 + qy * sv);
-  const double e0 = sw * (u0 * dp00 + u1 * dp10);
-  const double e1 = sw * (u0 * dp01 + u1 * dp11);
-  const double e2 = sw * (u0 * dp02 + u1 * dp12);
-  w[0] += double(p0.x) * e0 + double(p1.x) * e1 + double(p2.x) * e2;
-  w[1] += double(p0.y) * e0 + double(p1.y) * e1 + double(p2.y) * e2;
-  w[2] += double(p0.z) * e0 + double(p1.z) * e1 + double(p2.z) * e2;
+  const double x0 = __builtin_fma(f1, pt2, __builtin_fma(-f2, pt1, f3));
+  const double x1 = __builtin_fma(f2, pt0, __builtin_fma(-f0, pt2, f4));
+  const double x2 = __builtin_fma(f0, pt1, __builtin_fma(-f1, pt0, f5));
+  const double fr2 = f * r2, fr4 = fr2 * r2;
+  const double sv = __builtin_fma(radial, f6, __builtin_fma(fr2, f7, fr4 * f8));
+  const double y0 = __builtin_fma(qx, x2, x0), y1 = __builtin_fma(qy, x2, x1);
+  const double v0 = __builtin_fma(iz, __builtin_fma(a, y0, b * y1), qx * sv);
+  const double v1 = __builtin_fma(iz, __builtin_fma(b, y0, d * y1), qy * sv);
+  const double h0 = __builtin_fma(a, v0, b * v1), h1 = __builtin_fma(b, v0, d * v1);
+  const double wz = sw * sw * iz;
+  const double e0 = wz * h0, e1 = wz * h1, e2 = wz * __builtin_fma(qx, h0, qy * h1);
+  w[0] = __builtin_fma(double(p0.x), e0, __builtin_fma(double(p1.x), e1, __builtin_fma(double(p2.x), e2, w[0])));
+  w[1] = __builtin_fma(double(p0.y), e0, __builtin_fma(double(p1.y), e1, __builtin_fma(double(p2.y), e2, w[1])));
+  w[2] = __builtin_fma(double(p0.z), e0, __builtin_fma(double(p1.z), e1, __builtin_fma(double(p2.z), e2, w[2])));
 }
 
 // One lane's observations of a point-major pass (SELL chunk, two observations per step so
@@ -1146,35 +1160,47 @@ __device__ __forceinline__ void point_sweep(const PointArgs& pa, int chunk, int 
                                             Vec4<float>* __restrict__ coop = nullptr) {
   const int base = pa.chunk_base[chunk];
   if constexpr (COOP && MODE == kSchur && sizeof(T) == 4) {
-    // Step j's rows travel while step j + 1's camera index and record are fetched, so one
-    // memory round trip is exposed per step instead of three (index, rows, record).
+    // Step j's rows travel from L2 into LDS while the camera index and record of step j + 1 are
+    // fetched; the wait before step j's arithmetic covers the rows only (the two loads issued
+    // after them stay in flight through the arithmetic), and step j + 1's camera shuffles come
+    // after step j's arithmetic. Two register sets A (even steps) and B (odd steps) carry the
+    // index and record, so no copy waits on a load in flight. Loads past the chunk's last step
+    // re-read its last step (in bounds, unused), so exactly two loads follow the rows and
+    // vmcnt(2) is "rows landed". The index and record loads go through pointer copies the
+    // compiler cannot trace to the restrict-qualified arguments, so the compiler barriers keep
+    // them after the row fetch.
+    using V4f = float __attribute__((ext_vector_type(4)));
+    using GV = const __attribute__((address_space(1))) V4f;
+    using GI = const __attribute__((address_space(1))) int;
+    GV* rcv = (GV*)(rec);
+    GI* scv = (GI*)(pa.slot_cam);
+    __asm__ volatile("" : "+s"(rcv), "+s"(scv));
     const Vec4<float>* pr = reinterpret_cast<const Vec4<float>*>(fw);
-    const Vec4<float>* rc = reinterpret_cast<const Vec4<float>*>(rec);
-    // Step j's rows travel while step j + 1's camera index and record are fetched, so one
-    // memory round trip is exposed per step instead of three (index, rows, record); the
-    // shuffles of step j + 1's cameras are issued before step j's arithmetic.
-    int c_cur = -1;
-    Vec4<float> r_cur{0.f, 0.f, 0.f, 0.f};
-    if (j0 < j1) {
-      c_cur = pa.slot_cam[base + j0 * kWave + lane];
-      r_cur = rc[base + j0 * kWave + lane];
-    }
+    if (j0 >= j1) return;  // uniform over the wave
+    const int last = base + (j1 - 1) * kWave + lane;
+    auto slot = [&](int j) { return min(base + j * kWave + lane, last); };
     int cr[kRowPieces];
-    piece_cameras(c_cur, lane, cr);
-    for (int j = j0; j < j1; ++j) {
+    int c_a = scv[slot(j0)], c_b;
+    V4f r_a = rcv[slot(j0)], r_b;
+    piece_cameras(c_a, lane, cr);
+    auto step = [&](int j, int c, const V4f& r, int& c_next, V4f& r_next) {
       rows_to_lds_dma(pr, cr, lane, coop);
-      int c_nxt = -1;
-      Vec4<float> r_nxt{0.f, 0.f, 0.f, 0.f};
-      if (j + 1 < j1) {  // uniform over the wave
-        const int s1 = base + (j + 1) * kWave + lane;
-        c_nxt = pa.slot_cam[s1];
-        r_nxt = rc[s1];
-      }
-      rows_landed();
-      piece_cameras(c_nxt, lane, cr);
-      if (c_cur >= 0) schur_obs_lds(coop + lane * kRowPieces, r_cur, w);
-      c_cur = c_nxt;
-      r_cur = r_nxt;
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      c_next = scv[slot(j + 1)];
+      r_next = rcv[slot(j + 1)];
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_waitcnt(0x0F72);  // vmcnt(2): the rows and every load before them
+      __builtin_amdgcn_wave_barrier();
+      if (c >= 0) schur_obs_lds(coop + lane * kRowPieces, Vec4<float>{r.x, r.y, r.z, r.w}, w);
+      piece_cameras(c_next, lane, cr);
+    };
+    for (int j = j0;; j += 2) {
+      step(j, c_a, r_a, c_b, r_b);
+      if (j + 1 >= j1) break;
+      step(j + 1, c_b, r_b, c_a, r_a);
+      if (j + 2 >= j1) break;
     }
     return;
   } else if constexpr (COOP) {
@@ -1424,18 +1450,191 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
 }
 
 // ---------------------------------------------------------------------------------------
+// Camera half of the Schur product inside PCG: S p = (B + lambda I) p - E (C + lambda I)^-1 E^T p
+// (Reduced::apply, schur.rs:606-652) in its factored form lambda p + Dc^-1 T(acc) with
+// acc = sum over the camera's rows of the factors of Jc^T (u - Jp t), u = Jc Dc^-1 p (forward
+// table of p) and t the point's solve from k_points<kSchur>; no B is assembled and the
+// cancellation happens per row on 2-vectors. T(acc) maps the rotation slots by (R Jr)^T.
+//
+// One wave per (camera, point range): a camera's rows are grouped by the point-slot range of
+// their point (point_ranges ranges), and workgroup b (four waves) takes range g = b % kR of the
+// four cameras 4 (b / kR) .. + 3, so the XCD that runs it gathers the solves of that range only.
+// The wave writes its share y = Dc^-1 T(acc) of camera c to out[9 sh ..] with sh = kR c + g,
+// and p_c.y (plus lambda ||p_c||^2 in range 0) to parts[sh]; k_cg_update forms
+// q_c = lambda p_c + sum_g y in range order.
+//
+// Rows are software-pipelined per lane (rows k0 + lane + 64 m), with two register sets A (even
+// m) and B (odd m): while row m is evaluated, the record and solve of row m + 1 and the slot
+// index of row m + 2 are in flight, so the chain index -> solve -> arithmetic is exposed once
+// per wave instead of once per row. The trip count is uniform over the wave; a lane past its
+// last row re-reads the wave's last row (in bounds) with weight 0, which adds exact zeros. The
+// loop body has no divergent branch, so each wait covers only loads issued one row earlier.
+// acc += the factors of Jc^T (Jc p~ - Jp t) for one row of the Schur product (the camera half's
+// row: jc_apply, jp_apply and jct_factors of obs_geom in one pass). The row's projection
+// derivative is dp = iz A P with iz = -1/z, A = f (radial I + c2 q q^T) (2x2 symmetric, entries
+// a, b, d) and P = [I | q] (2x3), so with v = w' x (P - t) + z_t (the forward table of p~) and
+// x = v - R t:
+//   v' = iz A P x + q s          (s = radial z_i0 + f r2 z_i1 + f r2^2 z_i2; the row is sw v')
+//   e = sw^2 iz P^T A v'          (= sw dp^T (sw v'))
+//   acc_rot += (P - t) x e, acc_t += e, acc_i += sw^2 (q.v') (radial, f r2, f r2^2).
+// ct: the camera table (R rows with t, f, k1, k2), fw: the forward table; f64 arithmetic.
+__device__ __forceinline__ void schur_camera_row(double qx, double qy, double z, double sw, const double tp[3],
+                                                 const double* ct, const double* fw, double acc[9]) {
+  const double f = ct[tF];
+  const double r2 = __builtin_fma(qx, qx, qy * qy);
+  const double radial = __builtin_fma(r2, __builtin_fma(ct[tK2], r2, ct[tK1]), 1.0);
+  const double fc2 = f * __builtin_fma(4.0 * ct[tK2], r2, 2.0 * ct[tK1]);
+  const double frad = f * radial;
+  const double cx = fc2 * qx, cy = fc2 * qy;
+  const double a = __builtin_fma(cx, qx, frad), b = cx * qy, d = __builtin_fma(cy, qy, frad);
+  const double iz = neg_inv_depth(z);
+  const double pt0 = __builtin_fma(-qx, z, -ct[tT(0)]);
+  const double pt1 = __builtin_fma(-qy, z, -ct[tT(1)]);
+  const double pt2 = z - ct[tT(2)];
+  // x = w' x pt + z_t - R t
+  double x0 = __builtin_fma(fw[1], pt2, __builtin_fma(-fw[2], pt1, fw[3]));
+  double x1 = __builtin_fma(fw[2], pt0, __builtin_fma(-fw[0], pt2, fw[4]));
+  double x2 = __builtin_fma(fw[0], pt1, __builtin_fma(-fw[1], pt0, fw[5]));
+  x0 = __builtin_fma(-ct[tR(0, 0)], tp[0], __builtin_fma(-ct[tR(0, 1)], tp[1], __builtin_fma(-ct[tR(0, 2)], tp[2], x0)));
+  x1 = __builtin_fma(-ct[tR(1, 0)], tp[0], __builtin_fma(-ct[tR(1, 1)], tp[1], __builtin_fma(-ct[tR(1, 2)], tp[2], x1)));
+  x2 = __builtin_fma(-ct[tR(2, 0)], tp[0], __builtin_fma(-ct[tR(2, 1)], tp[1], __builtin_fma(-ct[tR(2, 2)], tp[2], x2)));
+  const double fr2 = f * r2, fr4 = fr2 * r2;
+  const double sv = __builtin_fma(radial, fw[6], __builtin_fma(fr2, fw[7], fr4 * fw[8]));
+  const double y0 = __builtin_fma(qx, x2, x0), y1 = __builtin_fma(qy, x2, x1);
+  const double v0 = __builtin_fma(iz, __builtin_fma(a, y0, b * y1), qx * sv);
+  const double v1 = __builtin_fma(iz, __builtin_fma(b, y0, d * y1), qy * sv);
+  const double h0 = __builtin_fma(a, v0, b * v1), h1 = __builtin_fma(b, v0, d * v1);
+  const double w2 = sw * sw;
+  const double wz = w2 * iz;
+  const double e0 = wz * h0, e1 = wz * h1, e2 = wz * __builtin_fma(qx, h0, qy * h1);
+  acc[0] = __builtin_fma(pt1, e2, __builtin_fma(-pt2, e1, acc[0]));
+  acc[1] = __builtin_fma(pt2, e0, __builtin_fma(-pt0, e2, acc[1]));
+  acc[2] = __builtin_fma(pt0, e1, __builtin_fma(-pt1, e0, acc[2]));
+  acc[3] += e0;
+  acc[4] += e1;
+  acc[5] += e2;
+  const double si = w2 * __builtin_fma(qx, v0, qy * v1);
+  acc[6] = __builtin_fma(radial, si, acc[6]);
+  acc[7] = __builtin_fma(fr2, si, acc[7]);
+  acc[8] = __builtin_fma(fr4, si, acc[8]);
+}
+
+template <typename T>
+constexpr int schur_camera_waves() { return sizeof(T) == 4 ? 6 : 5; }
+// Rows per register set: two per set measured no faster than one (30.4 against 30.1 us in FP32).
+template <typename T>
+constexpr int schur_rows_per_step() { return 1; }
+template <typename T>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(schur_camera_waves<T>())))
+k_schur_cameras(int nc, const int* __restrict__ cam_off_r, const int* __restrict__ k_pslot,
+                const Vec4<T>* __restrict__ rec_cm, const double* __restrict__ tabw, const double* __restrict__ tp,
+                const double* __restrict__ fww, const T* __restrict__ dc, const double* __restrict__ pin,
+                double lambda, double* __restrict__ out, double* __restrict__ parts, const CgState* st) {
+  constexpr int kR = point_ranges<T>();
+  constexpr int kWavesPerBlock = kBlock / kWave;
+  if (st != nullptr) {
+    if (st->done) return;
+    lambda = st->lambda;
+  }
+  const int lane = threadIdx.x % kWave;
+  const int g = (int)blockIdx.x % kR;
+  const int c = __builtin_amdgcn_readfirstlane(((int)blockIdx.x / kR) * kWavesPerBlock + (int)threadIdx.x / kWave);
+  if (c >= nc) return;  // whole waves; no workgroup barrier follows
+  const int sh = kR * c + g;
+  const int k0 = cam_off_r[sh], k1 = cam_off_r[sh + 1];
+  const double* __restrict__ ct = tabw + kCamTab * c;
+  double ctl[16], fwl[kFwd];
+#pragma unroll
+  for (int a = 0; a < 16; ++a) ctl[a] = ct[a];
+#pragma unroll
+  for (int a = 0; a < kFwd; ++a) fwl[a] = fww[kFwd * c + a];
+  double acc[9];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) acc[a] = 0.0;
+  // The pipelined loads go through copies of the row pointers that the compiler cannot trace
+  // to the restrict-qualified arguments, so the compiler barriers in the loop also order them
+  // (otherwise each load is moved next to its use a row later).
+  using V4 = T __attribute__((ext_vector_type(4)));
+  using GVec = const __attribute__((address_space(1))) V4;
+  using GInt = const __attribute__((address_space(1))) int;
+  GVec* tpv = (GVec*)(tp);
+  GVec* recv = (GVec*)(rec_cm);
+  GInt* psv = (GInt*)(k_pslot);
+  __asm__ volatile("" : "+s"(tpv), "+s"(recv), "+s"(psv));
+  // rows k0 + lane + 64 (P G + i), i < P, form group G; the lane's groups alternate between the
+  // register sets A and B
+  constexpr int P = schur_rows_per_step<T>();
+  const int ng = ((k1 - k0 + kWave - 1) / kWave + P - 1) / P;  // uniform over the wave
+  if (ng > 0) {
+    const int last = k1 - 1;
+    const int kk = k0 + lane;
+    auto at = [&](int G, int i) { return kk + kWave * (P * G + i); };
+    auto row = [&](const V4& r, const V4& t, bool valid) {
+      const double x[3] = {double(t.x), double(t.y), double(t.z)};
+      schur_camera_row(double(r.x), double(r.y), double(r.z), valid ? double(r.w) : 0.0, x, ctl, fwl, acc);
+    };
+    V4 rec_a[P], t_a[P];
+    int ps_b[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      rec_a[i] = recv[min(at(0, i), last)];
+      t_a[i] = tpv[psv[min(at(0, i), last)]];
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) ps_b[i] = psv[min(at(1, i), last)];
+    for (int G = 0;; G += 2) {
+      int ps_a[P];
+      V4 rec_b[P], t_b[P];
+#pragma unroll
+      for (int i = 0; i < P; ++i) ps_a[i] = psv[min(at(G + 2, i), last)];
+#pragma unroll
+      for (int i = 0; i < P; ++i) rec_b[i] = recv[min(at(G + 1, i), last)];
+#pragma unroll
+      for (int i = 0; i < P; ++i) t_b[i] = tpv[ps_b[i]];
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < P; ++i) row(rec_a[i], t_a[i], at(G, i) < k1);
+      if (G + 1 >= ng) break;
+#pragma unroll
+      for (int i = 0; i < P; ++i) ps_b[i] = psv[min(at(G + 3, i), last)];
+#pragma unroll
+      for (int i = 0; i < P; ++i) rec_a[i] = recv[min(at(G + 2, i), last)];
+#pragma unroll
+      for (int i = 0; i < P; ++i) t_a[i] = tpv[ps_a[i]];
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < P; ++i) row(rec_b[i], t_b[i], at(G + 1, i) < k1);
+      if (G + 2 >= ng) break;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 9; ++a) acc[a] = wave_sum(acc[a]);
+  if (lane == 0) {
+    double y[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) y[j] = ct[tRJ(0, j)] * acc[0] + ct[tRJ(1, j)] * acc[1] + ct[tRJ(2, j)] * acc[2];
+#pragma unroll
+    for (int j = 3; j < 9; ++j) y[j] = acc[j];
+    const double* p = pin + 9 * c;
+    double part = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double yi = y[i] / double(dc[9 * c + i]);
+      out[9 * sh + i] = yi;
+      part += p[i] * yi;
+      if (g == 0) part += lambda * (p[i] * p[i]);
+    }
+    parts[sh] = part;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Camera-major half (one workgroup per camera). Sums the factors of Jc^T v over the camera's
-// rows and finishes the camera block:
-//   kSchur:   q = lambda p + Dc^-1 T(acc),  v = Jc Dc^-1 p - Jp t, with p.q partials. This is
-//             S p = (B + lambda I) p - E (C + lambda I)^-1 E^T p (Reduced::apply,
-//             schur.rs:606-652) in its factored form Dc^-1 Jc^T (u - Jp t) with u = Jc Dc^-1 p,
-//             which needs no assembled B and cancels per row on 2-vectors. One workgroup per
-//             (camera, point range) (point_ranges<T>() per camera, rows off_r[b]..off_r[b + 1]):
-//             it writes its share y_b = Dc^-1 T(acc_b) to out[9 b ..] and p_c.y_b (plus
-//             lambda ||p_c||^2 in range 0) to parts[b]; k_cg_update forms
-//             q_c = lambda p_c + sum_b y_b in range order. With one range the partial is
-//             p_c.(lambda p_c + y_c), the same rounding as the q the update forms.
-//   kResid:   q as kSchur, minus rhs, with ||q - rhs||^2 and ||rhs||^2 partials
+// rows and finishes the camera block (the PCG product's mode cSchur is k_schur_cameras):
+//   kResid:   q = lambda p + Dc^-1 T(acc) with v = Jc Dc^-1 p - Jp t (S p in factored form),
+//             minus rhs, with ||q - rhs||^2 and ||rhs||^2 partials
 //   kRhs:     rhs = Dc^-1 (-gc + T(acc)),  v = Jp t
 //   kVerify:  nr = T(acc) + lambda Dc^2 delta_c, v = J delta + r; norms and ||J delta||^2
 //   kJtu:     x_c = T(acc), v = u of the row's reference index cm_ref[k]
@@ -1450,37 +1649,27 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
 //             partials in parts[nc + c]
 enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5, cCheck = 6 };
 
-// Waves per SIMD the camera-major kernel is compiled for: the f32 product modes fit six (80
+// Waves per SIMD the camera-major kernel is compiled for: the f32 residual mode fits six (80
 // VGPRs), so all of a 1,500-camera launch is resident at once; the other modes keep the
 // compiler's default.
 template <typename T, int MODE>
-constexpr int camera_waves() { return sizeof(T) == 4 && (MODE == 0 || MODE == 1) ? 6 : 1; }
-
-// Threads per workgroup of the camera-major kernel: one wave for the Schur product's
-// (camera, point range) workgroups when a camera's rows are split over ranges (a few hundred
-// rows, about four per lane, and no LDS round for the sums; 128 threads measured 51.9 us and
-// 256 threads 74 us against 50.3 us); a whole camera's rows (kBlock) otherwise.
-template <typename T, int MODE>
-constexpr int camera_block() { return MODE == cSchur && point_ranges<T>() > 1 ? kWave : kBlock; }
+constexpr int camera_waves() { return sizeof(T) == 4 && MODE == cResid ? 6 : 1; }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__((camera_block<T, MODE>())) __attribute__((amdgpu_waves_per_eu(camera_waves<T, MODE>()))) k_cameras(
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(camera_waves<T, MODE>()))) k_cameras(
     int nc, const int* __restrict__ cam_off, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
     const Vec2<T>* __restrict__ res_cm, const double* __restrict__ tabw, const double* __restrict__ tp,
     const double* __restrict__ delta_p, const double* __restrict__ rows_in, const double* __restrict__ fww,
     const T* __restrict__ dc, const T* __restrict__ gc, const double* __restrict__ vin,
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
     double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls,
-    const int* __restrict__ cam_off_r, const int* __restrict__ cm_ref) {
-  constexpr int kB = camera_block<T, MODE>();
-  constexpr int kR = point_ranges<T>();
+    const int* __restrict__ cm_ref) {
+  static_assert(MODE != cSchur, "the Schur product's camera half is k_schur_cameras");
+  constexpr int kB = kBlock;
   __shared__ double red[9][kB / kWave];
-  if (MODE == cSchur && st != nullptr) {
-    if (st->done) return;
-    lambda = st->lambda;
-  }
+  (void)st;
   if (ls != nullptr && ls->done) return;
-  const int c = MODE == cSchur ? (int)blockIdx.x / kR : (int)blockIdx.x;
+  const int c = (int)blockIdx.x;
   // the camera's table and forward table as f64 rows (the T values widened): uniform over the
   // workgroup, so they are read with scalar loads and stay out of the vector registers
   const double* __restrict__ ct = tabw + kCamTab * c;
@@ -1488,8 +1677,8 @@ __global__ void __launch_bounds__((camera_block<T, MODE>())) __attribute__((amdg
   double acc[9], acc2[9];
 #pragma unroll
   for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
-  const int k0 = MODE == cSchur ? cam_off_r[blockIdx.x] : cam_off[c];
-  const int k1 = MODE == cSchur ? cam_off_r[blockIdx.x + 1] : cam_off[c + 1];
+  const int k0 = cam_off[c];
+  const int k1 = cam_off[c + 1];
   auto one = [&](int k, const Vec4<T>& rec, const Vec4<double>& tv) {
     ObsGeom<double> geo;
     obs_geom(to_d(rec), ct, geo);
@@ -1521,7 +1710,7 @@ __global__ void __launch_bounds__((camera_block<T, MODE>())) __attribute__((amdg
       const Vec2<T> r = res_cm[k];
       v0 = u0 + w0 + double(r.x);
       v1 = u1 + w1 + double(r.y);
-    } else if (MODE == cSchur || MODE == cResid) {
+    } else if (MODE == cResid) {
       const double x[3] = {tv.x, tv.y, tv.z};
       double u0, u1, w0, w1;
       jc_apply(geo, cfw, u0, u1);
@@ -1620,21 +1809,7 @@ __global__ void __launch_bounds__((camera_block<T, MODE>())) __attribute__((amdg
       pv[4] = v4s;
       pv[5] = 0.0;
     }
-    if (MODE == cSchur) {
-      const double* p = vin + 9 * c;
-      const bool first = blockIdx.x % kR == 0;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double yi = y[i] / d[i];
-        out[9 * blockIdx.x + i] = yi;
-        if (kR == 1) {
-          part0 += p[i] * (lambda * p[i] + yi);
-        } else {
-          part0 += p[i] * yi;
-          if (first) part0 += lambda * (p[i] * p[i]);
-        }
-      }
-    } else if (MODE == cResid || MODE == cCheck) {
+    if (MODE == cResid || MODE == cCheck) {
       const double* p = vin + 9 * c;
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
@@ -1680,9 +1855,7 @@ __global__ void __launch_bounds__((camera_block<T, MODE>())) __attribute__((amdg
       }
       parts[c] = part0;
     }
-    if (MODE == cSchur) {
-      parts[blockIdx.x] = part0;  // p.q, one per (camera, range) (k_cg_update sums them contiguously)
-    } else if (MODE != cJtu && MODE != cLsmr) {
+    if (MODE != cJtu && MODE != cLsmr) {
       double* pc = parts + kParts * c;
       pc[0] = part0;
       pc[1] = part1;
