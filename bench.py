@@ -42,10 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # measured on (null elsewhere).
 TRAFFIC_SOURCE = "profiles/hbm_traffic_rings1500.md"
 MEASURED_TRAFFIC = {
-    ("rings1500", "k_points<float, kSchur>"): (2 * 44332.3 + 7812.5) * 1024,
-    ("rings1500", "k_cameras<float, cSchur>"): (2 * 70948.7 + 187.5) * 1024,
-    ("rings1500", "k_points<double, kSchur>"): (2 * 70314.5 + 15625.1) * 1024,
-    ("rings1500", "k_cameras<double, cSchur>"): (2 * 79479.2 + 1533.8) * 1024,
+    ("rings1500", "k_points<float, kSchur>"): (2 * 44329.0 + 7812.5) * 1024,
+    ("rings1500", "k_schur_cameras<float>"): (2 * 52463.7 + 1828.8) * 1024,
+    ("rings1500", "k_points<double, kSchur>"): (2 * 70317.0 + 15625.0) * 1024,
+    ("rings1500", "k_schur_cameras<double>"): (2 * 78369.9 + 1516.7) * 1024,
 }
 
 
