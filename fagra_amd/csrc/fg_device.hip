@@ -718,13 +718,14 @@ fg_status attempt_begin(fg_schur* s, double lambda_d, CgRun& run) {
   if (learned) {
     Phase ph(s, &s->tim.preconditioner);
     if (!s->mlp) return FG_INVALID_OPTIONS;
-    // per-camera features -> network -> G = I + s tril(y) -> explicit L^-T G G^T L^-1
-    const float* range = s->mlp->range;
-    hipLaunchKernelGGL(k_mlp_features<T>, dim3(grid_for(nc * kInPad)), dim3(kBlock), 0, b->stream, nc,
-                       b->d_cam_off, (const T*)s->hcc, lambda_d, s->mx, range ? s->mxf : (float*)nullptr);
-    mlp_forward(s->mlp, b->stream, pad64(nc), s->mx, s->my);
-    hipLaunchKernelGGL(k_mlp_build_g, dim3((nc + 3) / 4), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
-                       (const float*)s->my, (const double*)s->lc, (const float*)s->mxf, range, (double*)s->gmat);
+    // per-camera features -> network -> G = I + s tril(y) -> explicit L^-T G G^T L^-1, 16
+    // cameras per workgroup in one launch
+    const fg_mlp* m = s->mlp;
+    hipLaunchKernelGGL(k_mlp_precond<T>, dim3((unsigned)((nc + 15) / 16)), dim3(256), 0, b->stream, nc,
+                       b->d_cam_off, (const T*)s->hcc, lambda_d, (const uint16_t*)m->w1,
+                       (const float*)(m->params + kB1), (const uint16_t*)m->w2, (const float*)(m->params + kB2),
+                       (const uint16_t*)m->w3, (const float*)(m->params + kB3), (const double*)s->lc,
+                       (const float*)m->range, (double*)s->gmat);
     FG_TRY(hipGetLastError());
   }
   Phase ph(s, &s->tim.linear_solve);

@@ -18,15 +18,12 @@ __host__ __device__ inline void upper_index(int k, int& i, int& j) {
   j = r + (k - start);
 }
 
-// Features of one camera block: for the 45 upper-triangle entries of the unnormalized Gram
+// Feature k of camera c's block: for the 45 upper-triangle entries of the unnormalized Gram
 // block, log(H_ii) on the diagonal and the correlation H_ij / sqrt(H_ii H_jj) off it; then
-// log(observations + 1) and log10(lambda), all scaled to O(1). One lane per (camera, feature).
+// log(observations + 1) and log10(lambda), all scaled to O(1); 0 for the padding k >= 47.
 template <typename T>
-__global__ void k_mlp_features(int nc, const int* __restrict__ cam_off, const T* __restrict__ hcc, double lambda,
-                               uint16_t* __restrict__ x, float* __restrict__ xf) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = gid / kInPad, k = gid % kInPad;
-  if (c >= nc) return;
+__device__ __forceinline__ float camera_feature(int c, int k, const int* __restrict__ cam_off,
+                                                const T* __restrict__ hcc, double lambda) {
   const T* h = hcc + 45 * c;
   float f = 0.f;
   if (k < 45) {
@@ -44,6 +41,18 @@ __global__ void k_mlp_features(int nc, const int* __restrict__ cam_off, const T*
   } else if (k == 46) {
     f = float(log10(fmax(lambda, 1e-30)) / 8.0);
   }
+  return f;
+}
+
+// The features of every camera as padded bf16 rows (and f32 rows in xf when non-null). One lane
+// per (camera, feature).
+template <typename T>
+__global__ void k_mlp_features(int nc, const int* __restrict__ cam_off, const T* __restrict__ hcc, double lambda,
+                               uint16_t* __restrict__ x, float* __restrict__ xf) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = gid / kInPad, k = gid % kInPad;
+  if (c >= nc) return;
+  const float f = camera_feature<T>(c, k, cam_off, hcc, lambda);
   x[(size_t)kInPad * c + k] = f2bf(f);
   if (xf && k < kIn) xf[(size_t)kIn * c + k] = f;
 }
@@ -149,11 +158,38 @@ __device__ __forceinline__ void build_g(const float* yrow, float g[45]) {
 }
 
 // The learned inverse per observed camera, M = L^-T G G^T L^-1 (packed symmetric, f64), from
-// the network output y and the block-Jacobi factor L of B + lambda I. One 64-lane group per
-// camera: lanes 0-8 form column j of W = L^-T G (a triangular solve each), lanes 0-44 then
-// form one packed entry of M = W W^T.
-// With range (device [2][kIn], lower then upper bounds) a camera whose features xf (f32
-// [nc][kIn]) leave the range the network was trained on keeps G = I, i.e. block-Jacobi.
+// the network output y and the block-Jacobi factor L of B + lambda I, in two steps: column j of
+// W = L^-T G (a triangular solve), then entry e of M = W W^T.
+// With range (device [2][kIn], lower then upper bounds) a camera whose features xf (f32, kIn)
+// leave the range the network was trained on keeps G = I, i.e. block-Jacobi.
+__device__ __forceinline__ void inverse_column(int j, const float* yr, const double* __restrict__ l45,
+                                               const float* xf, const float* __restrict__ range, double col[9]) {
+  bool inside = true;
+  if (range)
+    for (int k = 0; k < kIn; ++k) {
+      const float f = xf[k];
+      inside = inside && f >= range[k] && f <= range[kIn + k];
+    }
+  double l[45];
+  for (int i = 0; i < 45; ++i) l[i] = l45[i];
+  for (int i = 0; i < 9; ++i)  // column j of G = I + s tril(y)
+    col[i] = i < j ? 0.0 : (inside ? double(kOutScale * yr[sym_index(9, i, j)]) : 0.0) + (i == j ? 1.0 : 0.0);
+  solve_upper_packed<double, 9>(l, col);
+}
+// Entry e (packed lower, column-major) of M = W W^T from W (9x9, w[i][t]).
+__device__ __forceinline__ void inverse_entry(int e, const double (*w)[9], double* __restrict__ m45) {
+  int j = 0, k = e;
+  while (k >= 9 - j) {  // packed lower, column-major: (i, j) with i >= j
+    k -= 9 - j;
+    ++j;
+  }
+  const int i = j + k;
+  double s = 0.0;
+  for (int t = 0; t < 9; ++t) s += w[i][t] * w[j][t];
+  m45[sym9(i, j)] = s;
+}
+
+// One 64-lane group per camera: lanes 0-8 form the columns of W, lanes 0-44 one entry of M each.
 static __global__ void __launch_bounds__(256) k_mlp_build_g(int nc, const int* __restrict__ cam_off,
                                                             const float* __restrict__ y,
                                                             const double* __restrict__ lc,
@@ -165,33 +201,84 @@ static __global__ void __launch_bounds__(256) k_mlp_build_g(int nc, const int* _
   const int c = blockIdx.x * 4 + grp;
   const bool active = c < nc && cam_off[c + 1] > cam_off[c];
   if (active && lane < 9) {
-    const int j = lane;
-    const float* yr = y + (size_t)kOutPad * c;
-    bool inside = true;
-    if (range)
-      for (int k = 0; k < kIn; ++k) {
-        const float f = xf[(size_t)kIn * c + k];
-        inside = inside && f >= range[k] && f <= range[kIn + k];
-      }
-    double l[45], col[9];
-    for (int i = 0; i < 45; ++i) l[i] = lc[45 * c + i];
-    for (int i = 0; i < 9; ++i)  // column j of G = I + s tril(y)
-      col[i] = i < j ? 0.0 : (inside ? double(kOutScale * yr[sym_index(9, i, j)]) : 0.0) + (i == j ? 1.0 : 0.0);
-    solve_upper_packed<double, 9>(l, col);
-    for (int i = 0; i < 9; ++i) w[grp][i][j] = col[i];
+    double col[9];
+    inverse_column(lane, y + (size_t)kOutPad * c, lc + 45 * c, range ? xf + (size_t)kIn * c : nullptr, range, col);
+    for (int i = 0; i < 9; ++i) w[grp][i][lane] = col[i];
   }
   __syncthreads();
-  if (active && lane < 45) {
-    int j = 0, k = lane;
-    while (k >= 9 - j) {  // packed lower, column-major: (i, j) with i >= j
-      k -= 9 - j;
-      ++j;
-    }
-    const int i = j + k;
-    double s = 0.0;
-    for (int t = 0; t < 9; ++t) s += w[grp][i][t] * w[grp][j][t];
-    mout[45 * c + sym9(i, j)] = s;
+  if (active && lane < 45) inverse_entry(lane, w[grp], mout + 45 * c);
+}
+
+// The whole learned preconditioner of 16 cameras in one workgroup (features, the network as
+// k_mlp_infer computes it, and M as k_mlp_build_g), every intermediate in LDS: one launch
+// instead of three and no round trips through memory between them. The same arithmetic on the
+// same operands as the three kernels, so M is the same.
+template <typename T>
+static __global__ void __launch_bounds__(256) k_mlp_precond(int nc, const int* __restrict__ cam_off,
+                                                            const T* __restrict__ hcc, double lambda,
+                                                            const uint16_t* __restrict__ w1,
+                                                            const float* __restrict__ b1,
+                                                            const uint16_t* __restrict__ w2,
+                                                            const float* __restrict__ b2,
+                                                            const uint16_t* __restrict__ w3,
+                                                            const float* __restrict__ b3,
+                                                            const double* __restrict__ lc,
+                                                            const float* __restrict__ range,
+                                                            double* __restrict__ mout) {
+  __shared__ uint16_t xs[16][kInPad];
+  __shared__ float xfs[16][kIn];
+  __shared__ uint16_t h1[16][kHidden];
+  __shared__ uint16_t h2[16][kHidden];
+  __shared__ float ys[16][kOutPad];
+  __shared__ double w[16][9][9];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const int c0 = blockIdx.x * 16;
+  for (int e = threadIdx.x; e < 16 * kInPad; e += blockDim.x) {
+    const int cl = e / kInPad, k = e % kInPad, c = c0 + cl;
+    const float f = c < nc ? camera_feature<T>(c, k, cam_off, hcc, lambda) : 0.f;
+    xs[cl][k] = f2bf(f);
+    if (k < kIn) xfs[cl][k] = f;
   }
+  __syncthreads();
+  for (int nt = wave; nt < kHidden / 16; nt += 4) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&xs[0][0], kInPad, w1 + (size_t)nt * 16 * kInPad, kInPad, kInPad, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      h1[row][col] = f2bf(gelu(acc[i] + b1[col]));
+    }
+  }
+  __syncthreads();
+  for (int nt = wave; nt < kHidden / 16; nt += 4) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&h1[0][0], kHidden, w2 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      h2[row][col] = f2bf(gelu(acc[i] + b2[col]));
+    }
+  }
+  __syncthreads();
+  for (int nt = wave; nt < kOutPad / 16; nt += 4) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma_tile(&h2[0][0], kHidden, w3 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * q + i, col = nt * 16 + r;
+      ys[row][col] = acc[i] + b3[col];
+    }
+  }
+  __syncthreads();
+  // 16 threads per camera: 9 of them form the columns of W, then all 16 the 45 entries of M
+  const int cl = threadIdx.x >> 4, t = threadIdx.x & 15, c = c0 + cl;
+  const bool active = c < nc && cam_off[c + 1] > cam_off[c];
+  if (active && t < 9) {
+    double col[9];
+    inverse_column(t, &ys[cl][0], lc + 45 * c, range ? &xfs[cl][0] : nullptr, range, col);
+    for (int i = 0; i < 9; ++i) w[cl][i][t] = col[i];
+  }
+  __syncthreads();
+  if (active)
+    for (int e = t; e < 45; e += 16) inverse_entry(e, w[cl], mout + 45 * c);
 }
 
 // Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).
