@@ -1199,14 +1199,28 @@ fg_status fg_ba_create(size_t nc, size_t np, size_t no, const double* cameras, c
       k_pslot[k] = (int)sl;
     }
   }
-  // Camera-major device rows: within each camera, the rows of points in slot range g (slots
-  // [g np / R, (g + 1) np / R)) before those of range g + 1, reference order within a range.
-  // Index arrays of the camera-major kernels (k_slot, k_pslot, pixels) follow the device rows;
-  // slot_row keeps the reference row, which is the order of J v and J^T u row vectors.
+  // Camera-major device rows: within each camera, the rows of points in slot range g before
+  // those of range g + 1, reference order within a range. The R ranges hold equal numbers of
+  // observations, not of points: slots run longest track first, so equal slot counts would put
+  // most rows in range 0 (65% at R = 2 with tracks of 3-10, over 70% on Zipf-distributed
+  // tracks), and the XCDs that run range 0 would carry that share of the camera pass alone.
+  // Range g is the slots [first[g], first[g + 1]), first[g] the first slot whose preceding
+  // slots hold at least g no / R observations. Index arrays of the camera-major kernels
+  // (k_slot, k_pslot, pixels) follow the device rows; slot_row keeps the reference row, which
+  // is the order of J v and J^T u row vectors.
   const int ranges = precision == FG_FP32 ? point_ranges<float>() : point_ranges<double>();
   std::vector<int> cam_off_r(nc * ranges + 1, 0);
   {
-    auto range_of = [&](int64_t k) { return (int)((int64_t)k_pslot[k] * ranges / (int64_t)np); };
+    std::vector<int> range_of_slot(np, 0);
+    {
+      int64_t before = 0;  // observations of the slots before sl
+      for (size_t sl = 0; sl < np; ++sl) {
+        // the range whose share of the observations the slot starts in
+        range_of_slot[sl] = (int)std::min<int64_t>(ranges - 1, before * ranges / std::max<int64_t>(1, (int64_t)no));
+        before += pt_len[sl];
+      }
+    }
+    auto range_of = [&](int64_t k) { return range_of_slot[k_pslot[k]]; };
     for (size_t c = 0; c < nc; ++c)
       for (int64_t k = b->cam_off[c]; k < b->cam_off[c + 1]; ++k) cam_off_r[c * ranges + range_of(k) + 1]++;
     for (size_t g = 0; g < nc * ranges; ++g) cam_off_r[g + 1] += cam_off_r[g];
