@@ -471,15 +471,29 @@ fg_status gradient_finish(fg_schur* s, bool spec, double* out) {
 // direction reads it. rhs_points: the point pass also forms the right-hand side's point solves
 // into s->tp.
 template <typename T>
-fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = false) {
+// With learned (and no timings), the point factorization's launch also forms the learned
+// preconditioner's explicit inverse in s->gmat (k_factor_points_precond).
+fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = false, bool learned = false) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
   hipLaunchKernelGGL(k_factor_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
-  hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
-                     b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail, (const T*)s->gp,
-                     (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp,
-                     (double*)s->lpp);
+  if (learned && !s->cfg.collect_timings) {
+    const fg_mlp* m = s->mlp;
+    const PrecondArgs pa{(int)b->nc, b->d_cam_off, s->hcc, lambda, m->w1, m->w2, m->w3,
+                         m->params + kB1, m->params + kB2, m->params + kB3, (const double*)s->lc, m->range,
+                         (double*)s->gmat};
+    const int lblocks = (int)((b->nc + 15) / 16);
+    hipLaunchKernelGGL(k_factor_points_precond<T>, dim3((unsigned)(lblocks + grid_for(b->np))), dim3(kBlock), 0,
+                       b->stream, lblocks, pa, (int)b->np, b->d_pt_len, (const double*)s->cs, lambda,
+                       (double*)s->mp, s->d_fail, (const T*)s->gp, (const T*)s->dp,
+                       rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp, (double*)s->lpp);
+  } else {
+    hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
+                       b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail, (const T*)s->gp,
+                       (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp,
+                       (double*)s->lpp);
+  }
   FG_TRY(hipGetLastError());
   if (fails == nullptr) return FG_OK;
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -703,29 +717,27 @@ template <typename T>
 fg_status attempt_begin(fg_schur* s, double lambda_d, CgRun& run) {
   fg_ba* b = s->ba;
   const double lambda = lambda_d;
-  const int nc = (int)b->nc;
+  const bool learned = s->cfg.preconditioner == FG_PRECOND_LEARNED;
+  if (learned && !s->mlp) return FG_INVALID_OPTIONS;
   {
     Phase ph(s, &s->tim.factorization);
     // a failed block is reported by the PCG (NotPositiveDefinite before its first iteration)
-    // the point factorization also forms M gp (the point half of the reduced right-hand side)
-    FG_CHECK(factor_impl<T>(s, lambda, nullptr, true));
+    // the point factorization also forms M gp (the point half of the reduced right-hand side),
+    // and with the learned preconditioner, in the same launch, its per-camera features ->
+    // network -> G = I + s tril(y) -> explicit L^-T G G^T L^-1
+    FG_CHECK(factor_impl<T>(s, lambda, nullptr, true, learned));
     // rhs = Dc^-1 (-gc + sum Jc^T Jp M gp)
     launch_cameras<T, cRhs>(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, lambda, (double*)s->rhs,
                             nullptr);
     FG_TRY(hipGetLastError());
   }
-  const bool learned = s->cfg.preconditioner == FG_PRECOND_LEARNED;
-  if (learned) {
+  if (learned && s->cfg.collect_timings) {  // timed separately, as SchurTimings::preconditioner
     Phase ph(s, &s->tim.preconditioner);
-    if (!s->mlp) return FG_INVALID_OPTIONS;
-    // per-camera features -> network -> G = I + s tril(y) -> explicit L^-T G G^T L^-1, 16
-    // cameras per workgroup in one launch
     const fg_mlp* m = s->mlp;
-    hipLaunchKernelGGL(k_mlp_precond<T>, dim3((unsigned)((nc + 15) / 16)), dim3(256), 0, b->stream, nc,
-                       b->d_cam_off, (const T*)s->hcc, lambda_d, (const uint16_t*)m->w1,
-                       (const float*)(m->params + kB1), (const uint16_t*)m->w2, (const float*)(m->params + kB2),
-                       (const uint16_t*)m->w3, (const float*)(m->params + kB3), (const double*)s->lc,
-                       (const float*)m->range, (double*)s->gmat);
+    const PrecondArgs pa{(int)b->nc, b->d_cam_off, s->hcc, lambda, m->w1, m->w2, m->w3,
+                         m->params + kB1, m->params + kB2, m->params + kB3, (const double*)s->lc, m->range,
+                         (double*)s->gmat};
+    hipLaunchKernelGGL(k_mlp_precond<T>, dim3((unsigned)((b->nc + 15) / 16)), dim3(kBlock), 0, b->stream, pa);
     FG_TRY(hipGetLastError());
   }
   Phase ph(s, &s->tim.linear_solve);

@@ -725,11 +725,11 @@ __device__ __forceinline__ void store_tp(double* tp, int i, const double t[3]);
 // (the Gram of the Jp the products re-form), which k_points<kSchur> solves with; a zero
 // factor where it fails (an unobserved point at lambda = 0), where kSchur's t is zero.
 template <typename T>
-__global__ void k_factor_points(int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
-                                double lambda, double* __restrict__ lp, int* fail_count,
-                                const T* __restrict__ gp, const T* __restrict__ dpv, double* __restrict__ t_out,
-                                const double* __restrict__ hpp, double* __restrict__ lpp) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void factor_point(int p, int np, const int* __restrict__ pt_len,
+                                             const double* __restrict__ cs, double lambda, double* __restrict__ lp,
+                                             int* fail_count, const T* __restrict__ gp, const T* __restrict__ dpv,
+                                             double* __restrict__ t_out, const double* __restrict__ hpp,
+                                             double* __restrict__ lpp) {
   if (p >= np) return;
   double* l = lp + 6 * p;
   if (lpp != nullptr) {
@@ -772,6 +772,14 @@ __global__ void k_factor_points(int np, const int* __restrict__ pt_len, const do
     point_solve(lp, dpv, p, v, t);
     store_tp<T>(t_out, p, t);
   }
+}
+template <typename T>
+__global__ void k_factor_points(int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
+                                double lambda, double* __restrict__ lp, int* fail_count,
+                                const T* __restrict__ gp, const T* __restrict__ dpv, double* __restrict__ t_out,
+                                const double* __restrict__ hpp, double* __restrict__ lpp) {
+  factor_point<T>(blockIdx.x * blockDim.x + threadIdx.x, np, pt_len, cs, lambda, lp, fail_count, gp, dpv, t_out, hpp,
+                  lpp);
 }
 
 // t = (Hpp + lambda D^2)^-1 v = D^-1 (C + lambda I)^-1 D^-1 v with the stored factor of point i.

@@ -210,21 +210,31 @@ static __global__ void __launch_bounds__(256) k_mlp_build_g(int nc, const int* _
 }
 
 // The whole learned preconditioner of 16 cameras in one workgroup (features, the network as
-// k_mlp_infer computes it, and M as k_mlp_build_g), every intermediate in LDS: one launch
-// instead of three and no round trips through memory between them. The same arithmetic on the
-// same operands as the three kernels, so M is the same.
+// k_mlp_infer computes it, and M as k_mlp_build_g), every intermediate in LDS: no round trips
+// through memory between them. The same arithmetic on the same operands as the three kernels,
+// so M is the same. Every thread of the workgroup must call it (it has workgroup barriers).
+struct PrecondArgs {
+  int nc;
+  const int* cam_off;
+  const void* hcc;  // T [nc][45]
+  double lambda;
+  const uint16_t *w1, *w2, *w3;
+  const float *b1, *b2, *b3;
+  const double* lc;
+  const float* range;
+  double* mout;
+};
 template <typename T>
-static __global__ void __launch_bounds__(256) k_mlp_precond(int nc, const int* __restrict__ cam_off,
-                                                            const T* __restrict__ hcc, double lambda,
-                                                            const uint16_t* __restrict__ w1,
-                                                            const float* __restrict__ b1,
-                                                            const uint16_t* __restrict__ w2,
-                                                            const float* __restrict__ b2,
-                                                            const uint16_t* __restrict__ w3,
-                                                            const float* __restrict__ b3,
-                                                            const double* __restrict__ lc,
-                                                            const float* __restrict__ range,
-                                                            double* __restrict__ mout) {
+__device__ __forceinline__ void mlp_precond_block(int blk, const PrecondArgs& pa) {
+  const int nc = pa.nc;
+  const int* __restrict__ cam_off = pa.cam_off;
+  const T* __restrict__ hcc = (const T*)pa.hcc;
+  const double lambda = pa.lambda;
+  const uint16_t *__restrict__ w1 = pa.w1, *__restrict__ w2 = pa.w2, *__restrict__ w3 = pa.w3;
+  const float *__restrict__ b1 = pa.b1, *__restrict__ b2 = pa.b2, *__restrict__ b3 = pa.b3;
+  const double* __restrict__ lc = pa.lc;
+  const float* __restrict__ range = pa.range;
+  double* __restrict__ mout = pa.mout;
   __shared__ uint16_t xs[16][kInPad];
   __shared__ float xfs[16][kIn];
   __shared__ uint16_t h1[16][kHidden];
@@ -233,7 +243,7 @@ static __global__ void __launch_bounds__(256) k_mlp_precond(int nc, const int* _
   __shared__ double w[16][9][9];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
-  const int c0 = blockIdx.x * 16;
+  const int c0 = blk * 16;
   for (int e = threadIdx.x; e < 16 * kInPad; e += blockDim.x) {
     const int cl = e / kInPad, k = e % kInPad, c = c0 + cl;
     const float f = c < nc ? camera_feature<T>(c, k, cam_off, hcc, lambda) : 0.f;
@@ -279,6 +289,29 @@ static __global__ void __launch_bounds__(256) k_mlp_precond(int nc, const int* _
   __syncthreads();
   if (active)
     for (int e = t; e < 45; e += 16) inverse_entry(e, w[cl], mout + 45 * c);
+}
+
+template <typename T>
+static __global__ void __launch_bounds__(256) k_mlp_precond(PrecondArgs pa) {
+  mlp_precond_block<T>(blockIdx.x, pa);
+}
+
+// The attempt's point factorization (k_factor_points) and the learned preconditioner in one
+// launch: workgroups 0 .. learned_blocks - 1 form M for 16 cameras each (mlp_precond_block; they
+// need the camera factors, which k_factor_cameras wrote before), the rest factor 256 points
+// each. The two are independent, and the network's few latency-bound workgroups run beside the
+// point factorization's instead of after it.
+template <typename T>
+static __global__ void __launch_bounds__(256) k_factor_points_precond(
+    int learned_blocks, PrecondArgs pa, int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
+    double lambda, double* __restrict__ lp, int* fail_count, const T* __restrict__ gp, const T* __restrict__ dpv,
+    double* __restrict__ t_out, const double* __restrict__ hpp, double* __restrict__ lpp) {
+  if ((int)blockIdx.x < learned_blocks) {  // uniform over the workgroup
+    mlp_precond_block<T>(blockIdx.x, pa);
+    return;
+  }
+  factor_point<T>(((int)blockIdx.x - learned_blocks) * blockDim.x + threadIdx.x, np, pt_len, cs, lambda, lp,
+                  fail_count, gp, dpv, t_out, hpp, lpp);
 }
 
 // Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).

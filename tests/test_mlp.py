@@ -137,6 +137,32 @@ def test_learned_preconditioner_solves_the_same_system(force_fallback):
 
 
 @pytest.mark.gpu
+def test_learned_preconditioner_fused_and_timed_launches_agree():
+    """Without timings the learned inverse is formed in the point factorization's launch; with
+    collect_timings it runs as its own kernel, timed as SchurTimings::preconditioner. Both apply
+    the same arithmetic to the same operands, so the damped step is the same bit for bit."""
+    import fagra_amd as fa
+    from fagra_amd import learned, synthetic
+    p = synthetic.generate("ring16", seed=23, n_cameras=40, n_points=1500, huber=1.0)
+    net = learned.Network(params=M.init_params(11))
+    out = {}
+    for timed in (False, True):
+        ba = fa.BalProblem(p, precision=fa.FP32)
+        s = fa.Schur(precision=fa.FP32, relative_tolerance=1e-6, preconditioner=fa.PRECOND_LEARNED,
+                     collect_timings=timed)
+        fa.lib().fg_schur_set_mlp(s.h, net.h)
+        assert s.prepare(ba) == fa.OK and s.linearize() == fa.OK and s.prepare_damping(1e-6) == fa.OK
+        st, d, pr = s.solve_damped(1e-2)
+        assert st == fa.OK and s.diagnostics().termination == fa.CG_CONVERGED
+        out[timed] = (d, pr, s.diagnostics().iterations, s.timings().preconditioner)
+        s.close()
+        ba.close()
+    assert np.array_equal(out[False][0], out[True][0]) and out[False][1] == out[True][1]
+    assert out[False][2] == out[True][2]
+    assert out[True][3] > 0.0 and out[False][3] == 0.0
+
+
+@pytest.mark.gpu
 def test_learned_without_network_is_rejected():
     import fagra_amd as fa
     from fagra_amd import synthetic
