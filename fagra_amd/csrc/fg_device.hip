@@ -135,6 +135,7 @@ struct fg_schur {
   void* lpp = nullptr;  // f64 [np][6]: factor of Hpp + lambda Dp^2 of the current attempt
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* qy = nullptr;  // the PCG product's shares of q, 9 per (camera, point range)
+  double* cshares = nullptr;  // the check's camera shares, 18 per (camera, point range)
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
   void* prow = nullptr;    // T [nc][kPRow]: table row + forward table (point-major reads)
@@ -776,8 +777,20 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   const int pblocks = point_blocks(b);
   launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
                            s->d_parts + (size_t)kParts * 2 * nc, nullptr);
-  launch_cameras<T, cCheck>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, (const double*)s->x, nullptr,
-                            (const double*)s->rhs, lambda, (double*)s->q, s->d_delta);
+  {
+    // the check's camera side: one wave per (camera, point range) into shares, then each
+    // camera's shares in range order (k_cameras<cCheck> in two launches)
+    constexpr int kR = point_ranges<T>(), kWavesPerBlock = kBlock / kWave;
+    const unsigned grid = (unsigned)((b->nc + kWavesPerBlock - 1) / kWavesPerBlock) * kR;
+    hipLaunchKernelGGL(k_check_cameras<T>, dim3(grid), dim3(kBlock), 0, b->stream, nc,
+                       (const int*)b->d_cam_off_r, (const int*)b->d_k_pslot, (const Vec4<T>*)s->rec_cm,
+                       (const Vec2<T>*)s->res_cm, (const double*)b->d_tabw[b->cur], (const double*)s->tp,
+                       (const double*)(s->d_delta + 9 * b->nc), fw_wide(s), s->cshares);
+    hipLaunchKernelGGL(k_check_finish<T>, dim3(grid_for(nc, kWave)), dim3(kWave), 0, b->stream, nc,
+                       (const double*)s->cshares, (const double*)b->d_tabw[b->cur], (const T*)s->dc,
+                       (const T*)s->gc, (const double*)s->x, (const double*)s->rhs, lambda, (double*)s->q,
+                       s->d_delta, s->d_parts);
+  }
   const SumJobs jobs{{nc, pblocks, nc, 0}, {0, 2 * nc, nc, 0}, {0, 1, 2, 0}};
   hipLaunchKernelGGL(k_sum_parts_jobs, dim3(3), dim3(kBlock), 0, b->stream, (const double*)s->d_parts, jobs,
                      s->d_sums);
@@ -859,6 +872,7 @@ fg_status alloc_model(fg_schur* s) {
   if (sizeof(T) == 4) FG_TRY(dalloc(&s->fw_w, kFwd * nc));
   FG_TRY(dalloc((T**)&s->prow, kPRow * nc));
   FG_TRY(dalloc(&s->qy, 9 * (size_t)point_ranges<T>() * nc));
+  FG_TRY(dalloc(&s->cshares, 18 * (size_t)point_ranges<T>() * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((double**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
@@ -899,6 +913,7 @@ void free_model(fg_schur* s) {
   }
   dfree(s->fw_w);
   dfree(s->qy);
+  dfree(s->cshares);
   if (s->prow) (void)hipFree(s->prow);
   s->prow = nullptr;
   dfree(s->d_delta);

@@ -1654,7 +1654,8 @@ k_schur_cameras(int nc, const int* __restrict__ cam_off_r, const int* __restrict
 //   kCheck:   cResid and cVerify of one attempt in one pass: per row u = Jc z (forward table
 //             of z), v1 = u - Jp t_p for q = S z, v2 = u + Jp delta_p + r for the normal
 //             residual, delta_c = z / D written; residual partials in parts[c], verification
-//             partials in parts[nc + c]
+//             partials in parts[nc + c] (k_check_cameras and k_check_finish; finish_camera
+//             holds the camera's end of every mode)
 enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5, cCheck = 6 };
 
 // Waves per SIMD the camera-major kernel is compiled for: the f32 residual mode fits six (80
@@ -1662,6 +1663,111 @@ enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr
 // compiler's default.
 template <typename T, int MODE>
 constexpr int camera_waves() { return sizeof(T) == 4 && MODE == cResid ? 6 : 1; }
+
+// The end of a camera-major pass for camera c, from the camera's column sums s of the factors
+// of Jc^T v (and, in cCheck, s2 of the second row vector): T(acc), the camera block's outputs
+// and its partial sums, as the modes above describe.
+template <typename T, int MODE>
+__device__ __forceinline__ void finish_camera(int nc, int c, const double s[9], const double s2[9],
+                                              const double* __restrict__ ct, const T* __restrict__ dc,
+                                              const T* __restrict__ gc, const double* __restrict__ vin,
+                                              const double* __restrict__ delta_c, const double* __restrict__ rhs,
+                                              double lambda, double* __restrict__ out, double* __restrict__ out_d,
+                                              double* parts, const LsmrState* ls) {
+  double part0 = 0.0, part1 = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
+  // T(acc): rotation slots through (R Jr)^T
+  double y[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) y[j] = ct[tRJ(0, j)] * s[0] + ct[tRJ(1, j)] * s[1] + ct[tRJ(2, j)] * s[2];
+#pragma unroll
+  for (int j = 3; j < 9; ++j) y[j] = s[j];
+  double d[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = double(dc[9 * c + i]);
+  if (MODE == cCheck) {
+    // verification of delta = (z / D, delta_p) in original coordinates
+    double y2[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) y2[j] = ct[tRJ(0, j)] * s2[0] + ct[tRJ(1, j)] * s2[1] + ct[tRJ(2, j)] * s2[2];
+#pragma unroll
+    for (int j = 3; j < 9; ++j) y2[j] = s2[j];
+    double v0s = 0.0, v1s = 0.0, v2s = 0.0, v3s = 0.0, v4s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double dd = d[i];
+      const double dc_i = vin[9 * c + i] / dd;
+      out_d[9 * c + i] = dc_i;
+      const double nr = y2[i] + (lambda * (dd * dc_i)) * dd;
+      v0s += nr * nr;
+      v1s += (nr / dd) * (nr / dd);
+      const double g = double(gc[9 * c + i]);
+      v2s += g * dc_i;
+      v3s += g * g;
+      v4s += (g / dd) * (g / dd);
+    }
+    double* pv = parts + kParts * (nc + c);
+    pv[0] = v0s;
+    pv[1] = v1s;
+    pv[2] = v2s;
+    pv[3] = v3s;
+    pv[4] = v4s;
+    pv[5] = 0.0;
+  }
+  if (MODE == cResid || MODE == cCheck) {
+    const double* p = vin + 9 * c;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double q = lambda * p[i] + y[i] / d[i];
+      out[9 * c + i] = q;
+      const double e = q - rhs[9 * c + i];
+      part0 += e * e;
+      part1 += rhs[9 * c + i] * rhs[9 * c + i];
+    }
+  } else if (MODE == cRhs) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double r = (-double(gc[9 * c + i]) + y[i]) / d[i];
+      out[9 * c + i] = r;
+      part0 += r * r;
+    }
+  } else if (MODE == cVerify) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double dd = d[i];
+      const double nr = y[i] + (lambda * (dd * delta_c[9 * c + i])) * dd;
+      if (out_d) out_d[9 * c + i] = nr;  // the normal residual itself, on request
+      part0 += nr * nr;
+      part1 += (nr / dd) * (nr / dd);
+      const double g = double(gc[9 * c + i]);
+      part2 += g * delta_c[9 * c + i];
+      part3 += g * g;
+      part4 += (g / dd) * (g / dd);
+    }
+  } else if (MODE == cJtu) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out_d[9 * c + i] = y[i];
+  } else if (MODE == cLsmr) {
+    const double coef = ls->alpha / ls->beta, root = ls->root;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      out_d[9 * c + i] = y[i];
+      if (vin != nullptr && out != nullptr) {  // damping rows of this camera
+        const double ub = root * d[i] * vin[9 * c + i] - coef * out[9 * c + i];
+        out[9 * c + i] = ub;
+        part0 += ub * ub;
+      }
+    }
+    parts[c] = part0;
+  }
+  if (MODE != cJtu && MODE != cLsmr) {
+    double* pc = parts + kParts * c;
+    pc[0] = part0;
+    pc[1] = part1;
+    pc[2] = part2;
+    pc[3] = part3;
+    pc[4] = part4;
+  }
+}
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(camera_waves<T, MODE>()))) k_cameras(
@@ -1672,7 +1778,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
     double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls,
     const int* __restrict__ cm_ref) {
-  static_assert(MODE != cSchur, "the Schur product's camera half is k_schur_cameras");
+  static_assert(MODE != cSchur && MODE != cCheck, "cSchur and cCheck are k_schur_cameras and k_check_cameras");
   constexpr int kB = kBlock;
   __shared__ double red[9][kB / kWave];
   (void)st;
@@ -1682,9 +1788,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
   // workgroup, so they are read with scalar loads and stay out of the vector registers
   const double* __restrict__ ct = tabw + kCamTab * c;
   const double* __restrict__ cfw = fww + kFwd * c;
-  double acc[9], acc2[9];
+  double acc[9];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
+  for (int a = 0; a < 9; ++a) acc[a] = 0.0;
   const int k0 = cam_off[c];
   const int k1 = cam_off[c + 1];
   auto one = [&](int k, const Vec4<T>& rec, const Vec4<double>& tv) {
@@ -1698,18 +1804,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
     } else if (MODE == cLsmr) {
       v0 = tv.x;
       v1 = tv.y;
-    } else if (MODE == cCheck) {
-      const int ps = k_pslot[k];
-      const double xt[3] = {tv.x, tv.y, tv.z};
-      const double xd[3] = {delta_p[3 * ps], delta_p[3 * ps + 1], delta_p[3 * ps + 2]};
-      double u0, u1, a0, a1, b0, b1;
-      jc_apply(geo, cfw, u0, u1);
-      jp_apply(geo, ct, xt, a0, a1);
-      jp_apply(geo, ct, xd, b0, b1);
-      const Vec2<T> r = res_cm[k];
-      v0 = u0 - a0;
-      v1 = u1 - a1;
-      jct_factors(geo, u0 + b0 + double(r.x), u1 + b1 + double(r.y), acc2);
     } else if (MODE == cVerify) {
       const double x[3] = {tv.x, tv.y, tv.z};
       double u0, u1, w0, w1;
@@ -1733,7 +1827,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
   };
   auto gather = [&](int k) -> Vec4<double> {
     if (MODE == cJtu) return Vec4<double>{0.0, 0.0, 0.0, 0.0};
-    // cCheck gathers t_p here and delta_p in the row itself
     const int ps = k_pslot[k];
     if (MODE == cLsmr) {
       const Vec2<double> u = reinterpret_cast<const Vec2<double>*>(rows_in)[ps];
@@ -1757,16 +1850,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
     const double v = wave_sum(acc[a]);
     if (lane == 0) red[a][wave] = v;
   }
-  __shared__ double red2[9][kB / kWave];
-  if (MODE == cCheck) {
-#pragma unroll
-    for (int a = 0; a < 9; ++a) {
-      const double v = wave_sum(acc2[a]);
-      if (lane == 0) red2[a][wave] = v;
-    }
-  }
   __syncthreads();
-  double part0 = 0.0, part1 = 0.0, part2 = 0.0, part3 = 0.0, part4 = 0.0;
   if (threadIdx.x == 0) {
     double s[9];
 #pragma unroll
@@ -1774,104 +1858,156 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
       s[a] = 0.0;
       for (int w = 0; w < kB / kWave; ++w) s[a] += red[a][w];
     }
-    // T(acc): rotation slots through (R Jr)^T
-    double y[9];
+    finish_camera<T, MODE>(nc, c, s, nullptr, ct, dc, gc, vin, delta_c, rhs, lambda, out, out_d, parts, ls);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// The attempt check's camera side (mode cCheck) split like k_schur_cameras: one wave per
+// (camera, point range), four cameras of one range per workgroup, so the XCD running the
+// workgroup gathers the solves t_p and steps delta_p of one range of points only. Per row
+// u = Jc z (forward table of z), v1 = u - Jp t_p and v2 = u + Jp delta_p + r; the wave writes its
+// share of the column sums of the factors of Jc^T v1 and Jc^T v2 (18 values) to
+// shares[18 sh ..], sh = kR c + g, and k_check_finish adds a camera's shares in range order and
+// finishes the camera (finish_camera<cCheck>). Rows are pipelined per lane as in
+// k_schur_cameras (register sets A and B; a lane past its last row re-reads the wave's last row
+// with weight 0).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_check_cameras(
+    int nc, const int* __restrict__ cam_off_r, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
+    const Vec2<T>* __restrict__ res_cm, const double* __restrict__ tabw, const double* __restrict__ tp,
+    const double* __restrict__ delta_p, const double* __restrict__ fww, double* __restrict__ shares) {
+  constexpr int kR = point_ranges<T>();
+  constexpr int kWavesPerBlock = kBlock / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int g = (int)blockIdx.x % kR;
+  const int c = __builtin_amdgcn_readfirstlane(((int)blockIdx.x / kR) * kWavesPerBlock + (int)threadIdx.x / kWave);
+  if (c >= nc) return;  // whole waves; no workgroup barrier follows
+  const int sh = kR * c + g;
+  const int k0 = cam_off_r[sh], k1 = cam_off_r[sh + 1];
+  double ctl[16], fwl[kFwd];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) y[j] = ct[tRJ(0, j)] * s[0] + ct[tRJ(1, j)] * s[1] + ct[tRJ(2, j)] * s[2];
+  for (int a = 0; a < 16; ++a) ctl[a] = tabw[kCamTab * c + a];
 #pragma unroll
-    for (int j = 3; j < 9; ++j) y[j] = s[j];
-    double d[9];
+  for (int a = 0; a < kFwd; ++a) fwl[a] = fww[kFwd * c + a];
+  double acc[9], acc2[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) d[i] = double(dc[9 * c + i]);
-    if (MODE == cCheck) {
-      // verification of delta = (z / D, delta_p) in original coordinates
-      double s2[9], y2[9];
-#pragma unroll
-      for (int a = 0; a < 9; ++a) {
-        s2[a] = 0.0;
-        for (int w = 0; w < kB / kWave; ++w) s2[a] += red2[a][w];
-      }
-#pragma unroll
-      for (int j = 0; j < 3; ++j) y2[j] = ct[tRJ(0, j)] * s2[0] + ct[tRJ(1, j)] * s2[1] + ct[tRJ(2, j)] * s2[2];
-#pragma unroll
-      for (int j = 3; j < 9; ++j) y2[j] = s2[j];
-      double v0s = 0.0, v1s = 0.0, v2s = 0.0, v3s = 0.0, v4s = 0.0;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double dd = d[i];
-        const double dc_i = vin[9 * c + i] / dd;
-        out_d[9 * c + i] = dc_i;
-        const double nr = y2[i] + (lambda * (dd * dc_i)) * dd;
-        v0s += nr * nr;
-        v1s += (nr / dd) * (nr / dd);
-        const double g = double(gc[9 * c + i]);
-        v2s += g * dc_i;
-        v3s += g * g;
-        v4s += (g / dd) * (g / dd);
-      }
-      double* pv = parts + kParts * (nc + c);
-      pv[0] = v0s;
-      pv[1] = v1s;
-      pv[2] = v2s;
-      pv[3] = v3s;
-      pv[4] = v4s;
-      pv[5] = 0.0;
-    }
-    if (MODE == cResid || MODE == cCheck) {
-      const double* p = vin + 9 * c;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double q = lambda * p[i] + y[i] / d[i];
-        out[9 * c + i] = q;
-        const double e = q - rhs[9 * c + i];
-        part0 += e * e;
-        part1 += rhs[9 * c + i] * rhs[9 * c + i];
-      }
-    } else if (MODE == cRhs) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double r = (-double(gc[9 * c + i]) + y[i]) / d[i];
-        out[9 * c + i] = r;
-        part0 += r * r;
-      }
-    } else if (MODE == cVerify) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double dd = d[i];
-        const double nr = y[i] + (lambda * (dd * delta_c[9 * c + i])) * dd;
-        if (out_d) out_d[9 * c + i] = nr;  // the normal residual itself, on request
-        part0 += nr * nr;
-        part1 += (nr / dd) * (nr / dd);
-        const double g = double(gc[9 * c + i]);
-        part2 += g * delta_c[9 * c + i];
-        part3 += g * g;
-        part4 += (g / dd) * (g / dd);
-      }
-    } else if (MODE == cJtu) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) out_d[9 * c + i] = y[i];
-    } else if (MODE == cLsmr) {
-      const double coef = ls->alpha / ls->beta, root = ls->root;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        out_d[9 * c + i] = y[i];
-        if (vin != nullptr && out != nullptr) {  // damping rows of this camera
-          const double ub = root * d[i] * vin[9 * c + i] - coef * out[9 * c + i];
-          out[9 * c + i] = ub;
-          part0 += ub * ub;
-        }
-      }
-      parts[c] = part0;
-    }
-    if (MODE != cJtu && MODE != cLsmr) {
-      double* pc = parts + kParts * c;
-      pc[0] = part0;
-      pc[1] = part1;
-      pc[2] = part2;
-      pc[3] = part3;
-      pc[4] = part4;
+  for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
+  // pipelined loads through pointer copies the compiler cannot trace to the restrict-qualified
+  // arguments, so the compiler barriers in the loop also order them
+  using V4 = T __attribute__((ext_vector_type(4)));
+  using V2 = T __attribute__((ext_vector_type(2)));
+  using D2 = double __attribute__((ext_vector_type(2)));
+  using GV4 = const __attribute__((address_space(1))) V4;
+  using GV2 = const __attribute__((address_space(1))) V2;
+  using GD = const __attribute__((address_space(1))) double;
+  using GInt = const __attribute__((address_space(1))) int;
+  GV4* tpv = (GV4*)(tp);
+  GV4* recv = (GV4*)(rec_cm);
+  GV2* resv = (GV2*)(res_cm);
+  GD* dpv = (GD*)(delta_p);
+  GInt* psv = (GInt*)(k_pslot);
+  __asm__ volatile("" : "+s"(tpv), "+s"(recv), "+s"(resv), "+s"(dpv), "+s"(psv));
+  struct Row {
+    V4 rec, t;
+    V2 res;
+    double d0, d1, d2;
+  };
+  const int n = (k1 - k0 + kWave - 1) / kWave;  // uniform over the wave
+  if (n > 0) {
+    const int last = k1 - 1;
+    int k = k0 + lane;
+    auto fetch = [&](int kk, int ps, Row& r) {
+      r.rec = recv[kk];
+      r.res = resv[kk];
+      r.t = tpv[ps];
+      r.d0 = dpv[3 * ps];
+      r.d1 = dpv[3 * ps + 1];
+      r.d2 = dpv[3 * ps + 2];
+    };
+    auto row = [&](const Row& r, bool valid) {
+      ObsGeom<double> geo;
+      obs_geom(Vec4<double>{double(r.rec.x), double(r.rec.y), double(r.rec.z), double(r.rec.w)}, ctl, geo);
+      if (!valid) geo.sw = 0.0;
+      const double xt[3] = {double(r.t.x), double(r.t.y), double(r.t.z)};
+      const double xd[3] = {r.d0, r.d1, r.d2};
+      double u0, u1, a0, a1, b0, b1;
+      jc_apply(geo, fwl, u0, u1);
+      jp_apply(geo, ctl, xt, a0, a1);
+      jp_apply(geo, ctl, xd, b0, b1);
+      jct_factors(geo, u0 - a0, u1 - a1, acc);
+      jct_factors(geo, u0 + b0 + double(r.res.x), u1 + b1 + double(r.res.y), acc2);
+    };
+    Row ra, rb;
+    fetch(min(k, last), psv[min(k, last)], ra);
+    int ps_b = psv[min(k + kWave, last)];
+    for (int m = 0;; m += 2) {
+      const int ps_a = psv[min(k + 2 * kWave, last)];
+      fetch(min(k + kWave, last), ps_b, rb);
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      row(ra, k < k1);
+      if (m + 1 >= n) break;
+      ps_b = psv[min(k + 3 * kWave, last)];
+      fetch(min(k + 2 * kWave, last), ps_a, ra);
+      __asm__ volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      row(rb, k + kWave < k1);
+      if (m + 2 >= n) break;
+      k += 2 * kWave;
     }
   }
+#pragma unroll
+  for (int a = 0; a < 9; ++a) {
+    acc[a] = wave_sum(acc[a]);
+    acc2[a] = wave_sum(acc2[a]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      shares[18 * sh + a] = acc[a];
+      shares[18 * sh + 9 + a] = acc2[a];
+    }
+  }
+}
+
+// One thread per camera (single-wave workgroups, spread over more CUs): the camera's shares of
+// k_check_cameras in range order, then the end of the camera-major check
+// (finish_camera<cCheck>).
+template <typename T>
+__global__ void __launch_bounds__(kWave) k_check_finish(int nc, const double* __restrict__ shares,
+                                                         const double* __restrict__ tabw, const T* __restrict__ dc,
+                                                         const T* __restrict__ gc, const double* __restrict__ vin,
+                                                         const double* __restrict__ rhs, double lambda,
+                                                         double* __restrict__ out, double* __restrict__ out_d,
+                                                         double* parts) {
+  constexpr int kR = point_ranges<T>();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  // all of the camera's shares are loaded (16-byte pieces) before the first sum
+  Vec2<double> v[9 * kR];
+  const Vec2<double>* sh = reinterpret_cast<const Vec2<double>*>(shares + 18 * kR * c);
+#pragma unroll
+  for (int i = 0; i < 9 * kR; ++i) v[i] = sh[i];
+  double s[9], s2[9];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) s[a] = s2[a] = 0.0;
+#pragma unroll
+  for (int g = 0; g < kR; ++g) {
+    double e[18];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      e[2 * i] = v[9 * g + i].x;
+      e[2 * i + 1] = v[9 * g + i].y;
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      s[a] += e[a];
+      s2[a] += e[9 + a];
+    }
+  }
+  finish_camera<T, cCheck>(nc, c, s, s2, tabw + kCamTab * c, dc, gc, vin, nullptr, rhs, lambda, out, out_d, parts,
+                           nullptr);
 }
 
 // ---------------------------------------------------------------------------------------
