@@ -136,6 +136,7 @@ struct fg_schur {
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* qy = nullptr;  // the PCG product's shares of q, 9 per (camera, point range)
   double* cshares = nullptr;  // the check's camera shares, 18 per (camera, point range)
+  double* lshares = nullptr;  // the linearization's camera shares, 54 per (camera, point range)
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
   void* prow = nullptr;    // T [nc][kPRow]: table row + forward table (point-major reads)
@@ -360,10 +361,23 @@ fg_status linearize_launch(fg_schur* s) {
                      b->huber, (Vec4<T>*)s->rec,
                      (Vec2<T>*)s->res, (double*)s->hpp, (T*)s->gp, s->d_flags + 1);
   FG_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
-                     b->d_k_pslot, (const double*)b->d_pts[buf], (const Vec2<double>*)b->d_cam_pix64,
-                     (const double*)b->d_tab64[buf], (const T*)b->d_tab[buf], b->huber, (Vec4<T>*)s->rec_cm,
-                     (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags + 1);
+  if constexpr (sizeof(T) == 8) {
+    hipLaunchKernelGGL(k_linearize_cameras<T>, dim3((unsigned)b->nc), dim3(kBlock), 0, b->stream, b->d_cam_off,
+                       b->d_k_pslot, (const double*)b->d_pts[buf], (const Vec2<double>*)b->d_cam_pix64,
+                       (const double*)b->d_tab64[buf], (const T*)b->d_tab[buf], b->huber, (Vec4<T>*)s->rec_cm,
+                       (Vec2<T>*)s->res_cm, (T*)s->hcc, (T*)s->gc, s->d_flags + 1);
+  } else {
+    // FP32: the camera-major pass as one wave per (camera, point range) into shares, then each
+    // camera's shares in range order
+    constexpr int kR = point_ranges<T>(), kWavesPerBlock = kBlock / kWave;
+    const unsigned grid = (unsigned)((b->nc + kWavesPerBlock - 1) / kWavesPerBlock) * kR;
+    hipLaunchKernelGGL(k_linearize_cameras_split<T>, dim3(grid), dim3(kBlock), 0, b->stream, (int)b->nc,
+                       (const int*)b->d_cam_off_r, b->d_k_pslot, (const double*)b->d_pts[buf],
+                       (const Vec2<double>*)b->d_cam_pix64, (const double*)b->d_tab64[buf], (const T*)b->d_tab[buf],
+                       b->huber, (Vec4<T>*)s->rec_cm, (Vec2<T>*)s->res_cm, s->lshares, s->d_flags + 1);
+    hipLaunchKernelGGL(k_linearize_finish<T>, dim3(grid_for(54 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                       (const double*)s->lshares, (T*)s->hcc, (T*)s->gc);
+  }
   FG_TRY(hipGetLastError());
   return FG_OK;
 }
@@ -873,6 +887,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((T**)&s->prow, kPRow * nc));
   FG_TRY(dalloc(&s->qy, 9 * (size_t)point_ranges<T>() * nc));
   FG_TRY(dalloc(&s->cshares, 18 * (size_t)point_ranges<T>() * nc));
+  FG_TRY(dalloc(&s->lshares, 54 * (size_t)point_ranges<T>() * nc));
   for (void** v : {&s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs, &s->zero}) {
     FG_TRY(dalloc((double**)v, 9 * nc));
     FG_TRY(hipMemset(*v, 0, 9 * nc * sizeof(double)));
@@ -914,6 +929,7 @@ void free_model(fg_schur* s) {
   dfree(s->fw_w);
   dfree(s->qy);
   dfree(s->cshares);
+  dfree(s->lshares);
   if (s->prow) (void)hipFree(s->prow);
   s->prow = nullptr;
   dfree(s->d_delta);

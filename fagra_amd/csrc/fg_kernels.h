@@ -521,6 +521,142 @@ __global__ void __launch_bounds__(kBlock) k_linearize_points(
 }
 
 // ---------------------------------------------------------------------------------------
+// Linearization, camera-major pass split like k_schur_cameras: one wave per (camera, point
+// range), four cameras of one range per workgroup, so the XCD running the workgroup gathers the
+// f64 points of one range only. Per row the record and weighted residual are formed from the
+// point, the f64 camera table and the pixel with record_of (as k_linearize_cameras does) and
+// written to the camera-major copies; the wave's share of the 45 Gram entries (accumulated in
+// T) and the 9 gradient entries (f64) goes to shares[54 sh ..], sh = kR c + g, and
+// k_linearize_finish adds a camera's shares in range order. Rows are pipelined per lane as in
+// k_schur_cameras; a lane past its last row re-forms the wave's last row (the same record, so
+// its stores write the same bits) with weight 0 in the sums. FP32 storage only: in f64 the 45
+// Gram sums take 90 registers, and the split pass ran at one or two waves per SIMD (171-236
+// against 132 us for k_linearize_cameras), which FP64 keeps.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_linearize_cameras_split(
+    int nc, const int* __restrict__ cam_off_r, const int* __restrict__ k_pslot, const double* __restrict__ pts,
+    const Vec2<double>* __restrict__ cam_pix, const double* __restrict__ tab64, const T* __restrict__ tab,
+    double huber, Vec4<T>* __restrict__ rec_cm, Vec2<T>* __restrict__ res_cm, double* __restrict__ shares,
+    int* flags) {
+  constexpr int kR = point_ranges<T>();
+  constexpr int kWavesPerBlock = kBlock / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int g = (int)blockIdx.x % kR;
+  const int c = __builtin_amdgcn_readfirstlane(((int)blockIdx.x / kR) * kWavesPerBlock + (int)threadIdx.x / kWave);
+  if (c >= nc) return;  // whole waves; no workgroup barrier follows
+  const int sh = kR * c + g;
+  const int k0 = cam_off_r[sh], k1 = cam_off_r[sh + 1];
+  double cd[16];
+  T ct[kCamTab];
+#pragma unroll
+  for (int a = 0; a < 16; ++a) cd[a] = tab64[kCamTab * c + a];
+#pragma unroll
+  for (int a = 0; a < kCamTab; ++a) ct[a] = tab[kCamTab * c + a];
+  T acc[45];
+  double gacc[9];
+#pragma unroll
+  for (int i = 0; i < 45; ++i) acc[i] = T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gacc[i] = 0.0;
+  bool bad = false;
+  using D2 = double __attribute__((ext_vector_type(2)));
+  using GD2 = const __attribute__((address_space(1))) D2;
+  using GD = const __attribute__((address_space(1))) double;
+  using GInt = const __attribute__((address_space(1))) int;
+  GD* ptv = (GD*)(pts);
+  GD2* pixv = (GD2*)(cam_pix);
+  GInt* psv = (GInt*)(k_pslot);
+  __asm__ volatile("" : "+s"(ptv), "+s"(pixv), "+s"(psv));
+  struct Row {
+    double x0, x1, x2;
+    D2 ob;
+  };
+  const int n = (k1 - k0 + kWave - 1) / kWave;  // uniform over the wave
+  if (n > 0) {
+    const int last = k1 - 1;
+    int k = k0 + lane;
+    auto fetch = [&](int kk, int ps, Row& r) {
+      r.x0 = ptv[3 * ps];
+      r.x1 = ptv[3 * ps + 1];
+      r.x2 = ptv[3 * ps + 2];
+      r.ob = pixv[kk];
+    };
+    auto row = [&](int kk, const Row& r, bool valid) {
+      Vec4<T> r4;
+      Vec2<T> rw;
+      double qx, qy, z, sw, wrx, wry;
+      bad |= !record_of<T>(cd, r.x0, r.x1, r.x2, Vec2<double>{r.ob.x, r.ob.y}, huber, r4, rw, qx, qy, z, sw, wrx,
+                           wry);
+      rec_cm[kk] = r4;
+      res_cm[kk] = rw;
+      ObsGeom<T> geo;
+      obs_geom(r4, ct, geo);
+      T jc[18];
+      jc_block(geo, ct, jc);
+#pragma unroll
+      for (int a = 0; a < 18; ++a) bad |= !isfinite(jc[a]);
+      const T wt = valid ? T(1) : T(0);
+#pragma unroll
+      for (int a = 0; a < 18; ++a) jc[a] *= wt;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) {
+#pragma unroll
+        for (int b = a; b < 9; ++b) acc[sym9(b, a)] += jc[a] * jc[b] + jc[9 + a] * jc[9 + b];
+      }
+#pragma unroll
+      for (int a = 0; a < 9; ++a) gacc[a] += double(jc[a]) * double(rw.x) + double(jc[9 + a]) * double(rw.y);
+    };
+    {
+      Row ra, rb;
+      fetch(min(k, last), psv[min(k, last)], ra);
+      int ps_b = psv[min(k + kWave, last)];
+      for (int m = 0;; m += 2) {
+        const int ps_a = psv[min(k + 2 * kWave, last)];
+        fetch(min(k + kWave, last), ps_b, rb);
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        row(min(k, last), ra, k < k1);
+        if (m + 1 >= n) break;
+        ps_b = psv[min(k + 3 * kWave, last)];
+        fetch(min(k + 2 * kWave, last), ps_a, ra);
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        row(min(k + kWave, last), rb, k + kWave < k1);
+        if (m + 2 >= n) break;
+        k += 2 * kWave;
+      }
+    }
+  }
+  if (__builtin_amdgcn_ballot_w64(bad) != 0 && lane == 0) atomicOr(flags, 1);
+#pragma unroll
+  for (int i = 0; i < 45; ++i) {
+    const T v = wave_sum(acc[i]);
+    if (lane == 0) shares[54 * sh + i] = double(v);
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const double v = wave_sum(gacc[i]);
+    if (lane == 0) shares[54 * sh + 45 + i] = v;
+  }
+}
+
+// One thread per (camera, entry): the camera's shares of k_linearize_cameras_split in range
+// order into Hcc (45 packed) and gc (9).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_linearize_finish(int nc, const double* __restrict__ shares,
+                                                             T* __restrict__ hcc, T* __restrict__ gc) {
+  constexpr int kR = point_ranges<T>();
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = id / 54, e = id % 54;
+  if (c >= nc) return;
+  double s = 0.0;
+#pragma unroll
+  for (int g = 0; g < kR; ++g) s += shares[54 * (kR * c + g) + e];
+  if (e < 45) hcc[45 * c + e] = T(s);
+  else gc[9 * c + e - 45] = T(s);
+}
+
+// ---------------------------------------------------------------------------------------
 // Linearization, camera-major pass (one workgroup per camera): the camera-major copies of the
 // records, formed again from the f64 points (gathered by point slot from a 12 MB array in the
 // 1,500-camera case, instead of gathering the 78 MB of slot-order records and residuals) with
