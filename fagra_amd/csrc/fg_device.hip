@@ -135,7 +135,7 @@ struct fg_schur {
   void* lpp = nullptr;  // f64 [np][6]: factor of Hpp + lambda Dp^2 of the current attempt
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* qy = nullptr;  // the PCG product's shares of q, 9 per (camera, point range)
-  double* cshares = nullptr;  // the check's camera shares, 18 per (camera, point range)
+  double* cshares = nullptr;  // shares of the split camera passes (k_camera_split), up to 18 per (camera, point range)
   double* lshares = nullptr;  // the linearization's camera shares, 54 per (camera, point range)
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
@@ -328,7 +328,18 @@ void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, c
   // the Schur, residual and verification modes read the forward table of their input, which
   // is always s->fw (fw names it at the call), widened to f64
   (void)fw;
-  if constexpr (MODE == cSchur) {
+  if constexpr (MODE == cCheck) {
+    // one wave per (camera, point range) into shares, then each camera's shares in range order
+    constexpr int kR = point_ranges<T>(), kWavesPerBlock = kBlock / kWave;
+    const unsigned grid = (unsigned)((b->nc + kWavesPerBlock - 1) / kWavesPerBlock) * kR;
+    hipLaunchKernelGGL((k_camera_split<T, MODE>), dim3(grid), dim3(kBlock), 0, b->stream, (int)b->nc,
+                       (const int*)b->d_cam_off_r, (const int*)b->d_k_pslot, (const Vec4<T>*)s->rec_cm,
+                       (const Vec2<T>*)s->res_cm, (const double*)b->d_tabw[b->cur], (const double*)s->tp, delta_p,
+                       fw_wide(s), s->cshares);
+    hipLaunchKernelGGL((k_camera_finish<T, MODE>), dim3(grid_for(b->nc, kWave)), dim3(kWave), 0, b->stream,
+                       (int)b->nc, (const double*)s->cshares, (const double*)b->d_tabw[b->cur], (const T*)s->dc,
+                       (const T*)s->gc, vin, rhs, lambda, out, out_d, s->d_parts);
+  } else if constexpr (MODE == cSchur) {
     // one wave per (camera, point range), four cameras of one range per workgroup; the shares
     // of q go to out (9 point_ranges<T>() nc)
     constexpr int kR = point_ranges<T>(), kWavesPerBlock = kBlock / kWave;
@@ -791,20 +802,8 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   const int pblocks = point_blocks(b);
   launch_points<T, kCheck>(s, (const T*)s->fw, nullptr, nullptr, lambda, s->d_delta + 9 * b->nc, nullptr,
                            s->d_parts + (size_t)kParts * 2 * nc, nullptr);
-  {
-    // the check's camera side: one wave per (camera, point range) into shares, then each
-    // camera's shares in range order (k_cameras<cCheck> in two launches)
-    constexpr int kR = point_ranges<T>(), kWavesPerBlock = kBlock / kWave;
-    const unsigned grid = (unsigned)((b->nc + kWavesPerBlock - 1) / kWavesPerBlock) * kR;
-    hipLaunchKernelGGL(k_check_cameras<T>, dim3(grid), dim3(kBlock), 0, b->stream, nc,
-                       (const int*)b->d_cam_off_r, (const int*)b->d_k_pslot, (const Vec4<T>*)s->rec_cm,
-                       (const Vec2<T>*)s->res_cm, (const double*)b->d_tabw[b->cur], (const double*)s->tp,
-                       (const double*)(s->d_delta + 9 * b->nc), fw_wide(s), s->cshares);
-    hipLaunchKernelGGL(k_check_finish<T>, dim3(grid_for(nc, kWave)), dim3(kWave), 0, b->stream, nc,
-                       (const double*)s->cshares, (const double*)b->d_tabw[b->cur], (const T*)s->dc,
-                       (const T*)s->gc, (const double*)s->x, (const double*)s->rhs, lambda, (double*)s->q,
-                       s->d_delta, s->d_parts);
-  }
+  launch_cameras<T, cCheck>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, (const double*)s->x, nullptr,
+                            (const double*)s->rhs, lambda, (double*)s->q, s->d_delta);
   const SumJobs jobs{{nc, pblocks, nc, 0}, {0, 2 * nc, nc, 0}, {0, 1, 2, 0}};
   hipLaunchKernelGGL(k_sum_parts_jobs, dim3(3), dim3(kBlock), 0, b->stream, (const double*)s->d_parts, jobs,
                      s->d_sums);

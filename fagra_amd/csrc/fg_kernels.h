@@ -1790,8 +1790,9 @@ k_schur_cameras(int nc, const int* __restrict__ cam_off_r, const int* __restrict
 //   kCheck:   cResid and cVerify of one attempt in one pass: per row u = Jc z (forward table
 //             of z), v1 = u - Jp t_p for q = S z, v2 = u + Jp delta_p + r for the normal
 //             residual, delta_c = z / D written; residual partials in parts[c], verification
-//             partials in parts[nc + c] (k_check_cameras and k_check_finish; finish_camera
-//             holds the camera's end of every mode)
+//             partials in parts[nc + c]
+// (cCheck runs as k_camera_split and k_camera_finish; finish_camera holds the camera's end of
+// every mode)
 enum CameraMode { cSchur = 0, cResid = 1, cRhs = 2, cVerify = 3, cJtu = 4, cLsmr = 5, cCheck = 6 };
 
 // Waves per SIMD the camera-major kernel is compiled for: the f32 residual mode fits six (80
@@ -1914,7 +1915,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
     const double* __restrict__ delta_c, const double* __restrict__ rhs, double lambda, double* __restrict__ out,
     double* __restrict__ out_d, double* parts, const CgState* st, const LsmrState* ls,
     const int* __restrict__ cm_ref) {
-  static_assert(MODE != cSchur && MODE != cCheck, "cSchur and cCheck are k_schur_cameras and k_check_cameras");
+  static_assert(MODE != cSchur && MODE != cCheck, "cSchur is k_schur_cameras, cCheck k_camera_split");
   constexpr int kB = kBlock;
   __shared__ double red[9][kB / kWave];
   (void)st;
@@ -1999,21 +2000,29 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(cam
 }
 
 // ---------------------------------------------------------------------------------------
-// The attempt check's camera side (mode cCheck) split like k_schur_cameras: one wave per
+// The camera side of the attempt check (mode cCheck) split like k_schur_cameras: one wave per
 // (camera, point range), four cameras of one range per workgroup, so the XCD running the
-// workgroup gathers the solves t_p and steps delta_p of one range of points only. Per row
-// u = Jc z (forward table of z), v1 = u - Jp t_p and v2 = u + Jp delta_p + r; the wave writes its
-// share of the column sums of the factors of Jc^T v1 and Jc^T v2 (18 values) to
-// shares[18 sh ..], sh = kR c + g, and k_check_finish adds a camera's shares in range order and
-// finishes the camera (finish_camera<cCheck>). Rows are pipelined per lane as in
-// k_schur_cameras (register sets A and B; a lane past its last row re-reads the wave's last row
-// with weight 0).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_check_cameras(
+// workgroup gathers the per-point vectors of one range of points only. Per row u = Jc z
+// (forward table of z), v1 = u - Jp t_p and v2 = u + Jp delta_p + r. The wave writes its share
+// of the column sums of the factors of Jc^T v1 and Jc^T v2 to shares[W sh ..], W = share_width,
+// sh = kR c + g, and k_camera_finish adds a camera's shares in range order and finishes the
+// camera (finish_camera). Rows are pipelined per lane as in k_schur_cameras (register sets A
+// and B; a lane past its last row re-reads the wave's last row with weight 0). The same split
+// of the reduced right-hand side (cRhs, v = Jp t_p) measured no faster in FP32 (28.6 + 5.0
+// against 32.8 us) and changed the rounding of the right-hand side enough to move a borderline
+// FP32 solve at lambda = 1e-10 from convergence to the iteration limit, so cRhs stays in
+// k_cameras.
+template <int MODE>
+constexpr int share_width() { return MODE == cCheck ? 18 : 10; }
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) k_camera_split(
     int nc, const int* __restrict__ cam_off_r, const int* __restrict__ k_pslot, const Vec4<T>* __restrict__ rec_cm,
     const Vec2<T>* __restrict__ res_cm, const double* __restrict__ tabw, const double* __restrict__ tp,
     const double* __restrict__ delta_p, const double* __restrict__ fww, double* __restrict__ shares) {
+  static_assert(MODE == cCheck, "split camera pass: cCheck");
+  constexpr bool kCheck = MODE == cCheck;
   constexpr int kR = point_ranges<T>();
+  constexpr int kW = share_width<MODE>();
   constexpr int kWavesPerBlock = kBlock / kWave;
   const int lane = threadIdx.x % kWave;
   const int g = (int)blockIdx.x % kR;
@@ -2025,7 +2034,7 @@ __global__ void __launch_bounds__(kBlock) k_check_cameras(
 #pragma unroll
   for (int a = 0; a < 16; ++a) ctl[a] = tabw[kCamTab * c + a];
 #pragma unroll
-  for (int a = 0; a < kFwd; ++a) fwl[a] = fww[kFwd * c + a];
+  for (int a = 0; a < kFwd; ++a) fwl[a] = kCheck ? fww[kFwd * c + a] : 0.0;
   double acc[9], acc2[9];
 #pragma unroll
   for (int a = 0; a < 9; ++a) acc[a] = acc2[a] = 0.0;
@@ -2033,7 +2042,6 @@ __global__ void __launch_bounds__(kBlock) k_check_cameras(
   // arguments, so the compiler barriers in the loop also order them
   using V4 = T __attribute__((ext_vector_type(4)));
   using V2 = T __attribute__((ext_vector_type(2)));
-  using D2 = double __attribute__((ext_vector_type(2)));
   using GV4 = const __attribute__((address_space(1))) V4;
   using GV2 = const __attribute__((address_space(1))) V2;
   using GD = const __attribute__((address_space(1))) double;
@@ -2055,24 +2063,31 @@ __global__ void __launch_bounds__(kBlock) k_check_cameras(
     int k = k0 + lane;
     auto fetch = [&](int kk, int ps, Row& r) {
       r.rec = recv[kk];
-      r.res = resv[kk];
       r.t = tpv[ps];
-      r.d0 = dpv[3 * ps];
-      r.d1 = dpv[3 * ps + 1];
-      r.d2 = dpv[3 * ps + 2];
+      if constexpr (kCheck) {
+        r.res = resv[kk];
+        r.d0 = dpv[3 * ps];
+        r.d1 = dpv[3 * ps + 1];
+        r.d2 = dpv[3 * ps + 2];
+      }
     };
     auto row = [&](const Row& r, bool valid) {
       ObsGeom<double> geo;
       obs_geom(Vec4<double>{double(r.rec.x), double(r.rec.y), double(r.rec.z), double(r.rec.w)}, ctl, geo);
       if (!valid) geo.sw = 0.0;
       const double xt[3] = {double(r.t.x), double(r.t.y), double(r.t.z)};
-      const double xd[3] = {r.d0, r.d1, r.d2};
-      double u0, u1, a0, a1, b0, b1;
-      jc_apply(geo, fwl, u0, u1);
+      double a0, a1;
       jp_apply(geo, ctl, xt, a0, a1);
-      jp_apply(geo, ctl, xd, b0, b1);
-      jct_factors(geo, u0 - a0, u1 - a1, acc);
-      jct_factors(geo, u0 + b0 + double(r.res.x), u1 + b1 + double(r.res.y), acc2);
+      if constexpr (kCheck) {
+        const double xd[3] = {r.d0, r.d1, r.d2};
+        double u0, u1, b0, b1;
+        jc_apply(geo, fwl, u0, u1);
+        jp_apply(geo, ctl, xd, b0, b1);
+        jct_factors(geo, u0 - a0, u1 - a1, acc);
+        jct_factors(geo, u0 + b0 + double(r.res.x), u1 + b1 + double(r.res.y), acc2);
+      } else {
+        jct_factors(geo, a0, a1, acc);
+      }
     };
     Row ra, rb;
     fetch(min(k, last), psv[min(k, last)], ra);
@@ -2096,54 +2111,54 @@ __global__ void __launch_bounds__(kBlock) k_check_cameras(
 #pragma unroll
   for (int a = 0; a < 9; ++a) {
     acc[a] = wave_sum(acc[a]);
-    acc2[a] = wave_sum(acc2[a]);
+    if (kCheck) acc2[a] = wave_sum(acc2[a]);
   }
   if (lane == 0) {
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
-      shares[18 * sh + a] = acc[a];
-      shares[18 * sh + 9 + a] = acc2[a];
+      shares[kW * sh + a] = acc[a];
+      if (kCheck) shares[kW * sh + 9 + a] = acc2[a];
     }
   }
 }
 
 // One thread per camera (single-wave workgroups, spread over more CUs): the camera's shares of
-// k_check_cameras in range order, then the end of the camera-major check
-// (finish_camera<cCheck>).
-template <typename T>
-__global__ void __launch_bounds__(kWave) k_check_finish(int nc, const double* __restrict__ shares,
+// k_camera_split in range order, then the end of the camera-major pass (finish_camera).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kWave) k_camera_finish(int nc, const double* __restrict__ shares,
                                                          const double* __restrict__ tabw, const T* __restrict__ dc,
                                                          const T* __restrict__ gc, const double* __restrict__ vin,
                                                          const double* __restrict__ rhs, double lambda,
                                                          double* __restrict__ out, double* __restrict__ out_d,
                                                          double* parts) {
   constexpr int kR = point_ranges<T>();
+  constexpr int kW = share_width<MODE>();
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   // all of the camera's shares are loaded (16-byte pieces) before the first sum
-  Vec2<double> v[9 * kR];
-  const Vec2<double>* sh = reinterpret_cast<const Vec2<double>*>(shares + 18 * kR * c);
+  Vec2<double> v[kW / 2 * kR];
+  const Vec2<double>* sh = reinterpret_cast<const Vec2<double>*>(shares + kW * kR * c);
 #pragma unroll
-  for (int i = 0; i < 9 * kR; ++i) v[i] = sh[i];
+  for (int i = 0; i < kW / 2 * kR; ++i) v[i] = sh[i];
   double s[9], s2[9];
 #pragma unroll
   for (int a = 0; a < 9; ++a) s[a] = s2[a] = 0.0;
 #pragma unroll
   for (int g = 0; g < kR; ++g) {
-    double e[18];
+    double e[kW];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-      e[2 * i] = v[9 * g + i].x;
-      e[2 * i + 1] = v[9 * g + i].y;
+    for (int i = 0; i < kW / 2; ++i) {
+      e[2 * i] = v[kW / 2 * g + i].x;
+      e[2 * i + 1] = v[kW / 2 * g + i].y;
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) {
       s[a] += e[a];
-      s2[a] += e[9 + a];
+      if (MODE == cCheck) s2[a] += e[9 + a];
     }
   }
-  finish_camera<T, cCheck>(nc, c, s, s2, tabw + kCamTab * c, dc, gc, vin, nullptr, rhs, lambda, out, out_d, parts,
-                           nullptr);
+  finish_camera<T, MODE>(nc, c, s, s2, tabw + kCamTab * c, dc, gc, vin, nullptr, rhs, lambda, out, out_d, parts,
+                         nullptr);
 }
 
 // ---------------------------------------------------------------------------------------
