@@ -42,10 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # measured on (null elsewhere).
 TRAFFIC_SOURCE = "profiles/hbm_traffic_rings1500.md"
 MEASURED_TRAFFIC = {
-    ("rings1500", "k_points<float, kSchur>"): (2 * 44329.0 + 7812.5) * 1024,
-    ("rings1500", "k_schur_cameras<float>"): (2 * 52463.7 + 1828.8) * 1024,
-    ("rings1500", "k_points<double, kSchur>"): (2 * 70317.0 + 15625.0) * 1024,
-    ("rings1500", "k_schur_cameras<double>"): (2 * 78369.9 + 1516.7) * 1024,
+    ("rings1500", "k_points<float, kSchur>"): (2 * 44331.2 + 7812.5) * 1024,
+    ("rings1500", "k_schur_cameras<float>"): (2 * 52957.9 + 1829.1) * 1024,
+    ("rings1500", "k_points<double, kSchur>"): (2 * 70308.6 + 15625.0) * 1024,
+    ("rings1500", "k_schur_cameras<double>"): (2 * 80921.6 + 1514.7) * 1024,
 }
 
 

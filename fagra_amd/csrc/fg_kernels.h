@@ -8,7 +8,7 @@
 //     Padding entries have camera -1.
 //   * Camera-major kernels walk camera c's rows k in [off[c], off[c+1]) and keep a
 //     camera-major copy of the records. Within a camera the rows are grouped by the point-slot
-//     range of their point (point_ranges<T>() ranges of equal slot count; the reference's row
+//     range of their point (point_ranges<T>() ranges of equal observation count; the reference's row
 //     order within a group), so the Schur product's camera pass can give each range to its own
 //     wave (k_schur_cameras); cm_ref maps a camera-major row to the reference row.
 //   * Camera tables: kCamTab scalars per camera (see fg_math.h).
@@ -23,11 +23,12 @@ constexpr int kChunksPerBlock = kBlock / kWave;
 // Point-slot ranges of the Schur product's camera pass (k_schur_cameras). Workgroup b takes
 // range b % R, and workgroup index mod 8 is the XCD that runs it; each of the 8 XCDs has its own
 // L2, so with R ranges an XCD gathers the per-point solves of 1/R of the points instead of all
-// of them. FP64 (32-byte solves, 16 MB at 500,000 points): 8 ranges, 2 MB per XCD (camera pass
-// 50.6 -> 48.2 us on the 1,500-camera workload; unsplit, every XCD touches every line of the
-// solves). FP32 (16-byte solves): 2 ranges, measured fastest of 1, 2, 4 and 8 (33.5, 30.2, 32.4
-// and 35.0 us): the finer splits launch more, shorter waves whose fixed costs (scalar loads,
-// pipeline fill, the 9-sum wave reduction) outweigh the L2 hits they gain.
+// of them. FP64 (32-byte solves, 16 MB at 500,000 points): 8 ranges, 2 MB per XCD (unsplit,
+// every XCD touches every line of the solves). FP32 (16-byte solves): 2 ranges; on the
+// 1,500-camera workload 2, 4 and 8 ranges fetch 110, 100 and 95 MB per launch in 28.7, 29.6
+// and 31.9 us: the pass waits on L1 misses in flight, which finer splits do not reduce, while
+// they launch more, shorter waves with the same fixed costs (scalar loads, pipeline fill, the
+// 9-sum wave reduction).
 template <typename T>
 constexpr int point_ranges() { return sizeof(T) == 4 ? 2 : 8; }
 
