@@ -136,6 +136,7 @@ struct fg_schur {
   void *tp = nullptr, *fw = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double* qy = nullptr;  // the PCG product's shares of q, 9 per (camera, point range)
   double* cshares = nullptr;  // shares of the split camera passes (k_camera_split), up to 18 per (camera, point range)
+  float* tpair = nullptr;     // f32 check: t and delta_p of each point side by side (8 floats)
   double* lshares = nullptr;  // the linearization's camera shares, 54 per (camera, point range)
   void *rhs = nullptr, *zero = nullptr;
   double* fw_w = nullptr;  // f32 storage: the forward table widened to f64 (camera-major reads)
@@ -313,11 +314,14 @@ void launch_points(fg_schur* s, const T* fw, const double* delta_p_in, const dou
   fg_ba* b = s->ba;
   const int blocks = point_blocks(b);
   // the point-major kernels read the product rows (s->prow) where a forward table is named
+  // the f32 check writes t and delta_p side by side into tpair (its camera pass gathers both)
+  double* t_out = MODE == kCheck && sizeof(T) == 4 ? (double*)s->tpair : (double*)s->tp;
   hipLaunchKernelGGL((k_points<T, MODE>), dim3(blocks), dim3(kBlock), 0, b->stream, point_args<T>(b),
                      (const Vec4<T>*)s->rec, (const Vec2<T>*)s->res, (const T*)b->d_tab[b->cur],
                      fw != nullptr ? (const T*)s->prow : nullptr,
                      (const double*)s->mp, (const T*)s->gp, (const T*)s->dp, delta_p_in, rows_in, lambda,
-                     (double*)s->tp, delta_out, rows_out, parts, st, nullptr, (const double*)s->lpp);
+                     t_out, delta_out, rows_out, parts, st, nullptr, (const double*)s->lpp,
+                     (const double*)s->hpp);
 }
 
 template <typename T, int MODE>
@@ -334,8 +338,9 @@ void launch_cameras(fg_schur* s, const double* delta_p, const double* rows_in, c
     const unsigned grid = (unsigned)((b->nc + kWavesPerBlock - 1) / kWavesPerBlock) * kR;
     hipLaunchKernelGGL((k_camera_split<T, MODE>), dim3(grid), dim3(kBlock), 0, b->stream, (int)b->nc,
                        (const int*)b->d_cam_off_r, (const int*)b->d_k_pslot, (const Vec4<T>*)s->rec_cm,
-                       (const Vec2<T>*)s->res_cm, (const double*)b->d_tabw[b->cur], (const double*)s->tp, delta_p,
-                       fw_wide(s), s->cshares);
+                       (const Vec2<T>*)s->res_cm, (const double*)b->d_tabw[b->cur],
+                       sizeof(T) == 4 ? (const double*)s->tpair : (const double*)s->tp, delta_p, fw_wide(s),
+                       s->cshares);
     hipLaunchKernelGGL((k_camera_finish<T, MODE>), dim3(grid_for(b->nc, kWave)), dim3(kWave), 0, b->stream,
                        (int)b->nc, (const double*)s->cshares, (const double*)b->d_tabw[b->cur], (const T*)s->dc,
                        (const T*)s->gc, vin, rhs, lambda, out, out_d, s->d_parts);
@@ -883,6 +888,7 @@ fg_status alloc_model(fg_schur* s) {
   FG_TRY(dalloc((double**)&s->tp, 4 * np));
   FG_TRY(dalloc((T**)&s->fw, kFwd * nc));
   if (sizeof(T) == 4) FG_TRY(dalloc(&s->fw_w, kFwd * nc));
+  if (sizeof(T) == 4) FG_TRY(dalloc(&s->tpair, 8 * np));
   FG_TRY(dalloc((T**)&s->prow, kPRow * nc));
   FG_TRY(dalloc(&s->qy, 9 * (size_t)point_ranges<T>() * nc));
   FG_TRY(dalloc(&s->cshares, 18 * (size_t)point_ranges<T>() * nc));
@@ -928,6 +934,7 @@ void free_model(fg_schur* s) {
   dfree(s->fw_w);
   dfree(s->qy);
   dfree(s->cshares);
+  dfree(s->tpair);
   dfree(s->lshares);
   if (s->prow) (void)hipFree(s->prow);
   s->prow = nullptr;
@@ -1968,7 +1975,7 @@ void lsmr_passes(fg_lsmr* l, const double* ws, bool damped) {
                      step ? (const T*)s->prow : nullptr, (const double*)s->mp, (const T*)s->gp, (const T*)s->dp,
                      step ? ws + 9 * nc : nullptr, (const double*)l->d_u, 0.0,
                      damped && step ? l->d_ub + 9 * nc : nullptr, l->d_zt + 9 * nc, l->d_u, l->d_up,
-                     (const CgState*)nullptr, st, (const double*)nullptr);
+                     (const CgState*)nullptr, st, (const double*)nullptr, (const double*)nullptr);
   // the camera pass gathers U by observation slot (k_slot in the gather-index argument)
   hipLaunchKernelGGL((k_cameras<T, cLsmr>), dim3((unsigned)nc), dim3(kBlock), 0, b->stream, nc, b->d_cam_off,
                      b->d_k_slot, (const Vec4<T>*)s->rec_cm, (const Vec2<T>*)s->res_cm,

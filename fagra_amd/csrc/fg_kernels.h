@@ -1293,6 +1293,110 @@ __device__ __forceinline__ void schur_obs_lds(const Vec4<float>* __restrict__ ro
   w[2] = __builtin_fma(double(p0.z), e0, __builtin_fma(double(p1.z), e1, __builtin_fma(double(p2.z), e2, w[2])));
 }
 
+// The attempt check's sums for one observation of the f32 pass (kCheck), its camera's product
+// row (the forward table of the solution z) read from LDS as in schur_obs_lds: with u = Jc z =
+// sw v' and the weighted residual r of the row,
+//   w += Jp^T u = R^T e,  e = sw^2 iz P^T A v';  uu += ||u||^2;  gr += Jp^T r = R^T e',
+//   e' = sw iz P^T A r;
+// f64 arithmetic.
+__device__ __forceinline__ void check_obs_lds(const Vec4<float>* __restrict__ row, const Vec4<float>& rec,
+                                              const Vec2<float>& rr, double w[3], double& uu, double gr[3]) {
+  const double qx = rec.x, qy = rec.y, z = rec.z, sw = rec.w;
+  const Vec4<float> p3 = row[3];
+  const double f = p3.x, k1 = p3.y, k2 = p3.z;
+  const double r2 = __builtin_fma(qx, qx, qy * qy);
+  const double radial = __builtin_fma(r2, __builtin_fma(k2, r2, k1), 1.0);
+  const double fc2 = f * __builtin_fma(4.0 * k2, r2, 2.0 * k1);
+  const double frad = f * radial;
+  const double cx = fc2 * qx, cy = fc2 * qy;
+  const double a = __builtin_fma(cx, qx, frad), b = cx * qy, d = __builtin_fma(cy, qy, frad);
+  const double iz = neg_inv_depth(z);
+  const Vec4<float> p0 = row[0], p1 = row[1], p2 = row[2];
+  const double pt0 = __builtin_fma(-qx, z, -double(p0.w));
+  const double pt1 = __builtin_fma(-qy, z, -double(p1.w));
+  const double pt2 = z - double(p2.w);
+  const Vec4<float> p4 = row[4], p5 = row[5];
+  const double f0 = p3.w, f1 = p4.x, f2 = p4.y, f3 = p4.z, f4 = p4.w, f5 = p5.x, f6 = p5.y, f7 = p5.z,
+               f8 = p5.w;
+  const double x0 = __builtin_fma(f1, pt2, __builtin_fma(-f2, pt1, f3));
+  const double x1 = __builtin_fma(f2, pt0, __builtin_fma(-f0, pt2, f4));
+  const double x2 = __builtin_fma(f0, pt1, __builtin_fma(-f1, pt0, f5));
+  const double fr2 = f * r2, fr4 = fr2 * r2;
+  const double sv = __builtin_fma(radial, f6, __builtin_fma(fr2, f7, fr4 * f8));
+  const double y0 = __builtin_fma(qx, x2, x0), y1 = __builtin_fma(qy, x2, x1);
+  const double v0 = __builtin_fma(iz, __builtin_fma(a, y0, b * y1), qx * sv);
+  const double v1 = __builtin_fma(iz, __builtin_fma(b, y0, d * y1), qy * sv);
+  const double h0 = __builtin_fma(a, v0, b * v1), h1 = __builtin_fma(b, v0, d * v1);
+  const double w2 = sw * sw;
+  const double wz = w2 * iz;
+  const double e0 = wz * h0, e1 = wz * h1, e2 = wz * __builtin_fma(qx, h0, qy * h1);
+  uu = __builtin_fma(w2, __builtin_fma(v0, v0, v1 * v1), uu);
+  const double rx = rr.x, ry = rr.y;
+  const double g0 = __builtin_fma(a, rx, b * ry), g1 = __builtin_fma(b, rx, d * ry);
+  const double sz = sw * iz;
+  const double q0 = sz * g0, q1 = sz * g1, q2 = sz * __builtin_fma(qx, g0, qy * g1);
+  w[0] = __builtin_fma(double(p0.x), e0, __builtin_fma(double(p1.x), e1, __builtin_fma(double(p2.x), e2, w[0])));
+  w[1] = __builtin_fma(double(p0.y), e0, __builtin_fma(double(p1.y), e1, __builtin_fma(double(p2.y), e2, w[1])));
+  w[2] = __builtin_fma(double(p0.z), e0, __builtin_fma(double(p1.z), e1, __builtin_fma(double(p2.z), e2, w[2])));
+  gr[0] = __builtin_fma(double(p0.x), q0, __builtin_fma(double(p1.x), q1, __builtin_fma(double(p2.x), q2, gr[0])));
+  gr[1] = __builtin_fma(double(p0.y), q0, __builtin_fma(double(p1.y), q1, __builtin_fma(double(p2.y), q2, gr[1])));
+  gr[2] = __builtin_fma(double(p0.z), q0, __builtin_fma(double(p1.z), q1, __builtin_fma(double(p2.z), q2, gr[2])));
+}
+
+// The f32 attempt check's sweep (kCheck): the rows of the solution's forward table move into
+// LDS with global_load_lds and the next step's camera index, record and weighted residual are
+// fetched behind them, as in the Schur product's sweep (register sets A and B; loads past the
+// chunk's last step re-read it, so exactly three loads follow the rows and vmcnt(3) is "rows
+// landed"). Accumulates w = sum Jp^T Jc z, uu = sum ||Jc z||^2 and gr = sum Jp^T r; the point's
+// Hpp and its damped factor come from the linearization and the attempt's factorization
+// instead of being re-formed here. Every lane of the wave must call this.
+__device__ __forceinline__ void check_sweep_dma(const PointArgs& pa, int chunk, int lane, int j0, int j1,
+                                                const Vec4<float>* __restrict__ rec,
+                                                const Vec2<float>* __restrict__ res, const float* __restrict__ fw,
+                                                double w[3], double& uu, double gr[3],
+                                                Vec4<float>* __restrict__ coop) {
+  using V4f = float __attribute__((ext_vector_type(4)));
+  using V2f = float __attribute__((ext_vector_type(2)));
+  using GV = const __attribute__((address_space(1))) V4f;
+  using GV2 = const __attribute__((address_space(1))) V2f;
+  using GI = const __attribute__((address_space(1))) int;
+  GV* rcv = (GV*)(rec);
+  GV2* rsv = (GV2*)(res);
+  GI* scv = (GI*)(pa.slot_cam);
+  __asm__ volatile("" : "+s"(rcv), "+s"(rsv), "+s"(scv));
+  const Vec4<float>* pr = reinterpret_cast<const Vec4<float>*>(fw);
+  if (j0 >= j1) return;  // uniform over the wave
+  const int base = pa.chunk_base[chunk];
+  const int last = base + (j1 - 1) * kWave + lane;
+  auto slot = [&](int j) { return min(base + j * kWave + lane, last); };
+  int cr[kRowPieces];
+  int c_a = scv[slot(j0)], c_b;
+  V4f r_a = rcv[slot(j0)], r_b;
+  V2f q_a = rsv[slot(j0)], q_b;
+  piece_cameras(c_a, lane, cr);
+  auto step = [&](int j, int c, const V4f& r, const V2f& q, int& c_next, V4f& r_next, V2f& q_next) {
+    rows_to_lds_dma(pr, cr, lane, coop);
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    c_next = scv[slot(j + 1)];
+    r_next = rcv[slot(j + 1)];
+    q_next = rsv[slot(j + 1)];
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0x0F73);  // vmcnt(3): the rows and every load before them
+    __builtin_amdgcn_wave_barrier();
+    if (c >= 0)
+      check_obs_lds(coop + lane * kRowPieces, Vec4<float>{r.x, r.y, r.z, r.w}, Vec2<float>{q.x, q.y}, w, uu, gr);
+    piece_cameras(c_next, lane, cr);
+  };
+  for (int j = j0;; j += 2) {
+    step(j, c_a, r_a, q_a, c_b, r_b, q_b);
+    if (j + 1 >= j1) break;
+    step(j + 1, c_b, r_b, q_b, c_a, r_a, q_a);
+    if (j + 2 >= j1) break;
+  }
+}
+
 // One lane's observations of a point-major pass (SELL chunk, two observations per step so
 // that both loads issue before either is used). With COOP (FP32 product rows) the rows come
 // through coop_prow, and every lane of the wave must take part.
@@ -1389,7 +1493,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
     const T* __restrict__ fw, const double* __restrict__ mp, const T* __restrict__ gp,
     const T* __restrict__ dpv, const double* __restrict__ delta_p_in, const double* __restrict__ rows_in,
     double lambda, double* __restrict__ t_out, double* __restrict__ delta_out, double* __restrict__ rows_out,
-    double* parts, const CgState* st, const LsmrState* ls, const double* __restrict__ lpp) {
+    double* parts, const CgState* st, const LsmrState* ls, const double* __restrict__ lpp,
+    const double* __restrict__ hpp) {
   __shared__ double smem[kBlock / kWave];
   if (MODE == kSchur && st != nullptr) {
     if (st->done) return;
@@ -1425,9 +1530,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
   // chunk's empty slots, whose camera is -1); not in kCheck, whose larger sweep went from 96 to
   // 132 VGPRs with it and ran slower (63.8 against 56.2 us per launch)
   constexpr bool kCoop = sizeof(T) == 4 && point_fwd_mode<kSweep>() && MODE != kCheck;
-  // kSchur's DMA rows need 6 pieces per row, the register exchange of the other modes 7
-  constexpr int kCoopRow = MODE == kSchur ? kRowPieces : kCoopStride;
-  __shared__ Vec4<float> coop[kCoop ? kChunksPerBlock * kWave * kCoopRow : 1];
+  // the f32 check gathers its rows with global_load_lds like kSchur (check_sweep_dma), and takes
+  // the point's Hpp and damped factor from memory
+  constexpr bool kDmaCheck = sizeof(T) == 4 && MODE == kCheck;
+  // kSchur's and the check's DMA rows need 6 pieces per row, the register exchange of the other
+  // modes 7
+  constexpr int kCoopRow = (MODE == kSchur || kDmaCheck) ? kRowPieces : kCoopStride;
+  __shared__ Vec4<float> coop[(kCoop || kDmaCheck) ? kChunksPerBlock * kWave * kCoopRow : 1];
   if (MODE != kRhs && chunk < pa.nchunks) {
     const int len = pa.chunk_len[chunk];
     const int j0 = split ? len * wave / kChunksPerBlock : 0;
@@ -1437,6 +1546,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
       // register budget is the sweep's own
       point_sweep<T, kSweep, kCoop>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu,
                                     coef, gr, coop + wave * kWave * kCoopRow);
+    } else if constexpr (kDmaCheck) {
+      check_sweep_dma(pa, chunk, lane, j0, j1, reinterpret_cast<const Vec4<float>*>(rec),
+                      reinterpret_cast<const Vec2<float>*>(res), reinterpret_cast<const float*>(fw), w, uu, gr,
+                      coop + wave * kWave * kCoopRow);
     } else if (kCoop && fw != nullptr) {
       point_sweep<T, kSweep, kCoop>(pa, chunk, lane, j0, j1, rec, res, tab, fw, dpx, rows_in, rows_out, w, h, uu,
                                     coef, gr, coop + wave * kWave * kCoopRow);
@@ -1482,28 +1595,43 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(poi
       // a second sweep:
       //   ||J delta||^2 over the point's rows = sum ||Jc z||^2 + 2 delta_p.w + delta_p^T Hpp delta_p
       //   Jp^T (J delta + r)                  = w + Hpp delta_p + sum Jp^T r
+      // (f32: Hpp and the factor of Hpp + lambda Dp^2 as the linearization and the attempt's
+      // factorization stored them, formed from the same rows in the same order)
       double hu[6];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) hu[k] = h[k];  // undamped Hpp
+      for (int k = 0; k < 6; ++k) hu[k] = kDmaCheck ? hpp[6 * i + k] : h[k];  // undamped Hpp
       double t[3] = {w[0], w[1], w[2]};
+      if constexpr (kDmaCheck) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double d = double(dpv[3 * i + a]);
-        h[sym3(a, a)] += lambda * d * d;
+        for (int k = 0; k < 6; ++k) h[k] = lpp[6 * i + k];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const double d = double(dpv[3 * i + a]);
+          h[sym3(a, a)] += lambda * d * d;
+        }
       }
-      if (chol_packed<double, 3>(h)) {
+      if (kDmaCheck ? h[0] != 0.0 : chol_packed<double, 3>(h)) {
         solve_lower_packed<double, 3>(h, t);
         solve_upper_packed<double, 3>(h, t);
       } else {
         t[0] = t[1] = t[2] = 0.0;
       }
-      store_tp<T>(t_out, i, t);
+      // f32: t and delta_p side by side in one 32-byte row per point (t_out holds 2 np Vec4),
+      // so the check's camera pass gathers one line per row
+      const Vec4<float> t4{float(t[0]), float(t[1]), float(t[2]), 0.f};
+      if constexpr (!kDmaCheck) store_tp<T>(t_out, i, t);
       const double v[3] = {double(gp[3 * i]) + w[0], double(gp[3 * i + 1]) + w[1], double(gp[3 * i + 2]) + w[2]};
       point_solve(mp, dpv, i, v, t);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         dpx[a] = -t[a];
         delta_out[3 * i + a] = dpx[a];
+      }
+      if constexpr (kDmaCheck) {
+        Vec4<float>* row = reinterpret_cast<Vec4<float>*>(t_out) + 2 * i;
+        row[0] = t4;
+        row[1] = Vec4<float>{float(dpx[0]), float(dpx[1]), float(dpx[2]), 0.f};
       }
       double hd[3];
 #pragma unroll
@@ -2053,8 +2181,11 @@ __global__ void __launch_bounds__(kBlock) k_camera_split(
   GD* dpv = (GD*)(delta_p);
   GInt* psv = (GInt*)(k_pslot);
   __asm__ volatile("" : "+s"(tpv), "+s"(recv), "+s"(resv), "+s"(dpv), "+s"(psv));
+  // f32: tp holds t and delta_p side by side, two Vec4 per point (one 32-byte line piece);
+  // f64: t rows in tp, delta_p (f64) in delta_p
+  constexpr bool kPair = sizeof(T) == 4;
   struct Row {
-    V4 rec, t;
+    V4 rec, t, td;
     V2 res;
     double d0, d1, d2;
   };
@@ -2064,12 +2195,19 @@ __global__ void __launch_bounds__(kBlock) k_camera_split(
     int k = k0 + lane;
     auto fetch = [&](int kk, int ps, Row& r) {
       r.rec = recv[kk];
-      r.t = tpv[ps];
+      if constexpr (kPair) {
+        r.t = tpv[2 * ps];
+        r.td = tpv[2 * ps + 1];
+      } else {
+        r.t = tpv[ps];
+      }
       if constexpr (kCheck) {
         r.res = resv[kk];
-        r.d0 = dpv[3 * ps];
-        r.d1 = dpv[3 * ps + 1];
-        r.d2 = dpv[3 * ps + 2];
+        if constexpr (!kPair) {
+          r.d0 = dpv[3 * ps];
+          r.d1 = dpv[3 * ps + 1];
+          r.d2 = dpv[3 * ps + 2];
+        }
       }
     };
     auto row = [&](const Row& r, bool valid) {
@@ -2080,7 +2218,8 @@ __global__ void __launch_bounds__(kBlock) k_camera_split(
       double a0, a1;
       jp_apply(geo, ctl, xt, a0, a1);
       if constexpr (kCheck) {
-        const double xd[3] = {r.d0, r.d1, r.d2};
+        const double xd[3] = {kPair ? double(r.td.x) : r.d0, kPair ? double(r.td.y) : r.d1,
+                              kPair ? double(r.td.z) : r.d2};
         double u0, u1, b0, b1;
         jc_apply(geo, fwl, u0, u1);
         jp_apply(geo, ctl, xd, b0, b1);
