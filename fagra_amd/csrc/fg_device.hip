@@ -1811,7 +1811,9 @@ __global__ void k_info_points(int np, size_t n, size_t base, const double* __res
 }
 
 // One workgroup per camera: rows of its observations, cross blocks J_p^T J_c into the lower
-// triangle (point rows, camera columns); a point seen twice by a camera adds twice.
+// triangle (point rows, camera columns). Each entry receives one addition per observation of
+// its point by its camera onto zero; one or two additions give the same bits in either order,
+// so the result is run-to-run identical unless a camera observes one point three or more times.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_info_cross(int nc, size_t n, size_t base, const int* __restrict__ cam_off,
                                                        const int* __restrict__ k_pslot,

@@ -535,35 +535,43 @@ __global__ void __launch_bounds__(kBlock) k_gs_absmax(int n, const double* __res
   }
 }
 
-// Dense undamped information J^T J (LinearizedModel::covariance, linear_model.rs:122-167):
-// one thread per emission adds its block products into the lower triangle.
-__global__ void k_gs_info(int ne, int rw, int ew, int retained, size_t n, const Em* __restrict__ em,
-                          const double* __restrict__ vals, double* __restrict__ h) {
+// Dense undamped information J^T J (LinearizedModel::covariance, linear_model.rs:122-167), lower
+// triangle, deterministic: every entry has one owner, which sums its contributions in emission
+// order. Thread r of k_gs_info_ret owns retained block r's diagonal block (its emissions through
+// the retained CSR list); thread e of k_gs_info_elim owns eliminated block e's diagonal block
+// and its cross blocks with the retained blocks (all through e's CSR list, since every emission
+// that couples r and e is one of e's). n: dimension, h: column-major n x n, zeroed.
+__device__ __forceinline__ void gs_block_product(const Em& m, const double* __restrict__ vals, int offx, int wx,
+                                                 int colx, int offy, int wy, int coly, bool diag, size_t n,
+                                                 double* __restrict__ h) {
+  for (int ca = 0; ca < wx; ++ca)
+    for (int cb = 0; cb < (diag ? ca + 1 : wy); ++cb) {
+      double v = 0.0;
+      for (int i = 0; i < m.rows; ++i) v += vals[offx + ca * m.rows + i] * vals[offy + cb * m.rows + i];
+      h[(size_t)(coly + cb) * n + (size_t)(colx + ca)] += v;
+    }
+}
+__global__ void k_gs_info_ret(int nrb, int rw, size_t n, const int* __restrict__ rptr, const int* __restrict__ ridx,
+                              const Em* __restrict__ em, const double* __restrict__ vals, double* __restrict__ h) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrb) return;
+  for (int k = rptr[r]; k < rptr[r + 1]; ++k) {
+    const Em m = em[ridx[k]];
+    gs_block_product(m, vals, m.roff, rw, r * rw, m.roff, rw, r * rw, true, n, h);
+  }
+}
+__global__ void k_gs_info_elim(int neb, int rw, int ew, int retained, size_t n, const int* __restrict__ eptr,
+                               const int* __restrict__ eidx, const Em* __restrict__ em,
+                               const double* __restrict__ vals, double* __restrict__ h) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const Em m = em[e];
-  const int nb = (m.rblk >= 0) + (m.eblk >= 0);
-  int col[2], w[2], off[2], k = 0;
-  if (m.rblk >= 0) {
-    col[k] = m.rblk * rw;
-    w[k] = rw;
-    off[k++] = m.roff;
+  if (e >= neb) return;
+  const int col = retained + e * ew;
+  for (int k = eptr[e]; k < eptr[e + 1]; ++k) {
+    const Em m = em[eidx[k]];
+    gs_block_product(m, vals, m.eoff, ew, col, m.eoff, ew, col, true, n, h);
+    // eliminated columns follow the retained ones, so the cross block lies below the diagonal
+    if (m.rblk >= 0) gs_block_product(m, vals, m.eoff, ew, col, m.roff, rw, m.rblk * rw, false, n, h);
   }
-  if (m.eblk >= 0) {
-    col[k] = retained + m.eblk * ew;
-    w[k] = ew;
-    off[k++] = m.eoff;
-  }
-  for (int x = 0; x < nb; ++x)
-    for (int y = 0; y <= x; ++y)  // block x has the larger columns (retained before eliminated)
-      for (int ca = 0; ca < w[x]; ++ca)
-        for (int cb = 0; cb < w[y]; ++cb) {
-          const size_t r = (size_t)col[x] + ca, c = (size_t)col[y] + cb;
-          if (r < c) continue;
-          double v = 0.0;
-          for (int i = 0; i < m.rows; ++i) v += vals[off[x] + ca * m.rows + i] * vals[off[y] + cb * m.rows + i];
-          atomicAdd(&h[c * n + r], v);
-        }
 }
 
 }  // namespace
@@ -1145,9 +1153,17 @@ fg_status fg_gschur_covariance(fg_gschur* g, const size_t* offsets, const size_t
   fg_status st = FG_DEVICE_ERROR;
   const int ne = (int)g->h_em.size();
   if (hipMemsetAsync(h, 0, n * n * sizeof(double), g->stream) == hipSuccess) {
-    if (ne)
-      hipLaunchKernelGGL(k_gs_info, dim3(blocks_for(ne)), dim3(kBlock), 0, g->stream, ne, (int)g->rw, (int)g->ew,
-                         (int)g->retained, n, (const Em*)g->d_em, (const double*)g->d_vals, h);
+    if (ne) {
+      const int R = (int)nrb(g), E = (int)neb(g);
+      if (R)
+        hipLaunchKernelGGL(k_gs_info_ret, dim3(blocks_for(R)), dim3(kBlock), 0, g->stream, R, (int)g->rw, n,
+                           (const int*)g->d_rptr, (const int*)g->d_ridx, (const Em*)g->d_em,
+                           (const double*)g->d_vals, h);
+      if (E)
+        hipLaunchKernelGGL(k_gs_info_elim, dim3(blocks_for(E)), dim3(kBlock), 0, g->stream, E, (int)g->rw,
+                           (int)g->ew, (int)g->retained, n, (const int*)g->d_eptr, (const int*)g->d_eidx,
+                           (const Em*)g->d_em, (const double*)g->d_vals, h);
+    }
     if (hipGetLastError() == hipSuccess) st = fg::dense_covariance(g->stream, h, n, cols.data(), cols.size(), tol, out);
   }
   (void)hipStreamSynchronize(g->stream);

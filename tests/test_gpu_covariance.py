@@ -77,6 +77,11 @@ def test_generic_model_covariance_matches_oracle_and_dense_inverse():
     st_o, C_o = o.s.covariance(blocks)
     assert st == st_o == fa.OK
     assert close(C, C_o)
+    # the information is assembled with one owner per entry summing in emission order (no
+    # atomics), so repeated calls return the same bits
+    for _ in range(3):
+        st2, C2 = g.covariance(blocks)
+        assert st2 == fa.OK and np.array_equal(C2, C)
     J, _ = m.dense_jacobian()
     idx = [c for off, w in blocks for c in range(off, off + w)]
     assert close(C, np.linalg.inv(J.T @ J)[np.ix_(idx, idx)], 1e-8)
