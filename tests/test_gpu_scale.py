@@ -19,7 +19,10 @@ FP64 GPU run of the same length must agree step by step (cost 1e-9 relative, sam
 in the sampled final state. Converged: tests/golden/converged_<workload>_0.json (written by
 tests/golden/make_converged.py) holds the oracle's converged cost and state; the FP32 runs
 (block-Jacobi and learned) at the benchmark's CG tolerance 1e-3 must end within 1e-5 of that
-cost, and the converged FP64 GPU run within 1e-9 of it.
+cost, and the converged FP64 GPU run within 1e-9 of it. Where the oracle's converged run is out
+of reach (zipf4000: its first 8 LM steps took 1,548 s on 2 host threads, so a converged run would
+take days), the FP32 runs are held to 1e-5 of the converged FP64 GPU run instead, whose first
+steps the trajectory fixture pins to the oracle.
 """
 import json
 import os
@@ -241,3 +244,21 @@ def test_converged_fp64_matches_oracle_fixture(scale):
           f"{fx['final_cost']!r} rel {dc:.2e}, worst segment rel {worst:.2e}, sampled state rel {e_state:.2e}")
     assert len(costs) == len(fx["segment_costs"])
     assert worst <= 1e-9 and abs(dc) <= 1e-9
+
+
+def test_fp32_final_cost_against_converged_fp64_gpu_run(scale):
+    name, p = scale["name"], scale["p"]
+    if os.path.exists(os.path.join(GOLDEN, CONVERGED[name])):
+        pytest.skip(f"{name}: the oracle's converged fixture is the reference")
+    load_trajectory(name, p)  # the FP64 path is pinned to the oracle on this workload's first steps
+    ba64 = device(scale, fa.FP64)
+    ref, _ = run_lm(ba64, p, fa.FP64, 300, 1e-10)
+    print(f"{name} fp64 reference: {ref.accepted_steps} steps, status {ref.status}, cost {ref.cost!r}")
+    assert ref.status in (fa.OK, fa.NO_CONVERGENCE, fa.NO_PROGRESS)
+    precs = [fa.PRECOND_BLOCK_JACOBI] + ([fa.PRECOND_LEARNED] if learned.available() else [])
+    for precond in precs:
+        out, _ = run_lm(device(scale, fa.FP32), p, fa.FP32, 150, 1e-3, precond)
+        dc = (out.cost - ref.cost) / ref.cost
+        print(f"{name} fp32 preconditioner {precond}: {out.accepted_steps} steps, final {out.cost!r}, "
+              f"rel to the converged fp64 run {dc:.2e}")
+        assert abs(dc) <= 1e-5
