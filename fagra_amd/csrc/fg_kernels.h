@@ -942,13 +942,29 @@ __device__ __forceinline__ void point_solve(const double* __restrict__ lp, const
   for (int a = 0; a < 3; ++a) t[a] /= d[a];
 }
 
+// The factor of camera block B + lambda I (B: the scaled Gram, packed lower, in T) in registers;
+// false where an entry is nonfinite or the factorization fails.
+template <typename T>
+__device__ __forceinline__ bool camera_chol(const T* __restrict__ b, double lambda, double a[45]) {
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 45; ++i) {
+    a[i] = double(b[i]);
+    ok = ok && isfinite(a[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) a[sym9(k, k)] += lambda;
+  return ok && chol_packed<double, 9>(a);
+}
+
 // Camera blocks of BlockFactors::factor: nine lanes per camera each factor B + lambda I in
 // registers (the same arithmetic, so the same factor) and solve one column of its explicit
-// inverse; lane 0 stores the factor and counts a failure.
+// inverse; lane 0 stores the factor and counts a failure. Thread gid: camera gid / 9, inverse
+// column gid % 9.
 template <typename T>
-__global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const T* __restrict__ bs, double lambda,
-                                 double* __restrict__ lc, double* __restrict__ mbj, int* fail_count) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void factor_camera(int gid, int nc, const int* __restrict__ cam_off,
+                                              const T* __restrict__ bs, double lambda, double* __restrict__ lc,
+                                              double* __restrict__ mbj, int* fail_count) {
   const int c = gid / 9, j = gid % 9;
   if (c >= nc) return;
   if (cam_off[c + 1] == cam_off[c]) {  // unobserved: no block; a zero factor marks it
@@ -957,15 +973,7 @@ __global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const 
     return;
   }
   double a[45];
-  bool ok = true;
-#pragma unroll
-  for (int i = 0; i < 45; ++i) {
-    a[i] = double(bs[45 * c + i]);
-    ok = ok && isfinite(a[i]);
-  }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) a[sym9(k, k)] += lambda;
-  ok = ok && chol_packed<double, 9>(a);
+  const bool ok = camera_chol<T>(bs + 45 * c, lambda, a);
   if (j == 0) {
     // a failed block stores a zero factor: Schur reports it, LSMR keeps diagonal treatment
 #pragma unroll
@@ -979,6 +987,11 @@ __global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const 
   solve_lower_packed<double, 9>(a, e);
   solve_upper_packed<double, 9>(a, e);
   for (int i = j; i < 9; ++i) mbj[45 * c + sym9(i, j)] = e[i];
+}
+template <typename T>
+__global__ void k_factor_cameras(int nc, const int* __restrict__ cam_off, const T* __restrict__ bs, double lambda,
+                                 double* __restrict__ lc, double* __restrict__ mbj, int* fail_count) {
+  factor_camera<T>(blockIdx.x * blockDim.x + threadIdx.x, nc, cam_off, bs, lambda, lc, mbj, fail_count);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -21,6 +21,15 @@ __host__ __device__ inline void upper_index(int k, int& i, int& j) {
 // Feature k of camera c's block: for the 45 upper-triangle entries of the unnormalized Gram
 // block, log(H_ii) on the diagonal and the correlation H_ij / sqrt(H_ii H_jj) off it; then
 // log(observations + 1) and log10(lambda), all scaled to O(1); 0 for the padding k >= 47.
+// The pieces, shared by camera_feature and the fused preconditioner block:
+__device__ __forceinline__ float diagonal_log_feature(double hii) { return float(log(fmax(hii, 1e-30)) / 8.0); }
+__device__ __forceinline__ float correlation_feature(double hii, double hjj, double hij) {
+  const double d = sqrt(hii * hjj);
+  return d > 0.0 ? float(hij / d) : 0.f;
+}
+__device__ __forceinline__ float count_feature(int n) { return float(log(double(n) + 1.0) / 8.0); }
+__device__ __forceinline__ float damping_feature(double lambda) { return float(log10(fmax(lambda, 1e-30)) / 8.0); }
+
 template <typename T>
 __device__ __forceinline__ float camera_feature(int c, int k, const int* __restrict__ cam_off,
                                                 const T* __restrict__ hcc, double lambda) {
@@ -31,15 +40,14 @@ __device__ __forceinline__ float camera_feature(int c, int k, const int* __restr
     upper_index(k, i, j);
     const double hii = double(h[sym9(i, i)]), hjj = double(h[sym9(j, j)]);
     if (i == j) {
-      f = float(log(fmax(hii, 1e-30)) / 8.0);
+      f = diagonal_log_feature(hii);
     } else {
-      const double d = sqrt(hii * hjj);
-      f = d > 0.0 ? float(double(h[sym9(i, j)]) / d) : 0.f;
+      f = correlation_feature(hii, hjj, double(h[sym9(i, j)]));
     }
   } else if (k == 45) {
-    f = float(log(double(cam_off[c + 1] - cam_off[c]) + 1.0) / 8.0);
+    f = count_feature(cam_off[c + 1] - cam_off[c]);
   } else if (k == 46) {
-    f = float(log10(fmax(lambda, 1e-30)) / 8.0);
+    f = damping_feature(lambda);
   }
   return f;
 }
@@ -220,7 +228,7 @@ struct PrecondArgs {
   double lambda;
   const uint16_t *w1, *w2, *w3;
   const float *b1, *b2, *b3;
-  const double* lc;
+  const void* bs;  // T [nc][45]: the scaled camera blocks B, factored here as k_factor_cameras does
   const float* range;
   double* mout;
 };
@@ -232,63 +240,196 @@ __device__ __forceinline__ void mlp_precond_block(int blk, const PrecondArgs& pa
   const double lambda = pa.lambda;
   const uint16_t *__restrict__ w1 = pa.w1, *__restrict__ w2 = pa.w2, *__restrict__ w3 = pa.w3;
   const float *__restrict__ b1 = pa.b1, *__restrict__ b2 = pa.b2, *__restrict__ b3 = pa.b3;
-  const double* __restrict__ lc = pa.lc;
+  const T* __restrict__ bs = (const T*)pa.bs;
   const float* __restrict__ range = pa.range;
   double* __restrict__ mout = pa.mout;
-  __shared__ uint16_t xs[16][kInPad];
-  __shared__ float xfs[16][kIn];
-  __shared__ uint16_t h1[16][kHidden];
-  __shared__ uint16_t h2[16][kHidden];
-  __shared__ float ys[16][kOutPad];
-  __shared__ double w[16][9][9];
+  // The stages' LDS arrays alias where their lifetimes do not overlap (xs: features and layer 1,
+  // h1: layers 1-2, h2: layers 2-3, w: the inverse stage after layer 3; the features' Gram
+  // blocks and logarithms share the space of h1 and h2): 22.6 KB.
+  struct Pool {
+    union {
+      struct {
+        uint16_t xs[16][kInPad];
+        union {
+          struct {
+            uint16_t h1[16][kHidden];
+            uint16_t h2[16][kHidden];
+          } l;  // layers 1-3
+          struct {
+            T hs[16][45];        // the cameras' Gram blocks
+            float logs[16][11];  // per camera: the 9 diagonal features, the count and the damping
+            int offs[17];        // camera offsets c0 .. c0 + 16
+          } f;  // features
+        } v;
+      } a;
+      double w[16][9][9];  // inverse stage
+    } u;
+    float xfs[16][kIn];
+    float ys[16][kOutPad];
+    double lcs[16][45];
+    float rng[2 * kIn];
+  };
+  __shared__ Pool pool;
+  auto& xs = pool.u.a.xs;
+  auto& h1 = pool.u.a.v.l.h1;
+  auto& h2 = pool.u.a.v.l.h2;
+  auto& hs = pool.u.a.v.f.hs;
+  auto& logs = pool.u.a.v.f.logs;
+  auto& offs = pool.u.a.v.f.offs;
+  auto& w = pool.u.w;
+  auto& xfs = pool.xfs;
+  auto& ys = pool.ys;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, q = lane >> 4;
   const int c0 = blk * 16;
-  for (int e = threadIdx.x; e < 16 * kInPad; e += blockDim.x) {
+  // Everything the block reads from memory is loaded here, in one round trip: the 16 cameras'
+  // Gram blocks (contiguous, 45 each: at most 3 per thread), their offsets, the feature range
+  // and the biases of this lane's output columns; the first 16 threads factor one camera block
+  // B + lambda I each (camera_chol, as k_factor_cameras does, so the block needs no other
+  // kernel's factors); the layers' weights are touched below.
+  const int ncb = max(0, min(16, nc - c0));
+  const int nl = 45 * ncb;
+  T hreg[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const int e = (int)threadIdx.x + n * (int)blockDim.x;
+    hreg[n] = e < nl ? hcc[(size_t)45 * c0 + e] : T(0);
+  }
+  const int oreg = (int)threadIdx.x <= ncb ? cam_off[c0 + threadIdx.x] : 0;
+  const float rreg = range && threadIdx.x < 2 * kIn ? range[threadIdx.x] : 0.f;
+  float bias1[2], bias2[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    bias1[u] = b1[(wave + 4 * u) * 16 + r];
+    bias2[u] = b2[(wave + 4 * u) * 16 + r];
+  }
+  const float bias3 = wave < kOutPad / 16 ? b3[wave * 16 + r] : 0.f;
+  double fac[45];
+  bool fok = false;
+  if ((int)threadIdx.x < ncb) fok = camera_chol<T>(bs + 45 * (size_t)(c0 + threadIdx.x), lambda, fac);
+  // The three layers' weights (60 KB, read by every workgroup) are touched once per 128-byte
+  // line now, so the layers find them in L2 instead of waiting on memory behind the point
+  // factorization's traffic; the loaded words meet in pf, which an empty asm consumes at the end.
+  constexpr int kW1 = kHidden * kInPad, kW2 = kHidden * kHidden, kW3 = kOutPad * kHidden;  // bf16
+  constexpr int kLines = (kW1 + kW2 + kW3) / 64;
+  const __attribute__((address_space(1))) uint32_t* wl[3] = {
+      (const __attribute__((address_space(1))) uint32_t*)w1, (const __attribute__((address_space(1))) uint32_t*)w2,
+      (const __attribute__((address_space(1))) uint32_t*)w3};
+  __asm__ volatile("" : "+s"(wl[0]), "+s"(wl[1]), "+s"(wl[2]));
+  uint32_t pf = 0;
+#pragma unroll
+  for (int n = 0; n < (kLines + 255) / 256; ++n) {
+    const int line = (int)threadIdx.x + 256 * n;
+    if (line < kLines) {
+      const int b16 = 64 * line;  // bf16 index of the line's first word
+      pf ^= b16 < kW1 ? wl[0][b16 / 2] : b16 < kW1 + kW2 ? wl[1][(b16 - kW1) / 2] : wl[2][(b16 - kW1 - kW2) / 2];
+    }
+  }
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const int e = (int)threadIdx.x + n * (int)blockDim.x;
+    if (e < 16 * 45) (&hs[0][0])[e] = hreg[n];
+  }
+  if ((int)threadIdx.x < ncb) {
+#pragma unroll
+    for (int i = 0; i < 45; ++i) pool.lcs[threadIdx.x][i] = fok ? fac[i] : 0.0;
+  }
+  if ((int)threadIdx.x <= ncb) offs[threadIdx.x] = oreg;
+  if (range && threadIdx.x < 2 * kIn) pool.rng[threadIdx.x] = rreg;
+  __syncthreads();
+  // the logarithms once per camera (camera_feature's pieces), then every feature from them and
+  // the off-diagonal correlations
+  if (threadIdx.x < 16 * 11) {
+    const int cl = threadIdx.x / 11, m = threadIdx.x % 11;
+    float f = 0.f;
+    if (cl < ncb)
+      f = m < 9 ? diagonal_log_feature(double(hs[cl][sym9(m, m)]))
+                : m == 9 ? count_feature(offs[cl + 1] - offs[cl]) : damping_feature(lambda);
+    logs[cl][m] = f;
+  }
+  __syncthreads();
+  static_assert(16 * kInPad % 256 == 0, "features per thread");
+#pragma unroll
+  for (int n = 0; n < 16 * kInPad / 256; ++n) {
+    const int e = (int)threadIdx.x + 256 * n;
     const int cl = e / kInPad, k = e % kInPad, c = c0 + cl;
-    const float f = c < nc ? camera_feature<T>(c, k, cam_off, hcc, lambda) : 0.f;
+    float f = 0.f;
+    if (c < nc && k < kIn) {
+      if (k >= 45) {
+        f = logs[cl][9 + (k - 45)];
+      } else {
+        int i, j;
+        upper_index(k, i, j);
+        f = i == j ? logs[cl][i]
+                   : correlation_feature(double(hs[cl][sym9(i, i)]), double(hs[cl][sym9(j, j)]),
+                                         double(hs[cl][sym9(i, j)]));
+      }
+    }
     xs[cl][k] = f2bf(f);
     if (k < kIn) xfs[cl][k] = f;
   }
   __syncthreads();
-  for (int nt = wave; nt < kHidden / 16; nt += 4) {
+  // each wave's two output tiles of a hidden layer unrolled, so both tiles' weight loads issue
+  // together
+  static_assert(kHidden / 16 == 8, "two hidden tiles per wave");
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int nt = wave + 4 * u;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = mfma_tile(&xs[0][0], kInPad, w1 + (size_t)nt * 16 * kInPad, kInPad, kInPad, acc);
     for (int i = 0; i < 4; ++i) {
       const int row = 4 * q + i, col = nt * 16 + r;
-      h1[row][col] = f2bf(gelu(acc[i] + b1[col]));
+      h1[row][col] = f2bf(gelu(acc[i] + bias1[u]));
     }
   }
   __syncthreads();
-  for (int nt = wave; nt < kHidden / 16; nt += 4) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int nt = wave + 4 * u;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = mfma_tile(&h1[0][0], kHidden, w2 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
     for (int i = 0; i < 4; ++i) {
       const int row = 4 * q + i, col = nt * 16 + r;
-      h2[row][col] = f2bf(gelu(acc[i] + b2[col]));
+      h2[row][col] = f2bf(gelu(acc[i] + bias2[u]));
     }
   }
   __syncthreads();
-  for (int nt = wave; nt < kOutPad / 16; nt += 4) {
+  static_assert(kOutPad / 16 <= 4, "one output tile per wave");
+  if (wave < kOutPad / 16) {
+    const int nt = wave;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = mfma_tile(&h2[0][0], kHidden, w3 + (size_t)nt * 16 * kHidden, kHidden, kHidden, acc);
     for (int i = 0; i < 4; ++i) {
       const int row = 4 * q + i, col = nt * 16 + r;
-      ys[row][col] = acc[i] + b3[col];
+      ys[row][col] = acc[i] + bias3;
     }
   }
   __syncthreads();
-  // 16 threads per camera: 9 of them form the columns of W, then all 16 the 45 entries of M
+  // 16 threads per camera: 9 of them form the columns of W (as inverse_column does, the factor
+  // and the range read from LDS), then all 16 the 45 entries of M
   const int cl = threadIdx.x >> 4, t = threadIdx.x & 15, c = c0 + cl;
   const bool active = c < nc && cam_off[c + 1] > cam_off[c];
+  // the camera's 16 lanes test three features each against the training range, then combine
+  int inside = 1;
+  if (range)
+    for (int k = t; k < kIn; k += 16) {
+      const float f = xfs[cl][k];
+      inside &= (f >= pool.rng[k]) & (f <= pool.rng[kIn + k]);
+    }
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) inside &= __shfl_xor(inside, o, 16);
   if (active && t < 9) {
     double col[9];
-    inverse_column(t, &ys[cl][0], lc + 45 * c, range ? &xfs[cl][0] : nullptr, range, col);
+    for (int i = 0; i < 9; ++i)  // column t of G = I + s tril(y)
+      col[i] = i < t ? 0.0 : (inside ? double(kOutScale * ys[cl][sym_index(9, i, t)]) : 0.0) + (i == t ? 1.0 : 0.0);
+    solve_upper_packed<double, 9>(&pool.lcs[cl][0], col);
     for (int i = 0; i < 9; ++i) w[cl][i][t] = col[i];
   }
   __syncthreads();
   if (active)
     for (int e = t; e < 45; e += 16) inverse_entry(e, w[cl], mout + 45 * c);
+  __asm__ volatile("" ::"v"(pf));
 }
 
 template <typename T>
@@ -296,22 +437,22 @@ static __global__ void __launch_bounds__(256) k_mlp_precond(PrecondArgs pa) {
   mlp_precond_block<T>(blockIdx.x, pa);
 }
 
-// The attempt's point factorization (k_factor_points) and the learned preconditioner in one
-// launch: workgroups 0 .. learned_blocks - 1 form M for 16 cameras each (mlp_precond_block; they
-// need the camera factors, which k_factor_cameras wrote before), the rest factor 256 points
-// each. The two are independent, and the network's few latency-bound workgroups run beside the
-// point factorization's instead of after it.
+// The attempt's camera factorization (k_factor_cameras) and the learned preconditioner in one
+// launch: workgroups 0 .. learned_blocks - 1 form M for 16 cameras each (mlp_precond_block, which
+// factors its cameras' blocks itself), the rest factor the cameras for block-Jacobi. The two are
+// independent; the network's latency-bound workgroups run beside a light kernel instead of the
+// point factorization, whose memory traffic slowed their loads several-fold.
 template <typename T>
-static __global__ void __launch_bounds__(256) k_factor_points_precond(
-    int learned_blocks, PrecondArgs pa, int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
-    double lambda, double* __restrict__ lp, int* fail_count, const T* __restrict__ gp, const T* __restrict__ dpv,
-    double* __restrict__ t_out, const double* __restrict__ hpp, double* __restrict__ lpp) {
+static __global__ void __launch_bounds__(256) k_factor_cameras_precond(int learned_blocks, PrecondArgs pa,
+                                                                       const T* __restrict__ bs, double lambda,
+                                                                       double* __restrict__ lc,
+                                                                       double* __restrict__ mbj, int* fail_count) {
   if ((int)blockIdx.x < learned_blocks) {  // uniform over the workgroup
     mlp_precond_block<T>(blockIdx.x, pa);
     return;
   }
-  factor_point<T>(((int)blockIdx.x - learned_blocks) * blockDim.x + threadIdx.x, np, pt_len, cs, lambda, lp,
-                  fail_count, gp, dpv, t_out, hpp, lpp);
+  factor_camera<T>(((int)blockIdx.x - learned_blocks) * blockDim.x + threadIdx.x, pa.nc, pa.cam_off, bs, lambda, lc,
+                   mbj, fail_count);
 }
 
 // Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).
