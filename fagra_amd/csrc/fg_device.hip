@@ -245,17 +245,23 @@ fg_status ensure_tables(fg_ba* b, int buf) {
 
 // The cost of buffer buf is enqueued by cost_launch and read by cost_finish once the stream
 // has been synchronized.
+// The cost of state buffer buf: k_cost's per-workgroup partials in b->d_parts (cost_blocks of
+// them) and their sum in b->d_out. flags_zeroed: a kernel earlier in stream order already
+// cleared the flag word (no memset); reduce == false: the partials are left to the caller's
+// reduction (k_attempt_tail sums them the way k_reduce_parts does).
+inline int cost_blocks(const fg_ba* b) { return std::max(1, grid_for(b->nchunks, kChunksPerBlock)); }
 template <typename T>
-fg_status cost_launch(fg_ba* b, int buf, bool copy_back = true) {
+fg_status cost_launch(fg_ba* b, int buf, bool copy_back = true, bool flags_zeroed = false, bool reduce = true) {
   FG_CHECK(ensure_tables<T>(b, buf));
   if (b->timing) (void)hipEventRecord(b->ev_a, b->stream);
-  FG_TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int), b->stream));
-  const int blocks = std::max(1, grid_for(b->nchunks, kChunksPerBlock));
+  if (!flags_zeroed) FG_TRY(hipMemsetAsync(b->d_flags, 0, sizeof(int), b->stream));
+  const int blocks = cost_blocks(b);
   hipLaunchKernelGGL(k_cost<T>, dim3(blocks), dim3(kBlock), 0, b->stream, b->nchunks, (int)b->np, b->d_chunk_base,
                      b->d_chunk_len, b->d_slot_cam, (const Vec2<T>*)b->d_slot_pix, (const T*)b->d_tab[buf],
                      (const T*)b->d_pts_t[buf], (T)b->huber, b->d_parts, b->d_flags);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)b->d_parts, blocks, 0,
-                     b->d_out);
+  if (reduce)
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)b->d_parts, blocks, 0,
+                       b->d_out);
   FG_TRY(hipGetLastError());
   if (copy_back) {
     FG_TRY(hipMemcpyAsync(b->h_pinned, b->d_out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
@@ -728,13 +734,16 @@ void cg_collect(fg_schur* s) {
 
 // The trial state (cur + delta, both parameter blocks) into the other buffer; its camera
 // tables are rebuilt on first use.
+// Workgroups of k_stage (grid-stride): at most kStageBlocks, so its norm partials fit the PCG's
+// partial-sum buffer, idle once the solve has ended.
+constexpr int kStageBlocks = 1024;
+inline int stage_blocks(const fg_ba* b) { return std::min(kStageBlocks, grid_for(9 * b->nc + 3 * b->np)); }
 template <typename T>
-fg_status stage_launch(fg_ba* b, const double* d_delta) {
+fg_status stage_launch(fg_ba* b, const double* d_delta, double* nparts = nullptr, int* zero_flag = nullptr) {
   const int other = b->cur ^ 1;
-  const size_t n = 9 * b->nc + 3 * b->np;
-  hipLaunchKernelGGL(k_stage<T>, dim3(grid_for(n)), dim3(kBlock), 0, b->stream, (int)b->nc, (int)b->np,
+  hipLaunchKernelGGL(k_stage<T>, dim3(stage_blocks(b)), dim3(kBlock), 0, b->stream, (int)b->nc, (int)b->np,
                      b->d_cams[b->cur], b->d_pts[b->cur], d_delta, d_delta + 9 * b->nc, b->d_cams[other],
-                     b->d_pts[other], (T*)b->d_pts_t[other]);
+                     b->d_pts[other], (T*)b->d_pts_t[other], nparts, zero_flag);
   FG_TRY(hipGetLastError());
   b->tab_valid[other] = false;
   return FG_OK;
@@ -806,21 +815,32 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
                            s->d_parts + (size_t)kParts * 2 * nc, nullptr);
   launch_cameras<T, cCheck>(s, s->d_delta + 9 * b->nc, nullptr, (const T*)s->fw, (const double*)s->x, nullptr,
                             (const double*)s->rhs, lambda, (double*)s->q, s->d_delta);
-  const SumJobs jobs{{nc, pblocks, nc, 0}, {0, 2 * nc, nc, 0}, {0, 1, 2, 0}};
-  hipLaunchKernelGGL(k_sum_parts_jobs, dim3(3), dim3(kBlock), 0, b->stream, (const double*)s->d_parts, jobs,
-                     s->d_sums);
-  FG_TRY(hipGetLastError());
+  TailArgs tail{};
+  tail.sc = (const DevScalars*)s->d_sc;
+  tail.fail = (const int*)s->d_fail;
+  tail.parts = (const double*)s->d_parts;
+  tail.jobs = SumJobs{{nc, pblocks, nc, 0}, {0, 2 * nc, nc, 0}, {0, 1, 2, 0}};
+  tail.njobs = 3;
+  tail.sums = s->d_sums;
+  tail.out = s->d_att;
   if (speculate_trial) {
-    FG_CHECK(stage_launch<T>(b, s->d_delta));
-    FG_CHECK(cost_launch<T>(b, b->cur ^ 1, false));
+    // the staging also forms ||delta||_inf's partials and clears the cost's flag; the cost's
+    // partials are summed by the tail kernel
+    FG_CHECK(stage_launch<T>(b, s->d_delta, s->d_cgparts, b->d_flags));
+    FG_CHECK(cost_launch<T>(b, b->cur ^ 1, false, true, false));
+    tail.cost_parts = (const double*)b->d_parts;
+    tail.cost_blocks = cost_blocks(b);
+    tail.cost_flag = (const int*)b->d_flags;
+    tail.norm_parts = (const double*)s->d_cgparts;
+    tail.norm_blocks = stage_blocks(b);
+  } else {
+    FG_CHECK(norm_inf_launch<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0));
+    tail.norm = (const double*)s->d_scalar;
+    tail.norm_flag = (const int*)s->d_flags;
   }
-  FG_CHECK(norm_inf_launch<double>(s, s->d_delta, 9 * b->nc + 3 * b->np, nullptr, 0));
-  // one kernel gathers the PCG state, the verification sums, the step norm and the trial
-  // cost into host memory; one synchronization
-  hipLaunchKernelGGL(k_collect_attempt, dim3(1), dim3(kBlock), 0, b->stream, (const DevScalars*)s->d_sc,
-                     (const int*)s->d_fail, speculate_trial ? (const double*)b->d_out : (const double*)nullptr,
-                     speculate_trial ? (const int*)b->d_flags : (const int*)nullptr, (const double*)s->d_scalar,
-                     (const int*)s->d_flags, (const double*)s->d_sums, s->d_att);
+  // one kernel forms the column sums, the step norm and the trial cost and gathers them with
+  // the PCG state into host memory; one synchronization
+  hipLaunchKernelGGL(k_attempt_tail, dim3(1), dim3(kBlock), 0, b->stream, tail);
   FG_TRY(hipGetLastError());
   FG_TRY(hipStreamSynchronize(b->stream));
   const AttemptReadback& att = *s->h_att;

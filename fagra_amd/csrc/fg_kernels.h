@@ -2317,37 +2317,118 @@ __global__ void __launch_bounds__(kWave) k_camera_finish(int nc, const double* _
 // ---------------------------------------------------------------------------------------
 // Column sums of a (rows x kParts) partial array in fixed order (deterministic).
 // ---------------------------------------------------------------------------------------
-// Everything the host reads after a damped attempt, gathered by one kernel into host memory:
-// the PCG state and block-factor failures, the trial cost and its flag, ||delta||_inf and its
-// nonfinite flag, and the verification sums.
+// Up to four column-sum jobs: job j sums rows[j] rows of the partial array starting at row
+// first[j] into out[slot[j] * kParts ...].
+struct SumJobs {
+  int rows[4], first[4], slot[4];
+};
+// Column sums of `rows` rows of kParts partials by one workgroup (fixed assignment and tree:
+// per-thread strided sums, wave sums, waves in order); valid in threads 0 .. kParts - 1.
+__device__ __forceinline__ double column_sums(const double* __restrict__ p, int rows, double (*smem)[kBlock / kWave]) {
+  double s[kParts];
+#pragma unroll
+  for (int col = 0; col < kParts; ++col) s[col] = 0.0;
+  for (int i = threadIdx.x; i < rows; i += blockDim.x)
+#pragma unroll
+    for (int col = 0; col < kParts; ++col) s[col] += p[kParts * i + col];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int col = 0; col < kParts; ++col) {
+    const double v = wave_sum(s[col]);
+    if (lane == 0) smem[col][wave] = v;
+  }
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < kParts)
+    for (int w = 0; w < kBlock / kWave; ++w) t += smem[threadIdx.x][w];
+  __syncthreads();
+  return t;
+}
+
+// The end of an attempt in one workgroup (no separate reduction launches): the check's column
+// sums (jobs, the tree of column_sums), the trial cost from k_cost's per-workgroup partials
+// (the tree of k_reduce_parts' sum) when cost_parts is set, ||delta||_inf and its nonfinite flag
+// from k_stage's partials when norm_parts is set (else from norm / norm_flag), and the PCG
+// state and factor failures, gathered into the host-visible readback.
 struct AttemptReadback {
   DevScalars sc;
   int fail, cost_flag, norm_flag, pad;
   double cost, norm;
   double sums[3 * kParts];
 };
-static __global__ void k_collect_attempt(const DevScalars* __restrict__ sc, const int* __restrict__ fail,
-                                         const double* __restrict__ cost, const int* __restrict__ cost_flag,
-                                         const double* __restrict__ norm, const int* __restrict__ norm_flag,
-                                         const double* __restrict__ sums, AttemptReadback* out) {
+struct TailArgs {
+  const DevScalars* sc;
+  const int* fail;
+  const double* parts;  // the check's partial rows
+  SumJobs jobs;
+  int njobs;
+  const double* cost_parts;  // k_cost's partials (nullptr: no trial cost)
+  int cost_blocks;
+  const int* cost_flag;
+  const double* norm_parts;  // k_stage's [max, bad] pairs (nullptr: norm, norm_flag)
+  int norm_blocks;
+  const double* norm;
+  const int* norm_flag;
+  double* sums;  // the column sums, also kept on the device
+  AttemptReadback* out;
+};
+static __global__ void __launch_bounds__(kBlock) k_attempt_tail(TailArgs a) {
+  __shared__ double smem[kParts][kBlock / kWave];
   const int t = threadIdx.x;
-  if (t < 3 * kParts) out->sums[t] = sums[t];
+  for (int j = 0; j < a.njobs; ++j) {
+    const double v = column_sums(a.parts + (size_t)kParts * a.jobs.first[j], a.jobs.rows[j], smem);
+    if (t < kParts) {
+      a.sums[kParts * a.jobs.slot[j] + t] = v;
+      a.out->sums[kParts * a.jobs.slot[j] + t] = v;
+    }
+  }
+  if (a.cost_parts != nullptr) {  // uniform
+    double v = 0.0;
+    for (int i = t; i < a.cost_blocks; i += blockDim.x) v += a.cost_parts[i];
+    v = block_sum(v, &smem[0][0]);
+    if (t == 0) {
+      a.out->cost = v;
+      a.out->cost_flag = *a.cost_flag;
+    }
+    __syncthreads();
+  } else if (t == 0) {
+    a.out->cost = 0.0;
+    a.out->cost_flag = 0;
+  }
+  if (a.norm_parts != nullptr) {  // uniform
+    double m = 0.0;
+    int bad = 0;
+    for (int i = t; i < a.norm_blocks; i += blockDim.x) {
+      m = fmax(m, a.norm_parts[2 * i]);
+      bad |= a.norm_parts[2 * i + 1] != 0.0;
+    }
+    m = wave_max(m);
+    bad = __any(bad) ? 1 : 0;
+    const int wave = t / kWave, lane = t % kWave;
+    if (lane == 0) {
+      smem[0][wave] = m;
+      smem[1][wave] = bad;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double r = 0.0, b = 0.0;
+      for (int w = 0; w < kBlock / kWave; ++w) {
+        r = fmax(r, smem[0][w]);
+        b = fmax(b, smem[1][w]);
+      }
+      a.out->norm = r;
+      a.out->norm_flag = b != 0.0 ? 2 : 0;
+    }
+  } else if (t == 0) {
+    a.out->norm = *a.norm;
+    a.out->norm_flag = *a.norm_flag;
+  }
   if (t == 0) {
-    out->sc = *sc;
-    out->fail = *fail;
-    out->cost_flag = cost_flag ? *cost_flag : 0;
-    out->cost = cost ? *cost : 0.0;
-    out->norm_flag = *norm_flag;
-    out->norm = *norm;
+    a.out->sc = *a.sc;
+    a.out->fail = *a.fail;
   }
 }
 
-// Up to four column-sum jobs in one launch, one workgroup each: job j sums rows[j] rows of the
-// partial array starting at row first[j] into out[slot[j] * kParts ...].
-struct SumJobs {
-  int rows[4], first[4], slot[4];
-};
-static __global__ void k_sum_parts_jobs(const double* __restrict__ parts, SumJobs jobs, double* __restrict__ sums);
 
 static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, double* __restrict__ out) {
   __shared__ double smem[kParts][kBlock / kWave];
@@ -2368,31 +2449,6 @@ static __global__ void k_sum_parts(const double* __restrict__ parts, int rows, d
     double t = 0.0;
     for (int w = 0; w < kBlock / kWave; ++w) t += smem[threadIdx.x][w];
     out[threadIdx.x] = t;
-  }
-}
-
-static __global__ void k_sum_parts_jobs(const double* __restrict__ parts, SumJobs jobs, double* __restrict__ sums) {
-  __shared__ double smem[kParts][kBlock / kWave];
-  const int j = blockIdx.x;
-  const double* p = parts + (size_t)kParts * jobs.first[j];
-  const int rows = jobs.rows[j];
-  double s[kParts];
-#pragma unroll
-  for (int col = 0; col < kParts; ++col) s[col] = 0.0;
-  for (int i = threadIdx.x; i < rows; i += blockDim.x)
-#pragma unroll
-    for (int col = 0; col < kParts; ++col) s[col] += p[kParts * i + col];
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-#pragma unroll
-  for (int col = 0; col < kParts; ++col) {
-    const double v = wave_sum(s[col]);
-    if (lane == 0) smem[col][wave] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < kParts) {
-    double t = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) t += smem[threadIdx.x][w];
-    sums[kParts * jobs.slot[j] + threadIdx.x] = t;
   }
 }
 
@@ -2701,17 +2757,49 @@ __global__ void k_recover_cameras(int n, const double* __restrict__ x, const T* 
 // State staging: x_trial = x + delta (additive BAL retraction, model.rs:20-28), plus the
 // compute-precision point copy.
 template <typename T>
-__global__ void k_stage(int nc, int np, const double* __restrict__ cams, const double* __restrict__ pts,
-                        const double* __restrict__ dcam, const double* __restrict__ dpt, double* __restrict__ tcams,
-                        double* __restrict__ tpts, T* __restrict__ tpts_t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 9 * nc) {
-    tcams[i] = cams[i] + dcam[i];
-  } else if (i < 9 * nc + 3 * np) {
-    const int j = i - 9 * nc;
-    const double v = pts[j] + dpt[j];
-    tpts[j] = v;
-    tpts_t[j] = T(v);
+// With nparts: also each workgroup's max |delta| and whether a delta entry is nonfinite
+// (nparts[2 b], nparts[2 b + 1]), which k_attempt_tail reduces to ||delta||_inf and its flag
+// (the same maximum k_norm_inf forms, max being exact in any order); with zero_flag: also clears
+// that flag word for the kernels that follow in stream order. Grid-stride.
+__global__ void __launch_bounds__(kBlock) k_stage(int nc, int np, const double* __restrict__ cams,
+                                                  const double* __restrict__ pts, const double* __restrict__ dcam,
+                                                  const double* __restrict__ dpt, double* __restrict__ tcams,
+                                                  double* __restrict__ tpts, T* __restrict__ tpts_t,
+                                                  double* __restrict__ nparts, int* zero_flag) {
+  __shared__ double smem[kBlock / kWave];
+  __shared__ int sbad;
+  const int n = 9 * nc + 3 * np;
+  double v = 0.0;
+  bool bad = false;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double d;
+    if (i < 9 * nc) {
+      d = dcam[i];
+      tcams[i] = cams[i] + d;
+    } else {
+      const int j = i - 9 * nc;
+      d = dpt[j];
+      const double x = pts[j] + d;
+      tpts[j] = x;
+      tpts_t[j] = T(x);
+    }
+    bad |= !isfinite(d);
+    v = fmax(v, fabs(d));
+  }
+  if (zero_flag != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *zero_flag = 0;
+  if (nparts == nullptr) return;  // uniform over the grid
+  if (threadIdx.x == 0) sbad = 0;
+  v = wave_max(v);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  __syncthreads();
+  if (bad) sbad = 1;
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) r = fmax(r, smem[w]);
+    nparts[2 * blockIdx.x] = r;
+    nparts[2 * blockIdx.x + 1] = sbad ? 1.0 : 0.0;
   }
 }
 
