@@ -838,9 +838,9 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
     tail.norm = (const double*)s->d_scalar;
     tail.norm_flag = (const int*)s->d_flags;
   }
-  // one kernel forms the column sums, the step norm and the trial cost and gathers them with
+  // one launch forms the column sums, the step norm and the trial cost and gathers them with
   // the PCG state into host memory; one synchronization
-  hipLaunchKernelGGL(k_attempt_tail, dim3(1), dim3(kBlock), 0, b->stream, tail);
+  hipLaunchKernelGGL(k_attempt_tail, dim3((unsigned)tail.njobs + 1), dim3(kBlock), 0, b->stream, tail);
   FG_TRY(hipGetLastError());
   FG_TRY(hipStreamSynchronize(b->stream));
   const AttemptReadback& att = *s->h_att;

@@ -2345,8 +2345,8 @@ __device__ __forceinline__ double column_sums(const double* __restrict__ p, int 
   return t;
 }
 
-// The end of an attempt in one workgroup (no separate reduction launches): the check's column
-// sums (jobs, the tree of column_sums), the trial cost from k_cost's per-workgroup partials
+// The end of an attempt in one launch (no separate reduction launches): the check's column
+// sums (jobs, the tree of column_sums, one workgroup each), the trial cost from k_cost's per-workgroup partials
 // (the tree of k_reduce_parts' sum) when cost_parts is set, ||delta||_inf and its nonfinite flag
 // from k_stage's partials when norm_parts is set (else from norm / norm_flag), and the PCG
 // state and factor failures, gathered into the host-visible readback.
@@ -2373,14 +2373,18 @@ struct TailArgs {
   AttemptReadback* out;
 };
 static __global__ void __launch_bounds__(kBlock) k_attempt_tail(TailArgs a) {
+  // workgroup j < njobs: column-sum job j; the last workgroup: cost, norm, state (launched with
+  // njobs + 1 workgroups, which share nothing)
   __shared__ double smem[kParts][kBlock / kWave];
   const int t = threadIdx.x;
-  for (int j = 0; j < a.njobs; ++j) {
+  const int j = blockIdx.x;
+  if (j < a.njobs) {
     const double v = column_sums(a.parts + (size_t)kParts * a.jobs.first[j], a.jobs.rows[j], smem);
     if (t < kParts) {
       a.sums[kParts * a.jobs.slot[j] + t] = v;
       a.out->sums[kParts * a.jobs.slot[j] + t] = v;
     }
+    return;
   }
   if (a.cost_parts != nullptr) {  // uniform
     double v = 0.0;
