@@ -738,14 +738,17 @@ void cg_collect(fg_schur* s) {
 // partial-sum buffer, idle once the solve has ended.
 constexpr int kStageBlocks = 1024;
 inline int stage_blocks(const fg_ba* b) { return std::min(kStageBlocks, grid_for(9 * b->nc + 3 * b->np)); }
+// The staging also forms the trial buffer's camera tables (what ensure_tables would launch).
 template <typename T>
 fg_status stage_launch(fg_ba* b, const double* d_delta, double* nparts = nullptr, int* zero_flag = nullptr) {
   const int other = b->cur ^ 1;
   hipLaunchKernelGGL(k_stage<T>, dim3(stage_blocks(b)), dim3(kBlock), 0, b->stream, (int)b->nc, (int)b->np,
                      b->d_cams[b->cur], b->d_pts[b->cur], d_delta, d_delta + 9 * b->nc, b->d_cams[other],
-                     b->d_pts[other], (T*)b->d_pts_t[other], nparts, zero_flag);
+                     b->d_pts[other], (T*)b->d_pts_t[other], b->d_tab64[other],
+                     b->fp32 ? (T*)b->d_tab[other] : (T*)nullptr, b->fp32 ? b->d_tabw[other] : (double*)nullptr,
+                     nparts, zero_flag);
   FG_TRY(hipGetLastError());
-  b->tab_valid[other] = false;
+  b->tab_valid[other] = true;
   return FG_OK;
 }
 

@@ -188,14 +188,11 @@ __device__ __forceinline__ double block_sum_parts_col(const double* parts, int n
 // ---------------------------------------------------------------------------------------
 // Camera tables: Projection::new (examples/bal/model.rs:126-146) in f64, stored in T.
 // ---------------------------------------------------------------------------------------
+// Projection::new for camera c from its 9 parameters cam (f64): the table row in tab64 and, with
+// lo, its rounding to T and that rounding widened back to f64 (lo_wide).
 template <typename T>
-// f64 table into tab64; with f32 storage (lo != nullptr) also its entry-by-entry rounding into
-// lo and that rounding widened back to f64 into lo_wide, in the same pass.
-__global__ void k_camera_tables(int nc, const double* __restrict__ cams, double* __restrict__ tab64,
-                                T* __restrict__ lo, double* __restrict__ lo_wide) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
-  const double* cam = cams + 9 * c;
+__device__ __forceinline__ void camera_table(int c, const double* cam, double* __restrict__ tab64,
+                                             T* __restrict__ lo, double* __restrict__ lo_wide) {
   const double w0 = cam[0], w1 = cam[1], w2 = cam[2];
   const double t2 = w0 * w0 + w1 * w1 + w2 * w2;
   double a, b;
@@ -253,6 +250,13 @@ __global__ void k_camera_tables(int nc, const double* __restrict__ cams, double*
       lo_wide[kCamTab * c + i] = double(r);
     }
   }
+}
+template <typename T>
+__global__ void k_camera_tables(int nc, const double* __restrict__ cams, double* __restrict__ tab64,
+                                T* __restrict__ lo, double* __restrict__ lo_wide) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  camera_table<T>(c, cams + 9 * c, tab64, lo, lo_wide);
 }
 
 // Blockwise Huber (src/factors.rs:42-52): cost and sqrt(weight).
@@ -2760,33 +2764,44 @@ __global__ void k_recover_cameras(int n, const double* __restrict__ x, const T* 
 
 // State staging: x_trial = x + delta (additive BAL retraction, model.rs:20-28), plus the
 // compute-precision point copy.
+// The trial state cur + delta into the other buffer: one thread per camera stages its 9
+// parameters and, with tab64, forms the trial camera tables from them (camera_table, as
+// k_camera_tables would: f64 table, and with lo its rounding to T and the widening); the points
+// are staged element by element. With nparts: also each workgroup's max |delta| and whether a
+// delta entry is nonfinite (nparts[2 b], nparts[2 b + 1]), which k_attempt_tail reduces to
+// ||delta||_inf and its flag (the same maximum k_norm_inf forms, max being exact in any order);
+// with zero_flag: also clears that flag word for the kernels that follow in stream order.
+// Grid-stride.
 template <typename T>
-// With nparts: also each workgroup's max |delta| and whether a delta entry is nonfinite
-// (nparts[2 b], nparts[2 b + 1]), which k_attempt_tail reduces to ||delta||_inf and its flag
-// (the same maximum k_norm_inf forms, max being exact in any order); with zero_flag: also clears
-// that flag word for the kernels that follow in stream order. Grid-stride.
 __global__ void __launch_bounds__(kBlock) k_stage(int nc, int np, const double* __restrict__ cams,
                                                   const double* __restrict__ pts, const double* __restrict__ dcam,
                                                   const double* __restrict__ dpt, double* __restrict__ tcams,
                                                   double* __restrict__ tpts, T* __restrict__ tpts_t,
-                                                  double* __restrict__ nparts, int* zero_flag) {
+                                                  double* __restrict__ tab64, T* __restrict__ lo,
+                                                  double* __restrict__ lo_wide, double* __restrict__ nparts,
+                                                  int* zero_flag) {
   __shared__ double smem[kBlock / kWave];
   __shared__ int sbad;
-  const int n = 9 * nc + 3 * np;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
   double v = 0.0;
   bool bad = false;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double d;
-    if (i < 9 * nc) {
-      d = dcam[i];
-      tcams[i] = cams[i] + d;
-    } else {
-      const int j = i - 9 * nc;
-      d = dpt[j];
-      const double x = pts[j] + d;
-      tpts[j] = x;
-      tpts_t[j] = T(x);
+  for (int c = gid; c < nc; c += gsz) {
+    double tc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const double d = dcam[9 * c + k];
+      tc[k] = cams[9 * c + k] + d;
+      tcams[9 * c + k] = tc[k];
+      bad |= !isfinite(d);
+      v = fmax(v, fabs(d));
     }
+    if (tab64 != nullptr) camera_table<T>(c, tc, tab64, lo, lo_wide);
+  }
+  for (int j = gid; j < 3 * np; j += gsz) {
+    const double d = dpt[j];
+    const double x = pts[j] + d;
+    tpts[j] = x;
+    tpts_t[j] = T(x);
     bad |= !isfinite(d);
     v = fmax(v, fabs(d));
   }
