@@ -508,27 +508,28 @@ fg_status gradient_finish(fg_schur* s, bool spec, double* out) {
 // direction reads it. rhs_points: the point pass also forms the right-hand side's point solves
 // into s->tp.
 template <typename T>
-// With learned (and no timings), the camera factorization's launch also forms the learned
-// preconditioner's explicit inverse in s->gmat (k_factor_cameras_precond).
+// With learned (and no timings), the point factorization's launch also forms the learned
+// preconditioner's explicit inverse in s->gmat (k_factor_points_precond).
 fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = false, bool learned = false) {
   fg_ba* b = s->ba;
   FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
+  hipLaunchKernelGGL(k_factor_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
+                     b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
   if (learned && !s->cfg.collect_timings) {
     const fg_mlp* m = s->mlp;
     const PrecondArgs pa{(int)b->nc, b->d_cam_off, s->hcc, lambda, m->w1, m->w2, m->w3,
                          m->params + kB1, m->params + kB2, m->params + kB3, s->bs, m->range, (double*)s->gmat};
     const int lblocks = (int)((b->nc + 15) / 16);
-    hipLaunchKernelGGL(k_factor_cameras_precond<T>, dim3((unsigned)(lblocks + grid_for(9 * b->nc))), dim3(kBlock),
-                       0, b->stream, lblocks, pa, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj,
-                       s->d_fail);
+    hipLaunchKernelGGL(k_factor_points_precond<T>, dim3((unsigned)(lblocks + grid_for(b->np))), dim3(kBlock), 0,
+                       b->stream, lblocks, pa, (int)b->np, b->d_pt_len, (const double*)s->cs, lambda,
+                       (double*)s->mp, s->d_fail, (const T*)s->gp, (const T*)s->dp,
+                       rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp, (double*)s->lpp);
   } else {
-    hipLaunchKernelGGL(k_factor_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
-                       b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
+    hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
+                       b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail, (const T*)s->gp,
+                       (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp,
+                       (double*)s->lpp);
   }
-  hipLaunchKernelGGL(k_factor_points<T>, dim3(grid_for(b->np)), dim3(kBlock), 0, b->stream, (int)b->np,
-                     b->d_pt_len, (const double*)s->cs, lambda, (double*)s->mp, s->d_fail, (const T*)s->gp,
-                     (const T*)s->dp, rhs_points ? (double*)s->tp : (double*)nullptr, (const double*)s->hpp,
-                     (double*)s->lpp);
   FG_TRY(hipGetLastError());
   if (fails == nullptr) return FG_OK;
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));

@@ -437,22 +437,22 @@ static __global__ void __launch_bounds__(256) k_mlp_precond(PrecondArgs pa) {
   mlp_precond_block<T>(blockIdx.x, pa);
 }
 
-// The attempt's camera factorization (k_factor_cameras) and the learned preconditioner in one
+// The attempt's point factorization (k_factor_points) and the learned preconditioner in one
 // launch: workgroups 0 .. learned_blocks - 1 form M for 16 cameras each (mlp_precond_block, which
-// factors its cameras' blocks itself), the rest factor the cameras for block-Jacobi. The two are
-// independent; the network's latency-bound workgroups run beside a light kernel instead of the
-// point factorization, whose memory traffic slowed their loads several-fold.
+// factors its cameras' blocks itself, so it needs nothing the camera factorization forms), the
+// rest factor 256 points each. The network's few latency-bound workgroups run beside the point
+// factorization's instead of after it; their one memory round trip waits behind its traffic.
 template <typename T>
-static __global__ void __launch_bounds__(256) k_factor_cameras_precond(int learned_blocks, PrecondArgs pa,
-                                                                       const T* __restrict__ bs, double lambda,
-                                                                       double* __restrict__ lc,
-                                                                       double* __restrict__ mbj, int* fail_count) {
+static __global__ void __launch_bounds__(256) k_factor_points_precond(
+    int learned_blocks, PrecondArgs pa, int np, const int* __restrict__ pt_len, const double* __restrict__ cs,
+    double lambda, double* __restrict__ lp, int* fail_count, const T* __restrict__ gp, const T* __restrict__ dpv,
+    double* __restrict__ t_out, const double* __restrict__ hpp, double* __restrict__ lpp) {
   if ((int)blockIdx.x < learned_blocks) {  // uniform over the workgroup
     mlp_precond_block<T>(blockIdx.x, pa);
     return;
   }
-  factor_camera<T>(((int)blockIdx.x - learned_blocks) * blockDim.x + threadIdx.x, pa.nc, pa.cam_off, bs, lambda, lc,
-                   mbj, fail_count);
+  factor_point<T>(((int)blockIdx.x - learned_blocks) * blockDim.x + threadIdx.x, np, pt_len, cs, lambda, lp,
+                  fail_count, gp, dpv, t_out, hpp, lpp);
 }
 
 // Loss ||G^T A G - I||_F^2 / n per row and its gradient w.r.t. y (bf16 for the backward GEMM).
