@@ -805,7 +805,8 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   }
   Phase ph(s, &s->tim.verification);
   // recover_step (schur.rs:312-344) and verify_step (schur.rs:346-397) in one point pass and
-  // one camera pass (k_points<kCheck>, k_cameras<cCheck>), launched right behind the PCG and
+  // one camera-major sweep (k_points<kCheck>; k_camera_split and k_camera_finish in mode
+  // cCheck), launched right behind the PCG and
   // read back with its state in one synchronization: the reduced residual ||S x - rhs||, the
   // full-model verification of delta in original coordinates (linear_model.rs:254-282), and
   // ||delta||_inf. With a failed block factorization the PCG did not iterate and these results
