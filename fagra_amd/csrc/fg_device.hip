@@ -114,7 +114,8 @@ struct fg_ba {
 // Host side of a PCG solve in progress (launched iterations, issued and read batches, the
 // last state read).
 struct CgRun {
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;       // kBatch iterations
+  hipGraphExec_t graph_short = nullptr;  // kShortBatch iterations, for the tail of a solve
   bool timed = false;
   int max_it = 0, launched = 0, issued = 0, read = 0;
   CgState last{};
@@ -165,8 +166,9 @@ struct fg_schur {
   size_t solves = 0, iterations = 0;
   fg_schur_diagnostics diag;
   fg_schur_timings tim;
-  hipGraphExec_t cg_graph[2] = {nullptr, nullptr};  // PCG batch per accepted-state buffer
-  int cg_graph_size[2] = {0, 0};                      // sizeof(T) it was captured for
+  // PCG batches per accepted-state buffer: [slot][0] kBatch iterations, [slot][1] kShortBatch
+  hipGraphExec_t cg_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int cg_graph_size[2][2] = {{0, 0}, {0, 0}};  // sizeof(T) each was captured for
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
   int cg_launched = 0;  // PCG iterations launched by the last cg_finish (state slot parity)
   std::vector<hipEvent_t> events;  // matvec kernel timing pool (pairs)
@@ -561,12 +563,15 @@ void launch_sum_parts(fg_schur* s, int rows, int slot, int first_row = 0) {
 
 // PCG on the reduced camera system (textbook PCG, x0 = 0, ||r|| <= tol ||b||).
 // Iteration i launches direction, the two product kernels and the update. Iterations go out in
-// batches of kBatch (even, so every batch starts on the same state slot); the host polls the
-// published done flag of the previous batch while the next one runs, and iterations after the
-// solve ended are no-ops. The damping and the PCG settings travel in device state, so one
-// captured HIP graph per accepted-state buffer serves every batch of every solve; with
-// collect_timings the batch is launched kernel by kernel between timing events instead.
+// batches of kBatch, or kShortBatch when the solve is predicted to end within that many more
+// (both even, so every batch starts on the same state slot); the host polls the published done
+// flag of the previous batch while the next one runs, and iterations after the solve ended are
+// no-ops. The damping and the PCG settings travel in device state, so one captured HIP graph per
+// batch size and accepted-state buffer serves every batch of every solve; with collect_timings
+// the batch is launched kernel by kernel between timing events instead.
 constexpr int kBatch = 4;
+// The batch that goes out when the solve is predicted to end within it (even, like kBatch).
+constexpr int kShortBatch = 2;
 
 template <typename T>
 void launch_cg_iteration(fg_schur* s, int i, bool timed) {
@@ -595,18 +600,20 @@ void launch_cg_iteration(fg_schur* s, int i, bool timed) {
 }
 
 template <typename T>
-fg_status cg_batch_graph(fg_schur* s, hipGraphExec_t* out) {
+fg_status cg_batch_graph(fg_schur* s, hipGraphExec_t* out, int nb) {
   fg_ba* b = s->ba;
-  const int slot = b->cur;
-  if (s->cg_graph[slot] && s->cg_graph_size[slot] == (int)sizeof(T)) {
-    *out = s->cg_graph[slot];
+  const int slot = b->cur, which = nb == kBatch ? 0 : 1;
+  hipGraphExec_t& g_exec = s->cg_graph[slot][which];
+  int& g_size = s->cg_graph_size[slot][which];
+  if (g_exec && g_size == (int)sizeof(T)) {
+    *out = g_exec;
     return FG_OK;
   }
-  if (s->cg_graph[slot]) (void)hipGraphExecDestroy(s->cg_graph[slot]);
-  s->cg_graph[slot] = nullptr;
+  if (g_exec) (void)hipGraphExecDestroy(g_exec);
+  g_exec = nullptr;
   hipGraph_t g = nullptr;
   FG_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < kBatch; ++i) launch_cg_iteration<T>(s, i, false);
+  for (int i = 0; i < nb; ++i) launch_cg_iteration<T>(s, i, false);
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
@@ -616,8 +623,8 @@ fg_status cg_batch_graph(fg_schur* s, hipGraphExec_t* out) {
   const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ie != hipSuccess) return FG_DEVICE_ERROR;
-  s->cg_graph[slot] = ge;
-  s->cg_graph_size[slot] = (int)sizeof(T);
+  g_exec = ge;
+  g_size = (int)sizeof(T);
   *out = ge;
   return FG_OK;
 }
@@ -625,15 +632,16 @@ fg_status cg_batch_graph(fg_schur* s, hipGraphExec_t* out) {
 // Host side of one PCG solve in progress: cg_begin initializes and launches the first batch,
 // cg_finish polls and launches the rest.
 template <typename T>
-fg_status cg_issue(fg_schur* s, CgRun& run) {
+fg_status cg_issue(fg_schur* s, CgRun& run, int nb = kBatch) {
   fg_ba* b = s->ba;
-  if (run.graph) {
-    FG_TRY(hipGraphLaunch(run.graph, b->stream));
+  hipGraphExec_t graph = nb == kBatch ? run.graph : run.graph_short;
+  if (graph) {
+    FG_TRY(hipGraphLaunch(graph, b->stream));
   } else {
-    for (int it = 0; it < kBatch; ++it) launch_cg_iteration<T>(s, run.launched + it, run.timed);
+    for (int it = 0; it < nb; ++it) launch_cg_iteration<T>(s, run.launched + it, run.timed);
     FG_TRY(hipGetLastError());
   }
-  run.launched += kBatch;
+  run.launched += nb;
   FG_TRY(hipEventRecord(s->poll_ev[run.issued & 1], b->stream));
   ++run.issued;
   return FG_OK;
@@ -664,7 +672,10 @@ fg_status cg_begin(fg_schur* s, double lambda, bool learned, CgRun& run) {
   run = CgRun{};
   run.max_it = max_it;
   run.timed = s->cfg.collect_timings != 0;
-  if (!run.timed && s->use_graphs) FG_CHECK(cg_batch_graph<T>(s, &run.graph));
+  if (!run.timed && s->use_graphs) {
+    FG_CHECK(cg_batch_graph<T>(s, &run.graph, kBatch));
+    FG_CHECK(cg_batch_graph<T>(s, &run.graph_short, kShortBatch));
+  }
   // The directions publish their state to coherent host memory (h_mirror); after a batch's
   // event the host reads it there, possibly already overtaken by the next batch (a later
   // state; a torn read only misguides the prediction in cg_finish, and a set done flag was set).
@@ -682,16 +693,20 @@ fg_status cg_finish(fg_schur* s, CgRun& run) {
   // that the solve ends inside the batch already in flight, the host reads that batch first
   // instead: a batch launched after the end is kBatch iterations of no-op kernels, a wrong
   // prediction costs one host turnaround.
-  auto ends_in_flight = [&](const CgState& h) {
-    if (h.done || h.iters < 1 || !(h.bb > 0.0) || !(h.rr > h.tol2) || !(h.tol2 > 0.0)) return false;
+  // predicted iterations still to run beyond those launched (<= 0: the solve ends in flight;
+  // a large value where no prediction can be made)
+  auto beyond = [&](const CgState& h) -> double {
+    if (h.done || h.iters < 1 || !(h.bb > 0.0) || !(h.rr > h.tol2) || !(h.tol2 > 0.0)) return 1e9;
     const double rate = std::log(h.rr / h.bb) / h.iters;  // log of the mean decrease per iteration
-    if (!(rate < 0.0)) return false;
+    if (!(rate < 0.0)) return 1e9;
     const double need = std::ceil(std::log(h.tol2 / h.rr) / rate);
-    return h.iters + need <= run.launched - 1;
+    return h.iters + need - (run.launched - 1);
   };
   while (run.launched < run.max_it) {  // past max_it the final decision is read with the verification
-    const bool ahead = !(run.have && ends_in_flight(run.last));
-    if (ahead) FG_CHECK(cg_issue<T>(s, run));
+    // the next batch goes out unless the solve is predicted to end in flight; a short batch when
+    // it is predicted to end within kShortBatch more iterations
+    const double more = run.have ? beyond(run.last) : 1e9;
+    if (more > 0.0) FG_CHECK(cg_issue<T>(s, run, more <= kShortBatch ? kShortBatch : kBatch));
     FG_TRY(hipEventSynchronize(s->poll_ev[run.read & 1]));
     DevScalars h;
     std::memcpy(&h, (const void*)s->h_mirror, sizeof(DevScalars));
@@ -943,9 +958,11 @@ fg_status alloc_model(fg_schur* s) {
 void free_model(fg_schur* s) {
   // captured PCG batches hold the model's device pointers
   for (int i = 0; i < 2; ++i) {
-    if (s->cg_graph[i]) (void)hipGraphExecDestroy(s->cg_graph[i]);
-    s->cg_graph[i] = nullptr;
-    s->cg_graph_size[i] = 0;
+    for (int k = 0; k < 2; ++k) {
+      if (s->cg_graph[i][k]) (void)hipGraphExecDestroy(s->cg_graph[i][k]);
+      s->cg_graph[i][k] = nullptr;
+      s->cg_graph_size[i][k] = 0;
+    }
   }
   for (void** v : {&s->rec, &s->res, &s->rec_cm, &s->res_cm, &s->hpp, &s->gp, &s->hcc, &s->gc, &s->dc, &s->dp,
                    &s->bs, &s->cs, &s->lc, &s->mp, &s->lpp, &s->tp, &s->fw, &s->x, &s->r, &s->z, &s->p, &s->q, &s->rhs,
