@@ -156,6 +156,8 @@ struct fg_schur {
   void* h_poll = nullptr;      // two DevScalars, pinned
   AttemptReadback* h_att = nullptr;  // coherent host memory: a damped attempt's results
   AttemptReadback* d_att = nullptr;
+  GradReadback* h_grad = nullptr;  // coherent host memory: the gradient stage's results
+  GradReadback* d_grad = nullptr;
   DevScalars* h_mirror = nullptr;  // coherent host memory the PCG directions publish their state to
   DevScalars* d_mirror = nullptr;  // its device address
   double* h_sums = nullptr;   // 4 x kParts
@@ -472,6 +474,7 @@ fg_status prepare_damping_impl(fg_schur* s, double floor) {
 // Enqueues the gradient norm of a linearization enqueued by linearize_launch and its readback
 // together with that linearization's evaluation flag; with a finite spec_floor the damping
 // scales at that floor are enqueued too and their flag comes back in h_int[6] (damping_spec).
+// One kernel (k_gradient_tail) reduces the norm and writes it with the flags to host memory.
 template <typename T>
 fg_status gradient_launch(fg_schur* s, bool* spec_out) {
   fg_ba* b = s->ba;
@@ -479,19 +482,20 @@ fg_status gradient_launch(fg_schur* s, bool* spec_out) {
   const int blocks = std::min(1024, std::max(1, grid_for(n + m)));
   hipLaunchKernelGGL(k_norm_inf<T>, dim3(blocks), dim3(kBlock), 0, b->stream, (int)n, (const T*)s->gc, (int)m,
                      (const T*)s->gp, s->d_cgparts, s->d_flags);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_cgparts, blocks, 1,
-                     s->d_scalar);
   const bool spec = std::isfinite(s->spec_floor);
   if (spec) launch_scales<T>(s, s->spec_floor, s->d_flags + 2);
+  hipLaunchKernelGGL(k_gradient_tail, dim3(1), dim3(kBlock), 0, b->stream, (const double*)s->d_cgparts, blocks,
+                     (const int*)s->d_flags, s->d_grad);
   FG_TRY(hipGetLastError());
-  FG_TRY(hipMemcpyAsync(s->h_norm, s->d_scalar, sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  FG_TRY(hipMemcpyAsync(s->h_int + 4, s->d_flags, 3 * sizeof(int), hipMemcpyDeviceToHost, b->stream));
   *spec_out = spec;
   return FG_OK;
 }
 
-// The gradient stage's results once gradient_launch's copies have landed.
+// The gradient stage's results once gradient_launch's kernels have finished.
 fg_status gradient_finish(fg_schur* s, bool spec, double* out) {
+  const GradReadback& g = *s->h_grad;
+  s->h_norm[0] = g.norm;
+  for (int i = 0; i < 3; ++i) s->h_int[4 + i] = g.flags[i];
   if (s->h_int[5] != 0) {
     s->linearized = false;
     return FG_INVALID_EVALUATION;
@@ -1545,6 +1549,8 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
       hipHostMalloc((void**)&s->h_mirror, sizeof(DevScalars), hipHostMallocCoherent) != hipSuccess ||
       hipHostMalloc((void**)&s->h_att, sizeof(AttemptReadback), hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer((void**)&s->d_att, s->h_att, 0) != hipSuccess ||
+      hipHostMalloc((void**)&s->h_grad, sizeof(GradReadback), hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&s->d_grad, s->h_grad, 0) != hipSuccess ||
       hipHostGetDevicePointer((void**)&s->d_mirror, s->h_mirror, 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_sums, 4 * kParts * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_norm, 2 * sizeof(double), 0) != hipSuccess ||
@@ -1573,6 +1579,7 @@ void fg_schur_destroy(fg_schur* s) {
   if (s->h_poll) (void)hipHostFree(s->h_poll);
   if (s->h_mirror) (void)hipHostFree(s->h_mirror);
   if (s->h_att) (void)hipHostFree(s->h_att);
+  if (s->h_grad) (void)hipHostFree(s->h_grad);
   if (s->h_sums) (void)hipHostFree(s->h_sums);
   if (s->h_norm) (void)hipHostFree(s->h_norm);
   if (s->h_int) (void)hipHostFree(s->h_int);

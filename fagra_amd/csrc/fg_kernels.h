@@ -378,6 +378,30 @@ __global__ void __launch_bounds__(kBlock) k_cost(int nchunks, int np, const int*
   if (threadIdx.x == 0) parts[blockIdx.x] = b;
 }
 
+// The gradient stage's end in one workgroup: ||g||_inf from k_norm_inf's per-workgroup partials
+// (the maximum k_reduce_parts forms, exact in any order) and the first three flag words
+// (norm, linearization, damping scales), into host-visible memory: one kernel instead of a
+// reduction and two readback copies.
+struct GradReadback {
+  double norm;
+  int flags[4];
+};
+static __global__ void __launch_bounds__(kBlock) k_gradient_tail(const double* __restrict__ parts, int n,
+                                                                 const int* __restrict__ flags, GradReadback* out) {
+  __shared__ double smem[kBlock / kWave];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, parts[i]);
+  v = wave_max(v);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) v = fmax(v, smem[i]);
+    out->norm = v;
+    for (int i = 0; i < 3; ++i) out->flags[i] = flags[i];
+  }
+}
+
 // Final reduction of per-workgroup partials by one workgroup: sum (op 0) or maximum (op 1).
 static __global__ void k_reduce_parts(const double* __restrict__ parts, int n, int op, double* __restrict__ out) {
   __shared__ double smem[kBlock / kWave];
