@@ -200,6 +200,17 @@ struct fg_schur {
 
 namespace {
 
+// The device failure count (d_fail) is zero between solves; the attempt's tail kernel clears
+// it. An attempt that returns before its tail was enqueued clears it through this guard, so
+// that the next factorization does not count this one's failed blocks.
+struct FailCountGuard {
+  fg_schur* s;
+  bool armed = true;
+  ~FailCountGuard() {
+    if (armed) (void)hipMemsetAsync(s->d_fail, 0, sizeof(int), s->ba->stream);
+  }
+};
+
 void reset_diag(fg_schur* s, double damping) {
   s->diag.termination = FG_CG_NONE;
   s->diag.iterations = 0;
@@ -518,7 +529,10 @@ template <typename T>
 // preconditioner's explicit inverse in s->gmat (k_factor_points_precond).
 fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = false, bool learned = false) {
   fg_ba* b = s->ba;
-  FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
+  // d_fail is zero between solves: the attempt tail clears it once it has read it, and so do
+  // this function's counting path, a dropped prefetched attempt, an attempt that ends early
+  // (FailCountGuard) and the object's creation
+  if (fails != nullptr) FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
   hipLaunchKernelGGL(k_factor_cameras<T>, dim3(grid_for(9 * b->nc)), dim3(kBlock), 0, b->stream, (int)b->nc,
                      b->d_cam_off, (const T*)s->bs, lambda, (double*)s->lc, (double*)s->mbj, s->d_fail);
   if (learned && !s->cfg.collect_timings) {
@@ -541,6 +555,7 @@ fg_status factor_impl(fg_schur* s, double lambda, int* fails, bool rhs_points = 
   FG_TRY(hipMemcpyAsync(s->h_int, s->d_fail, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   FG_TRY(hipStreamSynchronize(b->stream));
   *fails = s->h_int[0];
+  FG_TRY(hipMemsetAsync(s->d_fail, 0, sizeof(int), b->stream));
   return FG_OK;
 }
 
@@ -817,6 +832,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   const int nc = (int)b->nc;
   CgRun local;
   CgRun& run = prefetched ? *prefetched : local;
+  FailCountGuard fail_guard{s};
   if (!prefetched) FG_CHECK(attempt_begin<T>(s, lambda_d, run));
   {
     Phase ph(s, &s->tim.linear_solve);
@@ -841,7 +857,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
                             (const double*)s->rhs, lambda, (double*)s->q, s->d_delta);
   TailArgs tail{};
   tail.sc = (const DevScalars*)s->d_sc;
-  tail.fail = (const int*)s->d_fail;
+  tail.fail = s->d_fail;
   tail.parts = (const double*)s->d_parts;
   tail.jobs = SumJobs{{nc, pblocks, nc, 0}, {0, 2 * nc, nc, 0}, {0, 1, 2, 0}};
   tail.njobs = 3;
@@ -866,6 +882,7 @@ fg_status attempt_impl(fg_schur* s, double lambda_d, double* predicted, bool spe
   // the PCG state into host memory; one synchronization
   hipLaunchKernelGGL(k_attempt_tail, dim3((unsigned)tail.njobs + 1), dim3(kBlock), 0, b->stream, tail);
   FG_TRY(hipGetLastError());
+  fail_guard.armed = false;
   FG_TRY(hipStreamSynchronize(b->stream));
   const AttemptReadback& att = *s->h_att;
   *s->h_sc = att.sc;
@@ -1543,7 +1560,9 @@ fg_status fg_schur_create(const fg_schur_config* cfg, fg_schur** out) {
       dalloc(&s->d_flags, 4) != hipSuccess || dalloc(&s->d_scalar, 4) != hipSuccess ||
       dalloc(&s->d_sums, 4 * kParts) != hipSuccess)
     return fail();
-  if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return fail();
+  if (hipMemset(s->d_sc, 0, sizeof(DevScalars)) != hipSuccess || hipMemset(s->d_fail, 0, 4 * sizeof(int)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess)
+    return fail();
   if (hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc(&s->h_poll, 2 * sizeof(DevScalars), 0) != hipSuccess ||
       hipHostMalloc((void**)&s->h_mirror, sizeof(DevScalars), hipHostMallocCoherent) != hipSuccess ||
@@ -2424,6 +2443,7 @@ static void drop_prefetch(fg_schur* s) {
   if (!s->pf_active) return;
   s->pf_active = false;
   (void)hipStreamSynchronize(s->ba->stream);
+  (void)hipMemsetAsync(s->d_fail, 0, sizeof(int), s->ba->stream);  // its factorization's count
 }
 
 static void clear_deferred(fg_schur* s) {
@@ -2450,7 +2470,9 @@ fg_status gradient_and_prefetch(fg_schur* s, double* out) {
   const bool learned = s->cfg.preconditioner == FG_PRECOND_LEARNED;
   if (spec && std::isfinite(lambda) && lambda >= 0.0 && (!learned || s->mlp)) {
     FG_TRY(hipEventRecord(s->pf_ev, s->ba->stream));
+    FailCountGuard fail_guard{s};
     FG_CHECK(attempt_begin<T>(s, lambda, s->pf_run));
+    fail_guard.armed = false;  // drop_prefetch or the attempt's tail clears it from here on
     s->pf_active = true;
     s->pf_lambda = lambda;
     FG_TRY(hipEventSynchronize(s->pf_ev));

@@ -2386,7 +2386,7 @@ struct AttemptReadback {
 };
 struct TailArgs {
   const DevScalars* sc;
-  const int* fail;
+  int* fail;  // read, then cleared for the next attempt's factorization
   const double* parts;  // the check's partial rows
   SumJobs jobs;
   int njobs;
@@ -2458,6 +2458,7 @@ static __global__ void __launch_bounds__(kBlock) k_attempt_tail(TailArgs a) {
   if (t == 0) {
     a.out->sc = *a.sc;
     a.out->fail = *a.fail;
+    *a.fail = 0;
   }
 }
 

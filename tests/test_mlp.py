@@ -138,7 +138,7 @@ def test_learned_preconditioner_solves_the_same_system(force_fallback):
 
 @pytest.mark.gpu
 def test_learned_preconditioner_fused_and_timed_launches_agree():
-    """Without timings the learned inverse is formed in the camera factorization's launch; with
+    """Without timings the learned inverse is formed in the point factorization's launch; with
     collect_timings it runs as its own kernel, timed as SchurTimings::preconditioner. Both apply
     the same arithmetic to the same operands, so the damped step is the same bit for bit."""
     import fagra_amd as fa
