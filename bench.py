@@ -73,7 +73,7 @@ def kernel_bytes(nc, npts, no, precision):
     return points, cameras
 
 
-def survey_matvec_bytes(nc, npts, no):
+def stored_jacobian_matvec_bytes(nc, npts, no):
     """The stored-Jacobian two-pass design's bytes per FP32 matvec, for comparison."""
     return no * 2 * (96 + 4) + npts * (24 + 12 + 12) + nc * (180 + 36 + 36)
 
@@ -341,8 +341,8 @@ def main():
                          "for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (how the path would shard "
-                                   "over several GPUs is a design note, not a measurement)"}))
+        print(json.dumps({"error": "fagra_amd measures one GPU: --gpus must be 1 (the sharding of the "
+                                   "path over several GPUs is described, not measured)"}))
         sys.exit(2)
     if args.profile_products:
         return profile_products(args)
@@ -446,7 +446,7 @@ def main():
                 "schur_product": {"k_points_us": round(t_pts * 1e6, 2), "k_cameras_us": round(t_cams * 1e6, 2),
                                   "bytes": b_pts + b_cams,
                                   "frac": round((b_pts + b_cams) / (t_pts + t_cams) / 1e9 / HBM_PEAK_GBS, 4),
-                                  "stored_jacobian_design_bytes": survey_matvec_bytes(
+                                  "stored_jacobian_design_bytes": stored_jacobian_matvec_bytes(
                                       p.n_cameras, p.n_points, p.n_observations)}}
     log(f"bench: roofline {kname}: {ksec * 1e6:.1f} us, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%})")
 
