@@ -137,9 +137,11 @@ def cpu_model():
 def oracle_sample(O, problem, cg_tol):
     """One full LM step of the oracle at the benchmark's settings, timed by component:
     linearization, damping preparation, cost of a state, and two damped solves at the initial
-    damping 1e-3 (CG tolerance 1e-2 and cg_tol) whose difference gives the time per PCG
-    iteration and, from it, the fixed part of a solve (factorization, right-hand side,
-    recovery, verification)."""
+    damping 1e-3 (CG tolerance 1e-2 and cg_tol), and three reduced-camera products at that
+    damping, whose fastest is the time per PCG iteration (the iteration's vector work is O(cameras)
+    and left out). The fixed part of a solve (factorization, right-hand side, recovery,
+    verification) is the cg_tol solve less its iterations. Two solves a few iterations apart
+    are too close to difference: the first also pays for first-touch memory."""
     ba = O.BA(problem, huber=problem.huber)
     t = time.perf_counter()
     st, s = O.ba_schur(ba)
@@ -158,10 +160,13 @@ def oracle_sample(O, problem, cg_tol):
         t = time.perf_counter()
         st, _, _ = s.solve_damped(1e-3)
         solves.append((time.perf_counter() - t, s.statistics()[1] - n0))
-    (ta, na), (tb, nb) = solves
-    t_iter = (tb - ta) / (nb - na) if nb > na else 0.0
-    if t_iter <= 0.0:  # timer noise on tiny samples: charge the whole solve per iteration
-        t_iter = tb / max(nb, 1)
+    tb, nb = solves[1]
+    x = np.random.default_rng(0).standard_normal(9 * problem.n_cameras)
+    t_iter = float("inf")
+    for _ in range(3):
+        t = time.perf_counter()
+        assert s.reduced_apply(1e-3, x)[0] == O.OK
+        t_iter = min(t_iter, time.perf_counter() - t)
     t_fixed = max(tb - nb * t_iter, 0.0)
     return {"linearize": t_lin, "prepare_damping": t_prep, "cost": t_cost, "solve": tb, "solve_iterations": nb,
             "per_iteration": t_iter, "solve_fixed": t_fixed,
@@ -194,7 +199,8 @@ def cpu_baseline(problem, counts, cg_tol):
     r = lambda d: {k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in d.items()}
     return {"value": project(t1), "unit": "s", "cores": 1, "kind": "port",
             "sample": ("oracle f64, one LM step of the same workload (linearize, prepare_damping, cost, "
-                       f"solve_damped at lambda 1e-3 to CG tol 1e-2 and {cg_tol:g}), 1 thread pinned to cpu {pin}; "
+                       f"solve_damped at lambda 1e-3 to CG tol 1e-2 and {cg_tol:g}, 3 reduced-camera products for the "
+                       f"per-iteration time), 1 thread pinned to cpu {pin}; "
                        "projected to the target with the FP64 GPU run's counts "
                        f"({counts['linearizations']} linearizations, {counts['solves']} solves, "
                        f"{counts['iterations']} PCG iterations, {counts['cost_evaluations']} costs)"),
